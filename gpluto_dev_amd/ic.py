@@ -1,0 +1,166 @@
+"""Seeded synthetic initial conditions for the stage-update workloads.
+
+Every field is a deterministic function of (seed, resolution): the random
+Fourier coefficients are drawn once, in a fixed order over the mode cube
+[-KMAX, KMAX]^3, with numpy's PCG64 generator, and the field is evaluated by
+a separable direct sum (no FFT), so the same seed gives the same continuous
+field at every resolution and on every host.
+
+Layouts follow the reference's data model (Src/Array.hpp:476-479, i fastest):
+``vc[nv, k, j, i]`` cell-centred primitives in the reference's variable
+order (MHD: rho, vx1, vx2, vx3, Bx1, Bx2, Bx3, prs -- Src/MHD/mhd.hpp:21-41;
+HD: rho, vx1, vx2, vx3, prs -- Src/HD/hd.hpp:21-34) and ``vs[d, k, j, i]``
+the face-normal field on the lower face of cell (i,j,k) in direction d
+(staggered index i means face i-1/2, reference CHANGES:18).  Boxes are
+periodic, so one face per cell is stored.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+KMAX = 8
+
+
+def _modes(kmax: int = KMAX):
+    r = np.arange(-kmax, kmax + 1)
+    kx, ky, kz = np.meshgrid(r, r, r, indexing="ij")
+    k2 = kx * kx + ky * ky + kz * kz
+    return kx, ky, kz, k2
+
+
+def _coefficients(rng: np.random.Generator, slope: float, kmax: int = KMAX):
+    """Complex mode amplitudes with E|c_k|^2 = |k|^slope on 1 <= |k| <= kmax."""
+    kx, ky, kz, k2 = _modes(kmax)
+    g = rng.standard_normal((2,) + k2.shape)
+    mask = (k2 >= 1) & (k2 <= kmax * kmax)
+    amp = np.zeros(k2.shape)
+    amp[mask] = np.power(k2[mask].astype(np.float64), 0.5 * slope)
+    c = (g[0] + 1j * g[1]) * amp / np.sqrt(2.0)
+    # variance of Re(sum_k c_k e^{i k.x}) is sum_k E|c_k|^2 / 2
+    var = 0.5 * np.sum(amp * amp)
+    return c, var
+
+
+def _synth(c: np.ndarray, n, shift, kmax: int = KMAX) -> np.ndarray:
+    """Evaluate Re sum_k c_k exp(2 pi i k.x) at x_a = (a + shift)/n per axis.
+
+    Returns an array indexed [k, j, i] (x fastest).  ``n`` and ``shift`` are
+    3-tuples (x, y, z); an axis with n == 1 is evaluated at x = 0 only for
+    the zero mode (the box is flat in that direction).
+    """
+    r = np.arange(-kmax, kmax + 1)
+    ex = [np.exp(2j * np.pi * np.outer((np.arange(n[d]) + shift[d]) / n[d], r))
+          for d in range(3)]
+    # c[kx, ky, kz]
+    t1 = np.einsum("xyz,cz->xyc", c, ex[2])           # [kx, ky, z]
+    t2 = np.einsum("xyc,by->xbc", t1, ex[1])          # [kx, y, z]
+    nx = n[0]
+    t2r = t2.reshape(t2.shape[0], -1)
+    out = ex[0].real @ t2r.real - ex[0].imag @ t2r.imag  # [x, y*z]
+    out = out.reshape(nx, n[1], n[2])
+    return np.ascontiguousarray(out.transpose(2, 1, 0))
+
+
+def _flat_modes(c: np.ndarray, dims: int) -> np.ndarray:
+    """Keep only modes with zero wavenumber along the absent directions."""
+    if dims == 3:
+        return c
+    c = c.copy()
+    k = np.arange(-KMAX, KMAX + 1)
+    if dims <= 2:
+        c[:, :, k != 0] = 0.0
+    if dims <= 1:
+        c[:, k != 0, :] = 0.0
+    return c
+
+
+def mhd_turbulence(n, seed: int, gamma: float = 5.0 / 3.0, beta: float = 10.0,
+                   mach: float = 0.1, drho: float = 0.1):
+    """Random-field MHD box: the 3D/2D ideal-MHD workload.
+
+    rho = 1 + drho*G (G unit-variance Gaussian random field, per-mode power
+    |k|^-2, 1 <= |k| <= 8), p = 1, velocity components independent random
+    fields scaled to rms |v| = mach * c_s (c_s = sqrt(gamma p / rho0)), face B
+    the discrete curl of an edge-centred random vector potential with the same
+    spectrum, scaled so that 2 <p> / <B^2> = beta.
+
+    ``n`` is (nx, ny, nz); nz == 1 gives a 2D box (nz == ny == 1 is not an MHD
+    workload here).  Returns (vc[8, nz, ny, nx], vs[dims, nz, ny, nx]).
+    """
+    nx, ny, nz = n
+    dims = 3 if nz > 1 else 2
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fields = []
+    for _ in range(7):  # rho, vx, vy, vz, Ax, Ay, Az (fixed draw order)
+        c, var = _coefficients(rng, -2.0)
+        fields.append((_flat_modes(c, dims), var))
+    cen = (0.5, 0.5, 0.5 if dims == 3 else 0.0)
+    rho_c, rho_var = fields[0]
+    # flat-mode filtering changes the variance; use the empirical one
+    g = _synth(rho_c, n, cen)
+    rho = 1.0 + drho * g / max(np.std(g), 1e-300)
+    cs = np.sqrt(gamma * 1.0 / 1.0)
+    vel = []
+    for d in range(3):
+        vc_, _ = fields[1 + d]
+        vel.append(_synth(vc_, n, cen))
+    vrms = np.sqrt(sum(np.mean(v * v) for v in vel))
+    vel = [v * (mach * cs / vrms) for v in vel]
+    # edge-centred potential: A_x at (x_i, y_{j-1/2}, z_{k-1/2}), etc.
+    zf = 0.0
+    zc = 0.5 if dims == 3 else 0.0
+    ax = _synth(fields[4][0], n, (0.5, 0.0, zf))
+    ay = _synth(fields[5][0], n, (0.0, 0.5, zf))
+    az = _synth(fields[6][0], n, (0.0, 0.0, zc))
+
+    def curl(ax, ay, az):
+        idx, idy, idz = float(nx), float(ny), float(nz)  # 1/dx on [0,1]
+        # periodic forward differences; arrays are [k, j, i]
+        dj = lambda a: np.roll(a, -1, axis=1) - a
+        di = lambda a: np.roll(a, -1, axis=2) - a
+        dk = lambda a: np.roll(a, -1, axis=0) - a
+        bx = dj(az) * idy
+        by = -di(az) * idx
+        bz = di(ay) * idx - dj(ax) * idy
+        if dims == 3:
+            bx = bx - dk(ay) * idz
+            by = by + dk(ax) * idz
+        return bx, by, bz
+
+    bx, by, bz = curl(ax, ay, az)
+    bcx = 0.5 * (bx + np.roll(bx, -1, axis=2))
+    bcy = 0.5 * (by + np.roll(by, -1, axis=1))
+    bcz = 0.5 * (bz + np.roll(bz, -1, axis=0)) if dims == 3 else bz
+    b2 = np.mean(bcx * bcx + bcy * bcy + bcz * bcz)
+    s = np.sqrt(2.0 * 1.0 / beta / b2)
+    bx, by, bz = curl(ax * s, ay * s, az * s)
+    # in 2D only Bx, By are staggered; Bz stays cell-centred (Src/MHD/CT/ct.hpp:83-89)
+    vs = np.stack([bx, by, bz]) if dims == 3 else np.stack([bx, by])
+    bcx = 0.5 * (bx + np.roll(bx, -1, axis=2))
+    bcy = 0.5 * (by + np.roll(by, -1, axis=1))
+    bcz = 0.5 * (bz + np.roll(bz, -1, axis=0)) if dims == 3 else bz
+    prs = np.ones_like(rho)
+    vc = np.stack([rho, vel[0], vel[1], vel[2], bcx, bcy, bcz, prs])
+    return np.ascontiguousarray(vc), np.ascontiguousarray(vs)
+
+
+def hd_riemann(n: int, seed: int):
+    """Seeded 1D shock tube on [0,1]: jump at x = 0.5, rho and p on each side
+    drawn from U[0.1, 1), v = 0.  Returns vc[5, 1, 1, n]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rl, rr, pl, pr = rng.uniform(0.1, 1.0, size=4)
+    x = (np.arange(n) + 0.5) / n
+    left = x < 0.5
+    vc = np.zeros((5, 1, 1, n))
+    vc[0, 0, 0] = np.where(left, rl, rr)
+    vc[4, 0, 0] = np.where(left, pl, pr)
+    return vc
+
+
+def divb_max(vs: np.ndarray) -> float:
+    """max |div B| (per unit volume, dx = 1/n) of a periodic face field."""
+    _, nz, ny, nx = vs.shape
+    d = (np.roll(vs[0], -1, axis=2) - vs[0]) * nx + (np.roll(vs[1], -1, axis=1) - vs[1]) * ny
+    if nz > 1:
+        d = d + (np.roll(vs[2], -1, axis=0) - vs[2]) * nz
+    return float(np.max(np.abs(d)))

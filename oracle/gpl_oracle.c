@@ -1,0 +1,974 @@
+/* CPU oracle for the Godunov stage update -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm, used by tests/ and by
+ * bench.py's cpu_baseline leg as the checker.  Nothing in the product path
+ * links it.  Each function names the reference file:line it restates; the
+ * order of floating-point operations inside each expression is kept as the
+ * reference writes it (C evaluates a+b+c as (a+b)+c, the same as C++), and
+ * the file is compiled with -ffp-contract=off, as the x86-64 reference
+ * build (no FMA instructions without -march) effectively is.
+ *
+ * Pinning: see tests/test_oracle_golden.py (bitwise against the reference
+ * binary's own output on committed fixtures) and tests/test_oracle_laws.py
+ * (conservation, div B, symmetry, closed forms).
+ */
+#include "gpl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- reference macros (Src/macros.hpp:12-14,191-201) ------------------ */
+#define MAX(a,b)  ( (a) >= (b) ? (a) : (b) )
+#define MIN(a,b)  ( (a) <= (b) ? (a) : (b) )
+#define ABS_MIN(a,b)  (fabs(a) < fabs(b) ? (a):(b))
+
+/* ---- error bits (Src/gpluto.hpp:247-262) ------------------------------- */
+#define ERR_NEGATIVE_RHO    (1u<<0)
+#define ERR_NEGATIVE_ENG    (1u<<1)
+#define ERR_NEGATIVE_PRS    (1u<<2)
+#define ERR_CONS2PRIM_FAIL  (1u<<4)
+#define ERR_NAN             (1u<<11)
+#define ERR_QUIT            (1u<<15)
+
+/* ---- variable indices (Src/MHD/mhd.hpp:21-33, Src/HD/hd.hpp:21-28) ----- */
+#define RHO 0
+#define MX1 1
+#define VX1 1
+#define BX1 4
+#define NVMAX 8
+
+#define NG 2   /* Src/get_nghost.cpp:33, PLM */
+
+struct gplo_state {
+  gplo_config c;
+  int nv;                 /* NVAR */
+  int nt[3];              /* ntot per direction */
+  int beg[3], end[3];     /* active range (inclusive) */
+  int ns[3];              /* staggered array extents (nt + 1) */
+  size_t ncell, nstag;
+  double dx[3];
+  double *Vc, *Uc, *U0, *Rc, *Cdt;
+  double *Vs[3], *Bs0[3], *Rs[3];
+  /* UCT face data, Src/MHD/CT/ct_emf.cpp:63-147 */
+  double *e[3][3], *aL[3], *aR[3], *dL[3], *dR[3];
+  double *Ee[3];
+  /* pencil scratch */
+  double *pv, *pvL, *pvR, *pflux, *pcmax, *pBn, *prhs;
+  double *qv[2], *qL[2], *qR[2];
+  int stage;
+};
+
+#define IC(s,i,j,k)  (((size_t)(k)*(s)->nt[1] + (size_t)(j))*(s)->nt[0] + (size_t)(i))
+#define IS(s,i,j,k)  (((size_t)(k)*(s)->ns[1] + (size_t)(j))*(s)->ns[0] + (size_t)(i))
+#define VC(s,nv,i,j,k)  (s)->Vc[(size_t)(nv)*(s)->ncell + IC(s,i,j,k)]
+#define UC(s,nv,i,j,k)  (s)->Uc[(size_t)(nv)*(s)->ncell + IC(s,i,j,k)]
+
+static int is_mhd(const gplo_state *s) { return s->c.phys == GPLO_MHD; }
+
+/* ======================================================================
+ * Pointwise physics
+ * ====================================================================== */
+
+/* MHD PrimToCons, Src/MHD/mhd.hpp:906-934 */
+static void mhd_prim_to_cons(const double *v, double *u, double gamma)
+{
+  double gmm1 = gamma - 1.0;
+  u[RHO] = v[RHO];
+  u[1] = v[RHO]*v[1];
+  u[2] = v[RHO]*v[2];
+  u[3] = v[RHO]*v[3];
+  u[4] = v[4];
+  u[5] = v[5];
+  u[6] = v[6];
+  double kinb2 = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  kinb2 = v[RHO]*kinb2 + v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  kinb2 *= 0.5;
+  u[7] = kinb2 + v[7]/gmm1;
+}
+
+/* HD PrimToCons, Src/HD/hd.hpp:392-410 */
+static void hd_prim_to_cons(const double *v, double *u, double gamma)
+{
+  u[RHO] = v[RHO];
+  u[1] = v[RHO]*v[1];
+  u[2] = v[RHO]*v[2];
+  u[3] = v[RHO]*v[3];
+  double kin = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  kin = v[RHO]*kin*0.5;
+  u[4] = kin + v[4]/(gamma - 1.0);
+}
+
+/* MHD Flux<dir>, Src/MHD/mhd.hpp:164-218 (no background field) */
+static void mhd_flux(const double *v, const double *u, int n, double *flux, double *pt)
+{
+  const int VXn = 1 + n, BXn = 4 + n;
+  double vB = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  *pt = v[7] + 0.5*(v[4]*v[4] + v[5]*v[5] + v[6]*v[6]);
+  flux[RHO] = u[1 + n];
+  flux[1] = u[1]*v[VXn] - v[4]*v[BXn];
+  flux[2] = u[2]*v[VXn] - v[5]*v[BXn];
+  flux[3] = u[3]*v[VXn] - v[6]*v[BXn];
+  flux[4] = v[4]*v[VXn] - v[BXn]*v[1];
+  flux[5] = v[5]*v[VXn] - v[BXn]*v[2];
+  flux[6] = v[6]*v[VXn] - v[BXn]*v[3];
+  flux[7] = (u[7] + *pt)*v[VXn] - v[BXn]*vB;
+  flux[1 + n] += *pt;
+}
+
+/* HD Flux<dir>, Src/HD/hd.hpp:105-125 */
+static void hd_flux(const double *v, const double *u, int n, double *flux, double *p)
+{
+  const int VXn = 1 + n;
+  flux[RHO] = u[1 + n];
+  flux[1] = u[1]*v[VXn];
+  flux[2] = u[2]*v[VXn];
+  flux[3] = u[3]*v[VXn];
+  flux[4] = (u[4] + v[4])*v[VXn];
+  *p = v[4];
+  flux[1 + n] += *p;
+}
+
+/* SoundSpeed2, Src/eos.hpp:84-99 (ideal EOS, classical) */
+static double sound_speed2(const double *v, double gamma, int prs)
+{
+  return gamma*v[prs]/v[RHO];
+}
+
+/* MHD MaxSignalSpeed<dir>, Src/MHD/mhd.hpp:242-280 */
+static void mhd_max_signal_speed(const double *v, double a2, int n, double *cmin, double *cmax)
+{
+  const int VXn = 1 + n;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3;
+  double gpr = a2*v[RHO];
+  double b1 = v[BXn], b2 = v[BXt], b3 = v[BXb];
+  double Btmag2 = b2*b2 + b3*b3;
+  double Bmag2  = b1*b1 + Btmag2;
+  double cf = gpr - Bmag2;
+  cf = gpr + Bmag2 + sqrt(cf*cf + 4.0*gpr*Btmag2);
+  cf = sqrt(0.5*cf/v[RHO]);
+  *cmax = v[VXn] + cf;
+  *cmin = v[VXn] - cf;
+}
+
+/* HD MaxSignalSpeed<dir>, Src/HD/hd.hpp:141-150 */
+static void hd_max_signal_speed(const double *v, double a2, int n, double *cmin, double *cmax)
+{
+  const double a = sqrt(a2);
+  *cmax = v[1 + n] + a;
+  *cmin = v[1 + n] - a;
+}
+
+/* ======================================================================
+ * Riemann solvers.  Each returns the flux (NFLX components) and the
+ * Davis speeds Sl, Sr; cmax(i) = MAX(|Sl|,|Sr|) (Src/MHD/hlld.cpp:178).
+ * ====================================================================== */
+
+/* HLLD, Src/MHD/hlld.cpp:62-500 (ideal EOS, no GLM, no background field) */
+static uint32_t mhd_hlld(const double *vl, const double *vr, int n, double gamma,
+                         double *flux, double *pSl, double *pSr)
+{
+  const int VXn = 1 + n, VXt = 1 + (n + 1)%3, VXb = 1 + (n + 2)%3;
+  const int MXn = VXn, MXt = VXt, MXb = VXb;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3;
+  const int NFLX = 8, ENG = 7;
+  int nv, switch_to_hll = 0;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], usl[NVMAX], usr[NVMAX];
+  double Uhll[NVMAX], ptl, ptr, Sl, Sr, sl_min, sl_max, sr_min, sr_max;
+
+  mhd_prim_to_cons(vl, ul, gamma);
+  mhd_prim_to_cons(vr, ur, gamma);
+  double a2L = sound_speed2(vl, gamma, 7);
+  double a2R = sound_speed2(vr, gamma, 7);
+  mhd_max_signal_speed(vl, a2L, n, &sl_min, &sl_max);   /* HLL_Speed, mhd.hpp:296 */
+  mhd_max_signal_speed(vr, a2R, n, &sr_min, &sr_max);
+  Sl = MIN(sl_min, sr_min);
+  Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+
+  mhd_flux(vl, ul, n, fl, &ptl);
+  mhd_flux(vr, ur, n, fr, &ptr);
+
+  double dul, dur, Bx, Bx1, sBx, pts, SM;
+  {
+    double scrh = 1.0/(Sr - Sl);
+    Bx1 = Bx = (Sr*vr[BXn] - Sl*vl[BXn])*scrh;
+    sBx = (Bx > 0.0 ? 1.0 : -1.0);
+    dul = Sl - vl[VXn];
+    dur = Sr - vr[VXn];
+    scrh = 1.0/(dur*ur[RHO] - dul*ul[RHO]);
+    SM = (dur*ur[MXn] - dul*ul[MXn] - ptr + ptl)*scrh;
+    pts =   dur*ur[RHO]*ptl - dul*ul[RHO]*ptr
+          + vl[RHO]*vr[RHO]*dur*dul*(vr[VXn]- vl[VXn]);
+    pts *= scrh;
+  }
+  usl[RHO] = ul[RHO]*dul/(Sl - SM);
+  usr[RHO] = ur[RHO]*dur/(Sr - SM);
+  if (usl[RHO] < 0.0 || usr[RHO] < 0.0) switch_to_hll = 1;
+  double sqrL = sqrt(usl[RHO]);
+  double sqrR = sqrt(usr[RHO]);
+  double S1L = SM - fabs(Bx)/sqrL;
+  double S1R = SM + fabs(Bx)/sqrR;
+
+  if (Sl >= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv];
+  } else {
+    if (!switch_to_hll) {
+      int switch_to_hllc = 0;
+      if ( (S1L - Sl) <  1.e-4*(SM - Sl) ) switch_to_hllc = 1;
+      if ( (S1R - Sr) > -1.e-4*(Sr - SM) ) switch_to_hllc = 1;
+      double scrhL, scrhR;
+      if (!switch_to_hllc) {
+        scrhL = (ul[RHO]*dul*dul - Bx*Bx)/(ul[RHO]*dul*(Sl - SM) - Bx*Bx);
+        scrhR = (ur[RHO]*dur*dur - Bx*Bx)/(ur[RHO]*dur*(Sr - SM) - Bx*Bx);
+        usl[BXn] = Bx1;
+        usl[BXt] = ul[BXt]*scrhL;
+        usl[BXb] = ul[BXb]*scrhL;
+        usr[BXn] = Bx1;
+        usr[BXt] = ur[BXt]*scrhR;
+        usr[BXb] = ur[BXb]*scrhR;
+      } else {
+        double scrh = 1.0/(Sr - Sl);
+        for (nv = 0; nv < NFLX; nv++) {
+          Uhll[nv]  = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+          Uhll[nv] *= scrh;
+        }
+        usl[BXn] = usr[BXn] = Bx1;
+        usl[BXt] = usr[BXt] = Uhll[BXt];
+        usl[BXb] = usr[BXb] = Uhll[BXb];
+        S1L = S1R = SM;
+      }
+      scrhL = Bx/(ul[RHO]*dul);
+      scrhR = Bx/(ur[RHO]*dur);
+      double vsL = vl[VXt] - scrhL*(usl[BXt] - ul[BXt]);
+      double vsR = vr[VXt] - scrhR*(usr[BXt] - ur[BXt]);
+      double wsL = vl[VXb] - scrhL*(usl[BXb] - ul[BXb]);
+      double wsR = vr[VXb] - scrhR*(usr[BXb] - ur[BXb]);
+      usl[MXn] = usl[RHO]*SM;
+      usr[MXn] = usr[RHO]*SM;
+      usl[MXt] = usl[RHO]*vsL;
+      usr[MXt] = usr[RHO]*vsR;
+      usl[MXb] = usl[RHO]*wsL;
+      usr[MXb] = usr[RHO]*wsR;
+      scrhL     = vl[VXn]*Bx1 + vl[VXt]*ul[BXt] + vl[VXb]*ul[BXb];
+      scrhL    -= SM*Bx1      +    vsL*usl[BXt] + wsL*usl[BXb];
+      usl[ENG]  = dul*ul[ENG] - ptl*vl[VXn] + pts*SM + Bx*scrhL;
+      usl[ENG] /= Sl - SM;
+      scrhR     = vr[VXn]*Bx1 + vr[VXt]*ur[BXt] + vr[VXb]*ur[BXb];
+      scrhR    -= SM*Bx1      +    vsR*usr[BXt] +    wsR*usr[BXb];
+      usr[ENG]  = dur*ur[ENG] - ptr*vr[VXn] + pts*SM + Bx*scrhR;
+      usr[ENG] /= Sr - SM;
+
+      if (S1L >= 0.0) {
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv] + Sl*(usl[nv] - ul[nv]);
+      } else if (S1R <= 0.0) {
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv] + Sr*(usr[nv] - ur[nv]);
+      } else {
+        double ussl[NVMAX], ussr[NVMAX];
+        ussl[RHO] = usl[RHO];
+        ussr[RHO] = usr[RHO];
+        double vss  = sqrL*vsL + sqrR*vsR + (usr[BXt] - usl[BXt])*sBx;
+        vss /= sqrL + sqrR;
+        double wss  = sqrL*wsL + sqrR*wsR + (usr[BXb] - usl[BXb])*sBx;
+        wss /= sqrL + sqrR;
+        ussl[MXn] = ussl[RHO]*SM;
+        ussr[MXn] = ussr[RHO]*SM;
+        ussl[MXt] = ussl[RHO]*vss;
+        ussr[MXt] = ussr[RHO]*vss;
+        ussl[MXb] = ussl[RHO]*wss;
+        ussr[MXb] = ussr[RHO]*wss;
+        ussl[BXn] = ussr[BXn] = Bx1;
+        ussl[BXt]  = sqrL*usr[BXt] + sqrR*usl[BXt] + sqrL*sqrR*(vsR - vsL)*sBx;
+        ussl[BXt] /= sqrL + sqrR;
+        ussr[BXt]  = ussl[BXt];
+        ussl[BXb]  = sqrL*usr[BXb] + sqrR*usl[BXb] + sqrL*sqrR*(wsR - wsL)*sBx;
+        ussl[BXb] /= sqrL + sqrR;
+        ussr[BXb]  = ussl[BXb];
+        scrhL  = SM*Bx1 + vsL*usl [BXt] + wsL*usl [BXb];
+        scrhL -= SM*Bx1 + vss*ussl[BXt] + wss*ussl[BXb];
+        scrhR  = SM*Bx1 + vsR*usr [BXt] + wsR*usr [BXb];
+        scrhR -= SM*Bx1 + vss*ussr[BXt] + wss*ussr[BXb];
+        ussl[ENG] = usl[ENG] - sqrL*scrhL*sBx;
+        ussr[ENG] = usr[ENG] + sqrR*scrhR*sBx;
+        if (SM >= 0.0) {
+          for (nv = 0; nv < NFLX; nv++)
+            flux[nv] = fl[nv] + S1L*(ussl[nv] - usl[nv]) + Sl*(usl[nv] - ul[nv]);
+        } else {
+          for (nv = 0; nv < NFLX; nv++)
+            flux[nv] = fr[nv] + S1R*(ussr[nv] - usr[nv]) + Sr*(usr[nv] - ur[nv]);
+        }
+      }
+    } else {
+      double scrh = 1.0/(Sr - Sl);
+      for (nv = 0; nv < NFLX; nv++) {
+        flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+        flux[nv] *= scrh;
+      }
+    }
+  }
+  uint32_t err = 0;
+  for (nv = 0; nv < NFLX; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
+/* HD HLLC, Src/HD/hllc.cpp:54-250 (ideal EOS) */
+static uint32_t hd_hllc(const double *vl, const double *vr, int n, double gamma,
+                        double *flux, double *pSl, double *pSr)
+{
+  const int VXn = 1 + n, VXt = 1 + (n + 1)%3, VXb = 1 + (n + 2)%3;
+  const int MXn = VXn, MXt = VXt, MXb = VXb;
+  const int NFLX = 5, ENG = 4, PRS = 4;
+  int nv;
+  double ul[NVMAX], ur[NVMAX], fL[NVMAX], fR[NVMAX], usl[NVMAX], usr[NVMAX];
+  double pl, pr, Sl, Sr, sl_min, sl_max, sr_min, sr_max;
+  hd_prim_to_cons(vl, ul, gamma);
+  hd_prim_to_cons(vr, ur, gamma);
+  double a2L = sound_speed2(vl, gamma, PRS);
+  double a2R = sound_speed2(vr, gamma, PRS);
+  hd_flux(vl, ul, n, fL, &pl);
+  hd_flux(vr, ur, n, fR, &pr);
+  hd_max_signal_speed(vl, a2L, n, &sl_min, &sl_max);  /* HLL_Speed, hd.hpp:165 */
+  hd_max_signal_speed(vr, a2R, n, &sr_min, &sr_max);
+  Sl = MIN(sl_min, sr_min);
+  Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+  if (Sl > 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fL[nv];
+  } else if (Sr < 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fR[nv];
+  } else {
+    double vs;
+    {
+      double qL = vl[PRS] + ul[MXn]*(vl[VXn] - Sl);
+      double qR = vr[PRS] + ur[MXn]*(vr[VXn] - Sr);
+      double wL = vl[RHO]*(vl[VXn] - Sl);
+      double wR = vr[RHO]*(vr[VXn] - Sr);
+      vs = (qR - qL)/(wR - wL);
+    }
+    {
+      double vxr = vr[VXn];
+      double vxl = vl[VXn];
+      usl[RHO] = ul[RHO]*(Sl - vxl)/(Sl - vs);
+      usr[RHO] = ur[RHO]*(Sr - vxr)/(Sr - vs);
+      usl[MXn] = usl[RHO]*vs;      usr[MXn] = usr[RHO]*vs;
+      usl[MXt] = usl[RHO]*vl[VXt]; usr[MXt] = usr[RHO]*vr[VXt];
+      usl[MXb] = usl[RHO]*vl[VXb]; usr[MXb] = usr[RHO]*vr[VXb];
+      usl[ENG] =   ul[ENG]/vl[RHO]
+                + (vs - vxl)*(vs + vl[PRS]/(vl[RHO]*(Sl - vxl)));
+      usr[ENG] =   ur[ENG]/vr[RHO]
+                + (vs - vxr)*(vs + vr[PRS]/(vr[RHO]*(Sr - vxr)));
+    }
+    usl[ENG] *= usl[RHO];
+    usr[ENG] *= usr[RHO];
+    if (vs >= 0.0) {
+      for (nv = 0; nv < NFLX; nv++) flux[nv] = fL[nv] + Sl*(usl[nv] - ul[nv]);
+    } else {
+      for (nv = 0; nv < NFLX; nv++) flux[nv] = fR[nv] + Sr*(usr[nv] - ur[nv]);
+    }
+  }
+  uint32_t err = 0;
+  for (nv = 0; nv < NFLX; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
+/* HLL (MHD and HD), Src/MHD/hll.cpp, Src/HD/hll.cpp: flux formula as in
+   the HLL branch of Src/MHD/hlld.cpp:452-461 */
+static uint32_t hll(const double *vl, const double *vr, int n, double gamma, int mhd,
+                    double *flux, double *pSl, double *pSr)
+{
+  int nv, NFLX = mhd ? 8 : 5, PRS = mhd ? 7 : 4;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], pl, pr;
+  double sl_min, sl_max, sr_min, sr_max, Sl, Sr;
+  if (mhd) { mhd_prim_to_cons(vl, ul, gamma); mhd_prim_to_cons(vr, ur, gamma); }
+  else     { hd_prim_to_cons(vl, ul, gamma);  hd_prim_to_cons(vr, ur, gamma); }
+  double a2L = sound_speed2(vl, gamma, PRS);
+  double a2R = sound_speed2(vr, gamma, PRS);
+  if (mhd) {
+    mhd_flux(vl, ul, n, fl, &pl); mhd_flux(vr, ur, n, fr, &pr);
+    mhd_max_signal_speed(vl, a2L, n, &sl_min, &sl_max);
+    mhd_max_signal_speed(vr, a2R, n, &sr_min, &sr_max);
+  } else {
+    hd_flux(vl, ul, n, fl, &pl); hd_flux(vr, ur, n, fr, &pr);
+    hd_max_signal_speed(vl, a2L, n, &sl_min, &sl_max);
+    hd_max_signal_speed(vr, a2R, n, &sr_min, &sr_max);
+  }
+  Sl = MIN(sl_min, sr_min);
+  Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+  if (Sl > 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv];
+  } else if (Sr < 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv];
+  } else {
+    double scrh = 1.0/(Sr - Sl);
+    for (nv = 0; nv < NFLX; nv++) {
+      flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+      flux[nv] *= scrh;
+    }
+  }
+  uint32_t err = 0;
+  for (nv = 0; nv < NFLX; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
+/* ======================================================================
+ * PLM slope, Src/States/plm_states.cpp:121-138 with SET_MC_LIMITER /
+ * SET_VL_LIMITER on a uniform Cartesian grid (Src/plm_coeffs.hpp:92-100)
+ * ====================================================================== */
+static double plm_slope(double vm, double v0, double vp, int lim)
+{
+  double dvp = (vp - v0)*1.0;   /* geometry.wp = 1 (reconstruction_geometry.hpp:11) */
+  double dvm = (v0 - vm)*1.0;
+  double dv;
+  if (lim == GPLO_MC_LIM) {
+    if ( (dvp)*(dvm) > 0.0) {
+      double _qc  = 0.5*( (dvm) + (dvp) ), _scrh = 2.0*ABS_MIN( (dvp), (dvm) );
+      dv   = ABS_MIN(_qc, _scrh);
+    } else dv = 0.0;
+  } else {
+    dv = ( (dvp)*(dvm) > 0.0 ? 2.0*(dvp)*(dvm)/((dvp) + (dvm)) :0.0);
+  }
+  return dv;
+}
+
+/* ======================================================================
+ * Boundary conditions
+ * ====================================================================== */
+
+/* Cell-centred B of ghost zones from faces: Src/boundary.cpp:275-303 with
+   CT_AverageStaggeredFields, Src/MHD/CT/ct_field_average.cpp:182-184 */
+static void ghost_field_average(gplo_state *s, const int lo[3], const int hi[3])
+{
+  int i, j, k, d;
+  for (k = lo[2]; k <= hi[2]; k++)
+  for (j = lo[1]; j <= hi[1]; j++)
+  for (i = lo[0]; i <= hi[0]; i++) {
+    for (d = 0; d < s->c.dims; d++) {
+      int o[3] = {0, 0, 0}; o[d] = 1;
+      double b = 0.5*(s->Vs[d][IS(s,i,j,k)] + s->Vs[d][IS(s,i+o[0],j+o[1],k+o[2])]);
+      UC(s, BX1 + d, i, j, k) = b;
+      VC(s, BX1 + d, i, j, k) = b;
+    }
+  }
+}
+
+/* Boundary(), Src/boundary.cpp:67-303: sides X1_BEG..X3_END in order;
+   PeriodicBoundary :502-669; OutflowBoundary :677-760 (cell data only). */
+static void boundary(gplo_state *s)
+{
+  int side, i, j, k, nv, d;
+  for (side = 0; side < 2*s->c.dims; side++) {
+    int dir = side/2, upper = side%2;
+    int lo[3] = {0, 0, 0}, hi[3];
+    for (d = 0; d < 3; d++) hi[d] = s->nt[d] - 1;
+    if (!upper) { lo[dir] = 0; hi[dir] = s->beg[dir] - 1; }
+    else        { lo[dir] = s->end[dir] + 1; hi[dir] = s->nt[dir] - 1; }
+    int nint = s->c.dims > dir ? (dir == 0 ? s->c.nx : dir == 1 ? s->c.ny : s->c.nz) : 1;
+    if (s->c.periodic) {
+      int sh = upper ? -nint : nint;
+      int o[3] = {0, 0, 0}; o[dir] = sh;
+      for (nv = 0; nv < s->nv; nv++)
+        for (k = lo[2]; k <= hi[2]; k++)
+        for (j = lo[1]; j <= hi[1]; j++)
+        for (i = lo[0]; i <= hi[0]; i++)
+          VC(s,nv,i,j,k) = VC(s,nv,i+o[0],j+o[1],k+o[2]);
+      if (is_mhd(s)) {
+        for (d = 0; d < s->c.dims; d++) {
+          /* BOX_LOOP on the lower side for the normal component, BOX{I,J,K}_LOOP
+             (one more face along d) otherwise */
+          int ext[3] = {0, 0, 0};
+          if (!(d == dir && !upper)) ext[d] = 1;
+          for (k = lo[2]; k <= hi[2] + ext[2]; k++)
+          for (j = lo[1]; j <= hi[1] + ext[1]; j++)
+          for (i = lo[0]; i <= hi[0] + ext[0]; i++)
+            s->Vs[d][IS(s,i,j,k)] = s->Vs[d][IS(s,i+o[0],j+o[1],k+o[2])];
+        }
+      }
+    } else {
+      int i0[3] = {0, 0, 0};
+      for (nv = 0; nv < s->nv; nv++)
+        for (k = lo[2]; k <= hi[2]; k++)
+        for (j = lo[1]; j <= hi[1]; j++)
+        for (i = lo[0]; i <= hi[0]; i++) {
+          int p[3] = {i, j, k};
+          p[dir] = upper ? s->end[dir] : s->beg[dir];
+          (void)i0;
+          VC(s,nv,i,j,k) = VC(s,nv,p[0],p[1],p[2]);
+        }
+    }
+    if (is_mhd(s)) ghost_field_average(s, lo, hi);
+  }
+}
+
+/* ======================================================================
+ * One directional sweep: DataToSweep + Reconstruct + Riemann + CT store +
+ * RightHandSide + dUc accumulation (Src/update_stage.cpp:96-187 and the
+ * J/K analogues).
+ * ====================================================================== */
+static uint32_t sweep(gplo_state *s, int dir, double dt)
+{
+  const int nvar = s->nv, mhd = is_mhd(s);
+  const int t1 = (dir + 1)%3, t2 = (dir + 2)%3;
+  const int ntr = mhd ? NG : 0;           /* Src/update_stage.cpp:80-90 */
+  const int n = s->nt[dir];
+  uint32_t err = 0;
+  int lo[3], hi[3], a, b, i, nv;
+  for (int d = 0; d < 3; d++) { lo[d] = s->beg[d]; hi[d] = s->end[d]; }
+  for (int d = 0; d < 3; d++) if (d != dir && d < s->c.dims) { lo[d] -= ntr; hi[d] += ntr; }
+  double dtdx = dt/s->dx[dir];
+  double *v = s->pv, *vL = s->pvL, *vR = s->pvR, *flux = s->pflux, *cmax = s->pcmax;
+
+  for (b = lo[t2]; b <= hi[t2]; b++)
+  for (a = lo[t1]; a <= hi[t1]; a++) {
+    int p[3];
+    p[t1] = a; p[t2] = b;
+    /* DataToSweep, Src/data2sweep.cpp:75-115 */
+    for (i = 0; i < n; i++) {
+      p[dir] = i;
+      for (nv = 0; nv < nvar; nv++) v[i*NVMAX + nv] = VC(s, nv, p[0], p[1], p[2]);
+      if (mhd) s->pBn[i] = s->Vs[dir][IS(s, p[0], p[1], p[2])];
+    }
+    /* Reconstruct, Src/States/plm_states.cpp:99-141 then 184-189 */
+    int ib = s->beg[dir] - 1, ie = s->end[dir] + 1;
+    for (nv = 0; nv < nvar; nv++)
+      for (i = ib; i <= ie; i++) {
+        double dv = plm_slope(v[(i-1)*NVMAX + nv], v[i*NVMAX + nv], v[(i+1)*NVMAX + nv], s->c.limiter);
+        vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv] + 0.5*dv;
+        vR[i*NVMAX + nv]     = v[i*NVMAX + nv] - 0.5*dv;
+      }
+    if (mhd) {
+      for (i = ib; i <= ie; i++) {
+        vL[(i+1)*NVMAX + BX1 + dir] = s->pBn[i+1];
+        vR[i*NVMAX + BX1 + dir]     = s->pBn[i];
+      }
+    }
+    /* Riemann solver on faces beg..end+1 (Src/MHD/hlld.cpp:100-114) */
+    for (i = s->beg[dir]; i <= s->end[dir] + 1; i++) {
+      double Sl, Sr;
+      const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
+      if (mhd) {
+        if (s->c.solver == GPLO_HLLD) err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        else err |= hll(vl, vr, dir, s->c.gamma, 1, flux + i*NVMAX, &Sl, &Sr);
+      } else {
+        if (s->c.solver == GPLO_HLLC) err |= hd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        else err |= hll(vl, vr, dir, s->c.gamma, 0, flux + i*NVMAX, &Sl, &Sr);
+      }
+      cmax[i] = MAX(fabs(Sl), fabs(Sr));
+      if (mhd) {
+        /* CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177 */
+        const int VXt = 1 + t1, VXb = 1 + t2;
+        p[dir] = i;
+        size_t id = IC(s, p[0], p[1], p[2]);
+        double aR_1 = MAX(0.0, Sr);
+        double aL_1 = MIN(0.0, Sl);
+        double scrh = 1.0/(aR_1 - aL_1);
+        s->e[t2][dir][id] =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
+        s->e[t1][dir][id] = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
+        s->aL[dir][id] =  aR_1*scrh;
+        s->aR[dir][id] = -aL_1*scrh;
+        s->dR[dir][id] = -aL_1*aR_1*scrh;
+        s->dL[dir][id] = -aL_1*aR_1*scrh;
+      }
+    }
+    /* RightHandSide on the active box only, Src/MHD/rhs.cpp:330-527 */
+    if (a < s->beg[t1] || a > s->end[t1] || b < s->beg[t2] || b > s->end[t2]) continue;
+    for (i = s->beg[dir]; i <= s->end[dir]; i++) {
+      p[dir] = i;
+      size_t id = IC(s, p[0], p[1], p[2]);
+      for (nv = 0; nv < nvar; nv++) {
+        double rhs = -dtdx*(flux[(i+1)*NVMAX + nv] - flux[i*NVMAX + nv]);
+        rhs += 0.0;   /* src[] = 0, rhs.cpp:510-513 */
+        /* dUc = rhs (x); dUc += rhs (y, z): Src/update_stage.cpp:173-184,272-284,369-380 */
+        if (dir == 0) s->Rc[(size_t)nv*s->ncell + id] = rhs;
+        else          s->Rc[(size_t)nv*s->ncell + id] += rhs;
+      }
+      double cdt = 0.5*(cmax[i] + cmax[i+1])/s->dx[dir];
+      if (dir == 0) s->Cdt[id] = cdt;      /* rhs.cpp:523 */
+      else          s->Cdt[id] += cdt;     /* rhs.cpp:756,981 */
+    }
+  }
+  return err;
+}
+
+/* ArrayReconstruct<dir> linear branch, Src/array_reconstruct.cpp:46-206:
+   reconstruct array Q (staggered or cell layout via 'stag') along dir for
+   every pencil in the box; qL(i+1)/qR(i) for i in [1, ntot-2]. */
+static void array_reconstruct(gplo_state *s, int dir, const double *Q, int stag, int lim,
+                              const int lo[3], const int hi[3], double *qL, double *qR)
+{
+  const int t1 = (dir + 1)%3, t2 = (dir + 2)%3;
+  const int n = s->nt[dir];
+  int a, b, i;
+  for (b = lo[t2]; b <= hi[t2]; b++)
+  for (a = lo[t1]; a <= hi[t1]; a++) {
+    int p[3];
+    p[t1] = a; p[t2] = b;
+    for (i = 1; i < n - 1; i++) {
+      p[dir] = i;
+      size_t id = IS(s, p[0], p[1], p[2]);
+      int pm[3] = {p[0], p[1], p[2]}, pp[3] = {p[0], p[1], p[2]};
+      pm[dir] = i - 1; pp[dir] = i + 1;
+      double q0, qm, qp;
+      if (stag) { q0 = Q[id]; qm = Q[IS(s,pm[0],pm[1],pm[2])]; qp = Q[IS(s,pp[0],pp[1],pp[2])]; }
+      else      { q0 = Q[IC(s,p[0],p[1],p[2])]; qm = Q[IC(s,pm[0],pm[1],pm[2])]; qp = Q[IC(s,pp[0],pp[1],pp[2])]; }
+      double dq = plm_slope(qm, q0, qp, lim);
+      int pn[3] = {p[0], p[1], p[2]}; pn[dir] = i + 1;
+      qL[IS(s,pn[0],pn[1],pn[2])] = q0 + 0.5*dq;
+      qR[id] = q0 - 0.5*dq;
+    }
+  }
+}
+
+/* CT_EMF_Riemann2D for UCT_HLL, Src/MHD/CT/ct_emf_average.cpp:535-989 */
+static void ct_emf_uct(gplo_state *s)
+{
+  const int D = s->c.dims;
+  int lo[3], hi[3], i, j, k;
+  for (int d = 0; d < 3; d++) { lo[d] = s->beg[d]; hi[d] = s->end[d] + (d < D ? 1 : 0); }
+  const int recB = GPLO_MC_LIM, recV = s->c.limiter;
+  double *bL = s->qL[0], *bR = s->qR[0], *vL = s->qL[1], *vR = s->qR[1];
+  double *Ex = s->Ee[0], *Ey = s->Ee[1], *Ez = s->Ee[2];
+  /* (dir of reconstruction, face component Q, e array, target edge, sign of
+     phi, accumulate?) -- blocks X1a, X1b, X2a, X2b, X3a, X3b */
+  struct blk { int rdir, comp, ea, eb, edge, sgn, acc, cdir; int need[3]; };
+  const struct blk B[6] = {
+    {0, 1, 2, 1, 2, +1, 0, 1, {1,1,0}},  /* X1a: By, e[K][J] -> Ez, avg over j */
+    {0, 2, 1, 2, 1, -1, 0, 2, {1,0,1}},  /* X1b: Bz, e[J][K] -> Ey, avg over k */
+    {1, 2, 0, 2, 0, +1, 0, 2, {0,1,1}},  /* X2a: Bz, e[I][K] -> Ex, avg over k */
+    {1, 0, 2, 0, 2, -1, 1, 0, {1,1,0}},  /* X2b: Bx, e[K][I] -> Ez +=, avg over i */
+    {2, 0, 1, 0, 1, +1, 1, 0, {1,0,1}},  /* X3a: Bx, e[J][I] -> Ey +=, avg over i */
+    {2, 1, 0, 1, 0, -1, 1, 1, {0,1,1}},  /* X3b: By, e[I][J] -> Ex +=, avg over j */
+  };
+  for (int q = 0; q < 6; q++) {
+    const struct blk *bk = &B[q];
+    if (bk->rdir >= D || bk->comp >= D) continue;
+    int ok = 1;
+    for (int d = 0; d < 3; d++) if (bk->need[d] && d >= D) ok = 0;
+    if (!ok) continue;
+    array_reconstruct(s, bk->rdir, s->Vs[bk->comp], 1, recB, lo, hi, bL, bR);
+    array_reconstruct(s, bk->rdir, s->e[bk->ea][bk->eb], 0, recV, lo, hi, vL, vR);
+    double *E = bk->edge == 0 ? Ex : bk->edge == 1 ? Ey : Ez;
+    const int fd = bk->rdir;   /* face direction of the a/d coefficients */
+    for (k = lo[2]; k <= hi[2]; k++)
+    for (j = lo[1]; j <= hi[1]; j++)
+    for (i = lo[0]; i <= hi[0]; i++) {
+      int m[3] = {i, j, k}; m[bk->cdir] -= 1;
+      size_t id0 = IC(s, m[0], m[1], m[2]), id1 = IC(s, i, j, k);
+      size_t ie = IS(s, i, j, k);
+      double aL = 0.5*(s->aL[fd][id0] + s->aL[fd][id1]);
+      double aR = 0.5*(s->aR[fd][id0] + s->aR[fd][id1]);
+      double dL = 0.5*(s->dL[fd][id0] + s->dL[fd][id1]);
+      double dR = 0.5*(s->dR[fd][id0] + s->dR[fd][id1]);
+      double phi = dR*bR[ie] - dL*bL[ie];
+      double val = bk->sgn > 0 ? (aL*vL[ie]*bL[ie] + aR*vR[ie]*bR[ie]) + phi
+                               : (aL*vL[ie]*bL[ie] + aR*vR[ie]*bR[ie]) - phi;
+      if (bk->acc) E[ie] += val; else E[ie] = val;
+    }
+  }
+}
+
+/* CT_Update, Src/MHD/CT/ct_update.cpp:41-430 (Cartesian) */
+static void ct_update(gplo_state *s, double dt)
+{
+  const int D = s->c.dims;
+  int i, j, k;
+  double *Ex = s->Ee[0], *Ey = s->Ee[1], *Ez = s->Ee[2];
+  /* Bx1 faces: i in [ibeg, iend+1] */
+  for (k = s->beg[2]; k <= s->end[2]; k++)
+  for (j = s->beg[1]; j <= s->end[1]; j++)
+  for (i = s->beg[0]; i <= s->end[0] + 1; i++) {
+    double curlE = 0.0;
+    if (D >= 2) { double dtdx2 = dt/s->dx[1]; curlE = 0.0 + dtdx2*(Ez[IS(s,i,j+1,k)] - Ez[IS(s,i,j,k)]); }
+    if (D == 3) { double dtdx3 = dt/s->dx[2]; curlE = curlE - dtdx3*(Ey[IS(s,i,j,k+1)] - Ey[IS(s,i,j,k)]); }
+    s->Rs[0][IS(s,i,j,k)] = -curlE;
+  }
+  if (D >= 2) {
+    for (k = s->beg[2]; k <= s->end[2]; k++)
+    for (j = s->beg[1]; j <= s->end[1] + 1; j++)
+    for (i = s->beg[0]; i <= s->end[0]; i++) {
+      double dtdx1 = dt/s->dx[0];
+      double curlE = - dtdx1*(Ez[IS(s,i+1,j,k)] - Ez[IS(s,i,j,k)]);
+      if (D == 3) { double dtdx3 = dt/s->dx[2]; curlE = curlE + dtdx3*(Ex[IS(s,i,j,k+1)] - Ex[IS(s,i,j,k)]); }
+      s->Rs[1][IS(s,i,j,k)] = -curlE;
+    }
+  }
+  if (D == 3) {
+    for (k = s->beg[2]; k <= s->end[2] + 1; k++)
+    for (j = s->beg[1]; j <= s->end[1]; j++)
+    for (i = s->beg[0]; i <= s->end[0]; i++) {
+      double dtdx1 = dt/s->dx[0], dtdx2 = dt/s->dx[1];
+      double curlE = dtdx1*(Ey[IS(s,i+1,j,k)] - Ey[IS(s,i,j,k)])
+                   - dtdx2*(Ex[IS(s,i,j+1,k)] - Ex[IS(s,i,j,k)]);
+      s->Rs[2][IS(s,i,j,k)] = -curlE;
+    }
+  }
+}
+
+/* UpdateStage, Src/update_stage.cpp:57-440 */
+static uint32_t update_stage(gplo_state *s, double dt, double *inv_dt)
+{
+  uint32_t err = 0;
+  for (int d = 0; d < s->c.dims; d++) err |= sweep(s, d, dt);
+  if (is_mhd(s)) { ct_emf_uct(s); ct_update(s, dt); }
+  if (s->stage == 1) {
+    double local = -1.0;     /* getMax, :446-458 */
+    for (int k = s->beg[2]; k <= s->end[2]; k++)
+    for (int j = s->beg[1]; j <= s->end[1]; j++)
+    for (int i = s->beg[0]; i <= s->end[0]; i++) local = MAX(local, s->Cdt[IC(s,i,j,k)]);
+    *inv_dt = local/(double)s->c.dims;
+  }
+  return err;
+}
+
+/* ConsToPrim MHD (Src/MHD/mappers.cpp:32-128) / HD (Src/HD/mappers.cpp:33-110)
+   wrapped by ConsToPrim3D (Src/mappers3D.cpp:76-194) */
+static uint32_t cons_to_prim_3d(gplo_state *s)
+{
+  const double gmm1 = s->c.gamma - 1.0;
+  const int mhd = is_mhd(s), ENG = mhd ? 7 : 4;
+  uint32_t err_all = 0;
+  for (int k = s->beg[2]; k <= s->end[2]; k++)
+  for (int j = s->beg[1]; j <= s->end[1]; j++)
+  for (int i = s->beg[0]; i <= s->end[0]; i++) {
+    double u[NVMAX], v[NVMAX];
+    uint32_t err = 0;
+    size_t id = IC(s,i,j,k);
+    for (int nv = 0; nv < s->nv; nv++) {
+      u[nv] = s->Uc[(size_t)nv*s->ncell + id];
+      if (isnan(u[nv])) err |= ERR_NAN | ERR_QUIT;
+    }
+    uint32_t le = 0;
+    if (u[RHO] < 0.0) { u[RHO] = s->c.small_rho; le |= ERR_NEGATIVE_RHO | ERR_CONS2PRIM_FAIL; }
+    v[RHO] = u[RHO];
+    double tau = 1.0/u[RHO];
+    v[1] = u[1]*tau;
+    v[2] = u[2]*tau;
+    v[3] = u[3]*tau;
+    double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+    double kin;
+    if (mhd) {
+      v[4] = u[4]; v[5] = u[5]; v[6] = u[6];
+      double b2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+      kin = 0.5*(m2*tau + b2);
+    } else {
+      kin = 0.5*m2*tau;
+    }
+    if (u[ENG] > 0.0) {
+      v[ENG] = gmm1*(u[ENG] - kin);
+      if (v[ENG] < 0.0) {
+        v[ENG] = s->c.small_prs;
+        u[ENG] = v[ENG]/gmm1 + kin;
+        le |= ERR_NEGATIVE_PRS;
+      }
+    } else {
+      u[ENG] = s->c.small_prs/gmm1 + kin;
+      v[ENG] = s->c.small_prs;
+      le |= ERR_NEGATIVE_ENG;
+    }
+    err |= le;
+    if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
+    if (le) for (int nv = 0; nv < s->nv; nv++) s->Uc[(size_t)nv*s->ncell + id] = u[nv];
+    for (int nv = 0; nv < s->nv; nv++) s->Vc[(size_t)nv*s->ncell + id] = v[nv];
+    err_all |= err;
+  }
+  return err_all;
+}
+
+/* CT_AverageStaggeredFields on the active box, Src/MHD/CT/ct_field_average.cpp:182-184,393-395 */
+static void ct_average(gplo_state *s)
+{
+  for (int k = s->beg[2]; k <= s->end[2]; k++)
+  for (int j = s->beg[1]; j <= s->end[1]; j++)
+  for (int i = s->beg[0]; i <= s->end[0]; i++)
+    for (int d = 0; d < s->c.dims; d++) {
+      int o[3] = {0, 0, 0}; o[d] = 1;
+      UC(s, BX1 + d, i, j, k) = 0.5*(s->Vs[d][IS(s,i,j,k)] + s->Vs[d][IS(s,i+o[0],j+o[1],k+o[2])]);
+    }
+}
+
+/* AdvanceStep RK2/RK3, Src/rk_step.cpp:88-720 */
+uint32_t gplo_advance_step(gplo_state *s, double dt, double *inv_dt)
+{
+  static const double W[4][3] = { {0,0,0}, {1.0, 1.0, 1.0}, {0.5, 0.5, 0.5}, {0,0,0} };
+  const int nstage = s->c.time_stepping == 3 ? 3 : 2;
+  uint32_t err = 0;
+  double w[4][3];
+  memcpy(w, W, sizeof(w));
+  if (nstage == 3) {   /* Src/rk_step.cpp:35-51 */
+    w[2][0] = 0.75; w[2][1] = 0.25; w[2][2] = 0.25;
+    w[3][0] = 1.0/3.0; w[3][1] = 2.0/3.0; w[3][2] = 2.0/3.0;
+  }
+  const int D = s->c.dims, mhd = is_mhd(s);
+  for (int st = 1; st <= nstage; st++) {
+    s->stage = st;
+    boundary(s);
+    err |= update_stage(s, dt, inv_dt);
+    for (int nv = 0; nv < s->nv; nv++)
+    for (int k = s->beg[2]; k <= s->end[2]; k++)
+    for (int j = s->beg[1]; j <= s->end[1]; j++)
+    for (int i = s->beg[0]; i <= s->end[0]; i++) {
+      size_t id = (size_t)nv*s->ncell + IC(s,i,j,k);
+      if (st == 1) {
+        s->U0[id] = s->Uc[id];
+        s->Uc[id] = s->U0[id] + w[1][2]*s->Rc[id];
+      } else {
+        s->Uc[id] = w[st][0]*s->U0[id] + w[st][1]*s->Uc[id] + w[st][2]*s->Rc[id];
+      }
+    }
+    if (mhd) {   /* BOX_STAG_LOOP, Src/macros.hpp:82-84 */
+      for (int k = s->beg[2]; k <= s->end[2] + (D == 3); k++)
+      for (int j = s->beg[1]; j <= s->end[1] + (D >= 2); j++)
+      for (int i = s->beg[0]; i <= s->end[0] + 1; i++) {
+        size_t id = IS(s,i,j,k);
+        for (int d = 0; d < D; d++) {
+          if (st == 1) {
+            s->Bs0[d][id] = s->Vs[d][id];
+            s->Vs[d][id] = s->Bs0[d][id] + w[1][2]*s->Rs[d][id];
+          } else {
+            s->Vs[d][id] = w[st][0]*s->Bs0[d][id] + w[st][1]*s->Vs[d][id] + w[st][2]*s->Rs[d][id];
+          }
+        }
+      }
+      ct_average(s);
+    }
+    err |= cons_to_prim_3d(s);
+  }
+  return err;
+}
+
+uint32_t gplo_run(gplo_state *s, int nsteps, double first_dt, double *time_out, double *dt_out)
+{
+  double t = 0.0, dt = first_dt, inv_dt = 0.0;
+  uint32_t err = 0;
+  for (int n = 0; n < nsteps; n++) {
+    err |= gplo_advance_step(s, dt, &inv_dt);
+    t += dt;
+    double dt_new = s->c.cfl/inv_dt;                  /* next_time_step.cpp:80 */
+    dt = MIN(dt_new, s->c.cfl_max_var*dt);            /* next_time_step.cpp:125 */
+  }
+  if (time_out) *time_out = t;
+  if (dt_out) *dt_out = dt;
+  return err;
+}
+
+/* ======================================================================
+ * Allocation and state transfer
+ * ====================================================================== */
+gplo_state *gplo_create(const gplo_config *cfg)
+{
+  gplo_state *s = (gplo_state *)calloc(1, sizeof(*s));
+  s->c = *cfg;
+  if (s->c.small_rho == 0.0) s->c.small_rho = 1.e-12;   /* Src/structs.hpp:407-408 */
+  if (s->c.small_prs == 0.0) s->c.small_prs = 1.e-12;
+  s->nv = cfg->phys == GPLO_MHD ? 8 : 5;
+  int nint[3] = {cfg->nx, cfg->ny, cfg->nz};
+  for (int d = 0; d < 3; d++) {
+    if (d < cfg->dims) { s->nt[d] = nint[d] + 2*NG; s->beg[d] = NG; s->end[d] = NG + nint[d] - 1; }
+    else               { s->nt[d] = 1; s->beg[d] = 0; s->end[d] = 0; }
+    s->ns[d] = s->nt[d] + 1;
+    s->dx[d] = (1.0 - 0.0)/nint[d];   /* Src/set_grid.cpp:196 on [0,1] */
+  }
+  s->ncell = (size_t)s->nt[0]*s->nt[1]*s->nt[2];
+  s->nstag = (size_t)s->ns[0]*s->ns[1]*s->ns[2];
+  size_t nc = s->ncell, ns = s->nstag;
+  s->Vc = calloc(nc*s->nv, 8); s->Uc = calloc(nc*s->nv, 8);
+  s->U0 = calloc(nc*s->nv, 8); s->Rc = calloc(nc*s->nv, 8);
+  s->Cdt = calloc(nc, 8);
+  for (int d = 0; d < 3; d++) {
+    s->Vs[d] = calloc(ns, 8); s->Bs0[d] = calloc(ns, 8); s->Rs[d] = calloc(ns, 8);
+    s->Ee[d] = calloc(ns, 8);
+    s->aL[d] = calloc(nc, 8); s->aR[d] = calloc(nc, 8);
+    s->dL[d] = calloc(nc, 8); s->dR[d] = calloc(nc, 8);
+    for (int e = 0; e < 3; e++) s->e[d][e] = calloc(nc, 8);
+  }
+  int nmax = s->nt[0]; if (s->nt[1] > nmax) nmax = s->nt[1]; if (s->nt[2] > nmax) nmax = s->nt[2];
+  s->pv = calloc((size_t)(nmax + 2)*NVMAX, 8); s->pvL = calloc((size_t)(nmax + 2)*NVMAX, 8);
+  s->pvR = calloc((size_t)(nmax + 2)*NVMAX, 8); s->pflux = calloc((size_t)(nmax + 2)*NVMAX, 8);
+  s->pcmax = calloc(nmax + 2, 8); s->pBn = calloc(nmax + 2, 8); s->prhs = calloc((size_t)(nmax + 2)*NVMAX, 8);
+  for (int q = 0; q < 2; q++) { s->qv[q] = calloc(ns, 8); s->qL[q] = calloc(ns, 8); s->qR[q] = calloc(ns, 8); }
+  return s;
+}
+
+void gplo_destroy(gplo_state *s)
+{
+  if (!s) return;
+  free(s->Vc); free(s->Uc); free(s->U0); free(s->Rc); free(s->Cdt);
+  for (int d = 0; d < 3; d++) {
+    free(s->Vs[d]); free(s->Bs0[d]); free(s->Rs[d]); free(s->Ee[d]);
+    free(s->aL[d]); free(s->aR[d]); free(s->dL[d]); free(s->dR[d]);
+    for (int e = 0; e < 3; e++) free(s->e[d][e]);
+  }
+  free(s->pv); free(s->pvL); free(s->pvR); free(s->pflux); free(s->pcmax); free(s->pBn); free(s->prhs);
+  for (int q = 0; q < 2; q++) { free(s->qv[q]); free(s->qL[q]); free(s->qR[q]); }
+  free(s);
+}
+
+int gplo_nvar(const gplo_state *s) { return s->nv; }
+
+static int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+
+/* Startup semantics for the fields handed over (Src/startup.cpp:344-561):
+   InitDomain fills Vc and Vs everywhere (periodic images), the cell field
+   is the face average (CT_AverageStaggeredFields, :388), then
+   Uc = PrimToCons(Vc) on the active box (:514-561). */
+int gplo_set_state(gplo_state *s, const double *vc, const double *vs)
+{
+  const int nx = s->c.nx, ny = s->c.ny, nz = s->c.nz;
+  const size_t na = (size_t)nx*ny*nz;
+  for (int k = 0; k < s->nt[2]; k++)
+  for (int j = 0; j < s->nt[1]; j++)
+  for (int i = 0; i < s->nt[0]; i++) {
+    int a = i - s->beg[0], b = j - s->beg[1], c = k - s->beg[2];
+    if (s->c.periodic) { a = wrap(a, nx); b = wrap(b, ny); c = wrap(c, nz); }
+    else {
+      a = a < 0 ? 0 : (a >= nx ? nx - 1 : a);
+      b = b < 0 ? 0 : (b >= ny ? ny - 1 : b);
+      c = c < 0 ? 0 : (c >= nz ? nz - 1 : c);
+    }
+    for (int nv = 0; nv < s->nv; nv++)
+      VC(s,nv,i,j,k) = vc[(size_t)nv*na + ((size_t)c*ny + b)*nx + a];
+  }
+  if (is_mhd(s)) {
+    if (!vs) return -1;
+    for (int d = 0; d < s->c.dims; d++)
+      for (int k = 0; k < s->ns[2]; k++)
+      for (int j = 0; j < s->ns[1]; j++)
+      for (int i = 0; i < s->ns[0]; i++) {
+        int a = wrap(i - s->beg[0], nx), b = wrap(j - s->beg[1], ny), c = wrap(k - s->beg[2], nz);
+        s->Vs[d][IS(s,i,j,k)] = vs[(size_t)d*na + ((size_t)c*ny + b)*nx + a];
+      }
+    int lo[3] = {0, 0, 0}, hi[3] = {s->nt[0] - 1, s->nt[1] - 1, s->nt[2] - 1};
+    ghost_field_average(s, lo, hi);
+  }
+  for (int k = s->beg[2]; k <= s->end[2]; k++)
+  for (int j = s->beg[1]; j <= s->end[1]; j++)
+  for (int i = s->beg[0]; i <= s->end[0]; i++) {
+    double v[NVMAX], u[NVMAX];
+    for (int nv = 0; nv < s->nv; nv++) v[nv] = VC(s,nv,i,j,k);
+    if (is_mhd(s)) mhd_prim_to_cons(v, u, s->c.gamma); else hd_prim_to_cons(v, u, s->c.gamma);
+    for (int nv = 0; nv < s->nv; nv++) UC(s,nv,i,j,k) = u[nv];
+  }
+  return 0;
+}
+
+int gplo_get_state(const gplo_state *s, double *vc, double *vs, double *uc)
+{
+  const int nx = s->c.nx, ny = s->c.ny, nz = s->c.nz;
+  const size_t na = (size_t)nx*ny*nz;
+  for (int c = 0; c < nz; c++)
+  for (int b = 0; b < ny; b++)
+  for (int a = 0; a < nx; a++) {
+    size_t o = ((size_t)c*ny + b)*nx + a;
+    size_t id = IC(s, a + s->beg[0], b + s->beg[1], c + s->beg[2]);
+    for (int nv = 0; nv < s->nv; nv++) {
+      if (vc) vc[(size_t)nv*na + o] = s->Vc[(size_t)nv*s->ncell + id];
+      if (uc) uc[(size_t)nv*na + o] = s->Uc[(size_t)nv*s->ncell + id];
+    }
+    if (vs && is_mhd(s))
+      for (int d = 0; d < s->c.dims; d++)
+        vs[(size_t)d*na + o] = s->Vs[d][IS(s, a + s->beg[0], b + s->beg[1], c + s->beg[2])];
+  }
+  return 0;
+}

@@ -1,0 +1,59 @@
+/* CPU oracle: a plain-C restatement of the reference's Godunov stage update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The product path never links this code; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and only as the checker.
+ *
+ * Data layout and step semantics follow the reference (Src/rk_step.cpp:88,
+ * Src/update_stage.cpp:57, Src/main.cpp:320-485): arrays carry nghost = 2
+ * ghost zones per active direction, Vc/Uc are [nv][k][j][i], staggered
+ * fields carry one extra face along every direction.  Public state transfer
+ * uses the periodic "one face per cell" layout of gpluto_dev_amd.ic.
+ */
+#ifndef GPL_ORACLE_H
+#define GPL_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GPLO_HD = 1, GPLO_MHD = 2 };
+enum { GPLO_HLL = 1, GPLO_HLLC = 2, GPLO_HLLD = 3 };
+enum { GPLO_VANLEER_LIM = 26, GPLO_MC_LIM = 27 };    /* Src/gpluto.hpp:330-336 */
+enum { GPLO_UCT_HLL = 1 };
+
+typedef struct gplo_config {
+  int    phys;            /* GPLO_HD or GPLO_MHD */
+  int    dims;            /* 1, 2 or 3 */
+  int    nx, ny, nz;      /* active zones (1 for absent directions) */
+  int    periodic;        /* 1 periodic on every side, 0 outflow */
+  int    time_stepping;   /* 2 (RK2) or 3 (RK3) */
+  int    limiter;         /* GPLO_MC_LIM or GPLO_VANLEER_LIM */
+  int    solver;          /* GPLO_HLL / GPLO_HLLC / GPLO_HLLD */
+  double gamma;
+  double cfl, cfl_max_var;
+  double small_rho, small_prs;
+} gplo_config;
+
+typedef struct gplo_state gplo_state;
+
+gplo_state *gplo_create(const gplo_config *cfg);
+void        gplo_destroy(gplo_state *s);
+/* vc[nvar][nz][ny][nx]; vs[dims][nz][ny][nx] (lower faces, periodic) or NULL */
+int         gplo_set_state(gplo_state *s, const double *vc, const double *vs);
+/* uc may be NULL */
+int         gplo_get_state(const gplo_state *s, double *vc, double *vs, double *uc);
+/* One AdvanceStep with time step dt; returns the ERR_* bits; *inv_dt gets
+   Dts->invDt_hyp of stage 1 (Src/update_stage.cpp:422-425). */
+uint32_t    gplo_advance_step(gplo_state *s, double dt, double *inv_dt);
+/* The main-loop recurrence (Src/main.cpp:446-463, Src/next_time_step.cpp:80,125):
+   run nsteps steps from dt = first_dt; returns OR of the step error bits. */
+uint32_t    gplo_run(gplo_state *s, int nsteps, double first_dt,
+                     double *time_out, double *dt_out);
+int         gplo_nvar(const gplo_state *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

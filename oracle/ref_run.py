@@ -1,0 +1,143 @@
+"""Run the reference CPU build (oracle/_ref/<config>/pluto) on a given IC.
+
+Test infrastructure only: this drives the reference's own executable, built
+from its sources by oracle/ref/Makefile, through its normal command line
+(``-i pluto.ini -maxsteps N``, Src/cmd_line_opt.cpp:89-111) and reads back
+its ``.dbl`` output (Src/write_data.cpp:116-136: each variable's active
+zones, i fastest; staggered fields carry one extra face along their
+direction, Src/bin_io.cpp:319-324).
+
+Step semantics: ``-maxsteps N`` performs N+1 AdvanceStep calls
+(Src/main.cpp:363-371), so a run of S steps passes ``-maxsteps S-1`` and
+dumps with ``dn = S-1`` so that only the post-loop output fires
+(Src/main.cpp:497-500, CheckForOutput :619-623).
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_binary(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL") -> str:
+    name = f"{cfg}_{ts}_{lim}" + (f"_{emf}" if cfg.startswith("mhd") else "")
+    return os.path.join(REF_DIR, name, "pluto")
+
+
+def build_ref(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL",
+              ref_root: str = "/root/reference") -> str:
+    """Build one reference configuration (needs the reference sources)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "ref"), f"CFG={cfg}",
+                    f"TS={ts}", f"LIM={lim}", f"EMF={emf}", f"REF={ref_root}", "-j8"],
+                   check=True)
+    return ref_binary(cfg, ts, lim, emf)
+
+
+def write_ic(path: str, vc: np.ndarray, vs: np.ndarray | None, gamma: float, periodic: bool):
+    nvar, nz, ny, nx = vc.shape
+    nstag = 0 if vs is None else vs.shape[0]
+    with open(path, "wb") as f:
+        f.write(b"GPLIC01\0")
+        f.write(struct.pack("<6i", nx, ny, nz, nvar, nstag, 1 if periodic else 0))
+        f.write(struct.pack("<d", gamma))
+        f.write(np.ascontiguousarray(vc, dtype="<f8").tobytes())
+        if vs is not None:
+            f.write(np.ascontiguousarray(vs, dtype="<f8").tobytes())
+
+
+@dataclass
+class RefResult:
+    vc: np.ndarray          # [nvar, nz, ny, nx]
+    vs: np.ndarray | None   # [dims, nz, ny, nx] lower faces (periodic)
+    vs_upper: list | None   # extra upper face slab per direction
+    time: float
+    wall_s: float
+    sec_per_step: float | None
+    log: str
+
+
+def _grid_line(name: str, n: int) -> str:
+    return f"{name}  0.0  {n}  1.0\n"
+
+
+def run_reference(binary: str, vc: np.ndarray, vs: np.ndarray | None, *, gamma: float,
+                  steps: int, cfl: float, first_dt: float, solver: str,
+                  periodic: bool = True, cfl_max_var: float = 1.1,
+                  timeout: float = 3600.0, workdir: str | None = None) -> RefResult:
+    if steps < 2:
+        raise ValueError("the reference dumps its post-loop output only for >= 2 steps")
+    nvar, nz, ny, nx = vc.shape
+    bnd = "periodic" if periodic else "outflow"
+    ini = ["[Grid]\n\n", _grid_line("X1-grid", nx), _grid_line("X2-grid", ny),
+           _grid_line("X3-grid", nz), "\n[Time]\n\n", f"CFL  {cfl!r}\n",
+           f"CFL_max_var  {cfl_max_var!r}\n", "tstop  1.e30\n", f"first_dt  {first_dt!r}\n",
+           "\n[Solver]\n\n", f"Solver  {solver}\n", "\n[Boundary]\n\n"]
+    for ax in ("X1", "X2", "X3"):
+        ini.append(f"{ax}-beg  {bnd}\n{ax}-end  {bnd}\n")
+    ini += ["\n[Static Grid Output]\n\n", "uservar 0\n", f"dbl -1.0 {steps - 1} single_file\n",
+            "flt -1.0 -1 single_file\n", "vtk -1.0 -1 single_file\n", "tab -1.0 -1\n",
+            "ppm -1.0 -1\n", "png -1.0 -1\n", "log 1000000\n", "analysis -1.0 -1\n",
+            "\n[Parameters]\n\n", "DUMMY_PAR  0.0\n"]
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="gpl_ref_") if own else workdir
+    try:
+        with open(os.path.join(tmp, "pluto.ini"), "w") as f:
+            f.writelines(ini)
+        write_ic(os.path.join(tmp, "gpl_ic.bin"), vc, vs, gamma, periodic)
+        env = dict(os.environ, GPLUTO_FTZ_DAZ="0")  # IEEE mode (Src/cpu_fp.hpp:10-17)
+        t0 = time.perf_counter()
+        p = subprocess.run([binary, "-i", "pluto.ini", "-maxsteps", str(steps - 1)], cwd=tmp,
+                           env=env, capture_output=True, text=True, timeout=timeout)
+        wall = time.perf_counter() - t0
+        if p.returncode != 0:
+            raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
+        with open(os.path.join(tmp, "dbl.out")) as f:
+            last = f.read().strip().splitlines()[-1].split()
+        nfile, t = int(last[0]), float(last[1])
+        names = last[6:]
+        raw = np.fromfile(os.path.join(tmp, f"data.{nfile:04d}.dbl"), dtype="<f8")
+        spt = None
+        for line in p.stdout.splitlines():
+            if "Average time/step" in line:
+                try:
+                    spt = float(line.split()[-2 if line.strip().endswith("(sec)") else -1])
+                except ValueError:
+                    pass
+    finally:
+        if own:
+            import shutil
+            shutil.rmtree(tmp, ignore_errors=True)
+    ncell = nx * ny * nz
+    off = 0
+    cells = []
+    stag = []
+    upper = []
+    for name in names:
+        if name.endswith("s") and name.startswith("Bx"):
+            d = int(name[2]) - 1
+            shape = [nz, ny, nx]
+            shape[2 - d] += 1
+            cnt = shape[0] * shape[1] * shape[2]
+            a = raw[off:off + cnt].reshape(shape)
+            off += cnt
+            sl = [slice(None)] * 3
+            sl[2 - d] = slice(0, shape[2 - d] - 1)
+            stag.append(np.ascontiguousarray(a[tuple(sl)]))
+            su = [slice(None)] * 3
+            su[2 - d] = slice(shape[2 - d] - 1, shape[2 - d])
+            upper.append(np.ascontiguousarray(a[tuple(su)]))
+        else:
+            cells.append(raw[off:off + ncell].reshape(nz, ny, nx))
+            off += ncell
+    assert off == raw.size, (off, raw.size)
+    return RefResult(vc=np.stack(cells), vs=np.stack(stag) if stag else None,
+                     vs_upper=upper if stag else None, time=t, wall_s=wall,
+                     sec_per_step=spt, log=p.stdout)
