@@ -1,0 +1,88 @@
+"""ctypes binding of the C ABI in include/gpluto_amd.h (libgpluto_amd.so).
+
+The product path: every call goes to the HIP library.  Loading fails loudly
+when the library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpluto_amd.so")
+
+GPL_HD, GPL_MHD = 1, 2
+GPL_SOLVER_HLL, GPL_SOLVER_HLLC, GPL_SOLVER_HLLD = 1, 2, 3
+GPL_VANLEER_LIM, GPL_MC_LIM = 26, 27
+GPL_UCT_HLL = 1
+GPL_RK2, GPL_RK3 = 2, 3
+GPL_PERIODIC, GPL_OUTFLOW = 1, 2
+GPL_FP64, GPL_MIXED = 0, 1
+GPL_LAYOUT_ACTIVE, GPL_LAYOUT_ARY4D = 0, 1
+GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED = -1, -2, -3, -4
+
+# every symbol include/gpluto_amd.h declares
+EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl_download",
+           "gpl_advance_step", "gpl_next_time_step", "gpl_run", "gpl_set_timing",
+           "gpl_kernel_count", "gpl_kernel_stats", "gpl_last_run_ms", "gpl_device_bytes",
+           "gpl_strerror")
+
+
+class GplConfig(C.Structure):
+    _fields_ = [("physics", C.c_int), ("dimensions", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xbeg", C.c_double * 3), ("xend", C.c_double * 3),
+                ("boundary", C.c_int), ("time_stepping", C.c_int), ("limiter", C.c_int),
+                ("solver", C.c_int), ("ct_emf_average", C.c_int), ("precision", C.c_int),
+                ("nghost", C.c_int), ("device", C.c_int),
+                ("gamma", C.c_double), ("cfl", C.c_double), ("cfl_max_var", C.c_double),
+                ("small_rho", C.c_double), ("small_prs", C.c_double)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gpluto_dev_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.gpl_default_config.argtypes = [C.POINTER(GplConfig)]
+    L.gpl_default_config.restype = None
+    L.gpl_create.argtypes = [C.POINTER(GplConfig), C.POINTER(vp)]
+    L.gpl_create.restype = ip
+    L.gpl_destroy.argtypes = [vp]
+    L.gpl_destroy.restype = ip
+    L.gpl_upload.argtypes = [vp, vp, C.POINTER(vp), ip]
+    L.gpl_upload.restype = ip
+    L.gpl_download.argtypes = [vp, vp, C.POINTER(vp), vp, ip]
+    L.gpl_download.restype = ip
+    L.gpl_advance_step.argtypes = [vp, C.c_double, dp]
+    L.gpl_advance_step.restype = C.c_uint32
+    L.gpl_next_time_step.argtypes = [vp, C.c_double, C.c_double]
+    L.gpl_next_time_step.restype = C.c_double
+    L.gpl_run.argtypes = [vp, ip, C.c_double, dp, dp]
+    L.gpl_run.restype = C.c_uint32
+    L.gpl_set_timing.argtypes = [vp, ip]
+    L.gpl_set_timing.restype = ip
+    L.gpl_kernel_count.argtypes = [vp]
+    L.gpl_kernel_count.restype = ip
+    L.gpl_kernel_stats.argtypes = [vp, ip, C.POINTER(C.c_char_p), C.POINTER(C.c_longlong), dp]
+    L.gpl_kernel_stats.restype = ip
+    L.gpl_last_run_ms.argtypes = [vp]
+    L.gpl_last_run_ms.restype = C.c_double
+    L.gpl_device_bytes.argtypes = [vp]
+    L.gpl_device_bytes.restype = C.c_size_t
+    L.gpl_strerror.argtypes = [ip]
+    L.gpl_strerror.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {load().gpl_strerror(rc).decode()} ({rc})")

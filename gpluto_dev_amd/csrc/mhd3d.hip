@@ -1,0 +1,416 @@
+// 3D ideal-MHD stage update on a periodic box: PLM + HLLD/HLL + UCT_HLL
+// constrained transport + RK2/RK3, for gfx950.
+//
+// Device data model (one GPU, whole periodic box, no ghost zones: periodic
+// neighbours are addressed by wrapped indices, which is bitwise equivalent to
+// the reference's ghost copies because every ghost is an exact image):
+//   V [8][ncell]   primitives rho, v1..3, B1..3 (face averages), prs   (type R)
+//   U [5][ncell]   conserved rho, m1..3, E (FP64); the cell-centred field of
+//                  Uc is always the face average (ct_field_average.cpp:182),
+//                  so it is never stored
+//   Vs[3][ncell]   face-normal field of the lower face of every cell (FP64)
+//   F [3][5][ncell], CM[3][ncell]   interface flux / max speed  (type R)
+//   FD[3][5][ncell] UCT face data et, eb, aL, aR, d             (type R)
+//   E [3][ncell]   edge EMFs Ex(i,j-1/2,k-1/2), Ey(i-1/2,j,k-1/2),
+//                  Ez(i-1/2,j-1/2,k)                            (type R)
+// Cell index c = i + nx*(j + ny*k) (reference Ary4D order, i fastest).
+#include "physics.hpp"
+#include "mhd3d.hpp"
+
+namespace gpl {
+
+__device__ __forceinline__ int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+struct Cell { int i, j, k; };
+
+__device__ __forceinline__ Cell cell_of(int c, const Box3 &g)
+{
+  Cell r;
+  r.i = c % g.nx;
+  int t = c / g.nx;
+  r.j = t % g.ny;
+  r.k = t / g.ny;
+  return r;
+}
+
+__device__ __forceinline__ int cid(int i, int j, int k, const Box3 &g)
+{
+  return i + g.nx*(j + g.ny*k);
+}
+
+// index of the cell displaced by o along direction D (periodic)
+template <int D>
+__device__ __forceinline__ int shift(const Cell &p, int o, const Box3 &g)
+{
+  if (D == 0) return cid(wrapm(p.i + o, g.nx), p.j, p.k, g);
+  if (D == 1) return cid(p.i, wrapm(p.j + o, g.ny), p.k, g);
+  return cid(p.i, p.j, wrapm(p.k + o, g.nz), g);
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v)
+{
+  __shared__ T red[32];
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, 64);
+    v = v >= w ? v : w;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  if (threadIdx.x == 0) {
+    T m = red[0];
+    for (int w = 1; w < nw; w++) m = m >= red[w] ? m : red[w];
+    red[0] = m;
+  }
+  __syncthreads();
+  return red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Sweep along D: one thread per interface (lower face of cell c).
+// DataToSweep + Reconstruct (plm_states.cpp:99-189) + Riemann solver
+// (hlld.cpp:100-500) + CT_Store_UCTHll (ct_store.hpp:156).
+// ---------------------------------------------------------------------------
+template <typename R, int D, int LIM, int SOLVER>
+__global__ __launch_bounds__(256)
+void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
+             R *__restrict__ F, R *__restrict__ CM, R *__restrict__ FD,
+             R gamma, uint32_t *__restrict__ err)
+{
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t n = (size_t)g.ncell;
+  const Cell p = cell_of(c, g);
+  const int cm2 = shift<D>(p, -2, g), cm1 = shift<D>(p, -1, g), cp1 = shift<D>(p, 1, g);
+  R vl[8], vr[8];
+#pragma unroll
+  for (int nv = 0; nv < 8; nv++) {
+    if (nv == 4 + D) continue;   // overwritten by the face value below
+    const R a = V[nv*n + cm2], b = V[nv*n + cm1], q = V[nv*n + c], d = V[nv*n + cp1];
+    const R sl = plm_slope<R, LIM>(a, b, q);
+    const R sr = plm_slope<R, LIM>(b, q, d);
+    vl[nv] = b + R(0.5)*sl;
+    vr[nv] = q - R(0.5)*sr;
+  }
+  // normal field from the face (plm_states.cpp:184-189)
+  vl[4 + D] = vr[4 + D] = (R)Vs[D*n + c];
+  FaceFlux<R> ff;
+  if (SOLVER == 3) mhd_hlld<R, D>(vl, vr, gamma, ff);
+  else             mhd_hll<R, D>(vl, vr, gamma, ff);
+  R *Fd = F + (size_t)D*5*n;
+  Fd[0*n + c] = ff.f[0];
+  Fd[1*n + c] = ff.f[1];
+  Fd[2*n + c] = ff.f[2];
+  Fd[3*n + c] = ff.f[3];
+  Fd[4*n + c] = ff.f[7];
+  CM[(size_t)D*n + c] = rmax(rabs(ff.Sl), rabs(ff.Sr));
+  R et, eb, aL, aR, dd;
+  uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+  R *fd = FD + (size_t)D*5*n;
+  fd[0*n + c] = et;
+  fd[1*n + c] = eb;
+  fd[2*n + c] = aL;
+  fd[3*n + c] = aR;
+  fd[4*n + c] = dd;
+  if (ff.err) atomicOr(err, ff.err);
+}
+
+// ---------------------------------------------------------------------------
+// Flux divergence + RK combination of the hydro conserved variables.
+// RightHandSide (rhs.cpp:384-386,523; :671,756; :900,981), dUc accumulation
+// in x, y, z order (update_stage.cpp:173-184,272-284,369-380) and the RK
+// stage update (rk_step.cpp:366-374,509-518,670-708).  Stage 1 also emits the
+// per-block maximum of C_dt for getMax (update_stage.cpp:446-458).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256)
+void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
+                  double *__restrict__ U, double *__restrict__ U0,
+                  const StepCtl *__restrict__ ctl, RKW w, int first_stage,
+                  double *__restrict__ cdt_part)
+{
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  double cdt = -1.0;
+  if (c < g.ncell) {
+    const size_t n = (size_t)g.ncell;
+    const Cell p = cell_of(c, g);
+    const int ci = shift<0>(p, 1, g), cj = shift<1>(p, 1, g), ck = shift<2>(p, 1, g);
+    const double dt = ctl->dt;
+    const double dtdx = dt/g.dx, dtdy = dt/g.dy, dtdz = dt/g.dz;
+    const R *Fx = F, *Fy = F + 5*n, *Fz = F + 10*n;
+#pragma unroll
+    for (int nv = 0; nv < 5; nv++) {
+      const size_t o = nv*n;
+      double rx = -dtdx*((double)Fx[o + ci] - (double)Fx[o + c]);
+      rx += 0.0;
+      double ry = -dtdy*((double)Fy[o + cj] - (double)Fy[o + c]);
+      ry += 0.0;
+      double rz = -dtdz*((double)Fz[o + ck] - (double)Fz[o + c]);
+      rz += 0.0;
+      double du = rx;
+      du += ry;
+      du += rz;
+      if (first_stage) {
+        const double u0 = U[o + c];
+        U0[o + c] = u0;
+        U[o + c] = u0 + w.wR*du;
+      } else {
+        U[o + c] = w.w0*U0[o + c] + w.w1*U[o + c] + w.wR*du;
+      }
+    }
+    if (cdt_part) {
+      cdt = 0.5*((double)CM[c] + (double)CM[ci])/g.dx;
+      cdt += 0.5*((double)CM[n + c] + (double)CM[n + cj])/g.dy;
+      cdt += 0.5*((double)CM[2*n + c] + (double)CM[2*n + ck])/g.dz;
+    }
+  }
+  if (cdt_part) {
+    const double m = block_max(cdt);
+    if (threadIdx.x == 0) cdt_part[blockIdx.x] = m;
+  }
+}
+
+// PLM of a scalar column for the EMF average (array_reconstruct.cpp:167-205):
+// left state at the interface between q1 and q2 from (q0,q1,q2), right state
+// from (q1,q2,q3).
+template <typename R, int LIM>
+__device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
+{
+  L  = q1 + R(0.5)*plm_slope<R, LIM>(q0, q1, q2);
+  Rr = q2 - R(0.5)*plm_slope<R, LIM>(q1, q2, q3);
+}
+
+// one UCT contribution (ct_emf_average.cpp:636-643): returns
+// (aL vL bL + aR vR bR) + sgn*(d bR - d bL) with coefficients averaged over
+// the two faces fa, fb.
+template <typename R, int LIMV, int RD>
+__device__ __forceinline__ R uct_term(const Cell &p, const Box3 &g, const double *__restrict__ Q,
+                                      const R *__restrict__ ecomp, const R *__restrict__ coef,
+                                      int fa, int fb, int sgn)
+{
+  const size_t n = (size_t)g.ncell;
+  const int c = cid(p.i, p.j, p.k, g);
+  const int m2 = shift<RD>(p, -2, g), m1 = shift<RD>(p, -1, g), p1 = shift<RD>(p, 1, g);
+  R bL, bR, vL, vR;
+  rec4<R, MC_LIM>((R)Q[m2], (R)Q[m1], (R)Q[c], (R)Q[p1], bL, bR);   // recB = MC_LIM
+  rec4<R, LIMV>(ecomp[m2], ecomp[m1], ecomp[c], ecomp[p1], vL, vR);  // recV = LIMITER
+  const R aL = R(0.5)*(coef[0*n + fa] + coef[0*n + fb]);
+  const R aR = R(0.5)*(coef[1*n + fa] + coef[1*n + fb]);
+  const R dL = R(0.5)*(coef[2*n + fa] + coef[2*n + fb]);
+  const R dR = dL;
+  const R phi = dR*bR - dL*bL;
+  const R s = aL*vL*bL + aR*vR*bR;
+  return sgn > 0 ? s + phi : s - phi;
+}
+
+// ---------------------------------------------------------------------------
+// Edge EMFs, CT_EMF_Riemann2D for UCT_HLL (ct_emf_average.cpp:535-989):
+// Ez = X1a + X2b, Ey = X1b + X3a, Ex = X2a + X3b, same order as the reference.
+// ---------------------------------------------------------------------------
+template <typename R, int LIMV>
+__global__ __launch_bounds__(256)
+void k_emf(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
+{
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t n = (size_t)g.ncell;
+  const Cell p = cell_of(c, g);
+  const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
+  const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
+  // slots: 0 = et, 1 = eb, 2.. = aL, aR, d
+  const int im = shift<0>(p, -1, g), jm = shift<1>(p, -1, g), km = shift<2>(p, -1, g);
+  R ez = uct_term<R, LIMV, 0>(p, g, By, FDy + 1*n, FDx + 2*n, jm, c, +1);   // X1a: By, e[K][J]
+  ez  += uct_term<R, LIMV, 1>(p, g, Bx, FDx + 0*n, FDy + 2*n, im, c, -1);   // X2b: Bx, e[K][I]
+  R ey = uct_term<R, LIMV, 0>(p, g, Bz, FDz + 0*n, FDx + 2*n, km, c, -1);   // X1b: Bz, e[J][K]
+  ey  += uct_term<R, LIMV, 2>(p, g, Bx, FDx + 1*n, FDz + 2*n, im, c, +1);   // X3a: Bx, e[J][I]
+  R ex = uct_term<R, LIMV, 1>(p, g, Bz, FDz + 1*n, FDy + 2*n, km, c, +1);   // X2a: Bz, e[I][K]
+  ex  += uct_term<R, LIMV, 2>(p, g, By, FDy + 0*n, FDz + 2*n, jm, c, -1);   // X3b: By, e[I][J]
+  E[c] = ex;
+  E[n + c] = ey;
+  E[2*n + c] = ez;
+}
+
+// Rs = -dt curl E on the three lower faces of cell (i,j,k), Cartesian
+// CT_Update (ct_update.cpp:181-258, 266-339, 348-422).
+struct FaceR { double bx, by, bz; };
+
+template <typename R>
+__device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const R *__restrict__ E,
+                                       double dt)
+{
+  const size_t n = (size_t)g.ncell;
+  const int ip = wrapm(i + 1, g.nx), jp = wrapm(j + 1, g.ny), kp = wrapm(k + 1, g.nz);
+  const R *Ex = E, *Ey = E + n, *Ez = E + 2*n;
+  const double ex0 = Ex[cid(i, j, k, g)], exj = Ex[cid(i, jp, k, g)], exk = Ex[cid(i, j, kp, g)];
+  const double ey0 = Ey[cid(i, j, k, g)], eyi = Ey[cid(ip, j, k, g)], eyk = Ey[cid(i, j, kp, g)];
+  const double ez0 = Ez[cid(i, j, k, g)], ezi = Ez[cid(ip, j, k, g)], ezj = Ez[cid(i, jp, k, g)];
+  const double dtdx1 = dt/g.dx, dtdx2 = dt/g.dy, dtdx3 = dt/g.dz;
+  FaceR r;
+  double curlE = 0.0 + dtdx2*(ezj - ez0);
+  curlE = curlE - dtdx3*(eyk - ey0);
+  r.bx = -curlE;
+  curlE = - dtdx1*(ezi - ez0);
+  curlE = curlE + dtdx3*(exk - ex0);
+  r.by = -curlE;
+  curlE = dtdx1*(eyi - ey0) - dtdx2*(exj - ex0);
+  r.bz = -curlE;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Face update + cell-centred field + ConsToPrim.
+// Staggered RK combination (rk_step.cpp:378-385,540-552), field average
+// (ct_field_average.cpp:182-184) and ConsToPrim3D (mappers3D.cpp:104-194).
+// Each thread owns the three lower faces of its cell; the three upper faces
+// it needs for the average are recomputed from the same inputs (bitwise the
+// value their owner writes).  Vs is double-buffered (Vin -> Vout).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256)
+void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
+              double *__restrict__ Vout, double *__restrict__ Bs0,
+              double *__restrict__ U, R *__restrict__ V,
+              const StepCtl *__restrict__ ctl, RKW w, int first_stage,
+              double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err)
+{
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t n = (size_t)g.ncell;
+  const Cell p = cell_of(c, g);
+  const double dt = ctl->dt;
+  const int ci = shift<0>(p, 1, g), cj = shift<1>(p, 1, g), ck = shift<2>(p, 1, g);
+  const FaceR r0 = ct_rs<R>(p.i, p.j, p.k, g, E, dt);
+  const FaceR rx = ct_rs<R>(wrapm(p.i + 1, g.nx), p.j, p.k, g, E, dt);
+  const FaceR ry = ct_rs<R>(p.i, wrapm(p.j + 1, g.ny), p.k, g, E, dt);
+  const FaceR rz = ct_rs<R>(p.i, p.j, wrapm(p.k + 1, g.nz), g, E, dt);
+  double b_lo[3], b_hi[3];
+  const double rl[3] = {r0.bx, r0.by, r0.bz};
+  const double rh[3] = {rx.bx, ry.by, rz.bz};
+  const int up[3] = {ci, cj, ck};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double *vin = Vin + d*n;
+    if (first_stage) {
+      const double s0 = vin[c], s1 = vin[up[d]];
+      Bs0[d*n + c] = s0;
+      b_lo[d] = s0 + w.wR*rl[d];
+      b_hi[d] = s1 + w.wR*rh[d];
+    } else {
+      const double *b0 = Bs0 + d*n;
+      b_lo[d] = w.w0*b0[c] + w.w1*vin[c] + w.wR*rl[d];
+      b_hi[d] = w.w0*b0[up[d]] + w.w1*vin[up[d]] + w.wR*rh[d];
+    }
+    Vout[d*n + c] = b_lo[d];
+  }
+  double b[3], u[5], v[8];
+#pragma unroll
+  for (int d = 0; d < 3; d++) b[d] = 0.5*(b_lo[d] + b_hi[d]);
+#pragma unroll
+  for (int nv = 0; nv < 5; nv++) u[nv] = U[nv*n + c];
+  const uint32_t e = mhd_cons_to_prim(u, b, gmm1, small_rho, small_prs, v);
+  if (e) {
+#pragma unroll
+    for (int nv = 0; nv < 5; nv++) U[nv*n + c] = u[nv];
+    atomicOr(err, e);
+  }
+#pragma unroll
+  for (int nv = 0; nv < 8; nv++) V[nv*n + c] = (R)v[nv];
+}
+
+// ---------------------------------------------------------------------------
+// invDt_hyp = max(C_dt)/DIMENSIONS (update_stage.cpp:422-425); optionally the
+// NextTimeStep recurrence (next_time_step.cpp:80,123-125) and time += dt
+// (main.cpp:458).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024)
+void k_dt(const double *__restrict__ part, int nparts, double dims, StepCtl *ctl,
+          double cfl, double cfl_max_var, int advance)
+{
+  double m = -1.0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) m = m >= part[q] ? m : part[q];
+  m = block_max(m);
+  if (threadIdx.x == 0) {
+    const double inv = m/dims;
+    ctl->inv_dt = inv;
+    if (advance) {
+      const double dt = ctl->dt;
+      ctl->time += dt;
+      const double dt_new = cfl/inv;
+      ctl->dt = dt_new <= cfl_max_var*dt ? dt_new : cfl_max_var*dt;
+      ctl->nstep += 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers: one entry per kernel family, so that the host
+// orchestration can bracket each launch with timing events.
+// ---------------------------------------------------------------------------
+template <typename R, int LIM, int SOLVER>
+static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
+                     int first_stage, int want_dt, double gamma, double small_rho,
+                     double small_prs, hipStream_t s)
+{
+  const int T = 256;
+  const int nb = (g.ncell + T - 1)/T;
+  switch (which) {
+  case K_SWEEP_X:
+    hipLaunchKernelGGL((k_sweep<R, 0, LIM, SOLVER>), dim3(nb), dim3(T), 0, s,
+                       g, b.V, b.Vin, b.F, b.CM, b.FD, (R)gamma, b.err);
+    break;
+  case K_SWEEP_Y:
+    hipLaunchKernelGGL((k_sweep<R, 1, LIM, SOLVER>), dim3(nb), dim3(T), 0, s,
+                       g, b.V, b.Vin, b.F, b.CM, b.FD, (R)gamma, b.err);
+    break;
+  case K_SWEEP_Z:
+    hipLaunchKernelGGL((k_sweep<R, 2, LIM, SOLVER>), dim3(nb), dim3(T), 0, s,
+                       g, b.V, b.Vin, b.F, b.CM, b.FD, (R)gamma, b.err);
+    break;
+  case K_RHS:
+    hipLaunchKernelGGL((k_rhs_update<R>), dim3(nb), dim3(T), 0, s,
+                       g, b.F, b.CM, b.U, b.U0, ctl, w, first_stage,
+                       want_dt ? b.cdt_part : nullptr);
+    break;
+  case K_EMF:
+    hipLaunchKernelGGL((k_emf<R, LIM>), dim3(nb), dim3(T), 0, s, g, b.Vin, b.FD, b.E);
+    break;
+  case K_FINISH:
+    hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(T), 0, s,
+                       g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
+                       gamma - 1.0, small_rho, small_prs, b.err);
+    break;
+  default:
+    break;
+  }
+}
+
+template <typename R>
+void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
+                  int first_stage, int want_dt, int limiter, int solver, double gamma,
+                  double small_rho, double small_prs, hipStream_t s)
+{
+  if (limiter == MC_LIM) {
+    if (solver == 3) launch_t<R, MC_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+    else             launch_t<R, MC_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  } else {
+    if (solver == 3) launch_t<R, VANLEER_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+    else             launch_t<R, VANLEER_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  }
+}
+
+int mhd3d_blocks(const Box3 &g) { return (g.ncell + 255)/256; }
+
+void launch_dt(const double *part, int nparts, double dims, StepCtl *ctl, double cfl,
+               double cfl_max_var, int advance, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, s, part, nparts, dims, ctl, cfl, cfl_max_var, advance);
+}
+
+template void mhd3d_launch<double>(int, const Mhd3dBufs<double> &, const Box3 &, const StepCtl *, RKW,
+                                   int, int, int, int, double, double, double, hipStream_t);
+template void mhd3d_launch<float>(int, const Mhd3dBufs<float> &, const Box3 &, const StepCtl *, RKW,
+                                  int, int, int, int, double, double, double, hipStream_t);
+
+}  // namespace gpl

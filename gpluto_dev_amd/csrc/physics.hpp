@@ -1,0 +1,355 @@
+// Pointwise physics of the Godunov stage update, shared by every kernel.
+//
+// Templated on the arithmetic type R: R = double is the drop-in path and
+// keeps the reference's operation order exactly (the translation unit is
+// compiled with -ffp-contract=off, so a*b+c stays a rounded multiply and a
+// rounded add); R = float is the mixed-precision form's Riemann/PLM type.
+//
+// Variable order (reference Src/MHD/mhd.hpp:21-33): rho, v1, v2, v3, B1, B2,
+// B3, prs; HD (Src/HD/hd.hpp:21-28): rho, v1, v2, v3, prs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GPL_HD __host__ __device__ __forceinline__
+
+namespace gpl {
+
+// Reference error bits (Src/gpluto.hpp:247-262)
+constexpr uint32_t ERR_NEGATIVE_RHO   = 1u << 0;
+constexpr uint32_t ERR_NEGATIVE_ENG   = 1u << 1;
+constexpr uint32_t ERR_NEGATIVE_PRS   = 1u << 2;
+constexpr uint32_t ERR_CONS2PRIM_FAIL = 1u << 4;
+constexpr uint32_t ERR_NAN            = 1u << 11;
+constexpr uint32_t ERR_QUIT           = 1u << 15;
+
+constexpr int MC_LIM = 27, VANLEER_LIM = 26;   // Src/gpluto.hpp:335-336
+
+template <typename R> GPL_HD R rmax(R a, R b) { return a >= b ? a : b; }   // MAX, Src/macros.hpp:191
+template <typename R> GPL_HD R rmin(R a, R b) { return a <= b ? a : b; }   // MIN, Src/macros.hpp:194
+template <typename R> GPL_HD R rabs(R a) { return a < R(0) ? -a : a; }
+template <typename R> GPL_HD R abs_min(R a, R b) { return rabs(a) < rabs(b) ? a : b; }  // ABS_MIN
+template <typename R> GPL_HD R rsqrt_(R a);
+template <> GPL_HD double rsqrt_<double>(double a) { return sqrt(a); }
+template <> GPL_HD float rsqrt_<float>(float a) { return sqrtf(a); }
+template <typename R> GPL_HD bool risnan(R a) { return a != a; }
+
+// PLM limited slope, Src/States/plm_states.cpp:121-133 with SET_MC_LIMITER /
+// SET_VL_LIMITER on a uniform Cartesian grid (Src/plm_coeffs.hpp:92-100).
+template <typename R, int LIM>
+GPL_HD R plm_slope(R vm, R v0, R vp)
+{
+  R dvp = (vp - v0)*R(1);
+  R dvm = (v0 - vm)*R(1);
+  R dv;
+  if (LIM == MC_LIM) {
+    if (dvp*dvm > R(0)) {
+      R qc = R(0.5)*(dvm + dvp), scrh = R(2)*abs_min(dvp, dvm);
+      dv = abs_min(qc, scrh);
+    } else dv = R(0);
+  } else {
+    dv = (dvp*dvm > R(0) ? R(2)*dvp*dvm/(dvp + dvm) : R(0));
+  }
+  return dv;
+}
+
+// ---------------------------------------------------------------------
+// MHD
+// ---------------------------------------------------------------------
+
+// PrimToCons, Src/MHD/mhd.hpp:906-934
+template <typename R>
+GPL_HD void mhd_prim_to_cons(const R *v, R *u, R gmm1)
+{
+  u[0] = v[0];
+  u[1] = v[0]*v[1];
+  u[2] = v[0]*v[2];
+  u[3] = v[0]*v[3];
+  u[4] = v[4];
+  u[5] = v[5];
+  u[6] = v[6];
+  R kinb2 = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  kinb2 = v[0]*kinb2 + v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  kinb2 *= R(0.5);
+  u[7] = kinb2 + v[7]/gmm1;
+}
+
+// Flux<dir>, Src/MHD/mhd.hpp:164-218
+template <typename R, int N>
+GPL_HD void mhd_flux(const R *v, const R *u, R *flux, R &pt)
+{
+  constexpr int VXn = 1 + N, BXn = 4 + N;
+  R vB = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  pt = v[7] + R(0.5)*(v[4]*v[4] + v[5]*v[5] + v[6]*v[6]);
+  flux[0] = u[1 + N];
+  flux[1] = u[1]*v[VXn] - v[4]*v[BXn];
+  flux[2] = u[2]*v[VXn] - v[5]*v[BXn];
+  flux[3] = u[3]*v[VXn] - v[6]*v[BXn];
+  flux[4] = v[4]*v[VXn] - v[BXn]*v[1];
+  flux[5] = v[5]*v[VXn] - v[BXn]*v[2];
+  flux[6] = v[6]*v[VXn] - v[BXn]*v[3];
+  flux[7] = (u[7] + pt)*v[VXn] - v[BXn]*vB;
+  flux[1 + N] += pt;
+}
+
+// MaxSignalSpeed<dir>, Src/MHD/mhd.hpp:242-280; a2 = SoundSpeed2 (Src/eos.hpp:92)
+template <typename R, int N>
+GPL_HD void mhd_max_signal_speed(const R *v, R gamma, R &cmin, R &cmax)
+{
+  constexpr int VXn = 1 + N;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
+  R a2 = gamma*v[7]/v[0];
+  R gpr = a2*v[0];
+  R b1 = v[BXn], b2 = v[BXt], b3 = v[BXb];
+  R Btmag2 = b2*b2 + b3*b3;
+  R Bmag2  = b1*b1 + Btmag2;
+  R cf = gpr - Bmag2;
+  cf = gpr + Bmag2 + rsqrt_(cf*cf + R(4)*gpr*Btmag2);
+  cf = rsqrt_(R(0.5)*cf/v[0]);
+  cmax = v[VXn] + cf;
+  cmin = v[VXn] - cf;
+}
+
+// Result of one interface solve: hydro flux (rho, m1..3, E), the three
+// magnetic flux components (kept for 2D, where B3 is cell-centred) and the
+// Davis speeds.
+template <typename R>
+struct FaceFlux {
+  R f[8];
+  R Sl, Sr;
+  uint32_t err;
+};
+
+// HLLD, Src/MHD/hlld.cpp:62-500 (ideal EOS, no GLM, no background field)
+template <typename R, int N>
+GPL_HD void mhd_hlld(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
+{
+  constexpr int VXn = 1 + N, VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3;
+  constexpr int MXn = VXn, MXt = VXt, MXb = VXb;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
+  constexpr int ENG = 7;
+  const R gmm1 = gamma - R(1);
+  R ul[8], ur[8], fl[8], fr[8], usl[8], usr[8];
+  R ptl, ptr, sl_min, sl_max, sr_min, sr_max;
+  int switch_to_hll = 0;
+
+  mhd_prim_to_cons(vl, ul, gmm1);
+  mhd_prim_to_cons(vr, ur, gmm1);
+  mhd_max_signal_speed<R, N>(vl, gamma, sl_min, sl_max);    // HLL_Speed, mhd.hpp:296
+  mhd_max_signal_speed<R, N>(vr, gamma, sr_min, sr_max);
+  const R Sl = rmin(sl_min, sr_min);
+  const R Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  mhd_flux<R, N>(vl, ul, fl, ptl);
+  mhd_flux<R, N>(vr, ur, fr, ptr);
+
+  R dul, dur, Bx, Bx1, sBx, pts, SM;
+  {
+    R scrh = R(1)/(Sr - Sl);
+    Bx1 = Bx = (Sr*vr[BXn] - Sl*vl[BXn])*scrh;
+    sBx = (Bx > R(0) ? R(1) : R(-1));
+    dul = Sl - vl[VXn];
+    dur = Sr - vr[VXn];
+    scrh = R(1)/(dur*ur[0] - dul*ul[0]);
+    SM = (dur*ur[MXn] - dul*ul[MXn] - ptr + ptl)*scrh;
+    pts =   dur*ur[0]*ptl - dul*ul[0]*ptr
+          + vl[0]*vr[0]*dur*dul*(vr[VXn] - vl[VXn]);
+    pts *= scrh;
+  }
+  usl[0] = ul[0]*dul/(Sl - SM);
+  usr[0] = ur[0]*dur/(Sr - SM);
+  if (usl[0] < R(0) || usr[0] < R(0)) switch_to_hll = 1;
+  R sqrL = rsqrt_(usl[0]);
+  R sqrR = rsqrt_(usr[0]);
+  R S1L = SM - rabs(Bx)/sqrL;
+  R S1R = SM + rabs(Bx)/sqrR;
+
+  R *flux = out.f;
+  if (Sl >= R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
+  } else if (!switch_to_hll) {
+    int switch_to_hllc = 0;
+    if ((S1L - Sl) <  R(1.e-4)*(SM - Sl)) switch_to_hllc = 1;
+    if ((S1R - Sr) > -R(1.e-4)*(Sr - SM)) switch_to_hllc = 1;
+    R scrhL, scrhR;
+    if (!switch_to_hllc) {
+      scrhL = (ul[0]*dul*dul - Bx*Bx)/(ul[0]*dul*(Sl - SM) - Bx*Bx);
+      scrhR = (ur[0]*dur*dur - Bx*Bx)/(ur[0]*dur*(Sr - SM) - Bx*Bx);
+      usl[BXn] = Bx1;
+      usl[BXt] = ul[BXt]*scrhL;
+      usl[BXb] = ul[BXb]*scrhL;
+      usr[BXn] = Bx1;
+      usr[BXt] = ur[BXt]*scrhR;
+      usr[BXb] = ur[BXb]*scrhR;
+    } else {
+      R scrh = R(1)/(Sr - Sl);
+      R ut, ub;
+      ut = Sr*ur[BXt] - Sl*ul[BXt] + fl[BXt] - fr[BXt];  ut *= scrh;
+      ub = Sr*ur[BXb] - Sl*ul[BXb] + fl[BXb] - fr[BXb];  ub *= scrh;
+      usl[BXn] = usr[BXn] = Bx1;
+      usl[BXt] = usr[BXt] = ut;
+      usl[BXb] = usr[BXb] = ub;
+      S1L = S1R = SM;
+    }
+    scrhL = Bx/(ul[0]*dul);
+    scrhR = Bx/(ur[0]*dur);
+    R vsL = vl[VXt] - scrhL*(usl[BXt] - ul[BXt]);
+    R vsR = vr[VXt] - scrhR*(usr[BXt] - ur[BXt]);
+    R wsL = vl[VXb] - scrhL*(usl[BXb] - ul[BXb]);
+    R wsR = vr[VXb] - scrhR*(usr[BXb] - ur[BXb]);
+    usl[MXn] = usl[0]*SM;
+    usr[MXn] = usr[0]*SM;
+    usl[MXt] = usl[0]*vsL;
+    usr[MXt] = usr[0]*vsR;
+    usl[MXb] = usl[0]*wsL;
+    usr[MXb] = usr[0]*wsR;
+    scrhL  = vl[VXn]*Bx1 + vl[VXt]*ul[BXt] + vl[VXb]*ul[BXb];
+    scrhL -= SM*Bx1 + vsL*usl[BXt] + wsL*usl[BXb];
+    usl[ENG]  = dul*ul[ENG] - ptl*vl[VXn] + pts*SM + Bx*scrhL;
+    usl[ENG] /= Sl - SM;
+    scrhR  = vr[VXn]*Bx1 + vr[VXt]*ur[BXt] + vr[VXb]*ur[BXb];
+    scrhR -= SM*Bx1 + vsR*usr[BXt] + wsR*usr[BXb];
+    usr[ENG]  = dur*ur[ENG] - ptr*vr[VXn] + pts*SM + Bx*scrhR;
+    usr[ENG] /= Sr - SM;
+
+    if (S1L >= R(0)) {
+      for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv] + Sl*(usl[nv] - ul[nv]);
+    } else if (S1R <= R(0)) {
+      for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv] + Sr*(usr[nv] - ur[nv]);
+    } else {
+      R ussl[8], ussr[8];
+      ussl[0] = usl[0];
+      ussr[0] = usr[0];
+      R vss = sqrL*vsL + sqrR*vsR + (usr[BXt] - usl[BXt])*sBx;
+      vss /= sqrL + sqrR;
+      R wss = sqrL*wsL + sqrR*wsR + (usr[BXb] - usl[BXb])*sBx;
+      wss /= sqrL + sqrR;
+      ussl[MXn] = ussl[0]*SM;
+      ussr[MXn] = ussr[0]*SM;
+      ussl[MXt] = ussl[0]*vss;
+      ussr[MXt] = ussr[0]*vss;
+      ussl[MXb] = ussl[0]*wss;
+      ussr[MXb] = ussr[0]*wss;
+      ussl[BXn] = ussr[BXn] = Bx1;
+      ussl[BXt]  = sqrL*usr[BXt] + sqrR*usl[BXt] + sqrL*sqrR*(vsR - vsL)*sBx;
+      ussl[BXt] /= sqrL + sqrR;
+      ussr[BXt]  = ussl[BXt];
+      ussl[BXb]  = sqrL*usr[BXb] + sqrR*usl[BXb] + sqrL*sqrR*(wsR - wsL)*sBx;
+      ussl[BXb] /= sqrL + sqrR;
+      ussr[BXb]  = ussl[BXb];
+      scrhL  = SM*Bx1 + vsL*usl [BXt] + wsL*usl [BXb];
+      scrhL -= SM*Bx1 + vss*ussl[BXt] + wss*ussl[BXb];
+      scrhR  = SM*Bx1 + vsR*usr [BXt] + wsR*usr [BXb];
+      scrhR -= SM*Bx1 + vss*ussr[BXt] + wss*ussr[BXb];
+      ussl[ENG] = usl[ENG] - sqrL*scrhL*sBx;
+      ussr[ENG] = usr[ENG] + sqrR*scrhR*sBx;
+      if (SM >= R(0)) {
+        for (int nv = 0; nv < 8; nv++)
+          flux[nv] = fl[nv] + S1L*(ussl[nv] - usl[nv]) + Sl*(usl[nv] - ul[nv]);
+      } else {
+        for (int nv = 0; nv < 8; nv++)
+          flux[nv] = fr[nv] + S1R*(ussr[nv] - usr[nv]) + Sr*(usr[nv] - ur[nv]);
+      }
+    }
+  } else {
+    R scrh = R(1)/(Sr - Sl);
+    for (int nv = 0; nv < 8; nv++) {
+      flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+      flux[nv] *= scrh;
+    }
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+// HLL with Davis speeds (Src/MHD/hll.cpp; formula as Src/MHD/hlld.cpp:452-461)
+template <typename R, int N>
+GPL_HD void mhd_hll(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
+{
+  const R gmm1 = gamma - R(1);
+  R ul[8], ur[8], fl[8], fr[8], ptl, ptr, sl_min, sl_max, sr_min, sr_max;
+  mhd_prim_to_cons(vl, ul, gmm1);
+  mhd_prim_to_cons(vr, ur, gmm1);
+  mhd_max_signal_speed<R, N>(vl, gamma, sl_min, sl_max);
+  mhd_max_signal_speed<R, N>(vr, gamma, sr_min, sr_max);
+  const R Sl = rmin(sl_min, sr_min);
+  const R Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  mhd_flux<R, N>(vl, ul, fl, ptl);
+  mhd_flux<R, N>(vr, ur, fr, ptr);
+  R *flux = out.f;
+  if (Sl > R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
+  } else if (Sr < R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
+  } else {
+    R scrh = R(1)/(Sr - Sl);
+    for (int nv = 0; nv < 8; nv++) {
+      flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+      flux[nv] *= scrh;
+    }
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+// CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177.  Per face:
+//   et = e[DIRb][DIRn] (upwinded transverse velocity v_t),
+//   eb = e[DIRt][DIRn] (minus upwinded v_b),
+//   aL, aR weights and the diffusion coefficient d (= dL = dR).
+template <typename R, int N>
+GPL_HD void uct_hll_store(const R *vl, const R *vr, R Sl, R Sr, R &et, R &eb, R &aL, R &aR, R &d)
+{
+  constexpr int VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3;
+  R aR_1 = rmax(R(0), Sr);
+  R aL_1 = rmin(R(0), Sl);
+  R scrh = R(1)/(aR_1 - aL_1);
+  et =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
+  eb = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
+  aL =  aR_1*scrh;
+  aR = -aL_1*scrh;
+  d  = -aL_1*aR_1*scrh;
+}
+
+// ConsToPrim MHD (Src/MHD/mappers.cpp:32-128) for the hydro part; the cell
+// field b[] is the face average (CT_AverageStaggeredFields).  u[] holds
+// rho, m1, m2, m3, E and may be rewritten by the pressure/energy floors.
+GPL_HD uint32_t mhd_cons_to_prim(double *u, const double *b, double gmm1,
+                                  double small_rho, double small_prs, double *v)
+{
+  uint32_t err = 0;
+  for (int nv = 0; nv < 5; nv++) if (u[nv] != u[nv]) err |= ERR_NAN | ERR_QUIT;
+  for (int nv = 0; nv < 3; nv++) if (b[nv] != b[nv]) err |= ERR_NAN | ERR_QUIT;
+  uint32_t le = 0;
+  if (u[0] < 0.0) { u[0] = small_rho; le |= ERR_NEGATIVE_RHO | ERR_CONS2PRIM_FAIL; }
+  v[0] = u[0];
+  double tau = 1.0/u[0];
+  v[1] = u[1]*tau;
+  v[2] = u[2]*tau;
+  v[3] = u[3]*tau;
+  v[4] = b[0]; v[5] = b[1]; v[6] = b[2];
+  double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  double b2 = b[0]*b[0] + b[1]*b[1] + b[2]*b[2];
+  double kinb2 = 0.5*(m2*tau + b2);
+  if (u[4] > 0.0) {
+    v[7] = gmm1*(u[4] - kinb2);
+    if (v[7] < 0.0) {
+      v[7] = small_prs;
+      u[4] = v[7]/gmm1 + kinb2;
+      le |= ERR_NEGATIVE_PRS;
+    }
+  } else {
+    u[4] = small_prs/gmm1 + kinb2;
+    v[7] = small_prs;
+    le |= ERR_NEGATIVE_ENG;
+  }
+  err |= le;
+  if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
+  return err;
+}
+
+}  // namespace gpl

@@ -1,0 +1,196 @@
+"""Host-side mirror of the reference's time-step seam over the HIP library.
+
+``StageUpdate`` holds one device domain and exposes the reference's entry
+points with the same meaning:
+
+* ``advance_step(dt)``        -- AdvanceStep(data, grid, Dts)   (Src/rk_step.cpp:88)
+* ``next_time_step(dt, inv)`` -- NextTimeStep(Dts, ...)          (Src/next_time_step.cpp:20)
+* ``run(nsteps, first_dt)``   -- the main loop of Src/main.cpp:320-485 (dt kept on the device)
+
+Selections come from the same two places as in the reference: the
+definitions.hpp macros (``Definitions``, or ``parse_definitions``) and the
+pluto.ini sections (``RuntimeSetup``, or ``parse_pluto_ini``).  Errors are
+returned as the reference's ERR_* bitmask (Src/gpluto.hpp:247-265).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+ERR_NEGATIVE_RHO = 1 << 0
+ERR_NEGATIVE_ENG = 1 << 1
+ERR_NEGATIVE_PRS = 1 << 2
+ERR_CONS2PRIM_FAIL = 1 << 4
+ERR_NAN = 1 << 11
+ERR_QUIT = 1 << 15
+
+_LIMITERS = {"MC_LIM": _lib.GPL_MC_LIM, "VANLEER_LIM": _lib.GPL_VANLEER_LIM,
+             "DEFAULT": _lib.GPL_VANLEER_LIM}
+_SOLVERS = {"hlld": _lib.GPL_SOLVER_HLLD, "hll": _lib.GPL_SOLVER_HLL, "hllc": _lib.GPL_SOLVER_HLLC}
+_STEPPING = {"RK2": _lib.GPL_RK2, "RK3": _lib.GPL_RK3}
+
+
+@dataclass
+class Definitions:
+    """definitions.hpp selections the stage update reads (Src/gpluto.hpp:385-561)."""
+    physics: str = "MHD"
+    dimensions: int = 3
+    time_stepping: str = "RK2"
+    limiter: str = "MC_LIM"                  # default VANLEER_LIM (Src/gpluto.hpp:462-464)
+    ct_emf_average: str = "UCT_HLL"          # default (Src/MHD/CT/ct.hpp:89-96)
+    reconstruction: str = "LINEAR"
+
+
+@dataclass
+class RuntimeSetup:
+    """pluto.ini selections (Src/runtime_setup.cpp:34-160)."""
+    n: tuple = (64, 64, 64)
+    xbeg: tuple = (0.0, 0.0, 0.0)
+    xend: tuple = (1.0, 1.0, 1.0)
+    cfl: float = 0.4
+    cfl_max_var: float = 1.1
+    first_dt: float = 1.e-4
+    solver: str = "hlld"
+    boundary: str = "periodic"
+    gamma: float = 5.0 / 3.0
+    extra: dict = field(default_factory=dict)
+
+
+def parse_definitions(text: str) -> Definitions:
+    d = Definitions()
+    macros = dict(re.findall(r"^\s*#define\s+(\w+)\s+(\S+)", text, flags=re.M))
+    d.physics = macros.get("PHYSICS", d.physics)
+    d.dimensions = int(macros.get("DIMENSIONS", d.dimensions))
+    d.time_stepping = macros.get("TIME_STEPPING", d.time_stepping)
+    d.limiter = macros.get("LIMITER", "VANLEER_LIM")
+    d.ct_emf_average = macros.get("CT_EMF_AVERAGE", d.ct_emf_average)
+    d.reconstruction = macros.get("RECONSTRUCTION", d.reconstruction)
+    return d
+
+
+def parse_pluto_ini(text: str) -> RuntimeSetup:
+    r = RuntimeSetup()
+    kv = {}
+    for line in text.splitlines():
+        line = line.split("#")[0].strip()
+        if not line or line.startswith("["):
+            continue
+        parts = line.split()
+        kv[parts[0]] = parts[1:]
+    n, xb, xe = [], [], []
+    for ax in ("X1-grid", "X2-grid", "X3-grid"):
+        f = kv.get(ax, ["0.0", "1", "1.0"])
+        # legacy 3-field form "xbeg n xend" (Src/grid_input.hpp)
+        xb.append(float(f[0]))
+        n.append(int(f[1]))
+        xe.append(float(f[2]))
+    r.n, r.xbeg, r.xend = tuple(n), tuple(xb), tuple(xe)
+    r.cfl = float(kv.get("CFL", [r.cfl])[0])
+    r.cfl_max_var = float(kv.get("CFL_max_var", [r.cfl_max_var])[0])
+    r.first_dt = float(kv.get("first_dt", [r.first_dt])[0])
+    r.solver = kv.get("Solver", [r.solver])[0]
+    r.boundary = kv.get("X1-beg", [r.boundary])[0]
+    return r
+
+
+class StageUpdate:
+    """One device domain of the Godunov stage update."""
+
+    def __init__(self, defs: Definitions, ini: RuntimeSetup, precision: str = "fp64",
+                 device: int = 0):
+        L = _lib.load()
+        cfg = _lib.GplConfig()
+        L.gpl_default_config(C.byref(cfg))
+        cfg.physics = _lib.GPL_MHD if defs.physics.upper() == "MHD" else _lib.GPL_HD
+        cfg.dimensions = defs.dimensions
+        cfg.nx, cfg.ny, cfg.nz = ini.n
+        for d in range(3):
+            cfg.xbeg[d], cfg.xend[d] = ini.xbeg[d], ini.xend[d]
+        cfg.boundary = _lib.GPL_PERIODIC if ini.boundary == "periodic" else _lib.GPL_OUTFLOW
+        cfg.time_stepping = _STEPPING[defs.time_stepping]
+        cfg.limiter = _LIMITERS[defs.limiter]
+        cfg.solver = _SOLVERS[ini.solver]
+        cfg.ct_emf_average = _lib.GPL_UCT_HLL
+        cfg.precision = _lib.GPL_FP64 if precision == "fp64" else _lib.GPL_MIXED
+        cfg.device = device
+        cfg.gamma = ini.gamma
+        cfg.cfl = ini.cfl
+        cfg.cfl_max_var = ini.cfl_max_var
+        self.defs, self.ini, self.precision = defs, ini, precision
+        self._L = L
+        h = C.c_void_p()
+        _lib.check(L.gpl_create(C.byref(cfg), C.byref(h)), "gpl_create")
+        self._h = h
+        self.n = tuple(ini.n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.gpl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state transfer -----------------------------------------------------
+    def upload(self, vc: np.ndarray, vs: np.ndarray, layout: int = _lib.GPL_LAYOUT_ACTIVE):
+        """vc[8, nz, ny, nx] primitives, vs[3, nz, ny, nx] lower faces (periodic)."""
+        vc = np.ascontiguousarray(vc, dtype=np.float64)
+        comps = [np.ascontiguousarray(vs[d], dtype=np.float64) for d in range(3)]
+        arr = (C.c_void_p * 3)(*[c.ctypes.data for c in comps])
+        _lib.check(self._L.gpl_upload(self._h, vc.ctypes.data, arr, layout), "gpl_upload")
+
+    def download(self):
+        nx, ny, nz = self.n
+        vc = np.empty((8, nz, ny, nx))
+        uc = np.empty((8, nz, ny, nx))
+        vs = np.empty((3, nz, ny, nx))
+        arr = (C.c_void_p * 3)(*[vs[d].ctypes.data for d in range(3)])
+        _lib.check(self._L.gpl_download(self._h, vc.ctypes.data, arr, uc.ctypes.data,
+                                        _lib.GPL_LAYOUT_ACTIVE), "gpl_download")
+        return vc, vs, uc
+
+    # -- stepping -----------------------------------------------------------
+    def advance_step(self, dt: float):
+        inv = C.c_double(0.0)
+        err = self._L.gpl_advance_step(self._h, dt, C.byref(inv))
+        return int(err), inv.value
+
+    def next_time_step(self, dt: float, inv_dt: float) -> float:
+        return self._L.gpl_next_time_step(self._h, dt, inv_dt)
+
+    def run(self, nsteps: int, first_dt: float):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        err = self._L.gpl_run(self._h, nsteps, first_dt, C.byref(t), C.byref(dt))
+        return int(err), t.value, dt.value
+
+    # -- measurement ----------------------------------------------------------
+    def set_timing(self, on: bool):
+        _lib.check(self._L.gpl_set_timing(self._h, 1 if on else 0), "gpl_set_timing")
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for k in range(self._L.gpl_kernel_count(self._h)):
+            name, cnt, ms = C.c_char_p(), C.c_longlong(0), C.c_double(0.0)
+            self._L.gpl_kernel_stats(self._h, k, C.byref(name), C.byref(cnt), C.byref(ms))
+            out[name.value.decode()] = (cnt.value, ms.value)
+        return out
+
+    def last_run_ms(self) -> float:
+        return self._L.gpl_last_run_ms(self._h)
+
+    def device_bytes(self) -> int:
+        return int(self._L.gpl_device_bytes(self._h))

@@ -1,0 +1,138 @@
+/* gpluto_amd: MI355X (gfx950) Godunov stage update behind a C ABI.
+ *
+ * Drop-in for the reference's time-step seam: AdvanceStep()
+ * (Src/rk_step.cpp:88, called from Src/main.cpp:446) and NextTimeStep()
+ * (Src/next_time_step.cpp:20, called from Src/main.cpp:463).  The device
+ * owns every buffer from gpl_create() to gpl_destroy(); host arrays are
+ * authoritative only across gpl_upload()/gpl_download() (the reference's
+ * "acc enter data" / "acc update self" points, Src/startup.cpp:63-73).
+ *
+ * Plain C types only.  Every entry point returns 0 (or the reference's
+ * ERR_* bitmask, Src/gpluto.hpp:247-265, for the stepping calls) on success
+ * and a negative GPL_E* code on an API error.
+ */
+#ifndef GPLUTO_AMD_H
+#define GPLUTO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PHYSICS of definitions.hpp (Src/gpluto.hpp) */
+#define GPL_HD    1
+#define GPL_MHD   2
+/* pluto.ini [Solver] names (Src/MHD/set_solver.cpp:35-47, Src/HD/set_solver.cpp:17-19) */
+#define GPL_SOLVER_HLL   1
+#define GPL_SOLVER_HLLC  2
+#define GPL_SOLVER_HLLD  3
+/* LIMITER of definitions.hpp (Src/gpluto.hpp:330-336) */
+#define GPL_VANLEER_LIM  26
+#define GPL_MC_LIM       27
+/* CT_EMF_AVERAGE (Src/MHD/CT/ct.hpp:73-96) */
+#define GPL_UCT_HLL      1
+/* TIME_STEPPING (Src/gpluto.hpp:121-122) */
+#define GPL_RK2          2
+#define GPL_RK3          3
+/* pluto.ini [Boundary] (same type on every side) */
+#define GPL_PERIODIC     1
+#define GPL_OUTFLOW      2
+/* arithmetic of the stage update */
+#define GPL_FP64         0   /* drop-in: FP64 everywhere, reference operation order */
+#define GPL_MIXED        1   /* FP32 state/PLM/Riemann; FP64 U, faces, CT, dt */
+/* array layouts accepted by gpl_upload / gpl_download */
+#define GPL_LAYOUT_ACTIVE 0  /* vc[nv][nz][ny][nx]; vs[d][nz][ny][nx] = lower face of each cell */
+#define GPL_LAYOUT_ARY4D  1  /* reference Ary4D (Src/Array.hpp:476-479) with nghost ghost
+                                zones per active direction; Vs[d] with one extra face along d */
+
+/* Stepping calls return this bit (never raised by the numerics, whose
+   ERR_* bits stop at 1<<18) when the call itself failed. */
+#define GPL_STEP_API_ERROR 0x80000000u
+
+/* API error codes */
+#define GPL_EINVAL       (-1)
+#define GPL_EHIP         (-2)
+#define GPL_ENOMEM       (-3)
+#define GPL_EUNSUPPORTED (-4)
+
+/* The selections of definitions.hpp and pluto.ini that the stage update
+ * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
+typedef struct gpl_config {
+  int    physics;          /* GPL_MHD (3D: nz > 1) */
+  int    dimensions;       /* DIMENSIONS */
+  int    nx, ny, nz;       /* [Grid] active zones per direction */
+  double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
+  int    boundary;         /* GPL_PERIODIC */
+  int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
+  int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
+  int    solver;           /* GPL_SOLVER_HLLD or GPL_SOLVER_HLL */
+  int    ct_emf_average;   /* GPL_UCT_HLL */
+  int    precision;        /* GPL_FP64 or GPL_MIXED */
+  int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */
+  int    device;           /* HIP device ordinal */
+  double gamma;            /* run_config.gamma */
+  double cfl;              /* [Time] CFL */
+  double cfl_max_var;      /* [Time] CFL_max_var */
+  double small_rho;        /* run_config.small_rho (default 1e-12) */
+  double small_prs;        /* run_config.small_prs (default 1e-12) */
+} gpl_config;
+
+typedef struct gpl_ctx gpl_ctx;
+
+/* Fill *cfg with the reference defaults (3D MHD, HLLD, MC, RK2, UCT_HLL,
+ * periodic unit cube, gamma 5/3, CFL 0.4, CFL_max_var 1.1). */
+void gpl_default_config(gpl_config *cfg);
+
+/* Allocate the device state; replaces Initialize()'s allocations
+ * (Src/initialize.cpp:396-483) and AdvanceStep's static scratch
+ * (Src/rk_step.cpp:153-219). */
+int gpl_create(const gpl_config *cfg, gpl_ctx **out);
+int gpl_destroy(gpl_ctx *ctx);
+
+/* Host -> device copy of the primitive state: vc = Data::Vc and
+ * vs[d] = Data::Vs[BX1s+d] (Src/structs.hpp:640-712).  Follows Startup()
+ * (Src/startup.cpp:388,514-561): the cell-centred field becomes the face
+ * average and Uc = PrimToCons(Vc).  vc holds NVAR = 8 variables in the
+ * reference order rho, vx1, vx2, vx3, Bx1, Bx2, Bx3, prs. */
+int gpl_upload(gpl_ctx *ctx, const double *vc, const double *const *vs, int layout);
+
+/* Device -> host copy of Vc, Vs and (if uc != NULL) the conserved Uc. */
+int gpl_download(gpl_ctx *ctx, double *vc, double *const *vs, double *uc, int layout);
+
+/* AdvanceStep(data, grid, Dts) with Dts->dt = dt (Src/rk_step.cpp:88):
+ * one full RK step on the device.  Returns the OR of the ERR_* bits raised;
+ * *inv_dt_hyp receives Dts->invDt_hyp of stage 1
+ * (Src/update_stage.cpp:422-425). */
+uint32_t gpl_advance_step(gpl_ctx *ctx, double dt, double *inv_dt_hyp);
+
+/* NextTimeStep (Src/next_time_step.cpp:20-130) on one rank:
+ * min(cfl/inv_dt_hyp, cfl_max_var*dt). */
+double gpl_next_time_step(const gpl_ctx *ctx, double dt, double inv_dt_hyp);
+
+/* The main-loop recurrence of Src/main.cpp:446-463 for nsteps steps with the
+ * time step kept on the device (no host synchronisation between steps; the
+ * step is replayed from a HIP graph).  Starts from dt = first_dt at time 0
+ * unless first_dt <= 0, in which case it continues from the device's
+ * current dt and time.  Returns the OR of the ERR_* bits. */
+uint32_t gpl_run(gpl_ctx *ctx, int nsteps, double first_dt, double *time_out, double *dt_out);
+
+/* Kernel timing with HIP events on the library's stream: enable, then query
+ * the average duration of each kernel family over the recorded launches. */
+int gpl_set_timing(gpl_ctx *ctx, int enable);
+int gpl_kernel_count(const gpl_ctx *ctx);
+int gpl_kernel_stats(gpl_ctx *ctx, int idx, const char **name, long long *launches,
+                     double *total_ms);
+/* GPU time of the last gpl_run call, from events around the whole loop. */
+double gpl_last_run_ms(const gpl_ctx *ctx);
+
+/* Bytes of device memory owned by the context. */
+size_t gpl_device_bytes(const gpl_ctx *ctx);
+
+const char *gpl_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPLUTO_AMD_H */
