@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the 3D ideal-MHD Godunov step on one MI355X.
+
+Workload (the headline configuration): 256^3 periodic cube, FP64, gamma 5/3,
+PLM (MC limiter) + HLLD + RK2 + upwind CT (UCT_HLL) on staggered faces,
+CFL 0.4, seeded random-field IC (gpluto_dev_amd.ic.mhd_turbulence).  One
+"step" = one full RK2 time step (AdvanceStep) of the whole box; a zone-update
+is one cell advanced by one step.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--n 256]
+
+Prints ONE JSON line.  value = FP64 zone-updates/s with the state resident in
+HBM (host wall clock around exactly --steps steps, the library stream
+synchronised on both sides).  The mixed-precision form runs the same steps
+from the same IC and is reported beside it with its L1 error against the FP64
+run.  roofline: the dominant kernel's algorithmic bytes per launch over its
+average launch duration (HIP events on the library's stream during the timed
+region).  cpu_baseline: the reference's own serial CPU build (oracle/_ref) on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s, 3D MHD HLLD+CT 256³: FP64 vs mixed at stated L1; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+# Algorithmic bytes per launch unit (one cell / one face), per kernel family:
+# each kernel's compulsory HBM traffic if every input element were read once
+# and every output written once (w = bytes of the state type R).
+#   k_sweep_*   : V 8w + normal face B 8  -> F 5w + CM w + FD 5w          = 19w + 8
+#   k_rhs_update: F 15w + CM 3w + U 5*8 + U0 5*8 (read) -> U 5*8 (+U0 5*8 stage 1)
+#                 counted as the stage-2 form: 18w + 120
+#   k_emf       : FD 15w + Vs 3*8 -> E 3w                                 = 18w + 24
+#   k_finish    : E 3w + Vin 3*8 + Bs0 3*8 + U 5*8 -> Vout 3*8 + V 8w      = 11w + 112
+#   k_dt        : negligible
+def alg_bytes_per_cell(kernel: str, w: int) -> float:
+    if kernel.startswith("k_sweep"):
+        return 19 * w + 8
+    return {"k_rhs_update": 18 * w + 120, "k_emf": 18 * w + 24,
+            "k_finish": 11 * w + 112}.get(kernel, 0.0)
+
+
+# B_alg of the whole RK2 step per zone-update (4 NV w_V + 5 NF w_B + 5 NU w_U):
+B_ALG = {"fp64": 696.0, "mixed": 568.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(n, steps, warmup, precision, vc, vs, first_dt):
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    g = StageUpdate(Definitions(), RuntimeSetup(n=(n, n, n), first_dt=first_dt), precision=precision)
+    g.upload(vc, vs)
+    err, t, dt = g.run(warmup, first_dt)
+    if err:
+        raise RuntimeError(f"warmup error bits {err:#x}")
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    err, t1, dt1 = g.run(steps, -1.0)       # synchronous: returns after the stream drained
+    wall = time.perf_counter() - t0
+    if err:
+        raise RuntimeError(f"error bits {err:#x}")
+    stats = g.kernel_stats()
+    vcf, vsf, _ = g.download()
+    g.close()
+    return wall, stats, vcf, vsf, t1
+
+
+def dominant(stats, w, ncell):
+    """Kernel with the largest total time; its bandwidth from algorithmic bytes."""
+    best = max(stats.items(), key=lambda kv: kv[1][1])
+    name, (cnt, tot) = best
+    avg_ms = tot / cnt
+    abytes = alg_bytes_per_cell(name, w) * ncell
+    return name, avg_ms, abytes
+
+
+def traffic_from_profile(name: str, w: int):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic_mhd3d_256.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("fp64" if w == 8 else "mixed", {}).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seed: int):
+    """Reference serial CPU build on a bounded sample (64^3, 20 steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from gpluto_dev_amd import ic
+    m, steps = 64, 20
+    vc, vs = ic.mhd_turbulence((m, m, m), seed)
+    binary = os.path.join(ROOT, "oracle", "_ref", "mhd3d_RK2_MC_LIM_UCT_HLL", "pluto")
+    if os.path.exists(binary):
+        import ref_run
+        r = ref_run.run_reference(binary, vc, vs, gamma=5 / 3, steps=steps, cfl=0.4,
+                                  first_dt=5e-4 * 256 / m, solver="hlld")
+        mt = re.search(r"Elapsed time \(compute\)\s+([0-9.eE+-]+)", r.log)
+        secs = float(mt.group(1)) if mt else r.wall_s
+        return {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{m}^3 box of the same seeded IC generator, {steps} RK2 steps, "
+                          f"reference serial build (g++ -O3, oracle/_ref), compute time "
+                          f"{secs:.2f} s from its own 'Elapsed time (compute)' line"}
+    import oracle
+    o = oracle.Oracle(phys="mhd", n=(m, m, m), gamma=5 / 3)
+    o.set_state(vc, vs)
+    t0 = time.perf_counter()
+    o.run(steps, 5e-4 * 256 / m)
+    secs = time.perf_counter() - t0
+    return {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{m}^3, {steps} RK2 steps, C oracle (serial)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no halo exchange between GPUs); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from gpluto_dev_amd import ic
+    n = a.n
+    ncell = n ** 3
+    first_dt = 5e-4 * 256 / n
+    log(f"bench: generating {n}^3 IC (seed {a.seed})")
+    vc, vs = ic.mhd_turbulence((n, n, n), a.seed)
+
+    log("bench: FP64 run")
+    wall, stats, vc64, vs64, t64 = run_gpu(n, a.steps, a.warmup, "fp64", vc, vs, first_dt)
+    value = ncell * a.steps / wall
+    name, avg_ms, abytes = dominant(stats, 8, ncell)
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(name, 8)
+    step_gbs = value * B_ALG["fp64"] / 1e9
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D ideal MHD {n}^3 periodic, PLM(MC)+HLLD+UCT_HLL CT+RK2, "
+                               f"gamma 5/3, CFL 0.4, seeded random-field IC (beta 10)",
+                   "n": n, "zones": ncell, "precision": "fp64", "seed": a.seed},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": abytes, "avg_launch_ms": avg_ms},
+        "step_roofline": {"b_alg_per_zone_update": B_ALG["fp64"], "achieved_gbs": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in stats.items()},
+    }
+    if not a.no_mixed:
+        log("bench: mixed run")
+        wm, sm, vcm, vsm, tm = run_gpu(n, a.steps, a.warmup, "mixed", vc, vs, first_dt)
+        vm = ncell * a.steps / wm
+        l1 = {}
+        names = ["rho", "vx1", "vx2", "vx3", "Bx1", "Bx2", "Bx3", "prs"]
+        for v in range(8):
+            l1[names[v]] = float(np.sum(np.abs(vcm[v] - vc64[v])) / np.sum(np.abs(vc64[v])))
+        for d in range(3):
+            l1[f"Bx{d+1}s"] = float(np.sum(np.abs(vsm[d] - vs64[d])) / np.sum(np.abs(vs64[d])))
+        out["mixed"] = {"value": vm, "speedup_vs_fp64": vm / value, "l1_rel_vs_fp64": l1,
+                        "max_l1_rel": max(l1.values()), "same_time": tm == t64,
+                        "divb_max": ic.divb_max(vsm),
+                        "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in sm.items()}}
+    out["divb_max_fp64"] = ic.divb_max(vs64)
+    if not a.no_cpu:
+        log("bench: CPU baseline")
+        try:
+            out["cpu_baseline"] = cpu_baseline(a.seed)
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,18 @@
+"""Load the committed reference fixtures (tests/golden/*.npz)."""
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def cases(prefix=""):
+    return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith(prefix))
+
+
+def load(name):
+    d = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    p = json.loads(str(d["params"]))
+    arrays = {k: d[k] for k in d.files if k != "params"}
+    return p, arrays

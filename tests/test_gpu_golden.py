@@ -1,0 +1,29 @@
+"""The HIP path against the reference CPU build's own outputs, bit for bit
+(tests/golden/*.npz written by tests/golden/make_golden.py)."""
+import numpy as np
+import pytest
+
+from golden_io import cases, load
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", cases("mhd3d"))
+def test_gpu_matches_reference_fixture(name):
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    p, a = load(name)
+    print(f"case {name}", flush=True)
+    n = tuple(p["n"])
+    g = StageUpdate(Definitions(time_stepping=p["time_stepping"], limiter=p["limiter"]),
+                    RuntimeSetup(n=n, solver=p["solver"], cfl=p["cfl"], first_dt=p["first_dt"],
+                                 gamma=p["gamma"]))
+    g.upload(a["vc_in"], a["vs_in"])
+    err, t, _ = g.run(p["steps"], p["first_dt"])
+    vc, vs, _ = g.download()
+    g.close()
+    assert err == 0
+    assert abs(t - p["time"]) <= 1e-6 * abs(p["time"])
+    for v in range(8):
+        assert np.array_equal(vc[v], a["vc_out"][v]), f"var {v}: max diff " \
+            f"{np.max(np.abs(vc[v] - a['vc_out'][v])):.3e}"
+    assert np.array_equal(vs, a["vs_out"])

@@ -85,3 +85,21 @@ def test_advance_step_seam_matches_oracle():
     ovc, ovs, _ = o.get_state()
     assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
     g.close()
+
+
+def test_mixed_precision_l1_within_target():
+    """Mixed form: FP32 V/PLM/HLLD, FP64 U, faces, CT and dt.  Relative L1 per
+    variable against the FP64 oracle stays <= 1e-5; div B stays at FP64
+    round-off because the face update is FP64."""
+    n, steps = (32, 32, 32), 10
+    (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(
+        n, 13, steps, 1e-3, precision="mixed")
+    assert gerr == oerr == 0
+    for v in range(8):
+        l1 = np.sum(np.abs(gvc[v] - ovc[v])) / np.sum(np.abs(ovc[v]))
+        print(f"var {v}: rel L1 {l1:.3e}")
+        assert l1 <= 1e-5, (v, l1)
+    for d in range(3):
+        l1 = np.sum(np.abs(gvs[d] - ovs[d])) / np.sum(np.abs(ovs[d]))
+        assert l1 <= 1e-5, (d, l1)
+    assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * n[0]

@@ -1,0 +1,170 @@
+"""Checks that pin the oracle independently of the reference fixtures.
+
+Each one fails on a plausible restatement mistake:
+  conservation      -- wrong flux sign/index in the RHS, a dropped direction
+  div B round-off   -- a wrong sign or index in the EMF average or CT update
+  uniform state     -- a limiter or solver that is not exact on constants
+  axis permutation  -- a direction-specific component mix-up (t/b cycling)
+  Sod shock tube    -- the HLLC solver against the exact Riemann solution
+                       (Toro, "Riemann Solvers and Numerical Methods", ch. 4)
+"""
+import numpy as np
+import pytest
+
+from gpluto_dev_amd import ic
+
+
+def run_mhd(vc, vs, steps, first_dt, **kw):
+    import oracle
+    n = (vc.shape[3], vc.shape[2], vc.shape[1])
+    o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, **kw)
+    o.set_state(vc, vs)
+    err, t, dt = o.run(steps, first_dt)
+    assert err == 0
+    return o.get_state(), t
+
+
+def totals(uc):
+    return np.array([np.sum(uc[v]) for v in (0, 1, 2, 3, 7)])
+
+
+@pytest.mark.parametrize("ts", ["RK2", "RK3"])
+def test_periodic_conservation(ts):
+    import oracle
+    n = (16, 12, 10)
+    vc, vs = ic.mhd_turbulence(n, 4)
+    o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, time_stepping=int(ts[-1]))
+    o.set_state(vc, vs)
+    _, _, uc0 = o.get_state()
+    o.run(6, 3e-3)
+    _, _, uc1 = o.get_state()
+    t0, t1 = totals(uc0), totals(uc1)
+    scale = np.array([np.sum(np.abs(uc0[v])) for v in (0, 1, 2, 3, 7)])
+    assert np.all(np.abs(t1 - t0) <= 1e-13 * scale), (t1 - t0) / scale
+
+
+def test_divb_stays_at_roundoff():
+    vc, vs = ic.mhd_turbulence((16, 16, 16), 2)
+    d0 = ic.divb_max(vs)
+    (vc1, vs1, _), _ = run_mhd(vc, vs, 8, 3e-3)
+    bmax = np.max(np.abs(vs1))
+    assert ic.divb_max(vs1) <= max(10 * d0, 1e-12 * bmax * 16)
+
+
+def test_uniform_state_is_exact():
+    n = (8, 8, 8)
+    vc = np.zeros((8, 8, 8, 8))
+    vc[0] = 1.3; vc[1] = 0.2; vc[2] = -0.1; vc[3] = 0.05; vc[7] = 0.7
+    vs = np.zeros((3, 8, 8, 8))
+    vs[0] = 0.3; vs[1] = -0.2; vs[2] = 0.1
+    vc[4], vc[5], vc[6] = vs[0], vs[1], vs[2]
+    (vc1, vs1, _), _ = run_mhd(vc, vs, 3, 1e-3)
+    for v in range(8):
+        assert np.all(vc1[v] == vc1[v].flat[0])
+        assert abs(vc1[v].flat[0] - vc[v].flat[0]) <= 1e-15 * max(1.0, abs(vc[v].flat[0]))
+    assert np.array_equal(vs1, vs)
+
+
+def _relabel(vc, vs):
+    """Cyclic relabelling: new x = old y, new y = old z, new z = old x.
+    Arrays are [k, j, i]; new[k'=i, j'=k, i'=j] = old[k, j, i]."""
+    r = lambda f: np.ascontiguousarray(f.transpose(2, 0, 1))
+    comp = {0: 0, 1: 2, 2: 3, 3: 1, 4: 5, 5: 6, 6: 4, 7: 7}   # new <- old
+    nvc = np.stack([r(vc[comp[v]]) for v in range(8)])
+    nvs = np.stack([r(vs[1]), r(vs[2]), r(vs[0])])
+    return nvc, nvs
+
+
+def test_axis_permutation_symmetry():
+    """A cyclic relabelling of the axes (a proper rotation) relabels the
+    result; the sweep and EMF-term order only change summation order, so the
+    two agree to round-off."""
+    vc, vs = ic.mhd_turbulence((10, 12, 14), 6)
+    (a_vc, a_vs, _), ta = run_mhd(vc, vs, 3, 3e-3)
+    pvc, pvs = _relabel(vc, vs)
+    (b_vc, b_vs, _), tb = run_mhd(pvc, pvs, 3, 3e-3)
+    e_vc, e_vs = _relabel(a_vc, a_vs)
+    assert abs(ta - tb) <= 1e-14 * ta
+    for v in range(8):
+        s = np.max(np.abs(e_vc[v])) + 1e-30
+        assert np.max(np.abs(b_vc[v] - e_vc[v])) <= 1e-12 * s, v
+    assert np.max(np.abs(b_vs - e_vs)) <= 1e-12 * np.max(np.abs(e_vs))
+
+
+def exact_riemann(rl, ul, pl, rr, ur, pr, gamma, xi):
+    """Exact Euler Riemann solution sampled at x/t = xi (Toro ch. 4)."""
+    cl, cr = np.sqrt(gamma * pl / rl), np.sqrt(gamma * pr / rr)
+    g1 = (gamma - 1) / (2 * gamma)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((gamma + 1) * rk), (gamma - 1) / (gamma + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * ck / (gamma - 1) * ((p / pk) ** g1 - 1)
+
+    lo, hi = 1e-12, 10 * max(pl, pr)
+    for _ in range(200):
+        pm = 0.5 * (lo + hi)
+        if f(pm, rl, pl, cl) + f(pm, rr, pr, cr) + (ur - ul) > 0:
+            hi = pm
+        else:
+            lo = pm
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (f(ps, rr, pr, cr) - f(ps, rl, pl, cl))
+    rho = np.empty_like(xi)
+    for q, s in enumerate(xi):
+        if s <= us:   # left of contact
+            if ps > pl:
+                sl = ul - cl * np.sqrt((gamma + 1) / (2 * gamma) * ps / pl + g1)
+                rho[q] = rl if s < sl else rl * ((ps / pl + (gamma - 1) / (gamma + 1)) /
+                                                 ((gamma - 1) / (gamma + 1) * ps / pl + 1))
+            else:
+                shl, csl = ul - cl, cl * (ps / pl) ** g1
+                stl = us - csl
+                if s < shl:
+                    rho[q] = rl
+                elif s > stl:
+                    rho[q] = rl * (ps / pl) ** (1 / gamma)
+                else:
+                    rho[q] = rl * (2 / (gamma + 1) + (gamma - 1) / ((gamma + 1) * cl) * (ul - s)) ** (2 / (gamma - 1))
+        else:
+            if ps > pr:
+                sr = ur + cr * np.sqrt((gamma + 1) / (2 * gamma) * ps / pr + g1)
+                rho[q] = rr if s > sr else rr * ((ps / pr + (gamma - 1) / (gamma + 1)) /
+                                                 ((gamma - 1) / (gamma + 1) * ps / pr + 1))
+            else:
+                shr, csr = ur + cr, cr * (ps / pr) ** g1
+                str_ = us + csr
+                if s > shr:
+                    rho[q] = rr
+                elif s < str_:
+                    rho[q] = rr * (ps / pr) ** (1 / gamma)
+                else:
+                    rho[q] = rr * (2 / (gamma + 1) - (gamma - 1) / ((gamma + 1) * cr) * (ur - s)) ** (2 / (gamma - 1))
+    return rho
+
+
+def sod_error(n):
+    import oracle
+    vc = np.zeros((5, 1, 1, n))
+    x = (np.arange(n) + 0.5) / n
+    vc[0, 0, 0] = np.where(x < 0.5, 1.0, 0.125)
+    vc[4, 0, 0] = np.where(x < 0.5, 1.0, 0.1)
+    o = oracle.Oracle(phys="hd", n=(n, 1, 1), gamma=1.4, solver="hllc", periodic=False, cfl=0.8)
+    o.set_state(vc)
+    dt, t, tend = 1e-4, 0.0, 0.1
+    while t < tend * (1 - 1e-12):
+        dt = min(dt, tend - t)
+        err, inv = o.advance(dt)
+        assert err == 0
+        t += dt
+        dt = min(0.8 / inv, 1.1 * dt)
+    vcf, _, _ = o.get_state()
+    exact = exact_riemann(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, (x - 0.5) / t)
+    return np.mean(np.abs(vcf[0, 0, 0] - exact))
+
+
+def test_sod_against_exact_riemann_solution():
+    e1, e2 = sod_error(200), sod_error(800)
+    assert e1 < 6e-3 and e2 < 2.5e-3
+    assert e2 < 0.6 * e1          # first-order convergence at discontinuities
