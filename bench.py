@@ -127,6 +127,25 @@ def cpu_baseline(seed: int):
             "sample": f"{m}^3, {steps} RK2 steps, C oracle (serial)"}
 
 
+def pmc_probe(n, seed, precision):
+    """One step for the byte counters, then a plain copy of known size."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    vc, vs = ic.mhd_turbulence((n, n, n), seed)
+    g = StageUpdate(Definitions(), RuntimeSetup(n=(n, n, n)), precision=precision)
+    g.upload(vc, vs)
+    err, _, _ = g.run(1, 5e-4 * 256 / n)
+    g.close()
+    import torch
+    x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": precision, "n": n, "err": err, "copy_bytes": 2 * (1 << 30)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +155,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
+                    help="run one step of the workload plus a 1 GiB device copy (the HBM byte "
+                         "counter calibration) for rocprofv3 --pmc, then exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no halo exchange between GPUs); "
@@ -144,6 +166,9 @@ def main():
     from gpluto_dev_amd import ic
     n = a.n
     ncell = n ** 3
+    if a.pmc_probe:
+        pmc_probe(n, a.seed, a.pmc_probe)
+        return
     first_dt = 5e-4 * 256 / n
     log(f"bench: generating {n}^3 IC (seed {a.seed})")
     vc, vs = ic.mhd_turbulence((n, n, n), a.seed)
