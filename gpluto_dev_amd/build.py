@@ -21,7 +21,13 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgpluto_amd.so")
 ARCH = os.environ.get("GPL_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mhd3d.hip", "gpl_api.cpp"]
+# (source, extra flags): the FP64 drop-in keeps IEEE division/sqrt and no
+# contraction; the mixed form's FP32 kernels may contract and use the fast
+# FP32 divide/sqrt (its accuracy is stated against the FP64 run).
+SOURCES = [("mhd3d_fp64.hip", []),
+           ("mhd3d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+           ("mhd3d_common.hip", []),
+           ("gpl_api.cpp", [])]
 
 
 def hipcc() -> str:
@@ -42,10 +48,10 @@ def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     cc = hipcc()
-    for s in SOURCES:
+    for s, extra in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s + ".o")
-        cmd = [cc, *flags(), "-c", src, "-o", obj]
+        cmd = [cc, *flags(), *extra, "-c", src, "-o", obj]
         if s.endswith(".cpp"):
             cmd.insert(1, "-xhip")
         if verbose:

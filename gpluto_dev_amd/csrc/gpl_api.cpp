@@ -22,6 +22,7 @@ using namespace gpl;
 namespace {
 
 const char *kKernelNames[K_COUNT] = {"k_sweep_x", "k_sweep_y", "k_sweep_z", "k_rhs_update",
+                                     "k_sweepx_fused", "k_sweepy_fused", "k_sweepz_fused",
                                      "k_emf", "k_finish", "k_dt"};
 
 struct Timing {
@@ -53,6 +54,9 @@ struct gpl_ctx {
   double *U = nullptr, *U0 = nullptr;
   double *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
   void *F = nullptr, *CM = nullptr, *FD = nullptr, *E = nullptr;
+  double *dU = nullptr, *Cdt = nullptr;
+  bool fused = true;          // fused sweep kernels (nx <= 512) or the split path
+  int march = 16;             // cells per thread in the y/z marching sweeps
   double *cdt_part = nullptr;
   int nparts = 0;
   uint32_t *err = nullptr;
@@ -110,7 +114,7 @@ extern "C" void gpl_default_config(gpl_config *cfg)
 static void free_all(gpl_ctx *c)
 {
   void *ptrs[] = {c->V, c->U, c->U0, c->VsA, c->VsB, c->Bs0, c->F, c->CM, c->FD, c->E,
-                  c->cdt_part, c->err, c->ctl};
+                  c->dU, c->Cdt, c->cdt_part, c->err, c->ctl};
   for (void *p : ptrs) if (p) hipFree(p);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   for (auto &pe : c->timing.pending) { hipEventDestroy(pe.second.first); hipEventDestroy(pe.second.second); }
@@ -155,7 +159,12 @@ extern "C" int gpl_create(const gpl_config *cfg, gpl_ctx **out)
     return GPL_EHIP;
   }
   const size_t n = c->n, rs = c->fp64 ? sizeof(double) : sizeof(float);
-  c->nparts = mhd3d_blocks(c->g);
+  // the fused x sweep holds one periodic x-row per block
+  c->fused = cfg->nx <= 512 && getenv("GPL_SPLIT_KERNELS") == nullptr;
+  // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
+  if (const char *m = getenv("GPL_MARCH")) c->march = atoi(m);
+  if (c->march != 8 && c->march != 16 && c->march != 32) c->march = 16;
+  c->nparts = c->fused ? mhd3d_march_blocks(c->g, c->march) : mhd3d_blocks(c->g);
   int rc = 0;
   rc |= dev_alloc(c, &c->V, 8*n*rs);
   rc |= dev_alloc(c, (void **)&c->U, 5*n*sizeof(double));
@@ -163,8 +172,13 @@ extern "C" int gpl_create(const gpl_config *cfg, gpl_ctx **out)
   rc |= dev_alloc(c, (void **)&c->VsA, 3*n*sizeof(double));
   rc |= dev_alloc(c, (void **)&c->VsB, 3*n*sizeof(double));
   rc |= dev_alloc(c, (void **)&c->Bs0, 3*n*sizeof(double));
-  rc |= dev_alloc(c, &c->F, 15*n*rs);
-  rc |= dev_alloc(c, &c->CM, 3*n*rs);
+  if (c->fused) {
+    rc |= dev_alloc(c, (void **)&c->dU, 5*n*sizeof(double));
+    rc |= dev_alloc(c, (void **)&c->Cdt, n*sizeof(double));
+  } else {
+    rc |= dev_alloc(c, &c->F, 15*n*rs);
+    rc |= dev_alloc(c, &c->CM, 3*n*rs);
+  }
   rc |= dev_alloc(c, &c->FD, 15*n*rs);
   rc |= dev_alloc(c, &c->E, 3*n*rs);
   rc |= dev_alloc(c, (void **)&c->cdt_part, (size_t)c->nparts*sizeof(double));
@@ -377,12 +391,18 @@ static void enqueue_stage(gpl_ctx *c, double *vin, double *vout, RKW w, int firs
   Mhd3dBufs<R> b;
   b.V = (R *)c->V; b.U = c->U; b.U0 = c->U0; b.Vin = vin; b.Vout = vout; b.Bs0 = c->Bs0;
   b.F = (R *)c->F; b.CM = (R *)c->CM; b.FD = (R *)c->FD; b.E = (R *)c->E;
+  b.dU = c->dU; b.Cdt = c->Cdt;
   b.cdt_part = c->cdt_part; b.err = c->err;
-  for (int k = K_SWEEP_X; k <= K_FINISH; k++) {
+  static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
+  static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
+  const int *seq = c->fused ? fused_seq : split_seq;
+  const int nk = c->fused ? 5 : 6;
+  for (int q = 0; q < nk; q++) {
+    const int k = seq[q];
     hipEvent_t ev = nullptr;
     if (timed) record_launch(c, k, true, &ev);
     mhd3d_launch<R>(k, b, c->g, c->ctl, w, first_stage, want_dt, c->cfg.limiter, c->cfg.solver,
-                    c->cfg.gamma, c->cfg.small_rho, c->cfg.small_prs, c->stream);
+                    c->march, c->cfg.gamma, c->cfg.small_rho, c->cfg.small_prs, c->stream);
     if (timed) record_launch(c, k, false, &ev);
   }
 }

@@ -34,18 +34,22 @@ struct Mhd3dBufs {
   R *CM;           // [3][n]
   R *FD;           // [3][5][n]
   R *E;            // [3][n]
+  double *dU;      // [5][n] flux-divergence accumulator (fused path)
+  double *Cdt;     // [n]    C_dt accumulator (fused path)
   double *cdt_part;
   uint32_t *err;
 };
 
-enum KernelId { K_SWEEP_X = 0, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH, K_DT, K_COUNT };
+enum KernelId { K_SWEEP_X = 0, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F,
+                K_EMF, K_FINISH, K_DT, K_COUNT };
 
 template <typename R>
 void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
-                  int first_stage, int want_dt, int limiter, int solver, double gamma,
+                  int first_stage, int want_dt, int limiter, int solver, int march, double gamma,
                   double small_rho, double small_prs, hipStream_t s);
 
 int mhd3d_blocks(const Box3 &g);
+int mhd3d_march_blocks(const Box3 &g, int march);
 
 void launch_dt(const double *part, int nparts, double dims, StepCtl *ctl, double cfl,
                double cfl_max_var, int advance, hipStream_t s);

@@ -14,6 +14,18 @@
 //   E [3][ncell]   edge EMFs Ex(i,j-1/2,k-1/2), Ey(i-1/2,j,k-1/2),
 //                  Ez(i-1/2,j-1/2,k)                            (type R)
 // Cell index c = i + nx*(j + ny*k) (reference Ary4D order, i fastest).
+//
+// Two kernel paths compute the same numbers:
+//  * fused (nx <= 512): one kernel per sweep direction reconstructs, solves
+//    the Riemann problem and accumulates the flux divergence in place, with no
+//    interface flux written to HBM -- x: one block per periodic x-row, the row
+//    staged in LDS; y, z: each thread marches a register window along the
+//    column; the z kernel also applies the RK combination of U;
+//  * split: one thread per interface, fluxes through HBM, a separate
+//    divergence/RK kernel (any box).
+// This header holds the templates; mhd3d_fp64.hip and mhd3d_mixed.hip
+// instantiate them with their own compiler flags.
+#pragma once
 #include "physics.hpp"
 #include "mhd3d.hpp"
 
@@ -172,6 +184,206 @@ void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused x sweep: one block per periodic x-row (blockDim.x == nx <= 512).
+// DataToSweep/Reconstruct (plm_states.cpp:99-189) with every cell's slope
+// computed once and shared through LDS, HLLD (hlld.cpp:100-500), CT store,
+// and RightHandSide's x part (rhs.cpp:384-386, 523): dUc = rhs_x
+// (update_stage.cpp:173-184), C_dt = 0.5 (cmax_i + cmax_i+1)/dx.
+// ---------------------------------------------------------------------------
+template <typename R, int LIM, int SOLVER>
+__global__ __launch_bounds__(512)
+void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
+                    R *__restrict__ FD, double *__restrict__ dU, double *__restrict__ Cdt,
+                    const StepCtl *__restrict__ ctl, R gamma, int want_cdt,
+                    uint32_t *__restrict__ err)
+{
+  extern __shared__ __align__(16) unsigned char gpl_smem[];
+  const int nx = g.nx, i = threadIdx.x;
+  R *sV = (R *)gpl_smem;           // [7][nx] primitives of the row
+  R *sD = sV + 7*nx;               // [7][nx] limited slopes
+  const size_t n = (size_t)g.ncell;
+  const int c = i + nx*(int)blockIdx.x;
+  const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+  constexpr int VAR[7] = {0, 1, 2, 3, 5, 6, 7};   // B1 comes from the face
+  R v0[7], dv[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) { v0[q] = V[VAR[q]*n + c]; sV[q*nx + i] = v0[q]; }
+  const R bn = (R)Vs[c];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    dv[q] = plm_slope<R, LIM>(sV[q*nx + im], v0[q], sV[q*nx + ip]);
+    sD[q*nx + i] = dv[q];
+  }
+  __syncthreads();
+  R vl[8], vr[8];
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    vl[VAR[q]] = sV[q*nx + im] + R(0.5)*sD[q*nx + im];
+    vr[VAR[q]] = v0[q] - R(0.5)*dv[q];
+  }
+  vl[4] = vr[4] = bn;
+  FaceFlux<R> ff;
+  if (SOLVER == 3) mhd_hlld<R, 0>(vl, vr, gamma, ff);
+  else             mhd_hll<R, 0>(vl, vr, gamma, ff);
+  {
+    R et, eb, aL, aR, dd;
+    uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+    FD[0*n + c] = et; FD[1*n + c] = eb; FD[2*n + c] = aL; FD[3*n + c] = aR; FD[4*n + c] = dd;
+  }
+  __syncthreads();                 // everyone is done with sV/sD
+  R *sF = sV;                      // [6][nx]: hydro flux and cmax of face i
+  sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
+  sF[3*nx + i] = ff.f[3]; sF[4*nx + i] = ff.f[7];
+  sF[5*nx + i] = rmax(rabs(ff.Sl), rabs(ff.Sr));
+  __syncthreads();
+  const double dtdx = ctl->dt/g.dx;
+#pragma unroll
+  for (int nv = 0; nv < 5; nv++) {
+    double r = -dtdx*((double)sF[nv*nx + ip] - (double)sF[nv*nx + i]);
+    r += 0.0;
+    dU[nv*n + c] = r;
+  }
+  if (want_cdt) Cdt[c] = 0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx;
+  if (ff.err) atomicOr(err, ff.err);
+}
+
+// ---------------------------------------------------------------------------
+// Fused y / z sweep: each thread marches a segment of L cells along D with a
+// register window (V(f-1), V(f), left state of face f), computes faces
+// s0..s0+L (the last one is the first face of the next segment) and
+// accumulates the divergence of cell f-1 as soon as face f is known.  The
+// accumulator (and for z the conserved state) of cell f-1 is loaded at the
+// top of the step, before the Riemann solve, so its latency hides behind it.
+// y: dUc += rhs_y, C_dt += ... (update_stage.cpp:272-284, rhs.cpp:671,756);
+// z: dUc += rhs_z (:369-380, rhs.cpp:900,981), then the RK combination of U
+// (rk_step.cpp:366-374,509-518,670-708) and the per-block max of C_dt
+// (getMax, update_stage.cpp:446-458).
+// ---------------------------------------------------------------------------
+template <typename R, int D, int LIM, int SOLVER, int L>
+__global__ __launch_bounds__(256)
+void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
+                   R *__restrict__ FD, double *__restrict__ dU, double *__restrict__ Cdt,
+                   double *__restrict__ U, double *__restrict__ U0,
+                   const StepCtl *__restrict__ ctl, RKW w, int first_stage, int want_cdt,
+                   double *__restrict__ cdt_part, R gamma, uint32_t *__restrict__ err)
+{
+  static_assert(D == 1 || D == 2, "march along y or z");
+  constexpr int BXn = 4 + D;
+  constexpr int VAR[7] = {0, 1, 2, 3, D == 1 ? 4 : 4, D == 1 ? 6 : 5, 7};
+  const int t = blockIdx.x*blockDim.x + threadIdx.x;
+  const int nd = D == 1 ? g.ny : g.nz;          // cells along the sweep
+  const int na = D == 1 ? g.nz : g.ny;          // the other transverse extent
+  const int nseg = (nd + L - 1)/L;
+  double cdt_max = -1.0;
+  uint32_t e = 0;
+  if (t < g.nx*na*nseg) {
+    const size_t n = (size_t)g.ncell;
+    const int i = t % g.nx;
+    const int r = t / g.nx;
+    const int a = r % na, seg = r / na;
+    const int sd = D == 1 ? g.nx : g.nx*g.ny;     // stride along D
+    const int base = D == 1 ? i + g.nx*g.ny*a : i + g.nx*a;
+    const int s0 = seg*L, s1 = s0 + L < nd ? s0 + L : nd;
+    const double dtdx = ctl->dt/(D == 1 ? g.dy : g.dz);
+    const double ddx = D == 1 ? g.dy : g.dz;
+    R Vm[7], Vc[7], Vp[7], vlc[7];
+    {
+      const int cm2 = base + wrapm(s0 - 2, nd)*sd, cm1 = base + wrapm(s0 - 1, nd)*sd;
+      const int c0 = base + wrapm(s0, nd)*sd;
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        Vm[q] = V[VAR[q]*n + cm2]; Vc[q] = V[VAR[q]*n + cm1]; Vp[q] = V[VAR[q]*n + c0];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        vlc[q] = Vc[q] + R(0.5)*plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
+        Vm[q] = Vc[q]; Vc[q] = Vp[q];
+      }
+    }
+    double Fp[5] = {0, 0, 0, 0, 0}, cmp = 0.0;
+    for (int f = s0; f <= s1; f++) {
+      const int cf = base + wrapm(f, nd)*sd;
+      const int cn = base + wrapm(f + 1, nd)*sd;
+      const int cc = base + wrapm(f - 1, nd)*sd;    // the cell completed by face f
+      // early loads for cell f-1: accumulator, C_dt, and for z the state
+      double acc[5], u_old[5], u0_old[5], cacc = 0.0;
+      if (f > s0) {
+#pragma unroll
+        for (int nv = 0; nv < 5; nv++) acc[nv] = dU[nv*n + cc];
+        if (want_cdt) cacc = Cdt[cc];
+        if (D == 2) {
+#pragma unroll
+          for (int nv = 0; nv < 5; nv++) u_old[nv] = U[nv*n + cc];
+          if (!first_stage) {
+#pragma unroll
+            for (int nv = 0; nv < 5; nv++) u0_old[nv] = U0[nv*n + cc];
+          }
+        }
+      }
+      R vl[8], vr[8];
+#pragma unroll
+      for (int q = 0; q < 7; q++) Vp[q] = V[VAR[q]*n + cn];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
+        vl[VAR[q]] = vlc[q];
+        vr[VAR[q]] = Vc[q] - R(0.5)*dv;
+        vlc[q] = Vc[q] + R(0.5)*dv;
+      }
+      vl[BXn] = vr[BXn] = (R)Vs[D*n + cf];
+      FaceFlux<R> ff;
+      if (SOLVER == 3) mhd_hlld<R, D>(vl, vr, gamma, ff);
+      else             mhd_hll<R, D>(vl, vr, gamma, ff);
+      e |= ff.err;
+      if (f < s1) {
+        R et, eb, aL, aR, dd;
+        uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+        R *fd = FD + (size_t)D*5*n;
+        fd[0*n + cf] = et; fd[1*n + cf] = eb; fd[2*n + cf] = aL; fd[3*n + cf] = aR; fd[4*n + cf] = dd;
+      }
+      const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
+                           (double)ff.f[7]};
+      const double cm = (double)rmax(rabs(ff.Sl), rabs(ff.Sr));
+      if (f > s0) {
+#pragma unroll
+        for (int nv = 0; nv < 5; nv++) {
+          double rr = -dtdx*(F[nv] - Fp[nv]);
+          rr += 0.0;
+          double du = acc[nv];
+          du += rr;
+          const size_t o = nv*n + cc;
+          if (D == 1) {
+            dU[o] = du;
+          } else if (first_stage) {
+            U0[o] = u_old[nv];
+            U[o] = u_old[nv] + w.wR*du;
+          } else {
+            U[o] = w.w0*u0_old[nv] + w.w1*u_old[nv] + w.wR*du;
+          }
+        }
+        if (want_cdt) {
+          double cd = cacc;
+          cd += 0.5*(cmp + cm)/ddx;
+          if (D == 1) Cdt[cc] = cd;
+          else cdt_max = cdt_max >= cd ? cdt_max : cd;
+        }
+      }
+#pragma unroll
+      for (int nv = 0; nv < 5; nv++) Fp[nv] = F[nv];
+      cmp = cm;
+#pragma unroll
+      for (int q = 0; q < 7; q++) { Vm[q] = Vc[q]; Vc[q] = Vp[q]; }
+    }
+  }
+  if (e) atomicOr(err, e);
+  if (D == 2 && want_cdt) {
+    const double m = block_max(cdt_max);
+    if (threadIdx.x == 0) cdt_part[blockIdx.x] = m;
+  }
+}
+
 // PLM of a scalar column for the EMF average (array_reconstruct.cpp:167-205):
 // left state at the interface between q1 and q2 from (q0,q1,q2), right state
 // from (q1,q2,q3).
@@ -276,8 +488,8 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err)
 {
   const int c = blockIdx.x*blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
   const size_t n = (size_t)g.ncell;
+  if (c >= g.ncell) return;
   const Cell p = cell_of(c, g);
   const double dt = ctl->dt;
   const int ci = shift<0>(p, 1, g), cj = shift<1>(p, 1, g), ck = shift<2>(p, 1, g);
@@ -320,35 +532,10 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
 }
 
 // ---------------------------------------------------------------------------
-// invDt_hyp = max(C_dt)/DIMENSIONS (update_stage.cpp:422-425); optionally the
-// NextTimeStep recurrence (next_time_step.cpp:80,123-125) and time += dt
-// (main.cpp:458).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024)
-void k_dt(const double *__restrict__ part, int nparts, double dims, StepCtl *ctl,
-          double cfl, double cfl_max_var, int advance)
-{
-  double m = -1.0;
-  for (int q = threadIdx.x; q < nparts; q += blockDim.x) m = m >= part[q] ? m : part[q];
-  m = block_max(m);
-  if (threadIdx.x == 0) {
-    const double inv = m/dims;
-    ctl->inv_dt = inv;
-    if (advance) {
-      const double dt = ctl->dt;
-      ctl->time += dt;
-      const double dt_new = cfl/inv;
-      ctl->dt = dt_new <= cfl_max_var*dt ? dt_new : cfl_max_var*dt;
-      ctl->nstep += 1;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Host-side launch helpers: one entry per kernel family, so that the host
 // orchestration can bracket each launch with timing events.
 // ---------------------------------------------------------------------------
-template <typename R, int LIM, int SOLVER>
+template <typename R, int LIM, int SOLVER, int L>
 static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
                      int first_stage, int want_dt, double gamma, double small_rho,
                      double small_prs, hipStream_t s)
@@ -373,6 +560,26 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
                        g, b.F, b.CM, b.U, b.U0, ctl, w, first_stage,
                        want_dt ? b.cdt_part : nullptr);
     break;
+  case K_SWEEPX_F: {
+    const size_t shm = 14*(size_t)g.nx*sizeof(R);
+    hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER>), dim3(g.ny*g.nz), dim3(g.nx), shm, s,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
+    break;
+  }
+  case K_SWEEPY_F: {
+    const int nt = g.nx*g.nz*((g.ny + L - 1)/L);
+    hipLaunchKernelGGL((k_sweep_march<R, 1, LIM, SOLVER, L>), dim3((nt + T - 1)/T), dim3(T), 0, s,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
+                       b.cdt_part, (R)gamma, b.err);
+    break;
+  }
+  case K_SWEEPZ_F: {
+    const int nt = g.nx*g.ny*((g.nz + L - 1)/L);
+    hipLaunchKernelGGL((k_sweep_march<R, 2, LIM, SOLVER, L>), dim3((nt + T - 1)/T), dim3(T), 0, s,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
+                       b.cdt_part, (R)gamma, b.err);
+    break;
+  }
   case K_EMF:
     hipLaunchKernelGGL((k_emf<R, LIM>), dim3(nb), dim3(T), 0, s, g, b.Vin, b.FD, b.E);
     break;
@@ -386,31 +593,28 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
   }
 }
 
+template <typename R, int LIM, int SOLVER>
+static void launch_l(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
+                     int first_stage, int want_dt, int march, double gamma, double small_rho,
+                     double small_prs, hipStream_t s)
+{
+  if (march == 32) launch_t<R, LIM, SOLVER, 32>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  else if (march == 8) launch_t<R, LIM, SOLVER, 8>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  else launch_t<R, LIM, SOLVER, 16>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+}
+
 template <typename R>
 void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
-                  int first_stage, int want_dt, int limiter, int solver, double gamma,
+                  int first_stage, int want_dt, int limiter, int solver, int march, double gamma,
                   double small_rho, double small_prs, hipStream_t s)
 {
   if (limiter == MC_LIM) {
-    if (solver == 3) launch_t<R, MC_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
-    else             launch_t<R, MC_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+    if (solver == 3) launch_l<R, MC_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else             launch_l<R, MC_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   } else {
-    if (solver == 3) launch_t<R, VANLEER_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
-    else             launch_t<R, VANLEER_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+    if (solver == 3) launch_l<R, VANLEER_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else             launch_l<R, VANLEER_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   }
 }
-
-int mhd3d_blocks(const Box3 &g) { return (g.ncell + 255)/256; }
-
-void launch_dt(const double *part, int nparts, double dims, StepCtl *ctl, double cfl,
-               double cfl_max_var, int advance, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, s, part, nparts, dims, ctl, cfl, cfl_max_var, advance);
-}
-
-template void mhd3d_launch<double>(int, const Mhd3dBufs<double> &, const Box3 &, const StepCtl *, RKW,
-                                   int, int, int, int, double, double, double, hipStream_t);
-template void mhd3d_launch<float>(int, const Mhd3dBufs<float> &, const Box3 &, const StepCtl *, RKW,
-                                  int, int, int, int, double, double, double, hipStream_t);
 
 }  // namespace gpl

@@ -103,3 +103,19 @@ def test_mixed_precision_l1_within_target():
         l1 = np.sum(np.abs(gvs[d] - ovs[d])) / np.sum(np.abs(ovs[d]))
         assert l1 <= 1e-5, (d, l1)
     assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * n[0]
+
+
+@pytest.mark.parametrize("path", ["split", "march8", "march16", "march32"])
+def test_kernel_paths_bitwise(path, monkeypatch):
+    """Split (per-interface) and fused kernels (LDS x-row + register-window y/z
+    march, every march length), including segments that do not divide the
+    column (ny = 12, nz = 20, 17, 31)."""
+    if path == "split":
+        monkeypatch.setenv("GPL_SPLIT_KERNELS", "1")
+    elif path.startswith("march"):
+        monkeypatch.setenv("GPL_MARCH", path[5:])
+    for n in ((16, 12, 20), (20, 17, 31)):
+        (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(n, 21, 5, 2e-3)
+        assert gerr == oerr == 0
+        assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
+        assert gt == ot and gdt == odt
