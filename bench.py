@@ -36,20 +36,31 @@ METRIC = "zone-updates/s, 3D MHD HLLD+CT 256³: FP64 vs mixed at stated L1; % HB
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
-# Algorithmic bytes per launch unit (one cell / one face), per kernel family:
-# each kernel's compulsory HBM traffic if every input element were read once
-# and every output written once (w = bytes of the state type R).
-#   k_sweep_*   : V 8w + normal face B 8  -> F 5w + CM w + FD 5w          = 19w + 8
-#   k_rhs_update: F 15w + CM 3w + U 5*8 + U0 5*8 (read) -> U 5*8 (+U0 5*8 stage 1)
-#                 counted as the stage-2 form: 18w + 120
-#   k_emf       : FD 15w + Vs 3*8 -> E 3w                                 = 18w + 24
-#   k_finish    : E 3w + Vin 3*8 + Bs0 3*8 + U 5*8 -> Vout 3*8 + V 8w      = 11w + 112
-#   k_dt        : negligible
+# Algorithmic bytes per cell of one launch: each kernel's compulsory HBM
+# traffic if every element it touches were read once and written once
+# (w = bytes of the state type R; the conserved state, faces, dU and C_dt are
+# FP64).  A launch covers the whole box; the figure is the mean over the two
+# stages of an RK2 step, because the kernels' average launch time is too.
+#   k_sweepx_fused : V 7w + face 8  -> FD 5w + dU 40 (+ C_dt 8, stage 1)
+#                    = 12w + 56 | 12w + 48                         -> 12w + 52
+#   k_sweepy_fused : V 7w + face 8 + dU 40 (+ C_dt 8) -> FD 5w + dU 40 (+ C_dt 8)
+#                    = 12w + 104 | 12w + 88                        -> 12w + 96
+#   k_sweepz_fused : V 7w + face 8 + dU 40 + U 40 (+ C_dt 8 | + U0 40)
+#                    -> FD 5w + U 40 (+ U0 40, stage 1)
+#                    = 12w + 176 | 12w + 208                       -> 12w + 192
+#   k_emf          : FD 15w + faces 24 -> E 3w                     =  18w + 24
+#   k_finish       : E 3w + faces 24 + U 40 (+ Bs0 24, stage 2) -> faces 24 + V 8w
+#                    (+ Bs0 24, stage 1)                           =  11w + 112
+#   k_sweep_* (split, rows > 512): V 8w + face 8 -> F 5w + CM w + FD 5w = 19w + 8
+#   k_rhs_update   : F 15w + CM 3w + U 40 + U0 40 -> U 40          =  18w + 120
+ALG_BYTES = {"k_sweepx_fused": (12, 52), "k_sweepy_fused": (12, 96),
+             "k_sweepz_fused": (12, 192), "k_emf": (18, 24), "k_finish": (11, 112),
+             "k_rhs_update": (18, 120)}
+
+
 def alg_bytes_per_cell(kernel: str, w: int) -> float:
-    if kernel.startswith("k_sweep"):
-        return 19 * w + 8
-    return {"k_rhs_update": 18 * w + 120, "k_emf": 18 * w + 24,
-            "k_finish": 11 * w + 112}.get(kernel, 0.0)
+    a, b = ALG_BYTES.get(kernel, (19, 8) if kernel.startswith("k_sweep") else (0, 0))
+    return a * w + b
 
 
 # B_alg of the whole RK2 step per zone-update (4 NV w_V + 5 NF w_B + 5 NU w_U):
@@ -136,14 +147,17 @@ def pmc_probe(n, seed, precision):
     g.upload(vc, vs)
     err, _, _ = g.run(1, 5e-4 * 256 / n)
     g.close()
-    import torch
-    x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
-    x.fill_(1.0)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"pmc_probe": precision, "n": n, "err": err, "copy_bytes": 2 * (1 << 30)}))
+    import ctypes
+    probe = ctypes.CDLL(os.path.join(ROOT, "gpluto_dev_amd", "libgpl_probe.so"))
+    probe.gpl_probe_copy.argtypes = [ctypes.c_size_t]
+    probe.gpl_probe_copy.restype = ctypes.c_int
+    copy_bytes = 1 << 30
+    rc = probe.gpl_probe_copy(copy_bytes)
+    if rc:
+        raise RuntimeError(f"copy probe failed: hipError {rc}")
+    print(json.dumps({"pmc_probe": precision, "n": n, "err": err,
+                      "copy_kernel": "k_copy_probe", "copy_read_bytes": copy_bytes,
+                      "copy_write_bytes": copy_bytes}), flush=True)
 
 
 def main():
@@ -156,8 +170,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
-                    help="run one step of the workload plus a 1 GiB device copy (the HBM byte "
-                         "counter calibration) for rocprofv3 --pmc, then exit")
+                    help="run one step of the workload plus a 1 GiB device copy (k_copy_probe, the "
+                         "HBM byte-counter calibration) for rocprofv3 --pmc, then exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no halo exchange between GPUs); "

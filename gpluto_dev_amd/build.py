@@ -19,6 +19,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgpluto_amd.so")
+# the byte-counter calibration copy used when profiling (not part of the ABI)
+PROBE_LIB = os.path.join(HERE, "libgpl_probe.so")
 ARCH = os.environ.get("GPL_OFFLOAD_ARCH", "gfx950")
 
 # (source, extra flags): the FP64 drop-in keeps IEEE division/sqrt and no
@@ -43,12 +45,22 @@ def flags() -> list[str]:
             "-Wno-unused-command-line-argument", "-I" + os.path.join(HERE, "..", "include")]
 
 
+def _link(cc: str, objs: list[str], out: str) -> None:
+    tmp = out + ".tmp"
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"link failed: {out}")
+    os.replace(tmp, out)
+
+
 def build(verbose: bool = False) -> str:
     objs = []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     cc = hipcc()
-    for s, extra in SOURCES:
+    for s, extra in SOURCES + [("copy_probe.hip", [])]:
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s + ".o")
         cmd = [cc, *flags(), *extra, "-c", src, "-o", obj]
@@ -61,13 +73,8 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {s}")
         objs.append(obj)
-    tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("link failed")
-    os.replace(tmp, LIB)
+    _link(cc, objs[:-1], LIB)
+    _link(cc, objs[-1:], PROBE_LIB)
     return LIB
 
 
