@@ -29,6 +29,9 @@ ARCH = os.environ.get("GPL_OFFLOAD_ARCH", "gfx950")
 SOURCES = [("mhd3d_fp64.hip", []),
            ("mhd3d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
            ("mhd3d_common.hip", []),
+           ("hd_kernels.hip", []),
+           ("engine_mhd3d.cpp", []),
+           ("engine_hd.cpp", []),
            ("gpl_api.cpp", [])]
 
 

@@ -1,0 +1,91 @@
+// Host side of the stage update: one Engine per physics module owns the
+// device state of a context and enqueues whole AdvanceStep calls on the
+// context's stream.  The C ABI (gpl_api.cpp) holds the engine, the stream,
+// the step-control block, graph replay and timing.
+#pragma once
+#include "../../include/gpluto_amd.h"
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+namespace gpl {
+
+// Per-launch timing hook (begin/end around each kernel launch).
+struct LaunchTimer {
+  virtual void begin(int kernel) = 0;
+  virtual void end(int kernel) = 0;
+  virtual ~LaunchTimer() {}
+};
+
+struct EngineEnv {
+  int dev;
+  hipStream_t stream;
+  StepCtl *ctl;       // device step-control block
+  uint32_t *err;      // device error bits
+};
+
+class Engine {
+public:
+  virtual ~Engine() {}
+  virtual int upload(const double *vc, const double *const *vs, int layout) = 0;
+  virtual int download(double *vc, double *const *vs, double *uc, int layout) = 0;
+  // Enqueue one AdvanceStep; advance_dt != 0 also applies NextTimeStep and
+  // time += dt on the device.  t is null when launches are not timed.
+  virtual void enqueue_step(int advance_dt, LaunchTimer *t) = 0;
+  // Steps after which the ping-pong buffers are back where they started
+  // (a captured graph replays that many steps).
+  virtual int graph_period() const = 0;
+  // Identifies the buffer holding the current state (graph validity).
+  virtual const void *state_key() const = 0;
+  size_t bytes = 0;   // device memory held
+};
+
+int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
+int make_hd_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
+
+// Host-array layouts accepted by gpl_upload / gpl_download: the reference's
+// Ary4D with ghost zones along the used directions (Src/set_grid.cpp:57-58:
+// no ghosts along a direction that is not included), or active zones only.
+struct Layout {
+  int nx, ny, nz;
+  int gx, gy, gz;          // ghost zones per side
+  int tx, ty, tz;          // totals
+  bool ary;
+  size_t cell(int nv, int i, int j, int k) const
+  {
+    if (!ary) return (size_t)nv*nx*ny*nz + ((size_t)k*ny + j)*nx + i;
+    return (size_t)nv*tx*ty*tz + ((size_t)(k + gz)*ty + (j + gy))*tx + (i + gx);
+  }
+  // staggered component d: one extra face along d (Src/initialize.cpp:424-431)
+  size_t face(int d, int i, int j, int k) const
+  {
+    if (!ary) return ((size_t)k*ny + j)*nx + i;
+    const int sx = tx + (d == 0), sy = ty + (d == 1);
+    return ((size_t)(k + gz)*sy + (j + gy))*sx + (i + gx);
+  }
+};
+
+inline Layout make_layout(const gpl_config &c, int layout)
+{
+  Layout L;
+  L.nx = c.nx; L.ny = c.ny; L.nz = c.nz;
+  L.gx = c.dimensions > 0 ? c.nghost : 0;
+  L.gy = c.dimensions > 1 ? c.nghost : 0;
+  L.gz = c.dimensions > 2 ? c.nghost : 0;
+  L.ary = layout == GPL_LAYOUT_ARY4D;
+  L.tx = L.nx + 2*L.gx; L.ty = L.ny + 2*L.gy; L.tz = L.nz + 2*L.gz;
+  return L;
+}
+
+// device allocation that adds to an engine's byte count
+inline int dev_alloc(size_t &counter, void **p, size_t bytes)
+{
+  if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return GPL_ENOMEM; }
+  counter += bytes;
+  return 0;
+}
+
+}  // namespace gpl

@@ -1,0 +1,258 @@
+// 3D ideal-MHD engine: device state and AdvanceStep orchestration
+// (Src/rk_step.cpp:88-720) over the kernels of mhd3d_kernels.hpp.
+#include "engine.hpp"
+#include "mhd3d.hpp"
+#include "physics.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace gpl {
+
+namespace {
+
+template <typename T>
+int to_device(void *dst, const std::vector<double> &src)
+{
+  if (sizeof(T) == sizeof(double))
+    return hipMemcpy(dst, src.data(), src.size()*sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+           ? 0 : GPL_EHIP;
+  std::vector<T> tmp(src.size());
+  for (size_t q = 0; q < src.size(); q++) tmp[q] = (T)src[q];
+  return hipMemcpy(dst, tmp.data(), tmp.size()*sizeof(T), hipMemcpyHostToDevice) == hipSuccess
+         ? 0 : GPL_EHIP;
+}
+
+int from_device_v(void *src, bool fp64, std::vector<double> &dst)
+{
+  if (fp64)
+    return hipMemcpy(dst.data(), src, dst.size()*sizeof(double), hipMemcpyDeviceToHost) == hipSuccess
+           ? 0 : GPL_EHIP;
+  std::vector<float> tmp(dst.size());
+  if (hipMemcpy(tmp.data(), src, tmp.size()*sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_EHIP;
+  for (size_t q = 0; q < tmp.size(); q++) dst[q] = tmp[q];
+  return 0;
+}
+
+class Mhd3dEngine : public Engine {
+public:
+  gpl_config cfg;
+  EngineEnv env;
+  Box3 g;
+  bool fp64 = true;
+  size_t n = 0;
+  void *V = nullptr;          // [8][n] R
+  double *U = nullptr, *U0 = nullptr;
+  double *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
+  void *F = nullptr, *CM = nullptr, *FD = nullptr, *E = nullptr;
+  double *dU = nullptr, *Cdt = nullptr;
+  bool fused = true;          // fused sweep kernels (nx <= 512) or the split path
+  int march = 16;             // cells per thread in the y/z marching sweeps
+  double *cdt_part = nullptr;
+  int nparts = 0;
+  double *cur_vs = nullptr;   // VsA or VsB: faces of the current state
+
+  ~Mhd3dEngine() override
+  {
+    void *ptrs[] = {V, U, U0, VsA, VsB, Bs0, F, CM, FD, E, dU, Cdt, cdt_part};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+  }
+
+  int init(const gpl_config &c, const EngineEnv &e)
+  {
+    cfg = c; env = e;
+    fp64 = c.precision == GPL_FP64;
+    n = (size_t)c.nx*c.ny*c.nz;
+    g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+    g.ncell = (int)n;
+    // dx = (xend - xbeg)/N for every cell (Src/set_grid.cpp:196)
+    g.dx = (c.xend[0] - c.xbeg[0])/c.nx;
+    g.dy = (c.xend[1] - c.xbeg[1])/c.ny;
+    g.dz = (c.xend[2] - c.xbeg[2])/c.nz;
+    const size_t rs = fp64 ? sizeof(double) : sizeof(float);
+    // the fused x sweep holds one periodic x-row per block
+    fused = c.nx <= 512 && getenv("GPL_SPLIT_KERNELS") == nullptr;
+    // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
+    if (const char *m = getenv("GPL_MARCH")) march = atoi(m);
+    if (march != 8 && march != 16 && march != 32) march = 16;
+    nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
+    int rc = 0;
+    rc |= dev_alloc(bytes, &V, 8*n*rs);
+    rc |= dev_alloc(bytes, (void **)&U, 5*n*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&U0, 5*n*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&VsA, 3*n*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&VsB, 3*n*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&Bs0, 3*n*sizeof(double));
+    if (fused) {
+      rc |= dev_alloc(bytes, (void **)&dU, 5*n*sizeof(double));
+      rc |= dev_alloc(bytes, (void **)&Cdt, n*sizeof(double));
+    } else {
+      rc |= dev_alloc(bytes, &F, 15*n*rs);
+      rc |= dev_alloc(bytes, &CM, 3*n*rs);
+    }
+    rc |= dev_alloc(bytes, &FD, 15*n*rs);
+    rc |= dev_alloc(bytes, &E, 3*n*rs);
+    rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
+    cur_vs = VsA;
+    return rc ? GPL_ENOMEM : 0;
+  }
+
+  int upload(const double *vc, const double *const *vs, int layout) override
+  {
+    if (!vs || !vs[0] || !vs[1] || !vs[2]) return GPL_EINVAL;
+    const Layout L = make_layout(cfg, layout);
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    std::vector<double> hv(8*n), hu(5*n), hs(3*n);
+    for (int d = 0; d < 3; d++)
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int i = 0; i < nx; i++)
+            hs[d*n + ((size_t)k*ny + j)*nx + i] = vs[d][L.face(d, i, j, k)];
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const size_t o = ((size_t)k*ny + j)*nx + i;
+          double v[8], u[8];
+          for (int nv = 0; nv < 8; nv++) v[nv] = vc[L.cell(nv, i, j, k)];
+          // cell-centred field = face average (Src/startup.cpp:388 -> ct_field_average.cpp:182-184)
+          const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1, kp = k + 1 == nz ? 0 : k + 1;
+          v[4] = 0.5*(hs[0*n + o] + hs[0*n + ((size_t)k*ny + j)*nx + ip]);
+          v[5] = 0.5*(hs[1*n + o] + hs[1*n + ((size_t)k*ny + jp)*nx + i]);
+          v[6] = 0.5*(hs[2*n + o] + hs[2*n + ((size_t)kp*ny + j)*nx + i]);
+          // Uc = PrimToCons(Vc) (Src/startup.cpp:514-561)
+          mhd_prim_to_cons<double>(v, u, cfg.gamma - 1.0);
+          for (int nv = 0; nv < 8; nv++) hv[nv*n + o] = v[nv];
+          hu[0*n + o] = u[0]; hu[1*n + o] = u[1]; hu[2*n + o] = u[2]; hu[3*n + o] = u[3];
+          hu[4*n + o] = u[7];
+        }
+    int rc = fp64 ? to_device<double>(V, hv) : to_device<float>(V, hv);
+    if (rc) return rc;
+    if (hipMemcpy(U, hu.data(), 5*n*sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(VsA, hs.data(), 3*n*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return GPL_EHIP;
+    cur_vs = VsA;
+    return 0;
+  }
+
+  int download(double *vc, double *const *vs, double *uc, int layout) override
+  {
+    const Layout L = make_layout(cfg, layout);
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    if (vc) {
+      std::vector<double> hv(8*n);
+      if (from_device_v(V, fp64, hv)) return GPL_EHIP;
+      for (int nv = 0; nv < 8; nv++)
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+              vc[L.cell(nv, i, j, k)] = hv[nv*n + ((size_t)k*ny + j)*nx + i];
+    }
+    if (vs) {
+      std::vector<double> hs(3*n);
+      if (hipMemcpy(hs.data(), cur_vs, 3*n*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPL_EHIP;
+      for (int d = 0; d < 3; d++) {
+        if (!vs[d]) continue;
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+              vs[d][L.face(d, i, j, k)] = hs[d*n + ((size_t)k*ny + j)*nx + i];
+        if (L.ary) {   // the duplicate upper face equals the lower one (PeriodicBoundary)
+          for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+              for (int i = 0; i < nx; i++) {
+                if (d == 0 && i == 0) vs[0][L.face(0, nx, j, k)] = hs[((size_t)k*ny + j)*nx];
+                if (d == 1 && j == 0) vs[1][L.face(1, i, ny, k)] = hs[n + ((size_t)k*ny)*nx + i];
+                if (d == 2 && k == 0) vs[2][L.face(2, i, j, nz)] = hs[2*n + (size_t)j*nx + i];
+              }
+        }
+      }
+    }
+    if (uc) {
+      std::vector<double> hu(5*n), hv(8*n);
+      if (hipMemcpy(hu.data(), U, 5*n*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return GPL_EHIP;
+      // the cell-centred field of Uc is the face average (= Vc's B1..3)
+      if (from_device_v(V, fp64, hv)) return GPL_EHIP;
+      const int map[8] = {0, 1, 2, 3, -1, -1, -1, 4};
+      for (int nv = 0; nv < 8; nv++)
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+              const size_t o = ((size_t)k*ny + j)*nx + i;
+              uc[L.cell(nv, i, j, k)] = map[nv] >= 0 ? hu[map[nv]*n + o] : hv[nv*n + o];
+            }
+    }
+    return 0;
+  }
+
+  template <typename R>
+  void enqueue_stage(double *vin, double *vout, RKW w, int first_stage, int want_dt, LaunchTimer *t)
+  {
+    Mhd3dBufs<R> b;
+    b.V = (R *)V; b.U = U; b.U0 = U0; b.Vin = vin; b.Vout = vout; b.Bs0 = Bs0;
+    b.F = (R *)F; b.CM = (R *)CM; b.FD = (R *)FD; b.E = (R *)E;
+    b.dU = dU; b.Cdt = Cdt;
+    b.cdt_part = cdt_part; b.err = env.err;
+    static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
+    static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
+    const int *seq = fused ? fused_seq : split_seq;
+    const int nk = fused ? 5 : 6;
+    for (int q = 0; q < nk; q++) {
+      const int k = seq[q];
+      if (t) t->begin(k);
+      mhd3d_launch<R>(k, b, g, env.ctl, w, first_stage, want_dt, cfg.limiter, cfg.solver,
+                      march, cfg.gamma, cfg.small_rho, cfg.small_prs, env.stream);
+      if (t) t->end(k);
+    }
+  }
+
+  void enqueue_step(int advance_dt, LaunchTimer *t) override
+  {
+    // RK-SSP weights (Src/rk_step.cpp:20-51)
+    const RKW w1 = {0.0, 0.0, 1.0};
+    const RKW w2 = cfg.time_stepping == GPL_RK3 ? RKW{0.75, 0.25, 0.25} : RKW{0.5, 0.5, 0.5};
+    const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
+    const int nst = cfg.time_stepping == GPL_RK3 ? 3 : 2;
+    double *in = cur_vs, *out = cur_vs == VsA ? VsB : VsA;
+    for (int st = 1; st <= nst; st++) {
+      const RKW w = st == 1 ? w1 : (st == 2 ? w2 : w3);
+      if (fp64) enqueue_stage<double>(in, out, w, st == 1, st == 1, t);
+      else      enqueue_stage<float>(in, out, w, st == 1, st == 1, t);
+      double *tmp = in; in = out; out = tmp;
+    }
+    if (t) t->begin(K_DT);
+    launch_dt(cdt_part, nparts, 3.0, env.ctl, cfg.cfl, cfg.cfl_max_var, advance_dt, env.stream);
+    if (t) t->end(K_DT);
+    cur_vs = in;
+  }
+
+  // RK2 maps the face buffers A -> B -> A within a step; RK3 flips them
+  int graph_period() const override { return cfg.time_stepping == GPL_RK3 ? 2 : 1; }
+  const void *state_key() const override { return cur_vs; }
+};
+
+}  // namespace
+
+int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
+{
+  *out = nullptr;
+  if (cfg.dimensions != 3) return GPL_EUNSUPPORTED;
+  if (cfg.boundary != GPL_PERIODIC || cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
+  if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
+  // the PLM stencil wraps at most two cells: every direction needs >= 3 zones
+  if (cfg.nx < 3 || cfg.ny < 3 || cfg.nz < 3) return GPL_EINVAL;
+  Mhd3dEngine *e = new (std::nothrow) Mhd3dEngine();
+  if (!e) return GPL_ENOMEM;
+  const int rc = e->init(cfg, env);
+  if (rc) { delete e; return rc; }
+  *out = e;
+  return 0;
+}
+
+}  // namespace gpl

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.hpp"
+
 namespace gpl {
 
 struct Box3 {
@@ -10,17 +12,6 @@ struct Box3 {
   int ncell;
   double dx, dy, dz;
 };
-
-// Device-resident time-step control: the step loop never needs the host.
-struct StepCtl {
-  double dt;       // current time step (timeStep::dt)
-  double time;     // timeStep::time
-  double inv_dt;   // timeStep::invDt_hyp of the last stage-1 update
-  long long nstep;
-};
-
-// RK weights of one stage: U = w0*U0 + w1*U + wR*R  (rk_step.cpp:20-51)
-struct RKW { double w0, w1, wR; };
 
 template <typename R>
 struct Mhd3dBufs {
@@ -40,9 +31,6 @@ struct Mhd3dBufs {
   uint32_t *err;
 };
 
-enum KernelId { K_SWEEP_X = 0, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F,
-                K_EMF, K_FINISH, K_DT, K_COUNT };
-
 template <typename R>
 void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
                   int first_stage, int want_dt, int limiter, int solver, int march, double gamma,
@@ -50,8 +38,5 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
 
 int mhd3d_blocks(const Box3 &g);
 int mhd3d_march_blocks(const Box3 &g, int march);
-
-void launch_dt(const double *part, int nparts, double dims, StepCtl *ctl, double cfl,
-               double cfl_max_var, int advance, hipStream_t s);
 
 }  // namespace gpl

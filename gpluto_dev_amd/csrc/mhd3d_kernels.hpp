@@ -425,10 +425,14 @@ template <typename R, int LIMV>
 __global__ __launch_bounds__(256)
 void k_emf(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
 {
-  const int c = blockIdx.x*blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
+  // 2D blocks (x, y) so the y-stencil neighbours of a row are L1 hits
+  Cell p;
+  p.i = blockIdx.x*blockDim.x + threadIdx.x;
+  p.j = blockIdx.y*blockDim.y + threadIdx.y;
+  p.k = blockIdx.z;
+  if (p.i >= g.nx || p.j >= g.ny) return;
+  const int c = cid(p.i, p.j, p.k, g);
   const size_t n = (size_t)g.ncell;
-  const Cell p = cell_of(c, g);
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
   const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
   // slots: 0 = et, 1 = eb, 2.. = aL, aR, d
@@ -581,7 +585,8 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   }
   case K_EMF:
-    hipLaunchKernelGGL((k_emf<R, LIM>), dim3(nb), dim3(T), 0, s, g, b.Vin, b.FD, b.E);
+    hipLaunchKernelGGL((k_emf<R, LIM>), dim3((g.nx + 63)/64, (g.ny + 3)/4, g.nz), dim3(64, 4), 0, s,
+                       g, b.Vin, b.FD, b.E);
     break;
   case K_FINISH:
     hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(T), 0, s,

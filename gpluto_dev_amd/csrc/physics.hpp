@@ -352,4 +352,159 @@ GPL_HD uint32_t mhd_cons_to_prim(double *u, const double *b, double gmm1,
   return err;
 }
 
+
+// ---------------------------------------------------------------------
+// HD (ideal EOS): variables rho, v1, v2, v3, prs; conserved rho, m1..3, E
+// ---------------------------------------------------------------------
+
+// PrimToCons, Src/HD/hd.hpp:392-410
+template <typename R>
+GPL_HD void hd_prim_to_cons(const R *v, R *u, R gmm1)
+{
+  u[0] = v[0];
+  u[1] = v[0]*v[1];
+  u[2] = v[0]*v[2];
+  u[3] = v[0]*v[3];
+  R kin = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  kin = v[0]*kin*R(0.5);
+  u[4] = kin + v[4]/gmm1;
+}
+
+// Flux<dir>, Src/HD/hd.hpp:105-125
+template <typename R, int N>
+GPL_HD void hd_flux(const R *v, const R *u, R *flux)
+{
+  constexpr int VXn = 1 + N;
+  flux[0] = u[1 + N];
+  flux[1] = u[1]*v[VXn];
+  flux[2] = u[2]*v[VXn];
+  flux[3] = u[3]*v[VXn];
+  flux[4] = (u[4] + v[4])*v[VXn];
+  flux[1 + N] += v[4];
+}
+
+// MaxSignalSpeed<dir>, Src/HD/hd.hpp:141-150 with SoundSpeed2 (Src/eos.hpp:84-99)
+template <typename R, int N>
+GPL_HD void hd_max_signal_speed(const R *v, R gamma, R &cmin, R &cmax)
+{
+  const R a = rsqrt_(gamma*v[4]/v[0]);
+  cmax = v[1 + N] + a;
+  cmin = v[1 + N] - a;
+}
+
+// HLLC, Src/HD/hllc.cpp:54-250 (ideal EOS, Davis speeds, HLL_Speed hd.hpp:165).
+// out.f[0..4] = rho, m1, m2, m3, E fluxes.
+template <typename R, int N>
+GPL_HD void hd_hllc(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
+{
+  constexpr int VXn = 1 + N, VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3;
+  const R gmm1 = gamma - R(1);
+  R ul[5], ur[5], fl[5], fr[5], sl_min, sl_max, sr_min, sr_max;
+  hd_prim_to_cons(vl, ul, gmm1);
+  hd_prim_to_cons(vr, ur, gmm1);
+  hd_flux<R, N>(vl, ul, fl);
+  hd_flux<R, N>(vr, ur, fr);
+  hd_max_signal_speed<R, N>(vl, gamma, sl_min, sl_max);
+  hd_max_signal_speed<R, N>(vr, gamma, sr_min, sr_max);
+  const R Sl = rmin(sl_min, sr_min);
+  const R Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  R *flux = out.f;
+  if (Sl > R(0)) {
+    for (int nv = 0; nv < 5; nv++) flux[nv] = fl[nv];
+  } else if (Sr < R(0)) {
+    for (int nv = 0; nv < 5; nv++) flux[nv] = fr[nv];
+  } else {
+    // contact speed (hllc.cpp:194-200)
+    const R qL = vl[4] + ul[VXn]*(vl[VXn] - Sl);
+    const R qR = vr[4] + ur[VXn]*(vr[VXn] - Sr);
+    const R wL = vl[0]*(vl[VXn] - Sl);
+    const R wR = vr[0]*(vr[VXn] - Sr);
+    const R vs = (qR - qL)/(wR - wL);
+    // star states of the side that is used (hllc.cpp:202-232)
+    const bool left = vs >= R(0);
+    const R *v = left ? vl : vr;
+    const R *u = left ? ul : ur;
+    const R *f = left ? fl : fr;
+    const R S = left ? Sl : Sr;
+    R us[5];
+    us[0] = u[0]*(S - v[VXn])/(S - vs);
+    us[VXn] = us[0]*vs;
+    us[VXt] = us[0]*v[VXt];
+    us[VXb] = us[0]*v[VXb];
+    us[4] =   u[4]/v[0]
+            + (vs - v[VXn])*(vs + v[4]/(v[0]*(S - v[VXn])));
+    us[4] *= us[0];
+    for (int nv = 0; nv < 5; nv++) flux[nv] = f[nv] + S*(us[nv] - u[nv]);
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 5; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+// HLL with Davis speeds, Src/HD/hll.cpp (flux formula as Src/MHD/hlld.cpp:452-461)
+template <typename R, int N>
+GPL_HD void hd_hll(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
+{
+  const R gmm1 = gamma - R(1);
+  R ul[5], ur[5], fl[5], fr[5], sl_min, sl_max, sr_min, sr_max;
+  hd_prim_to_cons(vl, ul, gmm1);
+  hd_prim_to_cons(vr, ur, gmm1);
+  hd_flux<R, N>(vl, ul, fl);
+  hd_flux<R, N>(vr, ur, fr);
+  hd_max_signal_speed<R, N>(vl, gamma, sl_min, sl_max);
+  hd_max_signal_speed<R, N>(vr, gamma, sr_min, sr_max);
+  const R Sl = rmin(sl_min, sr_min);
+  const R Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  R *flux = out.f;
+  if (Sl > R(0)) {
+    for (int nv = 0; nv < 5; nv++) flux[nv] = fl[nv];
+  } else if (Sr < R(0)) {
+    for (int nv = 0; nv < 5; nv++) flux[nv] = fr[nv];
+  } else {
+    R scrh = R(1)/(Sr - Sl);
+    for (int nv = 0; nv < 5; nv++) {
+      flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+      flux[nv] *= scrh;
+    }
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 5; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+// ConsToPrim HD (Src/HD/mappers.cpp:33-110) inside ConsToPrim3D
+// (Src/mappers3D.cpp:104-194).  u[] may be rewritten by the floors.
+GPL_HD uint32_t hd_cons_to_prim(double *u, double gmm1, double small_rho, double small_prs,
+                                 double *v)
+{
+  uint32_t err = 0;
+  for (int nv = 0; nv < 5; nv++) if (u[nv] != u[nv]) err |= ERR_NAN | ERR_QUIT;
+  uint32_t le = 0;
+  if (u[0] < 0.0) { u[0] = small_rho; le |= ERR_NEGATIVE_RHO | ERR_CONS2PRIM_FAIL; }
+  v[0] = u[0];
+  double tau = 1.0/u[0];
+  v[1] = u[1]*tau;
+  v[2] = u[2]*tau;
+  v[3] = u[3]*tau;
+  double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  double kin = 0.5*m2*tau;
+  if (u[4] > 0.0) {
+    v[4] = gmm1*(u[4] - kin);
+    if (v[4] < 0.0) {
+      v[4] = small_prs;
+      u[4] = v[4]/gmm1 + kin;
+      le |= ERR_NEGATIVE_PRS;
+    }
+  } else {
+    u[4] = small_prs/gmm1 + kin;
+    v[4] = small_prs;
+    le |= ERR_NEGATIVE_ENG;
+  }
+  err |= le;
+  if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
+  return err;
+}
+
 }  // namespace gpl

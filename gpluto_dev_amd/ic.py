@@ -157,6 +157,27 @@ def hd_riemann(n: int, seed: int):
     return vc
 
 
+def hd_blast(n, seed: int, gamma: float = 1.4):
+    """Seeded HD box (1D, 2D or 3D, n = (nx, ny, nz)) on the unit cube: cell
+    noise of 25% on rho and p around 1, random velocities of rms ~0.2, and a
+    pressure of 10 inside radius 0.2 of a random centre, so the run has strong
+    shocks, contacts and limiter switches everywhere.  Returns vc[5, nz, ny, nx]."""
+    nx, ny, nz = n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (nz, ny, nx)
+    vc = np.empty((5,) + shape)
+    vc[0] = 1.0 + 0.25 * rng.uniform(-1.0, 1.0, size=shape)
+    for d in range(3):
+        vc[1 + d] = 0.35 * rng.uniform(-1.0, 1.0, size=shape) if (nx, ny, nz)[d] > 1 else 0.0
+    vc[4] = 1.0 + 0.25 * rng.uniform(-1.0, 1.0, size=shape)
+    c = rng.uniform(0.3, 0.7, size=3)
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny,
+                          (np.arange(nx) + 0.5) / nx, indexing="ij")
+    r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 * (ny > 1) + (z - c[2]) ** 2 * (nz > 1)
+    vc[4][r2 < 0.04] = 10.0
+    return vc
+
+
 def divb_max(vs: np.ndarray) -> float:
     """max |div B| (per unit volume, dx = 1/n) of a periodic face field."""
     _, nz, ny, nx = vs.shape

@@ -127,6 +127,8 @@ class StageUpdate:
         _lib.check(L.gpl_create(C.byref(cfg), C.byref(h)), "gpl_create")
         self._h = h
         self.n = tuple(ini.n)
+        self.mhd = cfg.physics == _lib.GPL_MHD
+        self.nvar = 8 if self.mhd else 5      # NVAR (Src/MHD/mhd.hpp:21-41, Src/HD/hd.hpp:21-34)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -146,19 +148,27 @@ class StageUpdate:
         self.close()
 
     # -- state transfer -----------------------------------------------------
-    def upload(self, vc: np.ndarray, vs: np.ndarray, layout: int = _lib.GPL_LAYOUT_ACTIVE):
-        """vc[8, nz, ny, nx] primitives, vs[3, nz, ny, nx] lower faces (periodic)."""
+    def upload(self, vc: np.ndarray, vs: np.ndarray | None = None,
+               layout: int = _lib.GPL_LAYOUT_ACTIVE):
+        """vc[NVAR, nz, ny, nx] primitives; MHD: vs[3, nz, ny, nx] lower faces."""
         vc = np.ascontiguousarray(vc, dtype=np.float64)
-        comps = [np.ascontiguousarray(vs[d], dtype=np.float64) for d in range(3)]
-        arr = (C.c_void_p * 3)(*[c.ctypes.data for c in comps])
+        if vc.shape[0] != self.nvar:
+            raise ValueError(f"vc holds {vc.shape[0]} variables, expected {self.nvar}")
+        arr = None
+        if self.mhd:
+            comps = [np.ascontiguousarray(vs[d], dtype=np.float64) for d in range(3)]
+            arr = (C.c_void_p * 3)(*[c.ctypes.data for c in comps])
         _lib.check(self._L.gpl_upload(self._h, vc.ctypes.data, arr, layout), "gpl_upload")
 
     def download(self):
+        """(vc, vs, uc) on the active zones; vs is None for HD."""
         nx, ny, nz = self.n
-        vc = np.empty((8, nz, ny, nx))
-        uc = np.empty((8, nz, ny, nx))
-        vs = np.empty((3, nz, ny, nx))
-        arr = (C.c_void_p * 3)(*[vs[d].ctypes.data for d in range(3)])
+        vc = np.empty((self.nvar, nz, ny, nx))
+        uc = np.empty((self.nvar, nz, ny, nx))
+        vs, arr = None, None
+        if self.mhd:
+            vs = np.empty((3, nz, ny, nx))
+            arr = (C.c_void_p * 3)(*[vs[d].ctypes.data for d in range(3)])
         _lib.check(self._L.gpl_download(self._h, vc.ctypes.data, arr, uc.ctypes.data,
                                         _lib.GPL_LAYOUT_ACTIVE), "gpl_download")
         return vc, vs, uc

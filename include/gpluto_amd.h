@@ -60,16 +60,16 @@ extern "C" {
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
 typedef struct gpl_config {
-  int    physics;          /* GPL_MHD (3D: nz > 1) */
-  int    dimensions;       /* DIMENSIONS */
+  int    physics;          /* GPL_MHD (3D) or GPL_HD (1D/2D/3D) */
+  int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
-  int    boundary;         /* GPL_PERIODIC */
+  int    boundary;         /* GPL_PERIODIC (MHD, HD) or GPL_OUTFLOW (HD) on every side */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
-  int    solver;           /* GPL_SOLVER_HLLD or GPL_SOLVER_HLL */
-  int    ct_emf_average;   /* GPL_UCT_HLL */
-  int    precision;        /* GPL_FP64 or GPL_MIXED */
+  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLL; HD: GPL_SOLVER_HLLC / _HLL */
+  int    ct_emf_average;   /* GPL_UCT_HLL (MHD) */
+  int    precision;        /* GPL_FP64, or GPL_MIXED (MHD) */
   int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */
   int    device;           /* HIP device ordinal */
   double gamma;            /* run_config.gamma */
@@ -94,11 +94,13 @@ int gpl_destroy(gpl_ctx *ctx);
 /* Host -> device copy of the primitive state: vc = Data::Vc and
  * vs[d] = Data::Vs[BX1s+d] (Src/structs.hpp:640-712).  Follows Startup()
  * (Src/startup.cpp:388,514-561): the cell-centred field becomes the face
- * average and Uc = PrimToCons(Vc).  vc holds NVAR = 8 variables in the
- * reference order rho, vx1, vx2, vx3, Bx1, Bx2, Bx3, prs. */
+ * average and Uc = PrimToCons(Vc).  MHD: vc holds NVAR = 8 variables in the
+ * reference order rho, vx1, vx2, vx3, Bx1, Bx2, Bx3, prs.  HD: NVAR = 5
+ * (rho, vx1, vx2, vx3, prs; Src/HD/hd.hpp:21-28) and vs is ignored. */
 int gpl_upload(gpl_ctx *ctx, const double *vc, const double *const *vs, int layout);
 
-/* Device -> host copy of Vc, Vs and (if uc != NULL) the conserved Uc. */
+/* Device -> host copy of Vc, Vs (MHD) and (if uc != NULL) the conserved Uc;
+ * a NULL vc or vs is skipped. */
 int gpl_download(gpl_ctx *ctx, double *vc, double *const *vs, double *uc, int layout);
 
 /* AdvanceStep(data, grid, Dts) with Dts->dt = dt (Src/rk_step.cpp:88):
