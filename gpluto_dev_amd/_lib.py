@@ -25,7 +25,10 @@ GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED = -1, -2, -3, -4
 EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl_download",
            "gpl_advance_step", "gpl_next_time_step", "gpl_run", "gpl_set_timing",
            "gpl_kernel_count", "gpl_kernel_stats", "gpl_last_run_ms", "gpl_device_bytes",
-           "gpl_strerror")
+           "gpl_strerror",
+           "gpl_batch_create", "gpl_batch_destroy", "gpl_batch_upload", "gpl_batch_download",
+           "gpl_batch_run", "gpl_batch_set_tuning", "gpl_batch_get_tuning",
+           "gpl_batch_last_run_ms", "gpl_batch_device_bytes")
 
 
 class GplConfig(C.Structure):
@@ -79,6 +82,25 @@ def load() -> C.CDLL:
     L.gpl_device_bytes.restype = C.c_size_t
     L.gpl_strerror.argtypes = [ip]
     L.gpl_strerror.restype = C.c_char_p
+    ipp = C.POINTER(C.c_int)
+    L.gpl_batch_create.argtypes = [C.POINTER(GplConfig), ip, ipp, ipp, C.POINTER(vp)]
+    L.gpl_batch_create.restype = ip
+    L.gpl_batch_destroy.argtypes = [vp]
+    L.gpl_batch_destroy.restype = ip
+    L.gpl_batch_upload.argtypes = [vp, ip, vp, C.POINTER(vp), ip]
+    L.gpl_batch_upload.restype = ip
+    L.gpl_batch_download.argtypes = [vp, ip, vp, C.POINTER(vp), vp, ip]
+    L.gpl_batch_download.restype = ip
+    L.gpl_batch_run.argtypes = [vp, ip, dp, dp, dp]
+    L.gpl_batch_run.restype = C.c_uint32
+    L.gpl_batch_set_tuning.argtypes = [vp, ip, ip]
+    L.gpl_batch_set_tuning.restype = ip
+    L.gpl_batch_get_tuning.argtypes = [vp, ipp, ipp, dp]
+    L.gpl_batch_get_tuning.restype = ip
+    L.gpl_batch_last_run_ms.argtypes = [vp]
+    L.gpl_batch_last_run_ms.restype = C.c_double
+    L.gpl_batch_device_bytes.argtypes = [vp]
+    L.gpl_batch_device_bytes.restype = C.c_size_t
     _lib = L
     return L
 

@@ -30,6 +30,10 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("mhd3d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
            ("mhd3d_common.hip", []),
            ("hd_kernels.hip", []),
+           ("batch2d_fp64.hip", []),
+           ("batch2d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+           ("batch2d_common.hip", []),
+           ("engine_batch2d.cpp", []),
            ("engine_mhd3d.cpp", []),
            ("engine_hd.cpp", []),
            ("gpl_api.cpp", [])]

@@ -1,0 +1,64 @@
+// Declarations of the batched 2D MHD stage kernels: many independent
+// periodic boxes of uneven sizes advanced together, one launch per phase,
+// each launch a persistent grid striding over a per-batch work list.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace gpl {
+
+// One nx x ny periodic box: cells off .. off + nx*ny - 1 of every batch
+// array, c = off + i + nx*j.
+struct BoxDesc {
+  int nx, ny;
+  int off;
+  double dx, dy;
+};
+
+// Work items.  Sweep items are one wavefront each:
+//   x: {box, row j, first cell i0, cells in the segment (<= 63)}
+//   y: {box, first column i0 (64 lanes), segment index, unused}
+// Cell tiles (EMF, face update): {box, first cell within the box}, 256 cells.
+struct Item4 { int box, a, b, c; };
+struct Item2 { int box, start; };
+
+template <typename R>
+struct Batch2dBufs {
+  R *V;              // [8][ncell] primitives (B1, B2 = face averages, B3 cell-centred)
+  double *U;         // [6][ncell] rho, m1, m2, m3, B3, E
+  double *U0;        // [6][ncell]
+  const double *Vin; // [2][ncell] lower x- and y-faces at stage start
+  double *Vout;      // [2][ncell] faces after the stage
+  double *Bs0;       // [2][ncell]
+  R *FD;             // [2][5][ncell] UCT face data et, eb, aL, aR, d
+  R *E;              // [ncell] Ez at the lower-left corner of every cell
+  R *dU;             // [6][ncell] x-sweep flux divergence
+  R *Cdt;            // [ncell]    x-sweep C_dt
+  double *cdt_part;  // one C_dt maximum per y item
+  uint32_t *err;
+  StepCtl *ctl;      // [nbox]
+  const BoxDesc *box;
+  const Item4 *xitems;
+  const Item4 *yitems;
+  const Item2 *tiles;
+  int nx_items, ny_items, ntiles;
+  int ncell;
+};
+
+enum Batch2dPhase { B2_SWEEP_X = 0, B2_SWEEP_Y, B2_EMF, B2_FINISH };
+
+// Launch one phase as a persistent grid of 'blocks' workgroups of 256
+// threads; 'march' is the y-segment length the y items were built for.
+template <typename R>
+void batch2d_launch(int phase, const Batch2dBufs<R> &b, RKW w, int first_stage, int want_dt,
+                    int limiter, int solver, int march, int blocks, double gamma,
+                    double small_rho, double small_prs, hipStream_t s);
+
+// Per-box dt: invDt_hyp = max over the box's y items / 2, then optionally
+// NextTimeStep and time += dt (one workgroup per box).
+void batch2d_launch_dt(const double *cdt_part, const int *part_beg, int nbox, StepCtl *ctl,
+                       double cfl, double cfl_max_var, int advance, hipStream_t s);
+
+}  // namespace gpl

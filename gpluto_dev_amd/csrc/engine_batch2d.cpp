@@ -1,0 +1,482 @@
+// Batched 2D ideal MHD: many independent periodic boxes of uneven sizes in
+// one device state, advanced together (one launch per phase over all boxes,
+// one HIP graph per step), each box with its own dt.  Launch shapes (the
+// y-march length and the persistent grid size) are chosen by an autotuner
+// that times the candidates on the run's own first steps: every candidate
+// computes the same bits, so tuning never changes a result.
+#include "../../include/gpluto_amd.h"
+#include "batch2d.hpp"
+#include "engine.hpp"
+#include "physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace gpl;
+
+namespace {
+
+constexpr int kMarch[3] = {8, 16, 32};
+constexpr int kBpc[4] = {2, 4, 8, 0};      // workgroups per CU; 0 = one per work group
+
+template <typename T>
+int upload_vec(T **dst, const std::vector<T> &v, size_t &bytes)
+{
+  if (dev_alloc(bytes, (void **)dst, (v.empty() ? 1 : v.size())*sizeof(T))) return GPL_ENOMEM;
+  if (!v.empty() && hipMemcpy(*dst, v.data(), v.size()*sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return GPL_EHIP;
+  return 0;
+}
+
+}  // namespace
+
+struct gpl_batch {
+  gpl_config cfg;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  bool fp64 = true;
+  int nbox = 0, ncell = 0, ncu = 0;
+  std::vector<BoxDesc> hbox;
+  size_t bytes = 0;
+  // device state
+  void *V = nullptr, *FD = nullptr, *E = nullptr, *dU = nullptr, *Cdt = nullptr;
+  double *U = nullptr, *U0 = nullptr, *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
+  double *cdt_part = nullptr;
+  double *cur_vs = nullptr;
+  uint32_t *err = nullptr;
+  StepCtl *ctl = nullptr;
+  // work lists
+  BoxDesc *box = nullptr;
+  Item4 *xitems = nullptr;
+  Item2 *tiles = nullptr;
+  int nx_items = 0, ntiles = 0;
+  Item4 *yitems[3] = {nullptr, nullptr, nullptr};   // per march length
+  int *part_beg[3] = {nullptr, nullptr, nullptr};
+  int ny_items[3] = {0, 0, 0};
+  // launch shape
+  int march_idx = 1, bpc_idx = 2;
+  bool tune_pending = true;
+  double tuned_ms = 0.0;
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  const void *graph_key = nullptr;
+  int graph_shape = -1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_run_ms = 0.0;
+};
+
+static void batch_free(gpl_batch *b)
+{
+  void *ptrs[] = {b->V, b->FD, b->E, b->dU, b->Cdt, b->U, b->U0, b->VsA, b->VsB, b->Bs0,
+                  b->cdt_part, b->err, b->ctl, b->box, b->xitems, b->tiles,
+                  b->yitems[0], b->yitems[1], b->yitems[2], b->part_beg[0], b->part_beg[1],
+                  b->part_beg[2]};
+  for (void *p : ptrs) if (p) (void)hipFree(p);
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+}
+
+extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny,
+                                gpl_batch **out)
+{
+  if (!cfg || !nx || !out || nbox < 1) return GPL_EINVAL;
+  if (!ny) ny = nx;
+  *out = nullptr;
+  if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
+  if (cfg->boundary != GPL_PERIODIC || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
+  if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
+  if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
+  if (cfg->precision != GPL_FP64 && cfg->precision != GPL_MIXED) return GPL_EINVAL;
+  long long total = 0;
+  for (int q = 0; q < nbox; q++) {
+    if (nx[q] < 3 || nx[q] > 8192 || ny[q] < 3 || ny[q] > 8192) return GPL_EINVAL;
+    total += (long long)nx[q]*ny[q];
+  }
+  if (total > (1LL << 30)) return GPL_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return GPL_EHIP;
+
+  gpl_batch *b = new (std::nothrow) gpl_batch();
+  if (!b) return GPL_ENOMEM;
+  b->cfg = *cfg;
+  b->dev = cfg->device;
+  b->fp64 = cfg->precision == GPL_FP64;
+  b->nbox = nbox;
+  b->ncell = (int)total;
+  if (hipSetDevice(b->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->dev) != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return GPL_EHIP;
+  }
+  hipEventCreate(&b->ev0);
+  hipEventCreate(&b->ev1);
+  // boxes and work lists
+  std::vector<Item4> xi;
+  std::vector<Item2> ti;
+  int off = 0;
+  for (int q = 0; q < nbox; q++) {
+    BoxDesc d;
+    d.nx = nx[q]; d.ny = ny[q]; d.off = off;
+    d.dx = (cfg->xend[0] - cfg->xbeg[0])/nx[q];   // Src/set_grid.cpp:196
+    d.dy = (cfg->xend[1] - cfg->xbeg[1])/ny[q];
+    b->hbox.push_back(d);
+    for (int j = 0; j < ny[q]; j++)
+      for (int i0 = 0; i0 < nx[q]; i0 += 63) xi.push_back({q, j, i0, nx[q] - i0 < 63 ? nx[q] - i0 : 63});
+    for (int s = 0; s < nx[q]*ny[q]; s += 256) ti.push_back({q, s});
+    off += nx[q]*ny[q];
+  }
+  const size_t nc = (size_t)b->ncell, rs = b->fp64 ? sizeof(double) : sizeof(float);
+  int rc = 0;
+  rc |= upload_vec(&b->box, b->hbox, b->bytes);
+  rc |= upload_vec(&b->xitems, xi, b->bytes);
+  rc |= upload_vec(&b->tiles, ti, b->bytes);
+  b->nx_items = (int)xi.size();
+  b->ntiles = (int)ti.size();
+  int max_y = 0;
+  for (int m = 0; m < 3; m++) {
+    const int L = kMarch[m];
+    std::vector<Item4> yi;
+    std::vector<int> pb;
+    for (int q = 0; q < nbox; q++) {
+      pb.push_back((int)yi.size());
+      const int nseg = (ny[q] + L - 1)/L;
+      for (int s = 0; s < nseg; s++)
+        for (int i0 = 0; i0 < nx[q]; i0 += 64) yi.push_back({q, i0, s, 0});
+    }
+    pb.push_back((int)yi.size());
+    rc |= upload_vec(&b->yitems[m], yi, b->bytes);
+    rc |= upload_vec(&b->part_beg[m], pb, b->bytes);
+    b->ny_items[m] = (int)yi.size();
+    if ((int)yi.size() > max_y) max_y = (int)yi.size();
+  }
+  rc |= dev_alloc(b->bytes, &b->V, 8*nc*rs);
+  rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->U0, 6*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->VsA, 2*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->VsB, 2*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->Bs0, 2*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, &b->FD, 10*nc*rs);
+  rc |= dev_alloc(b->bytes, &b->E, nc*rs);
+  rc |= dev_alloc(b->bytes, &b->dU, 6*nc*rs);
+  rc |= dev_alloc(b->bytes, &b->Cdt, nc*rs);
+  rc |= dev_alloc(b->bytes, (void **)&b->cdt_part, (size_t)max_y*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->err, sizeof(uint32_t));
+  rc |= dev_alloc(b->bytes, (void **)&b->ctl, (size_t)nbox*sizeof(StepCtl));
+  if (rc) {
+    batch_free(b);
+    delete b;
+    return rc == GPL_EHIP ? GPL_EHIP : GPL_ENOMEM;
+  }
+  std::vector<StepCtl> z(nbox, StepCtl{0.0, 0.0, 0.0, 0});
+  if (hipMemcpy(b->ctl, z.data(), nbox*sizeof(StepCtl), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->err, 0, sizeof(uint32_t)) != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return GPL_EHIP;
+  }
+  b->cur_vs = b->VsA;
+  if (const char *e = getenv("GPL_BATCH_SHAPE")) {   // "march,bpc" fixes the launch shape
+    int m = 16, p = 8;
+    if (sscanf(e, "%d,%d", &m, &p) == 2) gpl_batch_set_tuning(b, m, p);
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int gpl_batch_destroy(gpl_batch *b)
+{
+  if (!b) return GPL_EINVAL;
+  hipSetDevice(b->dev);
+  hipStreamSynchronize(b->stream);
+  batch_free(b);
+  delete b;
+  return 0;
+}
+
+static Layout box_layout(const gpl_batch *b, int box, int layout)
+{
+  gpl_config c = b->cfg;
+  c.nx = b->hbox[box].nx;
+  c.ny = b->hbox[box].ny;
+  c.nz = 1;
+  c.dimensions = 2;
+  return make_layout(c, layout);
+}
+
+extern "C" int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const double *const *vs,
+                                int layout)
+{
+  if (!b || box < 0 || box >= b->nbox || !vc || !vs || !vs[0] || !vs[1]) return GPL_EINVAL;
+  if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
+  hipSetDevice(b->dev);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return GPL_EHIP;
+  const Layout L = box_layout(b, box, layout);
+  const BoxDesc d = b->hbox[box];
+  const int NX = d.nx, NY = d.ny;
+  const size_t m = (size_t)NX*NY, nc = (size_t)b->ncell;
+  std::vector<double> hv(8*m), hu(6*m), hs(2*m);
+  for (int c = 0; c < 2; c++)
+    for (int j = 0; j < NY; j++)
+      for (int i = 0; i < NX; i++) hs[c*m + (size_t)j*NX + i] = vs[c][L.face(c, i, j, 0)];
+  for (int j = 0; j < NY; j++)
+    for (int i = 0; i < NX; i++) {
+      const size_t o = (size_t)j*NX + i;
+      double v[8], u[8];
+      for (int s = 0; s < 8; s++) v[s] = vc[L.cell(s, i, j, 0)];
+      // cell-centred B1, B2 = face averages (Src/startup.cpp:388, ct_field_average.cpp:182-184)
+      const int ip = i + 1 == NX ? 0 : i + 1, jp = j + 1 == NY ? 0 : j + 1;
+      v[4] = 0.5*(hs[o] + hs[(size_t)j*NX + ip]);
+      v[5] = 0.5*(hs[m + o] + hs[m + (size_t)jp*NX + i]);
+      mhd_prim_to_cons<double>(v, u, b->cfg.gamma - 1.0);   // Src/startup.cpp:514-561
+      for (int s = 0; s < 8; s++) hv[s*m + o] = v[s];
+      const int us[6] = {0, 1, 2, 3, 6, 7};
+      for (int s = 0; s < 6; s++) hu[s*m + o] = u[us[s]];
+    }
+  const size_t rs = b->fp64 ? sizeof(double) : sizeof(float);
+  for (int s = 0; s < 8; s++) {
+    char *dst = (char *)b->V + (s*nc + d.off)*rs;
+    if (b->fp64) {
+      if (hipMemcpy(dst, &hv[s*m], m*rs, hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
+    } else {
+      std::vector<float> t(m);
+      for (size_t q = 0; q < m; q++) t[q] = (float)hv[s*m + q];
+      if (hipMemcpy(dst, t.data(), m*rs, hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
+    }
+  }
+  for (int s = 0; s < 6; s++)
+    if (hipMemcpy(b->U + s*nc + d.off, &hu[s*m], m*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return GPL_EHIP;
+  for (int c = 0; c < 2; c++)
+    if (hipMemcpy(b->cur_vs + c*nc + d.off, &hs[c*m], m*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return GPL_EHIP;
+  const StepCtl z = {0.0, 0.0, 0.0, 0};
+  if (hipMemcpy(b->ctl + box, &z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
+  return 0;
+}
+
+extern "C" int gpl_batch_download(gpl_batch *b, int box, double *vc, double *const *vs, double *uc,
+                                  int layout)
+{
+  if (!b || box < 0 || box >= b->nbox) return GPL_EINVAL;
+  if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
+  hipSetDevice(b->dev);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return GPL_EHIP;
+  const Layout L = box_layout(b, box, layout);
+  const BoxDesc d = b->hbox[box];
+  const int NX = d.nx, NY = d.ny;
+  const size_t m = (size_t)NX*NY, nc = (size_t)b->ncell;
+  const size_t rs = b->fp64 ? sizeof(double) : sizeof(float);
+  std::vector<double> hv(8*m), hu(6*m), hs(2*m);
+  for (int s = 0; s < 8; s++) {
+    const char *src = (const char *)b->V + (s*nc + d.off)*rs;
+    if (b->fp64) {
+      if (hipMemcpy(&hv[s*m], src, m*rs, hipMemcpyDeviceToHost) != hipSuccess) return GPL_EHIP;
+    } else {
+      std::vector<float> t(m);
+      if (hipMemcpy(t.data(), src, m*rs, hipMemcpyDeviceToHost) != hipSuccess) return GPL_EHIP;
+      for (size_t q = 0; q < m; q++) hv[s*m + q] = t[q];
+    }
+  }
+  for (int s = 0; s < 6; s++)
+    if (hipMemcpy(&hu[s*m], b->U + s*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return GPL_EHIP;
+  for (int c = 0; c < 2; c++)
+    if (hipMemcpy(&hs[c*m], b->cur_vs + c*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return GPL_EHIP;
+  for (int j = 0; j < NY; j++)
+    for (int i = 0; i < NX; i++) {
+      const size_t o = (size_t)j*NX + i;
+      if (vc) for (int s = 0; s < 8; s++) vc[L.cell(s, i, j, 0)] = hv[s*m + o];
+      if (uc) {
+        // B1, B2 of Uc are the face averages (= Vc's)
+        const double u8[8] = {hu[o], hu[m + o], hu[2*m + o], hu[3*m + o], hv[4*m + o], hv[5*m + o],
+                              hu[4*m + o], hu[5*m + o]};
+        for (int s = 0; s < 8; s++) uc[L.cell(s, i, j, 0)] = u8[s];
+      }
+      if (vs) for (int c = 0; c < 2; c++) if (vs[c]) vs[c][L.face(c, i, j, 0)] = hs[c*m + o];
+    }
+  if (vs && L.ary) {   // the duplicate upper faces equal the lower ones (PeriodicBoundary)
+    for (int j = 0; j < NY; j++) if (vs[0]) vs[0][L.face(0, NX, j, 0)] = hs[(size_t)j*NX];
+    for (int i = 0; i < NX; i++) if (vs[1]) vs[1][L.face(1, i, NY, 0)] = hs[m + i];
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------
+static int grid_for(const gpl_batch *b, int groups)
+{
+  const int bpc = kBpc[b->bpc_idx];
+  if (bpc == 0 || groups <= bpc*b->ncu) return groups > 0 ? groups : 1;
+  return bpc*b->ncu;
+}
+
+template <typename R>
+static void enqueue_stage(gpl_batch *b, double *vin, double *vout, RKW w, int first_stage)
+{
+  Batch2dBufs<R> q;
+  q.V = (R *)b->V; q.U = b->U; q.U0 = b->U0; q.Vin = vin; q.Vout = vout; q.Bs0 = b->Bs0;
+  q.FD = (R *)b->FD; q.E = (R *)b->E; q.dU = (R *)b->dU; q.Cdt = (R *)b->Cdt;
+  q.cdt_part = b->cdt_part; q.err = b->err; q.ctl = b->ctl; q.box = b->box;
+  q.xitems = b->xitems; q.yitems = b->yitems[b->march_idx]; q.tiles = b->tiles;
+  q.nx_items = b->nx_items; q.ny_items = b->ny_items[b->march_idx]; q.ntiles = b->ntiles;
+  q.ncell = b->ncell;
+  const int march = kMarch[b->march_idx];
+  const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
+  const int gt = grid_for(b, q.ntiles);
+  const gpl_config &c = b->cfg;
+  batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, c.solver, march, gx,
+                    c.gamma, c.small_rho, c.small_prs, b->stream);
+  batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, c.solver, march, gy,
+                    c.gamma, c.small_rho, c.small_prs, b->stream);
+  batch2d_launch<R>(B2_EMF, q, w, first_stage, first_stage, c.limiter, c.solver, march, gt,
+                    c.gamma, c.small_rho, c.small_prs, b->stream);
+  batch2d_launch<R>(B2_FINISH, q, w, first_stage, first_stage, c.limiter, c.solver, march, gt,
+                    c.gamma, c.small_rho, c.small_prs, b->stream);
+}
+
+// one AdvanceStep of every box (Src/rk_step.cpp:88-720), then per-box dt
+static void enqueue_step(gpl_batch *b, int advance_dt)
+{
+  const RKW w1 = {0.0, 0.0, 1.0};
+  const RKW w2 = b->cfg.time_stepping == GPL_RK3 ? RKW{0.75, 0.25, 0.25} : RKW{0.5, 0.5, 0.5};
+  const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
+  const int nst = b->cfg.time_stepping == GPL_RK3 ? 3 : 2;
+  double *in = b->cur_vs, *out = in == b->VsA ? b->VsB : b->VsA;
+  for (int st = 1; st <= nst; st++) {
+    const RKW w = st == 1 ? w1 : (st == 2 ? w2 : w3);
+    if (b->fp64) enqueue_stage<double>(b, in, out, w, st == 1);
+    else         enqueue_stage<float>(b, in, out, w, st == 1);
+    double *t = in; in = out; out = t;
+  }
+  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
+                    b->cfg.cfl_max_var, advance_dt, b->stream);
+  b->cur_vs = in;
+}
+
+static int graph_period(const gpl_batch *b) { return b->cfg.time_stepping == GPL_RK3 ? 2 : 1; }
+
+static int build_graph(gpl_batch *b)
+{
+  const int shape = b->march_idx*8 + b->bpc_idx;
+  if (b->graph_exec && b->graph_key == b->cur_vs && b->graph_shape == shape) return 0;
+  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  const void *key = b->cur_vs;
+  hipGraph_t graph;
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return GPL_EHIP;
+  for (int s = 0; s < graph_period(b); s++) enqueue_step(b, 1);
+  if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) return GPL_EHIP;
+  if (b->cur_vs != key) { hipGraphDestroy(graph); return GPL_EHIP; }
+  const hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) { b->graph_exec = nullptr; return GPL_EHIP; }
+  b->graph_key = key;
+  b->graph_shape = shape;
+  return 0;
+}
+
+// Time every launch shape on one step each (eager launches); keep the fastest.
+static int autotune(gpl_batch *b, int *done)
+{
+  double best = 1e30;
+  int bm = b->march_idx, bp = b->bpc_idx;
+  for (int m = 0; m < 3; m++)
+    for (int p = 0; p < 4; p++) {
+      b->march_idx = m; b->bpc_idx = p;
+      hipEventRecord(b->ev0, b->stream);
+      enqueue_step(b, 1);
+      hipEventRecord(b->ev1, b->stream);
+      if (hipEventSynchronize(b->ev1) != hipSuccess) return GPL_EHIP;
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      if (ms < best) { best = ms; bm = m; bp = p; }
+      (*done)++;
+    }
+  b->march_idx = bm; b->bpc_idx = bp;
+  b->tuned_ms = best;
+  b->tune_pending = false;
+  return 0;
+}
+
+static constexpr int kTuneSteps = 12;
+
+extern "C" uint32_t gpl_batch_run(gpl_batch *b, int nsteps, const double *first_dt, double *time_out,
+                                  double *dt_out)
+{
+  if (!b || nsteps < 0) return GPL_STEP_API_ERROR;
+  hipSetDevice(b->dev);
+  std::vector<StepCtl> h(b->nbox);
+  if (hipStreamSynchronize(b->stream) != hipSuccess ||
+      hipMemcpy(h.data(), b->ctl, b->nbox*sizeof(StepCtl), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_STEP_API_ERROR;
+  if (first_dt)
+    for (int q = 0; q < b->nbox; q++) { h[q].dt = first_dt[q]; h[q].time = 0.0; h[q].nstep = 0; }
+  const uint32_t zero = 0;
+  if (hipMemcpy(b->ctl, h.data(), b->nbox*sizeof(StepCtl), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->err, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+    return GPL_STEP_API_ERROR;
+  int done = 0;
+  if (b->tune_pending && nsteps >= 2*kTuneSteps && getenv("GPL_NO_GRAPH") == nullptr)
+    if (autotune(b, &done) != 0) return GPL_STEP_API_ERROR;
+  hipEventRecord(b->ev0, b->stream);
+  const int period = graph_period(b);
+  if (getenv("GPL_NO_GRAPH") == nullptr && nsteps - done >= period) {
+    if (build_graph(b) != 0) return GPL_STEP_API_ERROR;
+    while (nsteps - done >= period) {
+      if (hipGraphLaunch(b->graph_exec, b->stream) != hipSuccess) return GPL_STEP_API_ERROR;
+      done += period;
+    }
+  }
+  while (done < nsteps) { enqueue_step(b, 1); done++; }
+  hipEventRecord(b->ev1, b->stream);
+  uint32_t err = 0;
+  if (hipStreamSynchronize(b->stream) != hipSuccess ||
+      hipMemcpy(h.data(), b->ctl, b->nbox*sizeof(StepCtl), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&err, b->err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_STEP_API_ERROR;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, b->ev0, b->ev1);
+  b->last_run_ms = ms;
+  for (int q = 0; q < b->nbox; q++) {
+    if (time_out) time_out[q] = h[q].time;
+    if (dt_out) dt_out[q] = h[q].dt;
+  }
+  return err;
+}
+
+extern "C" int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu)
+{
+  if (!b) return GPL_EINVAL;
+  if (march == 0) { b->tune_pending = true; return 0; }
+  int m = -1, p = -1;
+  for (int q = 0; q < 3; q++) if (kMarch[q] == march) m = q;
+  for (int q = 0; q < 4; q++) if (kBpc[q] == blocks_per_cu) p = q;
+  if (m < 0 || p < 0) return GPL_EINVAL;
+  b->march_idx = m; b->bpc_idx = p;
+  b->tune_pending = false;
+  return 0;
+}
+
+extern "C" int gpl_batch_get_tuning(const gpl_batch *b, int *march, int *blocks_per_cu, double *step_ms)
+{
+  if (!b) return GPL_EINVAL;
+  if (march) *march = kMarch[b->march_idx];
+  if (blocks_per_cu) *blocks_per_cu = kBpc[b->bpc_idx];
+  if (step_ms) *step_ms = b->tuned_ms;
+  return 0;
+}
+
+extern "C" double gpl_batch_last_run_ms(const gpl_batch *b) { return b ? b->last_run_ms : 0.0; }
+
+extern "C" size_t gpl_batch_device_bytes(const gpl_batch *b) { return b ? b->bytes : 0; }

@@ -49,9 +49,13 @@ public:
   double *U = nullptr, *U0 = nullptr;
   double *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
   void *F = nullptr, *CM = nullptr, *FD = nullptr, *E = nullptr;
-  double *dU = nullptr, *Cdt = nullptr;
+  void *dU = nullptr, *Cdt = nullptr;   // R: the within-stage sums (FP64 in the drop-in)
   bool fused = true;          // fused sweep kernels (nx <= 512) or the split path
   int march = 16;             // cells per thread in the y/z marching sweeps
+  // edges per thread along z in the EMF kernel: 16 (z-march) for FP64, one
+  // edge per thread (0) for the mixed form, where more waves in flight pay
+  // more than the register-carried z-stencil
+  int emf_march = 16;
   double *cdt_part = nullptr;
   int nparts = 0;
   double *cur_vs = nullptr;   // VsA or VsB: faces of the current state
@@ -79,6 +83,9 @@ public:
     // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
     if (const char *m = getenv("GPL_MARCH")) march = atoi(m);
     if (march != 8 && march != 16 && march != 32) march = 16;
+    emf_march = fp64 ? 16 : 0;
+    if (const char *m = getenv("GPL_EMF_MARCH")) emf_march = atoi(m);
+    if (emf_march != 0 && emf_march != 4 && emf_march != 16) emf_march = 16;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);
@@ -88,8 +95,8 @@ public:
     rc |= dev_alloc(bytes, (void **)&VsB, 3*n*sizeof(double));
     rc |= dev_alloc(bytes, (void **)&Bs0, 3*n*sizeof(double));
     if (fused) {
-      rc |= dev_alloc(bytes, (void **)&dU, 5*n*sizeof(double));
-      rc |= dev_alloc(bytes, (void **)&Cdt, n*sizeof(double));
+      rc |= dev_alloc(bytes, &dU, 5*n*rs);
+      rc |= dev_alloc(bytes, &Cdt, n*rs);
     } else {
       rc |= dev_alloc(bytes, &F, 15*n*rs);
       rc |= dev_alloc(bytes, &CM, 3*n*rs);
@@ -196,8 +203,9 @@ public:
     Mhd3dBufs<R> b;
     b.V = (R *)V; b.U = U; b.U0 = U0; b.Vin = vin; b.Vout = vout; b.Bs0 = Bs0;
     b.F = (R *)F; b.CM = (R *)CM; b.FD = (R *)FD; b.E = (R *)E;
-    b.dU = dU; b.Cdt = Cdt;
+    b.dU = (R *)dU; b.Cdt = (R *)Cdt;
     b.cdt_part = cdt_part; b.err = env.err;
+    b.emf_march = emf_march;
     static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
     static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
     const int *seq = fused ? fused_seq : split_seq;

@@ -25,10 +25,11 @@ struct Mhd3dBufs {
   R *CM;           // [3][n]
   R *FD;           // [3][5][n]
   R *E;            // [3][n]
-  double *dU;      // [5][n] flux-divergence accumulator (fused path)
-  double *Cdt;     // [n]    C_dt accumulator (fused path)
+  R *dU;           // [5][n] flux-divergence accumulator within a stage (fused path)
+  R *Cdt;          // [n]    C_dt accumulator within a stage (fused path)
   double *cdt_part;
   uint32_t *err;
+  int emf_march;   // edges per thread along z in the EMF kernel (0: one edge per thread)
 };
 
 template <typename R>

@@ -80,6 +80,16 @@ __device__ __forceinline__ T block_max(T v)
   return red[0];
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: block b runs on XCD b % 8.  Renumbering gives every XCD a contiguous
+// range of logical tiles, so tiles that share stencil rows share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+  constexpr int NXCD = 8;
+  const int x = b % NXCD, q = nb / NXCD, r = nb % NXCD;
+  return x*q + (x < r ? x : r) + b / NXCD;
+}
+
 // ---------------------------------------------------------------------------
 // Sweep along D: one thread per interface (lower face of cell c).
 // DataToSweep + Reconstruct (plm_states.cpp:99-189) + Riemann solver
@@ -194,7 +204,7 @@ void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
 template <typename R, int LIM, int SOLVER>
 __global__ __launch_bounds__(512)
 void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
-                    R *__restrict__ FD, double *__restrict__ dU, double *__restrict__ Cdt,
+                    R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
                     const StepCtl *__restrict__ ctl, R gamma, int want_cdt,
                     uint32_t *__restrict__ err)
 {
@@ -243,9 +253,9 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   for (int nv = 0; nv < 5; nv++) {
     double r = -dtdx*((double)sF[nv*nx + ip] - (double)sF[nv*nx + i]);
     r += 0.0;
-    dU[nv*n + c] = r;
+    dU[nv*n + c] = (R)r;
   }
-  if (want_cdt) Cdt[c] = 0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx;
+  if (want_cdt) Cdt[c] = (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx);
   if (ff.err) atomicOr(err, ff.err);
 }
 
@@ -264,7 +274,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
 template <typename R, int D, int LIM, int SOLVER, int L>
 __global__ __launch_bounds__(256)
 void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
-                   R *__restrict__ FD, double *__restrict__ dU, double *__restrict__ Cdt,
+                   R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
                    double *__restrict__ U, double *__restrict__ U0,
                    const StepCtl *__restrict__ ctl, RKW w, int first_stage, int want_cdt,
                    double *__restrict__ cdt_part, R gamma, uint32_t *__restrict__ err)
@@ -272,6 +282,8 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
   static_assert(D == 1 || D == 2, "march along y or z");
   constexpr int BXn = 4 + D;
   constexpr int VAR[7] = {0, 1, 2, 3, D == 1 ? 4 : 4, D == 1 ? 6 : 5, 7};
+  // block order: x fastest, then the other transverse index, then the
+  // segment, so that concurrent blocks stream whole planes
   const int t = blockIdx.x*blockDim.x + threadIdx.x;
   const int nd = D == 1 ? g.ny : g.nz;          // cells along the sweep
   const int na = D == 1 ? g.nz : g.ny;          // the other transverse extent
@@ -311,8 +323,8 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       double acc[5], u_old[5], u0_old[5], cacc = 0.0;
       if (f > s0) {
 #pragma unroll
-        for (int nv = 0; nv < 5; nv++) acc[nv] = dU[nv*n + cc];
-        if (want_cdt) cacc = Cdt[cc];
+        for (int nv = 0; nv < 5; nv++) acc[nv] = (double)dU[nv*n + cc];
+        if (want_cdt) cacc = (double)Cdt[cc];
         if (D == 2) {
 #pragma unroll
           for (int nv = 0; nv < 5; nv++) u_old[nv] = U[nv*n + cc];
@@ -355,7 +367,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           du += rr;
           const size_t o = nv*n + cc;
           if (D == 1) {
-            dU[o] = du;
+            dU[o] = (R)du;
           } else if (first_stage) {
             U0[o] = u_old[nv];
             U[o] = u_old[nv] + w.wR*du;
@@ -366,7 +378,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
         if (want_cdt) {
           double cd = cacc;
           cd += 0.5*(cmp + cm)/ddx;
-          if (D == 1) Cdt[cc] = cd;
+          if (D == 1) Cdt[cc] = (R)cd;
           else cdt_max = cdt_max >= cd ? cdt_max : cd;
         }
       }
@@ -394,23 +406,20 @@ __device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
   Rr = q2 - R(0.5)*plm_slope<R, LIM>(q1, q2, q3);
 }
 
-// one UCT contribution (ct_emf_average.cpp:636-643): returns
-// (aL vL bL + aR vR bR) + sgn*(d bR - d bL) with coefficients averaged over
-// the two faces fa, fb.
-template <typename R, int LIMV, int RD>
-__device__ __forceinline__ R uct_term(const Cell &p, const Box3 &g, const double *__restrict__ Q,
-                                      const R *__restrict__ ecomp, const R *__restrict__ coef,
-                                      int fa, int fb, int sgn)
+// one UCT contribution (ct_emf_average.cpp:636-643):
+// (aL vL bL + aR vR bR) +/- (d bR - d bL), from preloaded values: face-field stencil q[0..3] and
+// transverse-velocity stencil e[0..3] along the reconstruction direction, and
+// the (aL, aR, d) records of the two faces the edge averages over (fa is the
+// displaced face, fb the face of the edge's own cell).
+template <typename R, int LIMV>
+__device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const R *fb, int sgn)
 {
-  const size_t n = (size_t)g.ncell;
-  const int c = cid(p.i, p.j, p.k, g);
-  const int m2 = shift<RD>(p, -2, g), m1 = shift<RD>(p, -1, g), p1 = shift<RD>(p, 1, g);
   R bL, bR, vL, vR;
-  rec4<R, MC_LIM>((R)Q[m2], (R)Q[m1], (R)Q[c], (R)Q[p1], bL, bR);   // recB = MC_LIM
-  rec4<R, LIMV>(ecomp[m2], ecomp[m1], ecomp[c], ecomp[p1], vL, vR);  // recV = LIMITER
-  const R aL = R(0.5)*(coef[0*n + fa] + coef[0*n + fb]);
-  const R aR = R(0.5)*(coef[1*n + fa] + coef[1*n + fb]);
-  const R dL = R(0.5)*(coef[2*n + fa] + coef[2*n + fb]);
+  rec4<R, MC_LIM>(q[0], q[1], q[2], q[3], bL, bR);   // recB = MC_LIM
+  rec4<R, LIMV>(e[0], e[1], e[2], e[3], vL, vR);     // recV = LIMITER
+  const R aL = R(0.5)*(fa[0] + fb[0]);
+  const R aR = R(0.5)*(fa[1] + fb[1]);
+  const R dL = R(0.5)*(fa[2] + fb[2]);
   const R dR = dL;
   const R phi = dR*bR - dL*bL;
   const R s = aL*vL*bL + aR*vR*bR;
@@ -420,29 +429,147 @@ __device__ __forceinline__ R uct_term(const Cell &p, const Box3 &g, const double
 // ---------------------------------------------------------------------------
 // Edge EMFs, CT_EMF_Riemann2D for UCT_HLL (ct_emf_average.cpp:535-989):
 // Ez = X1a + X2b, Ey = X1b + X3a, Ex = X2a + X3b, same order as the reference.
+// A block is a 64 x 4 (x, y) tile that marches L edges along z: the
+// z-stencils (X3a, X3b) and the face records of plane k-1 (X1b, X2a) stay in
+// registers, so every HBM byte along z is read once; x- and y-stencils come
+// from the plane being processed, shared through L1/L2 by the tile and its
+// neighbours on the same XCD.
 // ---------------------------------------------------------------------------
-template <typename R, int LIMV>
+template <typename R, int LIMV, int L>
 __global__ __launch_bounds__(256)
-void k_emf(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
+void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
 {
-  // 2D blocks (x, y) so the y-stencil neighbours of a row are L1 hits
-  Cell p;
-  p.i = blockIdx.x*blockDim.x + threadIdx.x;
-  p.j = blockIdx.y*blockDim.y + threadIdx.y;
-  p.k = blockIdx.z;
-  if (p.i >= g.nx || p.j >= g.ny) return;
-  const int c = cid(p.i, p.j, p.k, g);
+  const int ntx = (g.nx + 63) >> 6, nty = (g.ny + 3) >> 2;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, rr = t / ntx, ty = rr % nty, seg = rr / nty;
+  const int i = tx*64 + (threadIdx.x & 63), j = ty*4 + (threadIdx.x >> 6);
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)g.ncell;
+  const int pl = nx*ny;
+  const int k0 = seg*L, k1 = k0 + L < nz ? k0 + L : nz;
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
   const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
-  // slots: 0 = et, 1 = eb, 2.. = aL, aR, d
-  const int im = shift<0>(p, -1, g), jm = shift<1>(p, -1, g), km = shift<2>(p, -1, g);
-  R ez = uct_term<R, LIMV, 0>(p, g, By, FDy + 1*n, FDx + 2*n, jm, c, +1);   // X1a: By, e[K][J]
-  ez  += uct_term<R, LIMV, 1>(p, g, Bx, FDx + 0*n, FDy + 2*n, im, c, -1);   // X2b: Bx, e[K][I]
-  R ey = uct_term<R, LIMV, 0>(p, g, Bz, FDz + 0*n, FDx + 2*n, km, c, -1);   // X1b: Bz, e[J][K]
-  ey  += uct_term<R, LIMV, 2>(p, g, Bx, FDx + 1*n, FDz + 2*n, im, c, +1);   // X3a: Bx, e[J][I]
-  R ex = uct_term<R, LIMV, 1>(p, g, Bz, FDz + 1*n, FDy + 2*n, km, c, +1);   // X2a: Bz, e[I][K]
-  ex  += uct_term<R, LIMV, 2>(p, g, By, FDy + 0*n, FDz + 2*n, jm, c, -1);   // X3b: By, e[I][J]
+  int xo[4], yo[4];               // in-plane offsets of the x- and y-stencils (m2, m1, 0, p1)
+#pragma unroll
+  for (int o = 0; o < 4; o++) {
+    xo[o] = wrapm(i + o - 2, nx) + nx*j;
+    yo[o] = i + nx*wrapm(j + o - 2, ny);
+  }
+  const int col = i + nx*j;
+  const int colim = xo[1], coljm = yo[1];
+  R zbx[4], zex[4], zby[4], zey[4];   // z-stencils of X3a (Bx, e[J][I]) and X3b (By, e[I][J])
+  R cxm[3], cym[3];                   // x- and y-face records of plane k-1
+#pragma unroll
+  for (int o = 0; o < 3; o++) {
+    const int c = col + wrapm(k0 + o - 2, nz)*pl;
+    zbx[o] = (R)Bx[c]; zex[o] = FDx[1*n + c];
+    zby[o] = (R)By[c]; zey[o] = FDy[0*n + c];
+  }
+  {
+    const int c = col + wrapm(k0 - 1, nz)*pl;
+#pragma unroll
+    for (int q = 0; q < 3; q++) { cxm[q] = FDx[(2 + q)*n + c]; cym[q] = FDy[(2 + q)*n + c]; }
+  }
+  for (int k = k0; k < k1; k++) {
+    const int base = k*pl;
+    {
+      const int c = col + wrapm(k + 1, nz)*pl;
+      zbx[3] = (R)Bx[c]; zex[3] = FDx[1*n + c];
+      zby[3] = (R)By[c]; zey[3] = FDy[0*n + c];
+    }
+    const int c = col + base;
+    R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      cx[q] = FDx[(2 + q)*n + c];  cy[q] = FDy[(2 + q)*n + c];  cz[q] = FDz[(2 + q)*n + c];
+      cxj[q] = FDx[(2 + q)*n + coljm + base];
+      cyi[q] = FDy[(2 + q)*n + colim + base];
+      czi[q] = FDz[(2 + q)*n + colim + base];
+      czj[q] = FDz[(2 + q)*n + coljm + base];
+    }
+    R q4[4], e4[4];
+    // X1a: By along x, e[K][J] (FDy slot 1), x-face records at (j-1, j)
+#pragma unroll
+    for (int o = 0; o < 4; o++) { q4[o] = (R)By[xo[o] + base]; e4[o] = FDy[1*n + xo[o] + base]; }
+    R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);
+    // X2b: Bx along y, e[K][I] (FDx slot 0), y-face records at (i-1, i)
+#pragma unroll
+    for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[yo[o] + base]; e4[o] = FDx[0*n + yo[o] + base]; }
+    ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);
+    // X1b: Bz along x, e[J][K] (FDz slot 0), x-face records at (k-1, k)
+#pragma unroll
+    for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[xo[o] + base]; e4[o] = FDz[0*n + xo[o] + base]; }
+    R ey = uct_val<R, LIMV>(q4, e4, cxm, cx, -1);
+    // X3a: Bx along z, e[J][I] (FDx slot 1), z-face records at (i-1, i)
+    ey += uct_val<R, LIMV>(zbx, zex, czi, cz, +1);
+    // X2a: Bz along y, e[I][K] (FDz slot 1), y-face records at (k-1, k)
+#pragma unroll
+    for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[yo[o] + base]; e4[o] = FDz[1*n + yo[o] + base]; }
+    R ex = uct_val<R, LIMV>(q4, e4, cym, cy, +1);
+    // X3b: By along z, e[I][J] (FDy slot 0), z-face records at (j-1, j)
+    ex += uct_val<R, LIMV>(zby, zey, czj, cz, -1);
+    E[c] = ex;
+    E[n + c] = ey;
+    E[2*n + c] = ez;
+#pragma unroll
+    for (int o = 0; o < 3; o++) { zbx[o] = zbx[o + 1]; zex[o] = zex[o + 1]; zby[o] = zby[o + 1]; zey[o] = zey[o + 1]; }
+#pragma unroll
+    for (int q = 0; q < 3; q++) { cxm[q] = cx[q]; cym[q] = cy[q]; }
+  }
+}
+
+// The same edge EMFs with one thread per edge and every stencil value loaded
+// from memory (2D thread blocks over x-y; blockIdx.z is the plane).
+template <typename R, int LIMV>
+__global__ __launch_bounds__(256)
+void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
+{
+  const int i = blockIdx.x*64 + (threadIdx.x & 63), j = blockIdx.y*4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)g.ncell;
+  const int pl = nx*ny, base = k*pl, c = i + nx*j + base;
+  const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
+  const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
+  int xo[4], yo[4], zo[4];
+#pragma unroll
+  for (int o = 0; o < 4; o++) {
+    xo[o] = wrapm(i + o - 2, nx) + nx*j + base;
+    yo[o] = i + nx*wrapm(j + o - 2, ny) + base;
+    zo[o] = i + nx*j + wrapm(k + o - 2, nz)*pl;
+  }
+  R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3], cxk[3], cyk[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    cx[q] = FDx[(2 + q)*n + c];  cy[q] = FDy[(2 + q)*n + c];  cz[q] = FDz[(2 + q)*n + c];
+    cxj[q] = FDx[(2 + q)*n + yo[1]];
+    cyi[q] = FDy[(2 + q)*n + xo[1]];
+    czi[q] = FDz[(2 + q)*n + xo[1]];
+    czj[q] = FDz[(2 + q)*n + yo[1]];
+    cxk[q] = FDx[(2 + q)*n + zo[1]];
+    cyk[q] = FDy[(2 + q)*n + zo[1]];
+  }
+  R q4[4], e4[4];
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)By[xo[o]]; e4[o] = FDy[1*n + xo[o]]; }
+  R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                      // X1a
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[yo[o]]; e4[o] = FDx[0*n + yo[o]]; }
+  ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                       // X2b
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[xo[o]]; e4[o] = FDz[0*n + xo[o]]; }
+  R ey = uct_val<R, LIMV>(q4, e4, cxk, cx, -1);                      // X1b
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[zo[o]]; e4[o] = FDx[1*n + zo[o]]; }
+  ey += uct_val<R, LIMV>(q4, e4, czi, cz, +1);                       // X3a
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[yo[o]]; e4[o] = FDz[1*n + yo[o]]; }
+  R ex = uct_val<R, LIMV>(q4, e4, cyk, cy, +1);                      // X2a
+#pragma unroll
+  for (int o = 0; o < 4; o++) { q4[o] = (R)By[zo[o]]; e4[o] = FDy[0*n + zo[o]]; }
+  ex += uct_val<R, LIMV>(q4, e4, czj, cz, -1);                       // X3b
   E[c] = ex;
   E[n + c] = ey;
   E[2*n + c] = ez;
@@ -584,10 +711,19 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
                        b.cdt_part, (R)gamma, b.err);
     break;
   }
-  case K_EMF:
-    hipLaunchKernelGGL((k_emf<R, LIM>), dim3((g.nx + 63)/64, (g.ny + 3)/4, g.nz), dim3(64, 4), 0, s,
-                       g, b.Vin, b.FD, b.E);
+  case K_EMF: {
+    if (b.emf_march == 0) {
+      hipLaunchKernelGGL((k_emf_edge<R, LIM>), dim3((g.nx + 63)/64, (g.ny + 3)/4, g.nz), dim3(256), 0, s,
+                         g, b.Vin, b.FD, b.E);
+    } else if (b.emf_march == 4) {
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 3)/4);
+      hipLaunchKernelGGL((k_emf_march<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+    } else {
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 15)/16);
+      hipLaunchKernelGGL((k_emf_march<R, LIM, 16>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+    }
     break;
+  }
   case K_FINISH:
     hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(T), 0, s,
                        g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,

@@ -157,6 +157,22 @@ def hd_riemann(n: int, seed: int):
     return vc
 
 
+def mhd_batch_boxes(seed: int, nbox: int = 64, nmin: int = 32, nmax: int = 512):
+    """Sizes and plasma betas of the batched 2D workload: N_b = floor(nmin *
+    (nmax/nmin)**u_b), u_b ~ U[0,1) (log-uniform in [nmin, nmax)), beta_b ~
+    U[1, 100].  Box b's IC is mhd_turbulence((N_b, N_b, 1), box_seed(seed, b),
+    beta=beta_b)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = rng.uniform(0.0, 1.0, size=nbox)
+    n = np.floor(nmin * (nmax / nmin) ** u).astype(int)
+    beta = rng.uniform(1.0, 100.0, size=nbox)
+    return [int(x) for x in n], [float(x) for x in beta]
+
+
+def box_seed(seed: int, b: int) -> int:
+    return 1000003 * (seed + 1) + b
+
+
 def hd_blast(n, seed: int, gamma: float = 1.4):
     """Seeded HD box (1D, 2D or 3D, n = (nx, ny, nz)) on the unit cube: cell
     noise of 25% on rho and p around 1, random velocities of rms ~0.2, and a

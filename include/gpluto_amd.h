@@ -134,6 +134,42 @@ size_t gpl_device_bytes(const gpl_ctx *ctx);
 
 const char *gpl_strerror(int code);
 
+/* ---------------------------------------------------------------------------
+ * Batched 2D MHD: many independent periodic N x N boxes of uneven sizes in
+ * one device state, advanced together (one launch per phase over all boxes,
+ * one HIP graph per step), each box with its own dt.  No counterpart in the
+ * reference, which runs one domain per process; each box computes exactly
+ * what gpl_create/gpl_run would compute for it alone (FP64: bit for bit).
+ * ------------------------------------------------------------------------- */
+typedef struct gpl_batch gpl_batch;
+
+/* cfg: physics GPL_MHD, dimensions 2, GPL_PERIODIC, GPL_UCT_HLL; cfg's
+ * nx/ny/nz are ignored: box q is nx[q] x ny[q] zones on [xbeg, xend]
+ * (3..8192 each; ny == NULL gives square boxes). */
+int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny, gpl_batch **out);
+int gpl_batch_destroy(gpl_batch *b);
+
+/* Per-box state transfer, as gpl_upload/gpl_download with dimensions 2:
+ * vc holds 8 variables (B3 cell-centred), vs[0..1] the B1/B2 faces. */
+int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const double *const *vs, int layout);
+int gpl_batch_download(gpl_batch *b, int box, double *vc, double *const *vs, double *uc, int layout);
+
+/* nsteps steps of every box with per-box dt on the device, as gpl_run.
+ * first_dt[nbox] restarts every box at time 0 (NULL: continue);
+ * time_out/dt_out receive nbox values each (may be NULL). */
+uint32_t gpl_batch_run(gpl_batch *b, int nsteps, const double *first_dt, double *time_out,
+                       double *dt_out);
+
+/* Launch shape: march = cells per lane of the y sweep (8, 16, 32) and
+ * blocks_per_cu = persistent workgroups per CU (2, 4, 8, or 0 for one
+ * workgroup per work-list group).  march 0 asks for the autotuner, which
+ * (the default) times every shape on the first steps of the next run of at
+ * least 24 steps and keeps the fastest; results do not depend on the shape. */
+int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu);
+int gpl_batch_get_tuning(const gpl_batch *b, int *march, int *blocks_per_cu, double *step_ms);
+double gpl_batch_last_run_ms(const gpl_batch *b);
+size_t gpl_batch_device_bytes(const gpl_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

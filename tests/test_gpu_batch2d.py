@@ -1,0 +1,133 @@
+"""GPU parity of the batched 2D MHD update: every box of a batch of uneven
+sizes computes, bit for bit (FP64), what a standalone 2D run of that box
+computes -- against the oracle (itself pinned bitwise to the reference CPU
+build) and against the reference build's own 2D output fixture.  Launch
+shapes chosen by the autotuner never change a bit; the FP32 variant stays
+within 1e-5 relative L1 of the FP64 batch."""
+import numpy as np
+import pytest
+
+from golden_io import load
+from gpluto_dev_amd import ic
+from gpluto_dev_amd.batch import BatchUpdate
+from gpluto_dev_amd.stage import Definitions, RuntimeSetup
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(sizes, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64"):
+    return BatchUpdate(Definitions(dimensions=2, time_stepping=ts, limiter=lim),
+                       RuntimeSetup(solver=solver, cfl=0.4), sizes, precision=precision)
+
+
+def _ics(sizes, seed):
+    out = []
+    for b, (nx, ny) in enumerate(sizes):
+        beta = 1.0 + 99.0 * ((b * 37) % 100) / 100.0
+        out.append(ic.mhd_turbulence((nx, ny, 1), ic.box_seed(seed, b), beta=beta))
+    return out
+
+
+def _oracle(n, vc, vs, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld"):
+    import oracle
+    o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, solver=solver, limiter=lim,
+                      time_stepping=int(ts[-1]), cfl=0.4)
+    o.set_state(vc, vs)
+    err, t, dt = o.run(steps, first_dt)
+    v, s, u = o.get_state()
+    return err, t, dt, v, s, u
+
+
+SIZES = [(5, 5), (33, 33), (63, 64), (64, 63), (65, 65), (100, 37), (130, 130), (32, 24)]
+
+
+@pytest.mark.parametrize("ts,lim,solver", [("RK2", "MC_LIM", "hlld"),
+                                           ("RK3", "VANLEER_LIM", "hll")])
+def test_every_box_matches_its_standalone_oracle_run(ts, lim, solver):
+    steps, first_dt = 6, 1e-3
+    print(f"case {ts} {lim} {solver} sizes={SIZES}", flush=True)
+    ics = _ics(SIZES, 4)
+    g = _batch(SIZES, ts, lim, solver)
+    for b, (vc, vs) in enumerate(ics):
+        g.upload(b, vc, vs)
+    err, t, dt = g.run(steps, first_dt)
+    assert err == 0
+    for b, (nx, ny) in enumerate(SIZES):
+        vc, vs = ics[b]
+        oerr, ot, odt, ov, osv, ou = _oracle((nx, ny, 1), vc, vs, steps, first_dt, ts, lim, solver)
+        gv, gs, gu = g.download(b)
+        assert oerr == 0
+        assert t[b] == ot and dt[b] == odt, f"box {b} ({nx}x{ny}): time/dt differ"
+        for v in range(8):
+            assert np.array_equal(gv[v], ov[v]), \
+                f"box {b} ({nx}x{ny}) var {v}: max diff {np.max(np.abs(gv[v] - ov[v])):.3e}"
+        assert np.array_equal(gs, osv), f"box {b}: faces differ"
+        for v in (0, 1, 2, 3, 6, 7):
+            assert np.array_equal(gu[v], ou[v]), f"box {b}: Uc var {v} differs"
+    g.close()
+
+
+def test_reference_fixture_box_inside_a_batch():
+    """The reference build's 2D output, reproduced by one box of a batch."""
+    p, a = load("mhd2d_32x24_rk2_mc")
+    nx, ny, _ = p["n"]
+    sizes = [(40, 40), (nx, ny), (70, 33)]
+    ics = _ics(sizes, 9)
+    g = _batch(sizes, p["time_stepping"], p["limiter"], p["solver"])
+    for b, (vc, vs) in enumerate(ics):
+        if b == 1:
+            vc, vs = a["vc_in"], a["vs_in"]
+        g.upload(b, vc, vs)
+    err, t, _ = g.run(p["steps"], p["first_dt"])
+    gv, gs, _ = g.download(1)
+    g.close()
+    assert err == 0
+    assert abs(t[1] - p["time"]) <= 1e-6 * abs(p["time"])
+    assert np.array_equal(gv, a["vc_out"])
+    assert np.array_equal(gs, a["vs_out"])
+
+
+def test_launch_shapes_and_autotuner_give_identical_bits():
+    sizes = [(47, 47), (130, 130), (75, 75), (300, 300), (34, 34), (200, 199)]
+    ics = _ics(sizes, 2)
+    results, tuned = [], None
+    for shape in [None, (8, 2), (32, 0), (16, 4)]:
+        print(f"shape {shape}", flush=True)
+        g = _batch(sizes)
+        if shape is not None:
+            g.set_tuning(*shape)
+        for b, (vc, vs) in enumerate(ics):
+            g.upload(b, vc, vs)
+        err, t, dt = g.run(30, 5e-4)
+        assert err == 0
+        if shape is None:
+            tuned = g.tuning()
+        results.append((t, dt, [g.download(b) for b in range(len(sizes))]))
+        g.close()
+    assert tuned[0] in (8, 16, 32) and tuned[1] in (0, 2, 4, 8) and tuned[2] > 0
+    t0, dt0, s0 = results[0]
+    for t, dt, s in results[1:]:
+        assert np.array_equal(t, t0) and np.array_equal(dt, dt0)
+        for (a1, b1, c1), (a2, b2, c2) in zip(s, s0):
+            assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(c1, c2)
+
+
+def test_fp32_batch_within_1e5_of_fp64():
+    sizes = [(64, 64), (150, 150), (37, 37), (256, 256)]
+    ics = _ics(sizes, 6)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        g = _batch(sizes, precision=prec)
+        for b, (vc, vs) in enumerate(ics):
+            g.upload(b, vc, vs)
+        err, t, _ = g.run(20, 5e-4)
+        assert err == 0
+        out[prec] = [g.download(b) for b in range(len(sizes))]
+        g.close()
+    for b in range(len(sizes)):
+        v64, s64, _ = out["fp64"][b]
+        v32, s32, _ = out["mixed"][b]
+        for v in range(8):
+            l1 = np.sum(np.abs(v32[v] - v64[v])) / np.sum(np.abs(v64[v]))
+            assert l1 <= 1e-5, f"box {b} var {v}: rel L1 {l1:.2e}"
+        assert ic.divb_max(s32) < 1e-10
