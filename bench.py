@@ -161,6 +161,67 @@ def pmc_probe(n, seed, precision):
                       "copy_write_bytes": copy_bytes}), flush=True)
 
 
+def run_batch2d(seed: int, steps: int = 100):
+    """The batched 2D workload: 64 periodic boxes, N log-uniform in [32, 512),
+    beta ~ U[1, 100], PLM(MC) + HLLD + UCT_HLL + RK2, CFL 0.4, per-box dt,
+    FP64 and FP32; zone-updates/s over all boxes (autotuned launch shape)."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.batch import BatchUpdate
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup
+    sizes, beta = ic.mhd_batch_boxes(seed)
+    ics = [ic.mhd_turbulence((n, n, 1), ic.box_seed(seed, b), beta=beta[b])
+           for b, n in enumerate(sizes)]
+    cells = sum(n * n for n in sizes)
+    res = {"boxes": len(sizes), "cells": cells, "steps": steps,
+           "sizes_min_max": [min(sizes), max(sizes)]}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        g = BatchUpdate(Definitions(dimensions=2), RuntimeSetup(solver="hlld", cfl=0.4), sizes,
+                        precision=prec)
+        for b, (vc, vs) in enumerate(ics):
+            g.upload(b, vc, vs)
+        err, _, _ = g.run(steps, 5e-4)          # includes the autotuner's first steps
+        if err:
+            raise RuntimeError(f"batch2d {prec}: error bits {err:#x}")
+        for b, (vc, vs) in enumerate(ics):      # timed run from the same IC, shape fixed
+            g.upload(b, vc, vs)
+        t0 = time.perf_counter()
+        err, _, _ = g.run(steps, 5e-4)
+        wall = time.perf_counter() - t0
+        march, bpc, _ = g.tuning()
+        finals[prec] = [g.download(b)[0] for b in range(len(sizes))]
+        g.close()
+        res[prec] = {"value": cells * steps / wall, "unit": "zone-updates/s",
+                     "ms_per_step": wall / steps * 1e3, "march": march, "blocks_per_cu": bpc}
+    l1 = max(float(np.sum(np.abs(finals["mixed"][b][v] - finals["fp64"][b][v]))
+                   / np.sum(np.abs(finals["fp64"][b][v])))
+             for b in range(len(sizes)) for v in range(8))
+    res["mixed"]["max_l1_rel_vs_fp64"] = l1
+    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    return res
+
+
+def run_hd1d(steps: int = 400):
+    """The 1D HD shock tube (4096 zones, outflow, gamma 1.4, PLM(MC) + HLLC +
+    RK2, CFL 0.8): the reference's CPU-runnable case, timed on the GPU."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    n = 4096
+    vc = ic.hd_riemann(n, 0)
+    g = StageUpdate(Definitions(physics="HD", dimensions=1),
+                    RuntimeSetup(n=(n, 1, 1), solver="hllc", cfl=0.8, gamma=1.4,
+                                 boundary="outflow"))
+    g.upload(vc)
+    g.run(steps, 1e-4)                           # warm-up (graph capture)
+    g.upload(vc)
+    t0 = time.perf_counter()
+    err, t, _ = g.run(steps, 1e-4)
+    wall = time.perf_counter() - t0
+    g.close()
+    return {"value": n * steps / wall, "unit": "zone-updates/s", "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "err": err, "time": t}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +231,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched 2D and 1D HD workloads reported beside the headline")
     ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
                     help="run one step of the workload plus a 1 GiB device copy (k_copy_probe, the "
                          "HBM byte-counter calibration) for rocprofv3 --pmc, then exit")
@@ -226,6 +289,10 @@ def main():
                         "divb_max": ic.divb_max(vsm),
                         "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in sm.items()}}
     out["divb_max_fp64"] = ic.divb_max(vs64)
+    if not a.no_extra:
+        log("bench: batched 2D and 1D HD workloads")
+        out["batch2d"] = run_batch2d(a.seed)
+        out["hd1d"] = run_hd1d()
     if not a.no_cpu:
         log("bench: CPU baseline")
         try:

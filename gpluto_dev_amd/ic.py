@@ -74,6 +74,24 @@ def _flat_modes(c: np.ndarray, dims: int) -> np.ndarray:
     return c
 
 
+def _curl(ax, ay, az, n, dims):
+    """Face field B = curl A of an edge-centred potential on [0,1]^dims with
+    periodic forward differences (the discrete curl of Src/vector_curl.cpp:109);
+    arrays are [k, j, i]."""
+    nx, ny, nz = n
+    idx, idy, idz = float(nx), float(ny), float(nz)  # 1/dx on [0,1]
+    dj = lambda a: np.roll(a, -1, axis=1) - a
+    di = lambda a: np.roll(a, -1, axis=2) - a
+    dk = lambda a: np.roll(a, -1, axis=0) - a
+    bx = dj(az) * idy
+    by = -di(az) * idx
+    bz = di(ay) * idx - dj(ax) * idy
+    if dims == 3:
+        bx = bx - dk(ay) * idz
+        by = by + dk(ax) * idz
+    return bx, by, bz
+
+
 def mhd_turbulence(n, seed: int, gamma: float = 5.0 / 3.0, beta: float = 10.0,
                    mach: float = 0.1, drho: float = 0.1):
     """Random-field MHD box: the 3D/2D ideal-MHD workload.
@@ -113,20 +131,7 @@ def mhd_turbulence(n, seed: int, gamma: float = 5.0 / 3.0, beta: float = 10.0,
     ay = _synth(fields[5][0], n, (0.0, 0.5, zf))
     az = _synth(fields[6][0], n, (0.0, 0.0, zc))
 
-    def curl(ax, ay, az):
-        idx, idy, idz = float(nx), float(ny), float(nz)  # 1/dx on [0,1]
-        # periodic forward differences; arrays are [k, j, i]
-        dj = lambda a: np.roll(a, -1, axis=1) - a
-        di = lambda a: np.roll(a, -1, axis=2) - a
-        dk = lambda a: np.roll(a, -1, axis=0) - a
-        bx = dj(az) * idy
-        by = -di(az) * idx
-        bz = di(ay) * idx - dj(ax) * idy
-        if dims == 3:
-            bx = bx - dk(ay) * idz
-            by = by + dk(ax) * idz
-        return bx, by, bz
-
+    curl = lambda ax, ay, az: _curl(ax, ay, az, n, dims)
     bx, by, bz = curl(ax, ay, az)
     bcx = 0.5 * (bx + np.roll(bx, -1, axis=2))
     bcy = 0.5 * (by + np.roll(by, -1, axis=1))
@@ -140,6 +145,49 @@ def mhd_turbulence(n, seed: int, gamma: float = 5.0 / 3.0, beta: float = 10.0,
     bcy = 0.5 * (by + np.roll(by, -1, axis=1))
     bcz = 0.5 * (bz + np.roll(bz, -1, axis=0)) if dims == 3 else bz
     prs = np.ones_like(rho)
+    vc = np.stack([rho, vel[0], vel[1], vel[2], bcx, bcy, bcz, prs])
+    return np.ascontiguousarray(vc), np.ascontiguousarray(vs)
+
+
+def rmhd_turbulence(n, seed: int, gamma: float = 4.0 / 3.0, lorentz_max: float = 3.0,
+                    sigma: float = 1.0, drho: float = 0.5):
+    """Random-field relativistic MHD box (the 3D RMHD workload's generator).
+
+    rho = exp(drho*G) (log-normal), p/rho = 10**clip(-0.5 + 0.5*G', -2, 1)
+    (so p/rho spans [0.01, 10]), spatial four-velocity u = Gamma v from random
+    modes scaled so that the largest Lorentz factor is ``lorentz_max``, and a
+    face field curl A scaled so that <B^2>/<rho h> = sigma.  G, G' are
+    unit-variance Gaussian random fields with |k|^-2 power on 1 <= |k| <= 8.
+    Returns (vc[8, nz, ny, nx], vs[3, nz, ny, nx])."""
+    nx, ny, nz = n
+    dims = 3 if nz > 1 else 2
+    rng = np.random.Generator(np.random.PCG64(seed))
+    fields = [_flat_modes(_coefficients(rng, -2.0)[0], dims) for _ in range(8)]
+    cen = (0.5, 0.5, 0.5 if dims == 3 else 0.0)
+    unit = lambda f: (lambda g: g / max(np.std(g), 1e-300))(_synth(f, n, cen))
+    rho = np.exp(drho * unit(fields[0]))
+    theta = 10.0 ** np.clip(-0.5 + 0.5 * unit(fields[1]), -2.0, 1.0)
+    prs = rho * theta
+    u4 = [unit(fields[2 + d]) for d in range(3)]
+    umag = np.sqrt(sum(u * u for u in u4))
+    s = np.sqrt(lorentz_max ** 2 - 1.0) / np.max(umag)
+    u4 = [u * s for u in u4]
+    lor = np.sqrt(1.0 + sum(u * u for u in u4))
+    vel = [u / lor for u in u4]
+    zc = 0.5 if dims == 3 else 0.0
+    ax = _synth(fields[5], n, (0.5, 0.0, 0.0))
+    ay = _synth(fields[6], n, (0.0, 0.5, 0.0))
+    az = _synth(fields[7], n, (0.0, 0.0, zc))
+    bx, by, bz = _curl(ax, ay, az, n, dims)
+    cell = lambda bx, by, bz: (0.5 * (bx + np.roll(bx, -1, axis=2)),
+                               0.5 * (by + np.roll(by, -1, axis=1)),
+                               0.5 * (bz + np.roll(bz, -1, axis=0)) if dims == 3 else bz)
+    bcx, bcy, bcz = cell(bx, by, bz)
+    h = 1.0 + gamma / (gamma - 1.0) * theta
+    scale = np.sqrt(sigma * np.mean(rho * h) / np.mean(bcx * bcx + bcy * bcy + bcz * bcz))
+    bx, by, bz = _curl(ax * scale, ay * scale, az * scale, n, dims)
+    bcx, bcy, bcz = cell(bx, by, bz)
+    vs = np.stack([bx, by, bz]) if dims == 3 else np.stack([bx, by])
     vc = np.stack([rho, vel[0], vel[1], vel[2], bcx, bcy, bcz, prs])
     return np.ascontiguousarray(vc), np.ascontiguousarray(vs)
 

@@ -22,12 +22,17 @@
 #define MAX(a,b)  ( (a) >= (b) ? (a) : (b) )
 #define MIN(a,b)  ( (a) <= (b) ? (a) : (b) )
 #define ABS_MIN(a,b)  (fabs(a) < fabs(b) ? (a):(b))
+#define DSIGN(x)      ( (x) >= 0.0 ? (1.0) : (-1.0))
 
 /* ---- error bits (Src/gpluto.hpp:247-262) ------------------------------- */
 #define ERR_NEGATIVE_RHO    (1u<<0)
 #define ERR_NEGATIVE_ENG    (1u<<1)
 #define ERR_NEGATIVE_PRS    (1u<<2)
 #define ERR_CONS2PRIM_FAIL  (1u<<4)
+#define ERR_PRESSURE_FIX    (1u<<5)
+#define ERR_ENG_SOLVE_NO_SOLUTION (1u<<7)
+#define ERR_SUPERLUMINAL_SPEED    (1u<<8)
+#define ERR_TOO_MANY_ITERATIONS   (1u<<9)
 #define ERR_NAN             (1u<<11)
 #define ERR_QUIT            (1u<<15)
 
@@ -64,7 +69,9 @@ struct gplo_state {
 #define VC(s,nv,i,j,k)  (s)->Vc[(size_t)(nv)*(s)->ncell + IC(s,i,j,k)]
 #define UC(s,nv,i,j,k)  (s)->Uc[(size_t)(nv)*(s)->ncell + IC(s,i,j,k)]
 
-static int is_mhd(const gplo_state *s) { return s->c.phys == GPLO_MHD; }
+/* staggered field + CT (MHD and RMHD) */
+static int is_mhd(const gplo_state *s) { return s->c.phys == GPLO_MHD || s->c.phys == GPLO_RMHD; }
+static int is_rmhd(const gplo_state *s) { return s->c.phys == GPLO_RMHD; }
 
 /* ======================================================================
  * Pointwise physics
@@ -414,6 +421,382 @@ static uint32_t hll(const double *vl, const double *vr, int n, double gamma, int
 }
 
 /* ======================================================================
+ * RMHD (ideal EOS, RMHD_REDUCED_ENERGY = YES, RMHD_FAST_EIGENVALUES = NO,
+ * no PCP modules): variables rho, v1..3, B1..3, prs; conserved D, m1..3,
+ * B1..3, E - D.
+ * ====================================================================== */
+
+/* Enthalpy, Src/eos.hpp:13-51 (relativistic, IDEAL) */
+static double rmhd_enthalpy(const double *v, double gamma)
+{
+  double gmmr  = gamma/(gamma - 1.0);
+  double theta = v[7]/v[RHO];
+  return 1.0 + gmmr*theta;
+}
+
+/* SoundSpeed2, Src/eos.hpp:84-125 (relativistic, IDEAL): a2 and h */
+static void rmhd_sound_speed2(const double *v, double gamma, double *a2, double *h)
+{
+  double hh = rmhd_enthalpy(v, gamma);
+  double theta = v[7]/v[RHO];
+  *h  = hh;
+  *a2 = gamma*theta/hh;
+}
+
+/* PrimToCons, Src/RMHD/rmhd.hpp:535-610 (B x (v x B) momentum form) */
+static void rmhd_prim_to_cons(const double *v, double *u, double gamma)
+{
+  double gmmr = gamma/(gamma - 1.0);
+  double h = rmhd_enthalpy(v, gamma);
+  double vx1 = v[1], vx2 = v[2], vx3 = v[3];
+  double vel2 = vx1*vx1 + vx2*vx2 + vx3*vx3;
+  double g2 = 1.0/(1.0 - vel2);
+  double g  = sqrt(g2);
+  double ex = vx2*v[6] - vx3*v[5];
+  double ey = vx3*v[4] - vx1*v[6];
+  double ez = vx1*v[5] - vx2*v[4];
+  double e2 = ex*ex + ey*ey + ez*ez;
+  double Bmag2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  double wt = v[RHO]*h*g2;
+  u[RHO] = g*v[RHO];
+  u[1] = wt*vx1 + v[5]*ez - v[6]*ey;
+  u[2] = wt*vx2 + v[6]*ex - v[4]*ez;
+  u[3] = wt*vx3 + v[4]*ey - v[5]*ex;
+  u[4] = v[4];
+  u[5] = v[5];
+  u[6] = v[6];
+  u[7] =   v[7]*(g2*gmmr - 1.0) + u[RHO]*g2*vel2/(g + 1.0)
+         + 0.5*(Bmag2 + e2);
+}
+
+/* Flux<dir>, Src/RMHD/rmhd.hpp:315-366 (reduced energy) */
+static void rmhd_flux(const double *v, const double *u, double h, int n, double *flux, double *pt)
+{
+  const int VXn = 1 + n, VXt = 1 + (n + 1)%3, VXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3;
+  double g  = u[RHO]/v[RHO];
+  double g2 = g*g;
+  double g_2 = 1.0/g2;
+  double Bmag2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  double vB = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  double b[3];
+  b[0] = g*(v[4]*g_2 + vB*v[1]);
+  b[1] = g*(v[5]*g_2 + vB*v[2]);
+  b[2] = g*(v[6]*g_2 + vB*v[3]);
+  double b2 = Bmag2*g_2 + vB*vB;
+  *pt = v[7] + 0.5*b2;
+  double wt = v[RHO]*h + b2;
+  double wtg2 = wt*g2;
+  flux[RHO] = u[RHO]*v[VXn];
+  flux[1] = wtg2*v[1]*v[VXn] - b[0]*b[n];
+  flux[2] = wtg2*v[2]*v[VXn] - b[1]*b[n];
+  flux[3] = wtg2*v[3]*v[VXn] - b[2]*b[n];
+  flux[BXn] = 0.0;
+  flux[BXt] = v[VXn]*v[BXt] - v[BXn]*v[VXt];
+  flux[BXb] = v[VXn]*v[BXb] - v[BXn]*v[VXb];
+  flux[7] = u[1 + n];
+  flux[7] -= flux[RHO];
+  flux[1 + n] += *pt;
+}
+
+/* SortArray, Src/RMHD/rmhd.hpp:370-387 (insertion sort) */
+static void sort_array(double *z, int n)
+{
+  for (int j = 1; j < n; j++) {
+    double f = z[j];
+    int i = j - 1;
+    while (i >= 0 && z[i] > f) { z[i+1] = z[i]; i--; }
+    z[i+1] = f;
+  }
+}
+
+/* CubicSolve, Src/RMHD/rmhd.hpp:390-452 (three real roots, trigonometric form) */
+static void cubic_solve(double b, double c, double d, double z[3])
+{
+  const double one_3 = 1.0/3.0, one_27 = 1.0/27.0, sqrt3 = sqrt(3.0);
+  double b2 = b*b;
+  double Q  = b2*one_3 - c*(1.0 - 1.e-16);
+  double R  = b*(2.0*b2 - 9.0*c)*one_27 + d;
+  Q = MAX(Q, 0.0);
+  double sQ  = sqrt(Q)/sqrt3;
+  double arg = -1.5*R/(Q*sQ);
+  arg = MAX(-1.0, arg);
+  arg = MIN( 1.0, arg);
+  double theta = acos(arg)*one_3;
+  double cs = cos(theta);
+  double sn = sqrt3*sin(theta);
+  double p  = -b*one_3;
+  z[0] = -sQ*(cs + sn) + p;
+  z[1] = -sQ*(cs - sn) + p;
+  z[2] =  2.0*sQ*cs    + p;
+}
+
+/* QuarticSolve, Src/RMHD/rmhd.hpp:455-505 (resolvent cubic) */
+static int quartic_solve(double b, double c, double d, double e, double *z)
+{
+  double b2 = b*b;
+  double p = c - b2*0.375;
+  double q = d + b2*b*0.125 - b*c*0.5;
+  double r = e - 3.0*b2*b2/256.0 + b2*c/16.0 - 0.25*b*d;
+  double a2 = 2.0*p;
+  double a1 = p*p - 4.0*r;
+  double a0 = -q*q;
+  double u[3];
+  cubic_solve(a2, a1, a0, u);
+  u[0] = MAX(u[0],0.0);
+  u[1] = MAX(u[1],0.0);
+  u[2] = MAX(u[2],0.0);
+  if (u[0] != u[0] || u[1] != u[1] || u[2] != u[2]) return 1;
+  double sq  = -0.5*DSIGN(q);
+  double sz1 = sq*sqrt(u[0]);
+  double sz2 = sq*sqrt(u[1]);
+  double sz3 = sq*sqrt(u[2]);
+  z[0] = -0.25*b + sz1 + sz2 + sz3;
+  z[1] = -0.25*b + sz1 - sz2 - sz3;
+  z[2] = -0.25*b - sz1 + sz2 - sz3;
+  z[3] = -0.25*b - sz1 - sz2 + sz3;
+  sort_array(z, 4);
+  return 0;
+}
+
+/* MaxSignalSpeed<dir>, Src/RMHD/rmhd.hpp:631-752 (exact fast speeds) */
+static void rmhd_max_signal_speed(const double *v, double cs2, double h, int n,
+                                  double *cmin, double *cmax)
+{
+  const int VXn = 1 + n, BXn = 4 + n;
+  double lfp, lfm;
+  double vB  = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  double u02 = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  double Bm2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  if (u02 < 1.e-14) {
+    /* zero velocity: biquadratic */
+    double w_1  = 1.0/(v[RHO]*h + Bm2);
+    double eps2 = cs2 + Bm2*w_1*(1.0 - cs2);
+    double a0   = cs2*v[BXn]*v[BXn]*w_1;
+    double a1   = - a0 - eps2;
+    double del  = a1*a1 - 4.0*a0;
+    del = MAX(del, 0.0);
+    del = sqrt(del);
+    lfp =  sqrt(0.5*(-a1 + del));
+    lfm = -lfp;
+  } else {
+    double vB2 = vB*vB;
+    u02 = 1.0/(1.0 - u02);
+    double b2  = Bm2/u02 + vB2;
+    double w_1 = 1.0/(v[RHO]*h + b2);
+    double vx  = v[VXn];
+    double vx2 = vx*vx;
+    if (fabs(v[BXn]) < 1.e-14) {
+      /* zero normal field: quadratic */
+      double w     = v[RHO]*h;
+      double scrh1 = b2 + cs2*(w - vB2);
+      double scrh2 = w*(1.0 - cs2)*u02;
+      double del   = 4.0*scrh1*(scrh2*(1.0 - vx2) + scrh1);
+      double a2 = scrh1 + scrh2;
+      double a1 = -2.0*vx*scrh2;
+      lfp = 0.5*(-a1 + sqrt(del))/a2;
+      lfm = 0.5*(-a1 - sqrt(del))/a2;
+    } else {
+      /* general case: quartic */
+      double scrh1 = v[BXn]/u02 + vB*vx;
+      double scrh2 = scrh1*scrh1;
+      double a2_w       = cs2*w_1;
+      double eps2       = (cs2*v[RHO]*h + b2)*w_1;
+      double one_m_eps2 = u02*v[RHO]*h*(1.0 - cs2)*w_1;
+      double scrh = 2.0*(a2_w*vB*scrh1 - eps2*vx);
+      double a4 = one_m_eps2 - a2_w*vB2 + eps2;
+      double a3 = - 4.0*vx*one_m_eps2 + scrh;
+      double a2 =   6.0*vx2*one_m_eps2 + a2_w*(vB2 - scrh2) + eps2*(vx2 - 1.0);
+      double a1 = - 4.0*vx*vx2*one_m_eps2 - scrh;
+      double a0 = vx2*vx2*one_m_eps2 + a2_w*scrh2 - eps2*vx2;
+      scrh = 1.0/a4;
+      a3 *= scrh;
+      a2 *= scrh;
+      a1 *= scrh;
+      a0 *= scrh;
+      double z[4] = {NAN, NAN, NAN, NAN};
+      (void)quartic_solve(a3, a2, a1, a0, z);
+      lfm = z[0];
+      lfp = z[3];
+    }
+  }
+  *cmax = lfp;
+  *cmin = lfm;
+}
+
+/* HLL_Solver, Src/RMHD/hll.cpp:150-230 with HLL_Speed (Src/RMHD/rmhd.hpp:818-835,
+   Davis): bL = MIN(Sl,0),
+   bR = MAX(0,Sr), one averaged flux for every wave configuration */
+static uint32_t rmhd_hll(const double *vl, const double *vr, int n, double gamma,
+                         double *flux, double *pSl, double *pSr)
+{
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], pl, pr;
+  double a2L, a2R, hL, hR, sl_min, sl_max, sr_min, sr_max;
+  rmhd_prim_to_cons(vl, ul, gamma);
+  rmhd_prim_to_cons(vr, ur, gamma);
+  rmhd_sound_speed2(vl, gamma, &a2L, &hL);
+  rmhd_sound_speed2(vr, gamma, &a2R, &hR);
+  rmhd_max_signal_speed(vl, a2L, hL, n, &sl_min, &sl_max);
+  rmhd_max_signal_speed(vr, a2R, hR, n, &sr_min, &sr_max);
+  double Sl = MIN(sl_min, sr_min);
+  double Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+  double bL = MIN(Sl, 0.0);
+  double bR = MAX(0.0, Sr);
+  rmhd_flux(vl, ul, hL, n, fl, &pl);
+  rmhd_flux(vr, ur, hR, n, fr, &pr);
+  double scrh = 1.0/(bR - bL);
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) {
+    flux[nv]  = bL*bR*(ur[nv] - ul[nv]) + bR*fl[nv] - bL*fr[nv];
+    flux[nv] *= scrh;
+  }
+  for (int nv = 0; nv < 8; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
+/* RMHD_EnergySolve, Src/RMHD/rmhd_energy_solve.cpp:35-209 (IDEAL, reduced
+   energy, Newton-Raphson on W' = W - D, tolerance 1e-11, at most 50 passes) */
+static uint32_t rmhd_energy_solve(double *u, double *v, double gamma)
+{
+  const double acc = 1.e-11;
+  int k, done;
+  double Y1, Y2, scrh, W, W2, S2_W2, fW, dfW, dW, dv2_dW, chi, p, lor = 1.0, lor2;
+  double dp, dp_drho, dp_dchi, dchi_dW, drho_dW, one_m_v2, vel2, Wp;
+  double D  = u[RHO];
+  double E  = u[7];
+  double S  = u[1]*u[4] + u[2]*u[5] + u[3]*u[6];
+  double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  double B2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+  double S2 = S*S;
+  if (E < 0.0) return ERR_NEGATIVE_ENG;
+  /* initial guess: positive root of Eq. (A27) */
+  Y1 = -4.0*(E + D - B2);
+  Y2 = m2 - 2.0*(E + D)*B2 + B2*B2;
+  chi = Y1*Y1 - 12.0*Y2;
+  chi = MAX(0.0,chi);
+  W = ( - Y1 + sqrt(chi))/6.0;
+  W = MAX(D, W);
+  Wp = W - D;
+  done = 0; p = -1.0;
+  for (k = 1; k < 50; k++) {
+    W = Wp + D;
+    W2    = W*W;
+    S2_W2 = S2/W2;
+    Y1    = 1.0/(W + B2);
+    Y2    = Y1*Y1;
+    vel2 = S2_W2*Y1*(Y1*W + 1.0) + m2*Y2;
+    one_m_v2 = 1.0 - vel2;
+    lor2 = 1.0/one_m_v2;
+    lor  = sqrt(lor2);
+    if (vel2 > 1.0) return ERR_ENG_SOLVE_NO_SOLUTION;
+    chi = Wp/lor2 - D*vel2/(lor + 1.0);
+    dv2_dW  = -2.0*Y2*(3.0*S2_W2 + Y1*(S2_W2*B2*B2/W + m2));
+    dchi_dW =  one_m_v2 - 0.5*lor*(D + 2.0*chi*lor)*dv2_dW;
+    drho_dW = -0.5*D*lor*dv2_dW;
+    dp_dchi = (gamma - 1.0)/gamma;
+    dp_drho = 0.0;
+    p       = chi*dp_dchi;
+    if (done) break;
+    dp  = dp_dchi*dchi_dW + dp_drho*drho_dW;
+    fW  = Wp + 0.5*(B2 + (B2*m2 - S2)*Y2) - (E + p);
+    dfW = 1.0 - dp - (B2*m2 - S2)*Y2*Y1;
+    dW  = fW/dfW;
+    Wp -= dW;
+    if (fabs(dW) < acc*Wp || fabs(fW) < acc) done = 1;
+  }
+  if (p < 0.0)  return ERR_NEGATIVE_PRS;
+  if (k == 50)  return ERR_TOO_MANY_ITERATIONS;
+  if (isnan(p)) return ERR_NAN;
+  W = Wp + D;
+  v[RHO] = u[RHO]/lor;
+  v[7] = p;
+  Y1   = 1.0/(W + B2);
+  scrh = S/W;
+  v[1] = Y1*(u[1] + scrh*u[4]);
+  v[2] = Y1*(u[2] + scrh*u[5]);
+  v[3] = Y1*(u[3] + scrh*u[6]);
+  return 0;
+}
+
+/* RMHD_PressureFix, Src/RMHD/rmhd_pressure_fix.cpp:22-179: p = small_prs,
+   bisection on v^2 of Eq. (A3), then E recomputed */
+static uint32_t rmhd_pressure_fix(double *u, double *v, double gamma, double small_prs)
+{
+  int k;
+  double v2, f, W, W2, S2_W2, fmin, v2min, v2max, lor, plor, scrh, w_1;
+  double D  = u[RHO];
+  double S  = u[1]*u[4] + u[2]*u[5] + u[3]*u[6];
+  double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  double B2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+  double S2 = S*S;
+  double p = small_prs;
+  v2min = v2 = 0.0;
+  lor  = 1.0/sqrt(1.0 - v2);
+  plor = p*lor;
+  W = (D + plor*gamma/(gamma - 1.0))*lor;
+  W2  = W*W;
+  f  = (S2*(2.0*W + B2) + m2*W2)/((W + B2)*(W + B2)*W2);
+  f -= v2;
+  fmin = f;
+  v2max = v2 = 1.0-1.e-9;
+  lor  = 1.0/sqrt(1.0 - v2);
+  plor = p*lor;
+  W = (D + plor*gamma/(gamma - 1.0))*lor;
+  W2  = W*W;
+  f  = (S2*(2.0*W + B2) + m2*W2)/((W + B2)*(W + B2)*W2);
+  f -= v2;
+  for (k = 1; k < 50; k++) {
+    v2 = 0.5*(v2min + v2max);
+    lor  = 1.0/sqrt(1.0 - v2);
+    plor = p*lor;
+    W = (D + plor*gamma/(gamma - 1.0))*lor;
+    W2  = W*W;
+    f  = (S2*(2.0*W + B2) + m2*W2)/((W + B2)*(W + B2)*W2);
+    f -= v2;
+    if (f*fmin > 0.0) { v2min = v2; fmin = f; }
+    else              { v2max = v2; }
+    if (fabs(f) < 1.e-9) break;
+  }
+  if (v2 >= 1.0 && k == 50) return ERR_PRESSURE_FIX;
+  v[RHO] = D/lor;
+  v[7] = p;
+  w_1  = 1.0/(W + B2);
+  scrh = S/W;
+  v[1] = w_1*(u[1] + scrh*u[4]);
+  v[2] = w_1*(u[2] + scrh*u[5]);
+  v[3] = w_1*(u[3] + scrh*u[6]);
+  S2_W2 = S2/W2;
+  u[7] = W - D - p + 0.5*(1.0 + v2)*B2 - 0.5*S2_W2;
+  return 0;
+}
+
+/* ConsToPrim, Src/RMHD/mappers.cpp:33-157 (energy solve, pressure fix on
+   failure, subluminal check) */
+static uint32_t rmhd_cons_to_prim(double *u, double *v, const gplo_config *c)
+{
+  int fix = 0;
+  uint32_t errm = 0, err;
+  v[4] = u[4];
+  v[5] = u[5];
+  v[6] = u[6];
+  if (u[RHO] < 0.0) { u[RHO] = c->small_rho; errm |= ERR_NEGATIVE_RHO; }
+  if (u[7] < 0.0)   { fix = 1; errm |= ERR_NEGATIVE_ENG; }
+  if (!fix) {
+    err = rmhd_energy_solve(u, v, c->gamma);
+    if (err) fix = 1;
+    errm |= err;
+  }
+  if (fix) {
+    err = rmhd_pressure_fix(u, v, c->gamma, c->small_prs);
+    if (err != 0) { errm |= err; errm |= ERR_CONS2PRIM_FAIL; }
+  }
+  double scrh = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  if (scrh >= 1.0) { errm |= ERR_SUPERLUMINAL_SPEED; errm |= ERR_CONS2PRIM_FAIL; }
+  return errm;
+}
+
+/* ======================================================================
  * PLM slope, Src/States/plm_states.cpp:121-138 with SET_MC_LIMITER /
  * SET_VL_LIMITER on a uniform Cartesian grid (Src/plm_coeffs.hpp:92-100)
  * ====================================================================== */
@@ -538,6 +921,18 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
         vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv] + 0.5*dv;
         vR[i*NVMAX + nv]     = v[i*NVMAX + nv] - 0.5*dv;
       }
+    if (is_rmhd(s)) {
+      /* VelocityLimiter, Src/vel_limiter.hpp:14-31 (plm_states.cpp:158-161):
+         a cell whose reconstructed velocity is superluminal on either side
+         is flattened */
+      for (i = ib; i <= ie; i++) {
+        const double *a = vR + i*NVMAX, *b = vL + (i+1)*NVMAX;
+        double v2m = a[1]*a[1] + a[2]*a[2] + a[3]*a[3];
+        double v2p = b[1]*b[1] + b[2]*b[2] + b[3]*b[3];
+        if (v2m >= 1.0 || v2p >= 1.0)
+          for (nv = 0; nv < nvar; nv++) vR[i*NVMAX + nv] = vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv];
+      }
+    }
     if (mhd) {
       for (i = ib; i <= ie; i++) {
         vL[(i+1)*NVMAX + BX1 + dir] = s->pBn[i+1];
@@ -548,7 +943,9 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
     for (i = s->beg[dir]; i <= s->end[dir] + 1; i++) {
       double Sl, Sr;
       const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
-      if (mhd) {
+      if (is_rmhd(s)) {
+        err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+      } else if (mhd) {
         if (s->c.solver == GPLO_HLLD) err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
         else err |= hll(vl, vr, dir, s->c.gamma, 1, flux + i*NVMAX, &Sl, &Sr);
       } else {
@@ -579,7 +976,9 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
       size_t id = IC(s, p[0], p[1], p[2]);
       for (nv = 0; nv < nvar; nv++) {
         double rhs = -dtdx*(flux[(i+1)*NVMAX + nv] - flux[i*NVMAX + nv]);
-        rhs += 0.0;   /* src[] = 0, rhs.cpp:510-513 */
+        /* MHD adds src[] = 0 (Src/MHD/rhs.cpp:519); RMHD with CT has no
+           source block (Src/RMHD/rhs.cpp:14-21,400-422) */
+        if (!is_rmhd(s)) rhs += 0.0;
         /* dUc = rhs (x); dUc += rhs (y, z): Src/update_stage.cpp:173-184,272-284,369-380 */
         if (dir == 0) s->Rc[(size_t)nv*s->ncell + id] = rhs;
         else          s->Rc[(size_t)nv*s->ncell + id] += rhs;
@@ -739,6 +1138,15 @@ static uint32_t cons_to_prim_3d(gplo_state *s)
       u[nv] = s->Uc[(size_t)nv*s->ncell + id];
       if (isnan(u[nv])) err |= ERR_NAN | ERR_QUIT;
     }
+    if (is_rmhd(s)) {
+      uint32_t le = rmhd_cons_to_prim(u, v, &s->c);
+      err |= le;
+      if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
+      if (le) for (int nv = 0; nv < s->nv; nv++) s->Uc[(size_t)nv*s->ncell + id] = u[nv];
+      for (int nv = 0; nv < s->nv; nv++) s->Vc[(size_t)nv*s->ncell + id] = v[nv];
+      err_all |= err;
+      continue;
+    }
     uint32_t le = 0;
     if (u[RHO] < 0.0) { u[RHO] = s->c.small_rho; le |= ERR_NEGATIVE_RHO | ERR_CONS2PRIM_FAIL; }
     v[RHO] = u[RHO];
@@ -862,7 +1270,7 @@ gplo_state *gplo_create(const gplo_config *cfg)
   s->c = *cfg;
   if (s->c.small_rho == 0.0) s->c.small_rho = 1.e-12;   /* Src/structs.hpp:407-408 */
   if (s->c.small_prs == 0.0) s->c.small_prs = 1.e-12;
-  s->nv = cfg->phys == GPLO_MHD ? 8 : 5;
+  s->nv = cfg->phys == GPLO_HD ? 5 : 8;
   int nint[3] = {cfg->nx, cfg->ny, cfg->nz};
   for (int d = 0; d < 3; d++) {
     if (d < cfg->dims) { s->nt[d] = nint[d] + 2*NG; s->beg[d] = NG; s->end[d] = NG + nint[d] - 1; }
@@ -947,7 +1355,9 @@ int gplo_set_state(gplo_state *s, const double *vc, const double *vs)
   for (int i = s->beg[0]; i <= s->end[0]; i++) {
     double v[NVMAX], u[NVMAX];
     for (int nv = 0; nv < s->nv; nv++) v[nv] = VC(s,nv,i,j,k);
-    if (is_mhd(s)) mhd_prim_to_cons(v, u, s->c.gamma); else hd_prim_to_cons(v, u, s->c.gamma);
+    if (is_rmhd(s))     rmhd_prim_to_cons(v, u, s->c.gamma);
+    else if (is_mhd(s)) mhd_prim_to_cons(v, u, s->c.gamma);
+    else                hd_prim_to_cons(v, u, s->c.gamma);
     for (int nv = 0; nv < s->nv; nv++) UC(s,nv,i,j,k) = u[nv];
   }
   return 0;

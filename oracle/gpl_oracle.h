@@ -18,13 +18,13 @@
 extern "C" {
 #endif
 
-enum { GPLO_HD = 1, GPLO_MHD = 2 };
+enum { GPLO_HD = 1, GPLO_MHD = 2, GPLO_RMHD = 3 };
 enum { GPLO_HLL = 1, GPLO_HLLC = 2, GPLO_HLLD = 3 };
 enum { GPLO_VANLEER_LIM = 26, GPLO_MC_LIM = 27 };    /* Src/gpluto.hpp:330-336 */
 enum { GPLO_UCT_HLL = 1 };
 
 typedef struct gplo_config {
-  int    phys;            /* GPLO_HD or GPLO_MHD */
+  int    phys;            /* GPLO_HD, GPLO_MHD or GPLO_RMHD */
   int    dims;            /* 1, 2 or 3 */
   int    nx, ny, nz;      /* active zones (1 for absent directions) */
   int    periodic;        /* 1 periodic on every side, 0 outflow */

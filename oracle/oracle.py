@@ -14,7 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpl_oracle.so")
 
-HD, MHD = 1, 2
+HD, MHD, RMHD = 1, 2, 3
+PHYS = {"hd": HD, "mhd": MHD, "rmhd": RMHD}
 HLL, HLLC, HLLD = 1, 2, 3
 VANLEER_LIM, MC_LIM = 26, 27
 SOLVERS = {"hll": HLL, "hllc": HLLC, "hlld": HLLD}
@@ -66,8 +67,8 @@ class Oracle:
         nx, ny, nz = n
         dims = 3 if nz > 1 else (2 if ny > 1 else 1)
         self.n, self.dims = (nx, ny, nz), dims
-        self.phys = MHD if phys.lower() == "mhd" else HD
-        self.nvar = 8 if self.phys == MHD else 5
+        self.phys = PHYS[phys.lower()]
+        self.nvar = 5 if self.phys == HD else 8
         cfg = Config(self.phys, dims, nx, ny, nz, 1 if periodic else 0, time_stepping,
                      LIMITERS[limiter], SOLVERS[solver], gamma, cfl, cfl_max_var, 1e-12, 1e-12)
         self._h = lib().gplo_create(C.byref(cfg))
@@ -87,7 +88,7 @@ class Oracle:
         nx, ny, nz = self.n
         vc = np.empty((self.nvar, nz, ny, nx))
         uc = np.empty((self.nvar, nz, ny, nx))
-        vs = np.empty((self.dims, nz, ny, nx)) if self.phys == MHD else None
+        vs = np.empty((self.dims, nz, ny, nx)) if self.phys != HD else None
         lib().gplo_get_state(self._h, _ptr(vc), _ptr(vs), _ptr(uc))
         return vc, vs, uc
 

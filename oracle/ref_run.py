@@ -28,7 +28,7 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 
 def ref_binary(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL") -> str:
-    name = f"{cfg}_{ts}_{lim}" + (f"_{emf}" if cfg.startswith("mhd") else "")
+    name = f"{cfg}_{ts}_{lim}" + (f"_{emf}" if "mhd" in cfg else "")
     return os.path.join(REF_DIR, name, "pluto")
 
 

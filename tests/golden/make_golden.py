@@ -34,12 +34,26 @@ CASES = [
     # the CPU-runnable headline case: 1D Euler shock tube, 4096 cells, 400 steps
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),
+    # relativistic MHD (HLL with exact quartic speeds, energy-solve con2prim);
+    # the first case drives cells into the pressure-fix branch
+    ("rmhd3d_16_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (16, 16, 16), 3, 5, 1e-3, 0.4, "hll", True, 4 / 3),
+    ("rmhd3d_12x14x10_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (12, 14, 10), 4, 6, 1e-3, 0.4, "hll", True, 4 / 3),
+    ("rmhd3d_10x12x14_rk3_vl_hll", "rmhd3d", "RK3", "VANLEER_LIM", (10, 12, 14), 6, 4, 2e-3, 0.4, "hll", True, 4 / 3),
 ]
+
+# relativistic IC parameters per case: (lorentz_max, sigma)
+RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
+           "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0)}
 
 
 def make(case):
     name, cfg, ts, lim, n, seed, steps, first_dt, cfl, solver, periodic, gamma = case
-    if cfg.startswith("mhd"):
+    extra = {}
+    if cfg.startswith("rmhd"):
+        lmax, sigma = RMHD_IC[name]
+        vc, vs = ic.rmhd_turbulence(n, seed, gamma=gamma, lorentz_max=lmax, sigma=sigma)
+        extra = dict(lorentz_max=lmax, sigma=sigma)
+    elif cfg.startswith("mhd"):
         vc, vs = ic.mhd_turbulence(n, seed, gamma=gamma)
     else:
         vc, vs = ic.hd_riemann(n[0], seed), None
@@ -48,7 +62,7 @@ def make(case):
                               first_dt=first_dt, solver=solver, periodic=periodic)
     params = dict(name=name, cfg=cfg, time_stepping=ts, limiter=lim, n=list(n), seed=seed,
                   steps=steps, first_dt=first_dt, cfl=cfl, cfl_max_var=1.1, solver=solver,
-                  periodic=periodic, gamma=gamma, time=r.time)
+                  periodic=periodic, gamma=gamma, time=r.time, err_log="cells reported errors" in r.log, **extra)
     arrays = dict(vc_in=vc, vc_out=r.vc)
     if vs is not None:
         arrays.update(vs_in=vs, vs_out=r.vs)
@@ -58,5 +72,7 @@ def make(case):
 
 
 if __name__ == "__main__":
+    only = sys.argv[1:]
     for c in CASES:
-        make(c)
+        if not only or c[0] in only:
+            make(c)
