@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpluto_amd.so")
 
-GPL_HD, GPL_MHD = 1, 2
+GPL_HD, GPL_MHD, GPL_RMHD = 1, 2, 3
 GPL_SOLVER_HLL, GPL_SOLVER_HLLC, GPL_SOLVER_HLLD = 1, 2, 3
 GPL_VANLEER_LIM, GPL_MC_LIM = 26, 27
 GPL_UCT_HLL = 1
@@ -28,7 +28,9 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_strerror",
            "gpl_batch_create", "gpl_batch_destroy", "gpl_batch_upload", "gpl_batch_download",
            "gpl_batch_run", "gpl_batch_set_tuning", "gpl_batch_get_tuning",
-           "gpl_batch_last_run_ms", "gpl_batch_device_bytes")
+           "gpl_batch_last_run_ms", "gpl_batch_device_bytes",
+           "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
+           "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network")
 
 
 class GplConfig(C.Structure):
@@ -101,6 +103,22 @@ def load() -> C.CDLL:
     L.gpl_batch_last_run_ms.restype = C.c_double
     L.gpl_batch_device_bytes.argtypes = [vp]
     L.gpl_batch_device_bytes.restype = C.c_size_t
+    fp = C.POINTER(C.c_float)
+    L.gpl_c2p_net_size.argtypes = []
+    L.gpl_c2p_net_size.restype = ip
+    L.gpl_c2p_net_make_data.argtypes = [ip, C.c_uint64, ip, C.c_double, dp, fp, fp]
+    L.gpl_c2p_net_make_data.restype = ip
+    L.gpl_c2p_net_train.argtypes = [ip, C.c_uint64, ip, ip, ip, C.c_float, C.c_double, fp, fp]
+    L.gpl_c2p_net_train.restype = ip
+    L.gpl_c2p_net_grad.argtypes = [ip, fp, fp, fp, ip, fp, fp]
+    L.gpl_c2p_net_grad.restype = ip
+    L.gpl_c2p_net_infer.argtypes = [ip, fp, fp, ip, fp]
+    L.gpl_c2p_net_infer.restype = ip
+    L.gpl_c2p_solve.argtypes = [ip, fp, dp, ip, C.c_double, dp, C.POINTER(C.c_int),
+                                C.POINTER(C.c_uint32), dp]
+    L.gpl_c2p_solve.restype = ip
+    L.gpl_set_c2p_network.argtypes = [vp, fp]
+    L.gpl_set_c2p_network.restype = ip
     _lib = L
     return L
 

@@ -34,6 +34,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("batch2d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
            ("batch2d_common.hip", []),
            ("engine_batch2d.cpp", []),
+           ("c2p_net.hip", []),
            ("engine_mhd3d.cpp", []),
            ("engine_hd.cpp", []),
            ("gpl_api.cpp", [])]

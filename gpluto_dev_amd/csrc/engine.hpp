@@ -40,6 +40,8 @@ public:
   virtual int graph_period() const = 0;
   // Identifies the buffer holding the current state (graph validity).
   virtual const void *state_key() const = 0;
+  // RMHD: learned con2prim guess weights (nullptr: the reference's guess).
+  virtual int set_c2p_network(const float *) { return GPL_EUNSUPPORTED; }
   size_t bytes = 0;   // device memory held
 };
 

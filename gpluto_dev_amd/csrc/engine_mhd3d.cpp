@@ -1,7 +1,9 @@
-// 3D ideal-MHD engine: device state and AdvanceStep orchestration
-// (Src/rk_step.cpp:88-720) over the kernels of mhd3d_kernels.hpp.
+// 3D MHD engine (ideal MHD, or relativistic MHD with the same CT machinery):
+// device state and AdvanceStep orchestration (Src/rk_step.cpp:88-720) over
+// the kernels of mhd3d_kernels.hpp.
 #include "engine.hpp"
 #include "mhd3d.hpp"
+#include "c2p_net.hpp"
 #include "physics.hpp"
 
 #include <cstdio>
@@ -44,6 +46,7 @@ public:
   EngineEnv env;
   Box3 g;
   bool fp64 = true;
+  bool rmhd = false;
   size_t n = 0;
   void *V = nullptr;          // [8][n] R
   double *U = nullptr, *U0 = nullptr;
@@ -59,10 +62,12 @@ public:
   double *cdt_part = nullptr;
   int nparts = 0;
   double *cur_vs = nullptr;   // VsA or VsB: faces of the current state
+  float *net = nullptr;       // RMHD learned con2prim guess, when enabled
+  bool net_on = false;
 
   ~Mhd3dEngine() override
   {
-    void *ptrs[] = {V, U, U0, VsA, VsB, Bs0, F, CM, FD, E, dU, Cdt, cdt_part};
+    void *ptrs[] = {V, U, U0, VsA, VsB, Bs0, F, CM, FD, E, dU, Cdt, cdt_part, net};
     for (void *p : ptrs) if (p) (void)hipFree(p);
   }
 
@@ -70,6 +75,7 @@ public:
   {
     cfg = c; env = e;
     fp64 = c.precision == GPL_FP64;
+    rmhd = c.physics == GPL_RMHD;
     n = (size_t)c.nx*c.ny*c.nz;
     g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
     g.ncell = (int)n;
@@ -131,7 +137,8 @@ public:
           v[5] = 0.5*(hs[1*n + o] + hs[1*n + ((size_t)k*ny + jp)*nx + i]);
           v[6] = 0.5*(hs[2*n + o] + hs[2*n + ((size_t)kp*ny + j)*nx + i]);
           // Uc = PrimToCons(Vc) (Src/startup.cpp:514-561)
-          mhd_prim_to_cons<double>(v, u, cfg.gamma - 1.0);
+          if (rmhd) rmhd_prim_to_cons<double>(v, u, cfg.gamma);
+          else      mhd_prim_to_cons<double>(v, u, cfg.gamma - 1.0);
           for (int nv = 0; nv < 8; nv++) hv[nv*n + o] = v[nv];
           hu[0*n + o] = u[0]; hu[1*n + o] = u[1]; hu[2*n + o] = u[2]; hu[3*n + o] = u[3];
           hu[4*n + o] = u[7];
@@ -206,6 +213,8 @@ public:
     b.dU = (R *)dU; b.Cdt = (R *)Cdt;
     b.cdt_part = cdt_part; b.err = env.err;
     b.emf_march = emf_march;
+    b.phys = rmhd ? 1 : 0;
+    b.net = net_on ? net : nullptr;
     static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
     static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
     const int *seq = fused ? fused_seq : split_seq;
@@ -242,6 +251,16 @@ public:
   // RK2 maps the face buffers A -> B -> A within a step; RK3 flips them
   int graph_period() const override { return cfg.time_stepping == GPL_RK3 ? 2 : 1; }
   const void *state_key() const override { return cur_vs; }
+
+  int set_c2p_network(const float *w) override
+  {
+    if (!rmhd) return GPL_EUNSUPPORTED;
+    if (!w) { net_on = false; return 0; }
+    if (!net && dev_alloc(bytes, (void **)&net, NET_SIZE*sizeof(float))) return GPL_ENOMEM;
+    if (hipMemcpy(net, w, NET_SIZE*sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
+    net_on = true;
+    return 0;
+  }
 };
 
 }  // namespace
@@ -253,6 +272,11 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   if (cfg.boundary != GPL_PERIODIC || cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
   if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
+  // RMHD: HLL, FP64 and the fused kernels (rows of at most 512 zones)
+  if (cfg.physics == GPL_RMHD &&
+      (cfg.solver != GPL_SOLVER_HLL || cfg.precision != GPL_FP64 || cfg.nx > 512 ||
+       getenv("GPL_SPLIT_KERNELS") != nullptr))
+    return GPL_EUNSUPPORTED;
   // the PLM stencil wraps at most two cells: every direction needs >= 3 zones
   if (cfg.nx < 3 || cfg.ny < 3 || cfg.nz < 3) return GPL_EINVAL;
   Mhd3dEngine *e = new (std::nothrow) Mhd3dEngine();

@@ -128,7 +128,8 @@ extern "C" int gpl_create(const gpl_config *cfg, gpl_ctx **out)
 {
   if (!cfg || !out) return GPL_EINVAL;
   *out = nullptr;
-  if (cfg->physics != GPL_MHD && cfg->physics != GPL_HD) return GPL_EUNSUPPORTED;
+  if (cfg->physics != GPL_MHD && cfg->physics != GPL_HD && cfg->physics != GPL_RMHD)
+    return GPL_EUNSUPPORTED;
   if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
   if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
   if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return GPL_EINVAL;
@@ -156,8 +157,8 @@ extern "C" int gpl_create(const gpl_config *cfg, gpl_ctx **out)
     return GPL_ENOMEM;
   }
   const EngineEnv env = {c->dev, c->stream, c->ctl, c->err};
-  rc = cfg->physics == GPL_MHD ? make_mhd3d_engine(*cfg, env, &c->engine)
-                               : make_hd_engine(*cfg, env, &c->engine);
+  rc = cfg->physics == GPL_HD ? make_hd_engine(*cfg, env, &c->engine)
+                              : make_mhd3d_engine(*cfg, env, &c->engine);
   if (rc) {
     free_all(c);
     delete c;
@@ -302,6 +303,19 @@ extern "C" uint32_t gpl_run(gpl_ctx *c, int nsteps, double first_dt, double *tim
   if (time_out) *time_out = h_out.time;
   if (dt_out) *dt_out = h_out.dt;
   return err;
+}
+
+extern "C" int gpl_set_c2p_network(gpl_ctx *c, const float *weights)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
+  const int rc = c->engine->set_c2p_network(weights);
+  if (rc == 0 && c->graph_exec) {      // the captured kernels carry the old argument
+    hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  return rc;
 }
 
 extern "C" int gpl_set_timing(gpl_ctx *c, int enable)

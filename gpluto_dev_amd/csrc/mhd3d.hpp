@@ -30,6 +30,8 @@ struct Mhd3dBufs {
   double *cdt_part;
   uint32_t *err;
   int emf_march;   // edges per thread along z in the EMF kernel (0: one edge per thread)
+  int phys;        // 0: ideal MHD, 1: relativistic MHD (FP64, fused kernels)
+  const float *net; // RMHD: learned con2prim guess (nullptr: the reference's A27 guess)
 };
 
 template <typename R>

@@ -26,6 +26,8 @@
 // This header holds the templates; mhd3d_fp64.hip and mhd3d_mixed.hip
 // instantiate them with their own compiler flags.
 #pragma once
+#include <type_traits>
+#include "c2p_net.hpp"
 #include "physics.hpp"
 #include "mhd3d.hpp"
 
@@ -78,6 +80,28 @@ __device__ __forceinline__ T block_max(T v)
   }
   __syncthreads();
   return red[0];
+}
+
+// Riemann solver of the physics module: PH 0 = MHD (HLLD or HLL), PH 1 =
+// RMHD (HLL with the exact quartic fast speeds; FP64 only).
+template <typename R, int N, int SOLVER, int PH>
+__device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
+{
+  if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
+  else if constexpr (SOLVER == 3) mhd_hlld<R, N>(vl, vr, gamma, ff);
+  else mhd_hll<R, N>(vl, vr, gamma, ff);
+}
+
+// VelocityLimiter (Src/vel_limiter.hpp:14-31, applied in plm_states.cpp:158-
+// 161): a cell whose reconstructed velocity is superluminal on either face
+// is flattened (both states = the cell value).  lo = vR(i), hi = vL(i+1),
+// indices 1..3 are the velocity components.
+template <typename R>
+__device__ __forceinline__ bool superluminal(const R *lo, const R *hi)
+{
+  const R v2m = lo[1]*lo[1] + lo[2]*lo[2] + lo[3]*lo[3];
+  const R v2p = hi[1]*hi[1] + hi[2]*hi[2] + hi[3]*hi[3];
+  return v2m >= R(1) || v2p >= R(1);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -201,7 +225,7 @@ void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
 // and RightHandSide's x part (rhs.cpp:384-386, 523): dUc = rhs_x
 // (update_stage.cpp:173-184), C_dt = 0.5 (cmax_i + cmax_i+1)/dx.
 // ---------------------------------------------------------------------------
-template <typename R, int LIM, int SOLVER>
+template <typename R, int LIM, int SOLVER, int PH>
 __global__ __launch_bounds__(512)
 void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
                     R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
@@ -211,7 +235,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   extern __shared__ __align__(16) unsigned char gpl_smem[];
   const int nx = g.nx, i = threadIdx.x;
   R *sV = (R *)gpl_smem;           // [7][nx] primitives of the row
-  R *sD = sV + 7*nx;               // [7][nx] limited slopes
+  R *sD = sV + 7*nx;               // [7][nx] left state vL(i+1) made by cell i
   const size_t n = (size_t)g.ncell;
   const int c = i + nx*(int)blockIdx.x;
   const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
@@ -221,22 +245,29 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   for (int q = 0; q < 7; q++) { v0[q] = V[VAR[q]*n + c]; sV[q*nx + i] = v0[q]; }
   const R bn = (R)Vs[c];
   __syncthreads();
+  R lo[7], hi[7];                  // this cell's vR(i) and vL(i+1)
 #pragma unroll
   for (int q = 0; q < 7; q++) {
     dv[q] = plm_slope<R, LIM>(sV[q*nx + im], v0[q], sV[q*nx + ip]);
-    sD[q*nx + i] = dv[q];
+    lo[q] = v0[q] - R(0.5)*dv[q];
+    hi[q] = v0[q] + R(0.5)*dv[q];
   }
+  if (PH == 1 && superluminal(lo, hi)) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) lo[q] = hi[q] = v0[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 7; q++) sD[q*nx + i] = hi[q];
   __syncthreads();
   R vl[8], vr[8];
 #pragma unroll
   for (int q = 0; q < 7; q++) {
-    vl[VAR[q]] = sV[q*nx + im] + R(0.5)*sD[q*nx + im];
-    vr[VAR[q]] = v0[q] - R(0.5)*dv[q];
+    vl[VAR[q]] = sD[q*nx + im];
+    vr[VAR[q]] = lo[q];
   }
   vl[4] = vr[4] = bn;
   FaceFlux<R> ff;
-  if (SOLVER == 3) mhd_hlld<R, 0>(vl, vr, gamma, ff);
-  else             mhd_hll<R, 0>(vl, vr, gamma, ff);
+  riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
   {
     R et, eb, aL, aR, dd;
     uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
@@ -252,7 +283,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
 #pragma unroll
   for (int nv = 0; nv < 5; nv++) {
     double r = -dtdx*((double)sF[nv*nx + ip] - (double)sF[nv*nx + i]);
-    r += 0.0;
+    if (PH == 0) r += 0.0;         // MHD adds src[] = 0 (rhs.cpp:519); RMHD+CT has no source
     dU[nv*n + c] = (R)r;
   }
   if (want_cdt) Cdt[c] = (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx);
@@ -271,7 +302,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
 // (rk_step.cpp:366-374,509-518,670-708) and the per-block max of C_dt
 // (getMax, update_stage.cpp:446-458).
 // ---------------------------------------------------------------------------
-template <typename R, int D, int LIM, int SOLVER, int L>
+template <typename R, int D, int LIM, int SOLVER, int L, int PH>
 __global__ __launch_bounds__(256)
 void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
                    R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
@@ -308,11 +339,19 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       for (int q = 0; q < 7; q++) {
         Vm[q] = V[VAR[q]*n + cm2]; Vc[q] = V[VAR[q]*n + cm1]; Vp[q] = V[VAR[q]*n + c0];
       }
+      R lo[7];
 #pragma unroll
       for (int q = 0; q < 7; q++) {
-        vlc[q] = Vc[q] + R(0.5)*plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
-        Vm[q] = Vc[q]; Vc[q] = Vp[q];
+        const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
+        lo[q] = Vc[q] - R(0.5)*dv;
+        vlc[q] = Vc[q] + R(0.5)*dv;
       }
+      if (PH == 1 && superluminal(lo, vlc)) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) vlc[q] = Vc[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; q++) { Vm[q] = Vc[q]; Vc[q] = Vp[q]; }
     }
     double Fp[5] = {0, 0, 0, 0, 0}, cmp = 0.0;
     for (int f = s0; f <= s1; f++) {
@@ -344,10 +383,16 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
         vr[VAR[q]] = Vc[q] - R(0.5)*dv;
         vlc[q] = Vc[q] + R(0.5)*dv;
       }
+      if (PH == 1) {
+        const R lo[4] = {R(0), vr[1], vr[2], vr[3]};
+        if (superluminal(lo, vlc)) {
+#pragma unroll
+          for (int q = 0; q < 7; q++) { vr[VAR[q]] = Vc[q]; vlc[q] = Vc[q]; }
+        }
+      }
       vl[BXn] = vr[BXn] = (R)Vs[D*n + cf];
       FaceFlux<R> ff;
-      if (SOLVER == 3) mhd_hlld<R, D>(vl, vr, gamma, ff);
-      else             mhd_hll<R, D>(vl, vr, gamma, ff);
+      riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff);
       e |= ff.err;
       if (f < s1) {
         R et, eb, aL, aR, dd;
@@ -362,7 +407,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
 #pragma unroll
         for (int nv = 0; nv < 5; nv++) {
           double rr = -dtdx*(F[nv] - Fp[nv]);
-          rr += 0.0;
+          if (PH == 0) rr += 0.0;
           double du = acc[nv];
           du += rr;
           const size_t o = nv*n + cc;
@@ -610,17 +655,20 @@ __device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const
 // it needs for the average are recomputed from the same inputs (bitwise the
 // value their owner writes).  Vs is double-buffered (Vin -> Vout).
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, int PH>
 __global__ __launch_bounds__(256)
 void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               double *__restrict__ Vout, double *__restrict__ Bs0,
               double *__restrict__ U, R *__restrict__ V,
               const StepCtl *__restrict__ ctl, RKW w, int first_stage,
-              double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err)
+              double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err,
+              const float *__restrict__ net)
 {
-  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  // no early exit: the learned con2prim guess is evaluated by whole wavefronts
+  const int c0 = blockIdx.x*blockDim.x + threadIdx.x;
+  const bool live = c0 < g.ncell;
+  const int c = live ? c0 : g.ncell - 1;
   const size_t n = (size_t)g.ncell;
-  if (c >= g.ncell) return;
   const Cell p = cell_of(c, g);
   const double dt = ctl->dt;
   const int ci = shift<0>(p, 1, g), cj = shift<1>(p, 1, g), ck = shift<2>(p, 1, g);
@@ -637,7 +685,7 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
     const double *vin = Vin + d*n;
     if (first_stage) {
       const double s0 = vin[c], s1 = vin[up[d]];
-      Bs0[d*n + c] = s0;
+      if (live) Bs0[d*n + c] = s0;
       b_lo[d] = s0 + w.wR*rl[d];
       b_hi[d] = s1 + w.wR*rh[d];
     } else {
@@ -645,18 +693,40 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
       b_lo[d] = w.w0*b0[c] + w.w1*vin[c] + w.wR*rl[d];
       b_hi[d] = w.w0*b0[up[d]] + w.w1*vin[up[d]] + w.wR*rh[d];
     }
-    Vout[d*n + c] = b_lo[d];
+    if (live) Vout[d*n + c] = b_lo[d];
   }
   double b[3], u[5], v[8];
 #pragma unroll
   for (int d = 0; d < 3; d++) b[d] = 0.5*(b_lo[d] + b_hi[d]);
 #pragma unroll
   for (int nv = 0; nv < 5; nv++) u[nv] = U[nv*n + c];
-  const uint32_t e = mhd_cons_to_prim(u, b, gmm1, small_rho, small_prs, v);
-  if (e) {
+  if constexpr (PH == 1) {
+    // ConsToPrim RMHD (mappers.cpp:33-157); written back when the mapper reports
+    // anything (mappers3D.cpp:182-190).  gmm1 carries gamma here.
+    double u8[8] = {u[0], u[1], u[2], u[3], b[0], b[1], b[2], u[4]};
+    double scale = 1.0;
+    if (net) {                     // uniform: a kernel argument
+      float x[4];
+      net_features(u8, x);
+      scale = exp((double)(net[NET_YSC]*net_eval64(net, x)));
+    }
+    uint32_t local = 0;
+    const uint32_t e = rmhd_cons_to_prim(u8, v, gmm1, small_rho, small_prs, local, net != nullptr,
+                                         scale);
+    if (!live) return;
+    if (local) {
+      U[0*n + c] = u8[0]; U[1*n + c] = u8[1]; U[2*n + c] = u8[2]; U[3*n + c] = u8[3];
+      U[4*n + c] = u8[7];
+    }
+    if (e) atomicOr(err, e);
+  } else {
+    const uint32_t e = mhd_cons_to_prim(u, b, gmm1, small_rho, small_prs, v);
+    if (!live) return;
+    if (e) {
 #pragma unroll
-    for (int nv = 0; nv < 5; nv++) U[nv*n + c] = u[nv];
-    atomicOr(err, e);
+      for (int nv = 0; nv < 5; nv++) U[nv*n + c] = u[nv];
+      atomicOr(err, e);
+    }
   }
 #pragma unroll
   for (int nv = 0; nv < 8; nv++) V[nv*n + c] = (R)v[nv];
@@ -666,7 +736,7 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
 // Host-side launch helpers: one entry per kernel family, so that the host
 // orchestration can bracket each launch with timing events.
 // ---------------------------------------------------------------------------
-template <typename R, int LIM, int SOLVER, int L>
+template <typename R, int LIM, int SOLVER, int L, int PH>
 static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
                      int first_stage, int want_dt, double gamma, double small_rho,
                      double small_prs, hipStream_t s)
@@ -693,20 +763,20 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   case K_SWEEPX_F: {
     const size_t shm = 14*(size_t)g.nx*sizeof(R);
-    hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER>), dim3(g.ny*g.nz), dim3(g.nx), shm, s,
+    hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER, PH>), dim3(g.ny*g.nz), dim3(g.nx), shm, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
     break;
   }
   case K_SWEEPY_F: {
     const int nt = g.nx*g.nz*((g.ny + L - 1)/L);
-    hipLaunchKernelGGL((k_sweep_march<R, 1, LIM, SOLVER, L>), dim3((nt + T - 1)/T), dim3(T), 0, s,
+    hipLaunchKernelGGL((k_sweep_march<R, 1, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
     break;
   }
   case K_SWEEPZ_F: {
     const int nt = g.nx*g.ny*((g.nz + L - 1)/L);
-    hipLaunchKernelGGL((k_sweep_march<R, 2, LIM, SOLVER, L>), dim3((nt + T - 1)/T), dim3(T), 0, s,
+    hipLaunchKernelGGL((k_sweep_march<R, 2, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
     break;
@@ -725,23 +795,25 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   }
   case K_FINISH:
-    hipLaunchKernelGGL((k_finish<R>), dim3(nb), dim3(T), 0, s,
+    // the RMHD mapper takes gamma, the MHD one gamma - 1
+    hipLaunchKernelGGL((k_finish<R, PH>), dim3(nb), dim3(T), 0, s,
                        g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
-                       gamma - 1.0, small_rho, small_prs, b.err);
+                       PH == 1 ? gamma : gamma - 1.0, small_rho, small_prs, b.err,
+                       PH == 1 ? b.net : nullptr);
     break;
   default:
     break;
   }
 }
 
-template <typename R, int LIM, int SOLVER>
+template <typename R, int LIM, int SOLVER, int PH>
 static void launch_l(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
                      int first_stage, int want_dt, int march, double gamma, double small_rho,
                      double small_prs, hipStream_t s)
 {
-  if (march == 32) launch_t<R, LIM, SOLVER, 32>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
-  else if (march == 8) launch_t<R, LIM, SOLVER, 8>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
-  else launch_t<R, LIM, SOLVER, 16>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  if (march == 32) launch_t<R, LIM, SOLVER, 32, PH>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  else if (march == 8) launch_t<R, LIM, SOLVER, 8, PH>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
+  else launch_t<R, LIM, SOLVER, 16, PH>(which, b, g, ctl, w, first_stage, want_dt, gamma, small_rho, small_prs, s);
 }
 
 template <typename R>
@@ -749,12 +821,19 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
                   int first_stage, int want_dt, int limiter, int solver, int march, double gamma,
                   double small_rho, double small_prs, hipStream_t s)
 {
+  if constexpr (std::is_same<R, double>::value) {
+    if (b.phys == 1) {            // RMHD: HLL only, FP64 only
+      if (limiter == MC_LIM) launch_l<R, MC_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+      else                   launch_l<R, VANLEER_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+      return;
+    }
+  }
   if (limiter == MC_LIM) {
-    if (solver == 3) launch_l<R, MC_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
-    else             launch_l<R, MC_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    if (solver == 3) launch_l<R, MC_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else             launch_l<R, MC_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   } else {
-    if (solver == 3) launch_l<R, VANLEER_LIM, 3>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
-    else             launch_l<R, VANLEER_LIM, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    if (solver == 3) launch_l<R, VANLEER_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else             launch_l<R, VANLEER_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   }
 }
 

@@ -507,4 +507,386 @@ GPL_HD uint32_t hd_cons_to_prim(double *u, double gmm1, double small_rho, double
   return err;
 }
 
+// ---------------------------------------------------------------------
+// RMHD (ideal EOS, reduced energy E - D, exact fast speeds): variables
+// rho, v1..3, B1..3, prs (Src/RMHD/rmhd.hpp:21-42)
+// ---------------------------------------------------------------------
+
+// Enthalpy and SoundSpeed2 (Src/eos.hpp:13-51, 84-125), relativistic IDEAL
+template <typename R>
+GPL_HD R rmhd_enthalpy(const R *v, R gamma)
+{
+  const R gmmr = gamma/(gamma - R(1));
+  const R theta = v[7]/v[0];
+  return R(1) + gmmr*theta;
+}
+
+// PrimToCons, Src/RMHD/rmhd.hpp:535-610 (B x (v x B) momentum form)
+template <typename R>
+GPL_HD void rmhd_prim_to_cons(const R *v, R *u, R gamma)
+{
+  const R gmmr = gamma/(gamma - R(1));
+  const R h = rmhd_enthalpy(v, gamma);
+  const R vx1 = v[1], vx2 = v[2], vx3 = v[3];
+  const R vel2 = vx1*vx1 + vx2*vx2 + vx3*vx3;
+  const R g2 = R(1)/(R(1) - vel2);
+  const R g  = rsqrt_(g2);
+  const R ex = vx2*v[6] - vx3*v[5];
+  const R ey = vx3*v[4] - vx1*v[6];
+  const R ez = vx1*v[5] - vx2*v[4];
+  const R e2 = ex*ex + ey*ey + ez*ez;
+  const R Bmag2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  const R wt = v[0]*h*g2;
+  u[0] = g*v[0];
+  u[1] = wt*vx1 + v[5]*ez - v[6]*ey;
+  u[2] = wt*vx2 + v[6]*ex - v[4]*ez;
+  u[3] = wt*vx3 + v[4]*ey - v[5]*ex;
+  u[4] = v[4];
+  u[5] = v[5];
+  u[6] = v[6];
+  u[7] =   v[7]*(g2*gmmr - R(1)) + u[0]*g2*vel2/(g + R(1))
+         + R(0.5)*(Bmag2 + e2);
+}
+
+// Flux<dir>, Src/RMHD/rmhd.hpp:315-366 (reduced energy)
+template <typename R, int N>
+GPL_HD void rmhd_flux(const R *v, const R *u, R h, R *flux)
+{
+  constexpr int VXn = 1 + N, VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
+  const R g  = u[0]/v[0];
+  const R g2 = g*g;
+  const R g_2 = R(1)/g2;
+  const R Bmag2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  const R vB = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  R b[3];
+  b[0] = g*(v[4]*g_2 + vB*v[1]);
+  b[1] = g*(v[5]*g_2 + vB*v[2]);
+  b[2] = g*(v[6]*g_2 + vB*v[3]);
+  const R b2 = Bmag2*g_2 + vB*vB;
+  const R pt = v[7] + R(0.5)*b2;
+  const R wt = v[0]*h + b2;
+  const R wtg2 = wt*g2;
+  flux[0] = u[0]*v[VXn];
+  flux[1] = wtg2*v[1]*v[VXn] - b[0]*b[N];
+  flux[2] = wtg2*v[2]*v[VXn] - b[1]*b[N];
+  flux[3] = wtg2*v[3]*v[VXn] - b[2]*b[N];
+  flux[BXn] = R(0);
+  flux[BXt] = v[VXn]*v[BXt] - v[BXn]*v[VXt];
+  flux[BXb] = v[VXn]*v[BXb] - v[BXn]*v[VXb];
+  flux[7] = u[1 + N];
+  flux[7] -= flux[0];
+  flux[1 + N] += pt;
+}
+
+// CubicSolve, Src/RMHD/rmhd.hpp:390-452 (three real roots, trigonometric form)
+GPL_HD void rmhd_cubic_solve(double b, double c, double d, double z[3])
+{
+  const double one_3 = 1.0/3.0, one_27 = 1.0/27.0, sqrt3 = sqrt(3.0);
+  const double b2 = b*b;
+  double Q = b2*one_3 - c*(1.0 - 1.e-16);
+  const double R = b*(2.0*b2 - 9.0*c)*one_27 + d;
+  Q = rmax(Q, 0.0);
+  const double sQ = sqrt(Q)/sqrt3;
+  double arg = -1.5*R/(Q*sQ);
+  arg = rmax(-1.0, arg);
+  arg = rmin( 1.0, arg);
+  const double theta = acos(arg)*one_3;
+  // cos and sin of one argument: the reference's g++ build evaluates them with
+  // one sincos call, whose last bit can differ from separate cos/sin calls
+  double sn1, cs;
+  sincos(theta, &sn1, &cs);
+  const double sn = sqrt3*sn1;
+  const double p  = -b*one_3;
+  z[0] = -sQ*(cs + sn) + p;
+  z[1] = -sQ*(cs - sn) + p;
+  z[2] =  2.0*sQ*cs    + p;
+}
+
+// QuarticSolve, Src/RMHD/rmhd.hpp:455-505: the smallest and largest roots
+// (the sorted z[0] and z[3]); returns false when the resolvent fails.
+GPL_HD bool rmhd_quartic_extremes(double b, double c, double d, double e, double &zmin, double &zmax)
+{
+  const double b2 = b*b;
+  const double p = c - b2*0.375;
+  const double q = d + b2*b*0.125 - b*c*0.5;
+  const double r = e - 3.0*b2*b2/256.0 + b2*c/16.0 - 0.25*b*d;
+  double u[3];
+  rmhd_cubic_solve(2.0*p, p*p - 4.0*r, -q*q, u);
+  u[0] = rmax(u[0], 0.0);
+  u[1] = rmax(u[1], 0.0);
+  u[2] = rmax(u[2], 0.0);
+  if (u[0] != u[0] || u[1] != u[1] || u[2] != u[2]) return false;
+  const double sq  = -0.5*(q >= 0.0 ? 1.0 : -1.0);          // DSIGN, Src/macros.hpp:197
+  const double sz1 = sq*sqrt(u[0]);
+  const double sz2 = sq*sqrt(u[1]);
+  const double sz3 = sq*sqrt(u[2]);
+  double z[4];
+  z[0] = -0.25*b + sz1 + sz2 + sz3;
+  z[1] = -0.25*b + sz1 - sz2 - sz3;
+  z[2] = -0.25*b - sz1 + sz2 - sz3;
+  z[3] = -0.25*b - sz1 - sz2 + sz3;
+  // SortArray (insertion sort, strict >): the extremes of the sorted list
+  for (int j = 1; j < 4; j++) {
+    const double f = z[j];
+    int i = j - 1;
+    while (i >= 0 && z[i] > f) { z[i + 1] = z[i]; i--; }
+    z[i + 1] = f;
+  }
+  zmin = z[0];
+  zmax = z[3];
+  return true;
+}
+
+// MaxSignalSpeed<dir>, Src/RMHD/rmhd.hpp:631-752 (exact fast speeds)
+template <int N>
+GPL_HD void rmhd_max_signal_speed(const double *v, double cs2, double h, double &cmin, double &cmax)
+{
+  constexpr int VXn = 1 + N, BXn = 4 + N;
+  double lfp, lfm;
+  const double vB  = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  double u02 = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  const double Bm2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  if (u02 < 1.e-14) {
+    const double w_1  = 1.0/(v[0]*h + Bm2);
+    const double eps2 = cs2 + Bm2*w_1*(1.0 - cs2);
+    const double a0   = cs2*v[BXn]*v[BXn]*w_1;
+    const double a1   = - a0 - eps2;
+    double del  = a1*a1 - 4.0*a0;
+    del = rmax(del, 0.0);
+    del = sqrt(del);
+    lfp =  sqrt(0.5*(-a1 + del));
+    lfm = -lfp;
+  } else {
+    const double vB2 = vB*vB;
+    u02 = 1.0/(1.0 - u02);
+    const double b2  = Bm2/u02 + vB2;
+    const double w_1 = 1.0/(v[0]*h + b2);
+    const double vx  = v[VXn];
+    const double vx2 = vx*vx;
+    if (fabs(v[BXn]) < 1.e-14) {
+      const double w     = v[0]*h;
+      const double scrh1 = b2 + cs2*(w - vB2);
+      const double scrh2 = w*(1.0 - cs2)*u02;
+      const double del   = 4.0*scrh1*(scrh2*(1.0 - vx2) + scrh1);
+      const double a2 = scrh1 + scrh2;
+      const double a1 = -2.0*vx*scrh2;
+      lfp = 0.5*(-a1 + sqrt(del))/a2;
+      lfm = 0.5*(-a1 - sqrt(del))/a2;
+    } else {
+      const double scrh1 = v[BXn]/u02 + vB*vx;
+      const double scrh2 = scrh1*scrh1;
+      const double a2_w       = cs2*w_1;
+      const double eps2       = (cs2*v[0]*h + b2)*w_1;
+      const double one_m_eps2 = u02*v[0]*h*(1.0 - cs2)*w_1;
+      double scrh = 2.0*(a2_w*vB*scrh1 - eps2*vx);
+      const double a4 = one_m_eps2 - a2_w*vB2 + eps2;
+      double a3 = - 4.0*vx*one_m_eps2 + scrh;
+      double a2 =   6.0*vx2*one_m_eps2 + a2_w*(vB2 - scrh2) + eps2*(vx2 - 1.0);
+      double a1 = - 4.0*vx*vx2*one_m_eps2 - scrh;
+      double a0 = vx2*vx2*one_m_eps2 + a2_w*scrh2 - eps2*vx2;
+      scrh = 1.0/a4;
+      a3 *= scrh;
+      a2 *= scrh;
+      a1 *= scrh;
+      a0 *= scrh;
+      lfm = lfp = __builtin_nan("");
+      (void)rmhd_quartic_extremes(a3, a2, a1, a0, lfm, lfp);
+    }
+  }
+  cmax = lfp;
+  cmin = lfm;
+}
+
+// HLL_Solver, Src/RMHD/hll.cpp:150-230 with HLL_Speed (Src/RMHD/rmhd.hpp:818-835):
+// bL = MIN(Sl,0), bR = MAX(0,Sr), one averaged flux for every wave pattern.
+template <int N>
+GPL_HD void rmhd_hll(const double *vl, const double *vr, double gamma, FaceFlux<double> &out)
+{
+  double ul[8], ur[8], fl[8], fr[8], sl_min, sl_max, sr_min, sr_max;
+  rmhd_prim_to_cons<double>(vl, ul, gamma);
+  rmhd_prim_to_cons<double>(vr, ur, gamma);
+  const double hL = rmhd_enthalpy<double>(vl, gamma), hR = rmhd_enthalpy<double>(vr, gamma);
+  const double a2L = gamma*(vl[7]/vl[0])/hL, a2R = gamma*(vr[7]/vr[0])/hR;
+  rmhd_max_signal_speed<N>(vl, a2L, hL, sl_min, sl_max);
+  rmhd_max_signal_speed<N>(vr, a2R, hR, sr_min, sr_max);
+  const double Sl = rmin(sl_min, sr_min);
+  const double Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  const double bL = rmin(Sl, 0.0);
+  const double bR = rmax(0.0, Sr);
+  rmhd_flux<double, N>(vl, ul, hL, fl);
+  rmhd_flux<double, N>(vr, ur, hR, fr);
+  const double scrh = 1.0/(bR - bL);
+  double *flux = out.f;
+  for (int nv = 0; nv < 8; nv++) {
+    flux[nv]  = bL*bR*(ur[nv] - ul[nv]) + bR*fl[nv] - bL*fr[nv];
+    flux[nv] *= scrh;
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+constexpr uint32_t ERR_PRESSURE_FIX          = 1u << 5;
+constexpr uint32_t ERR_ENG_SOLVE_NO_SOLUTION = 1u << 7;
+constexpr uint32_t ERR_SUPERLUMINAL_SPEED    = 1u << 8;
+constexpr uint32_t ERR_TOO_MANY_ITERATIONS   = 1u << 9;
+
+// Initial guess of RMHD_EnergySolve: positive root of Eq. (A27)
+// (Src/RMHD/rmhd_energy_solve.cpp:86-96), bounded below by D.
+GPL_HD double rmhd_guess_a27(const double *u)
+{
+  const double D  = u[0];
+  const double E  = u[7];
+  const double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  const double B2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+  const double Y1 = -4.0*(E + D - B2);
+  const double Y2 = m2 - 2.0*(E + D)*B2 + B2*B2;
+  double chi = Y1*Y1 - 12.0*Y2;
+  chi = rmax(0.0, chi);
+  const double W = ( - Y1 + sqrt(chi))/6.0;
+  return rmax(D, W);
+}
+
+// RMHD_EnergySolve, Src/RMHD/rmhd_energy_solve.cpp:35-209: Newton-Raphson on
+// W' = W - D, tolerance 1e-11, at most 49 passes, from the Eq. (A27) guess
+// (the reference) or, when W0 > 0, from max(D, W0).  u = D, m1..3, B1..3,
+// E - D.  On success writes rho, v, prs of v[]; *iters = passes made.
+GPL_HD uint32_t rmhd_energy_solve(const double *u, double *v, double gamma, int *iters = nullptr,
+                                  double W0 = -1.0)
+{
+  const double acc = 1.e-11;
+  const double D  = u[0];
+  const double E  = u[7];
+  const double S  = u[1]*u[4] + u[2]*u[5] + u[3]*u[6];
+  const double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  const double B2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+  const double S2 = S*S;
+  if (E < 0.0) return ERR_NEGATIVE_ENG;
+  double Y1, Y2, chi;
+  double W = W0 > 0.0 ? rmax(D, W0) : rmhd_guess_a27(u);
+  double Wp = W - D;
+  int k, done = 0;
+  double p = -1.0, lor = 1.0;
+  for (k = 1; k < 50; k++) {
+    W = Wp + D;
+    const double W2    = W*W;
+    const double S2_W2 = S2/W2;
+    Y1 = 1.0/(W + B2);
+    Y2 = Y1*Y1;
+    const double vel2 = S2_W2*Y1*(Y1*W + 1.0) + m2*Y2;
+    const double one_m_v2 = 1.0 - vel2;
+    const double lor2 = 1.0/one_m_v2;
+    lor = sqrt(lor2);
+    if (vel2 > 1.0) { if (iters) *iters = k; return ERR_ENG_SOLVE_NO_SOLUTION; }
+    chi = Wp/lor2 - D*vel2/(lor + 1.0);
+    const double dv2_dW  = -2.0*Y2*(3.0*S2_W2 + Y1*(S2_W2*B2*B2/W + m2));
+    const double dchi_dW =  one_m_v2 - 0.5*lor*(D + 2.0*chi*lor)*dv2_dW;
+    const double drho_dW = -0.5*D*lor*dv2_dW;
+    const double dp_dchi = (gamma - 1.0)/gamma;
+    const double dp_drho = 0.0;
+    p = chi*dp_dchi;
+    if (done) break;
+    const double dp  = dp_dchi*dchi_dW + dp_drho*drho_dW;
+    const double fW  = Wp + 0.5*(B2 + (B2*m2 - S2)*Y2) - (E + p);
+    const double dfW = 1.0 - dp - (B2*m2 - S2)*Y2*Y1;
+    const double dW  = fW/dfW;
+    Wp -= dW;
+    if (fabs(dW) < acc*Wp || fabs(fW) < acc) done = 1;
+  }
+  if (iters) *iters = k;
+  if (p < 0.0)  return ERR_NEGATIVE_PRS;
+  if (k == 50)  return ERR_TOO_MANY_ITERATIONS;
+  if (p != p)   return ERR_NAN;
+  W = Wp + D;
+  v[0] = u[0]/lor;
+  v[7] = p;
+  Y1 = 1.0/(W + B2);
+  const double scrh = S/W;
+  v[1] = Y1*(u[1] + scrh*u[4]);
+  v[2] = Y1*(u[2] + scrh*u[5]);
+  v[3] = Y1*(u[3] + scrh*u[6]);
+  return 0;
+}
+
+// RMHD_PressureFix, Src/RMHD/rmhd_pressure_fix.cpp:22-179: p = small_prs,
+// bisection on v^2 of Eq. (A3), then E recomputed into u[7].
+GPL_HD uint32_t rmhd_pressure_fix(double *u, double *v, double gamma, double small_prs)
+{
+  const double D  = u[0];
+  const double S  = u[1]*u[4] + u[2]*u[5] + u[3]*u[6];
+  const double m2 = u[1]*u[1] + u[2]*u[2] + u[3]*u[3];
+  const double B2 = u[4]*u[4] + u[5]*u[5] + u[6]*u[6];
+  const double S2 = S*S;
+  const double p = small_prs;
+  double v2min, v2max, v2, lor, plor, W, W2, f, fmin;
+  v2min = v2 = 0.0;
+  lor  = 1.0/sqrt(1.0 - v2);
+  plor = p*lor;
+  W = (D + plor*gamma/(gamma - 1.0))*lor;
+  W2 = W*W;
+  f  = (S2*(2.0*W + B2) + m2*W2)/((W + B2)*(W + B2)*W2);
+  f -= v2;
+  fmin = f;
+  v2max = v2 = 1.0-1.e-9;
+  int k;
+  for (k = 1; k < 50; k++) {
+    v2 = 0.5*(v2min + v2max);
+    lor  = 1.0/sqrt(1.0 - v2);
+    plor = p*lor;
+    W = (D + plor*gamma/(gamma - 1.0))*lor;
+    W2 = W*W;
+    f  = (S2*(2.0*W + B2) + m2*W2)/((W + B2)*(W + B2)*W2);
+    f -= v2;
+    if (f*fmin > 0.0) { v2min = v2; fmin = f; }
+    else              { v2max = v2; }
+    if (fabs(f) < 1.e-9) break;
+  }
+  if (v2 >= 1.0 && k == 50) return ERR_PRESSURE_FIX;
+  v[0] = D/lor;
+  v[7] = p;
+  const double w_1  = 1.0/(W + B2);
+  const double scrh = S/W;
+  v[1] = w_1*(u[1] + scrh*u[4]);
+  v[2] = w_1*(u[2] + scrh*u[5]);
+  v[3] = w_1*(u[3] + scrh*u[6]);
+  const double S2_W2 = S2/W2;
+  u[7] = W - D - p + 0.5*(1.0 + v2)*B2 - 0.5*S2_W2;
+  return 0;
+}
+
+// ConsToPrim RMHD (Src/RMHD/mappers.cpp:33-157) inside ConsToPrim3D
+// (Src/mappers3D.cpp:104-194): u[8] = D, m1..3, B1..3, E - D, rewritten by the
+// recovery branches; returns the mapper's bits plus the NaN check's.
+GPL_HD uint32_t rmhd_cons_to_prim(double *u, double *v, double gamma, double small_rho,
+                                  double small_prs, uint32_t &local, bool guided = false,
+                                  double guess_scale = 1.0)
+{
+  uint32_t nan_err = 0;
+  for (int nv = 0; nv < 8; nv++) if (u[nv] != u[nv]) nan_err |= ERR_NAN | ERR_QUIT;
+  uint32_t errm = 0;
+  bool fix = false;
+  v[4] = u[4]; v[5] = u[5]; v[6] = u[6];
+  if (u[0] < 0.0) { u[0] = small_rho; errm |= ERR_NEGATIVE_RHO; }
+  if (u[7] < 0.0) { fix = true; errm |= ERR_NEGATIVE_ENG; }
+  if (!fix) {
+    // guided: Newton from guess_scale * W_A27 first, the A27 start as the judge
+    uint32_t e = guided ? rmhd_energy_solve(u, v, gamma, nullptr, guess_scale*rmhd_guess_a27(u)) : 1u;
+    if (e) e = rmhd_energy_solve(u, v, gamma);
+    if (e) fix = true;
+    errm |= e;
+  }
+  if (fix) {
+    const uint32_t e = rmhd_pressure_fix(u, v, gamma, small_prs);
+    if (e) { errm |= e; errm |= ERR_CONS2PRIM_FAIL; }
+  }
+  const double scrh = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  if (scrh >= 1.0) { errm |= ERR_SUPERLUMINAL_SPEED; errm |= ERR_CONS2PRIM_FAIL; }
+  local = errm;
+  uint32_t err = nan_err | errm;
+  if (errm & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
+  return err;
+}
+
 }  // namespace gpl

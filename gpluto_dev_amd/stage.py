@@ -106,7 +106,7 @@ class StageUpdate:
         L = _lib.load()
         cfg = _lib.GplConfig()
         L.gpl_default_config(C.byref(cfg))
-        cfg.physics = _lib.GPL_MHD if defs.physics.upper() == "MHD" else _lib.GPL_HD
+        cfg.physics = {"MHD": _lib.GPL_MHD, "RMHD": _lib.GPL_RMHD, "HD": _lib.GPL_HD}[defs.physics.upper()]
         cfg.dimensions = defs.dimensions
         cfg.nx, cfg.ny, cfg.nz = ini.n
         for d in range(3):
@@ -127,7 +127,7 @@ class StageUpdate:
         _lib.check(L.gpl_create(C.byref(cfg), C.byref(h)), "gpl_create")
         self._h = h
         self.n = tuple(ini.n)
-        self.mhd = cfg.physics == _lib.GPL_MHD
+        self.mhd = cfg.physics != _lib.GPL_HD   # staggered field + CT (MHD, RMHD)
         self.nvar = 8 if self.mhd else 5      # NVAR (Src/MHD/mhd.hpp:21-41, Src/HD/hd.hpp:21-34)
 
     def close(self):
@@ -186,6 +186,16 @@ class StageUpdate:
         t, dt = C.c_double(0.0), C.c_double(0.0)
         err = self._L.gpl_run(self._h, nsteps, first_dt, C.byref(t), C.byref(dt))
         return int(err), t.value, dt.value
+
+    def set_c2p_network(self, weights=None):
+        """RMHD: learned con2prim guess (gpluto_dev_amd.c2p_net.train); None
+        restores the reference's Eq. (A27) start."""
+        if weights is None:
+            _lib.check(self._L.gpl_set_c2p_network(self._h, None), "gpl_set_c2p_network")
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        _lib.check(self._L.gpl_set_c2p_network(self._h, w.ctypes.data_as(C.POINTER(C.c_float))),
+                   "gpl_set_c2p_network")
 
     # -- measurement ----------------------------------------------------------
     def set_timing(self, on: bool):

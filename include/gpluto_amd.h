@@ -24,6 +24,7 @@ extern "C" {
 /* PHYSICS of definitions.hpp (Src/gpluto.hpp) */
 #define GPL_HD    1
 #define GPL_MHD   2
+#define GPL_RMHD  3
 /* pluto.ini [Solver] names (Src/MHD/set_solver.cpp:35-47, Src/HD/set_solver.cpp:17-19) */
 #define GPL_SOLVER_HLL   1
 #define GPL_SOLVER_HLLC  2
@@ -60,15 +61,15 @@ extern "C" {
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
 typedef struct gpl_config {
-  int    physics;          /* GPL_MHD (3D) or GPL_HD (1D/2D/3D) */
+  int    physics;          /* GPL_MHD (3D), GPL_RMHD (3D, HLL, FP64) or GPL_HD (1D/2D/3D) */
   int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
   int    boundary;         /* GPL_PERIODIC (MHD, HD) or GPL_OUTFLOW (HD) on every side */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
-  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLL; HD: GPL_SOLVER_HLLC / _HLL */
-  int    ct_emf_average;   /* GPL_UCT_HLL (MHD) */
+  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLL; RMHD: _HLL; HD: _HLLC / _HLL */
+  int    ct_emf_average;   /* GPL_UCT_HLL (MHD, RMHD) */
   int    precision;        /* GPL_FP64, or GPL_MIXED (MHD) */
   int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */
   int    device;           /* HIP device ordinal */
@@ -169,6 +170,39 @@ int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu);
 int gpl_batch_get_tuning(const gpl_batch *b, int *march, int *blocks_per_cu, double *step_ms);
 double gpl_batch_last_run_ms(const gpl_batch *b);
 size_t gpl_batch_device_bytes(const gpl_batch *b);
+
+/* ---------------------------------------------------------------------------
+ * Learned con2prim initial guess for RMHD.  No counterpart in the reference,
+ * whose RMHD_EnergySolve (Src/RMHD/rmhd_energy_solve.cpp:86-96) always starts
+ * Newton from the Eq. (A27) root.  A 4-32-32-1 network (weights: a float
+ * vector of gpl_c2p_net_size() entries) corrects that guess; Newton, its
+ * tolerance and the A27 restart on failure stay the judge.
+ * ------------------------------------------------------------------------- */
+int gpl_c2p_net_size(void);
+/* n (conserved, target) pairs from sampled primitive states: u[n][8]
+ * (D, m1..3, B1..3, E - D), raw features x[n][4], y[n] = log(W/W_A27). */
+int gpl_c2p_net_make_data(int device, uint64_t seed, int n, double gamma, double *u_out,
+                          float *x_out, float *y_out);
+/* Train on the GPU (FP32 forward/backward kernels, Adam); writes the weights
+ * and the last mini-batch loss (normalised target). */
+int gpl_c2p_net_train(int device, uint64_t seed, int nsamples, int steps, int batch, float lr,
+                      double gamma, float *weights_out, float *loss_out);
+/* Gradient of the mean-squared error over (x, y) -- the training kernel on
+ * one batch -- and its loss. */
+int gpl_c2p_net_grad(int device, const float *weights, const float *x, const float *y, int n,
+                     float *grad_out, float *loss_out);
+/* Matrix-core inference (FP16 MFMA, FP32 accumulate): y_out[n] = net(x). */
+int gpl_c2p_net_infer(int device, const float *weights, const float *x, int n, float *y_out);
+/* Use the learned guess inside an RMHD context's con2prim (weights copied;
+ * NULL restores the reference's A27 start, the bitwise drop-in).  With it on,
+ * results agree with the drop-in to the Newton tolerance, not bit for bit. */
+int gpl_set_c2p_network(gpl_ctx *ctx, const float *weights);
+/* Batched RMHD energy solve of n states u[n][8] from the A27 guess
+ * (weights == NULL, the reference method) or from the learned guess with
+ * the A27 restart; v_out[n][8], Newton passes and error bits per state, and
+ * the kernel time. */
+int gpl_c2p_solve(int device, const float *weights, const double *u, int n, double gamma,
+                  double *v_out, int *passes_out, uint32_t *err_out, double *ms_out);
 
 #ifdef __cplusplus
 }

@@ -1382,3 +1382,21 @@ int gplo_get_state(const gplo_state *s, double *vc, double *vs, double *uc)
   }
   return 0;
 }
+
+/* ======================================================================
+ * Pointwise entry points for tests: one RMHD interface solve and one RMHD
+ * con2prim, as restated above.
+ * ====================================================================== */
+uint32_t gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamma,
+                        double *flux, double *sl, double *sr)
+{
+  return rmhd_hll(vl, vr, dir, gamma, flux, sl, sr);
+}
+
+uint32_t gplo_rmhd_c2p(double *u, double *v, double gamma, double small_rho, double small_prs)
+{
+  gplo_config c;
+  memset(&c, 0, sizeof(c));
+  c.gamma = gamma; c.small_rho = small_rho; c.small_prs = small_prs;
+  return rmhd_cons_to_prim(u, v, &c);
+}

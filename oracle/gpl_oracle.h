@@ -52,6 +52,11 @@ uint32_t    gplo_advance_step(gplo_state *s, double dt, double *inv_dt);
 uint32_t    gplo_run(gplo_state *s, int nsteps, double first_dt,
                      double *time_out, double *dt_out);
 int         gplo_nvar(const gplo_state *s);
+/* pointwise RMHD pieces for tests: HLL interface solve along dir (flux[8],
+   Davis speeds) and con2prim of u[8] (rewrites u like the reference) */
+uint32_t    gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamma,
+                           double *flux, double *sl, double *sr);
+uint32_t    gplo_rmhd_c2p(double *u, double *v, double gamma, double small_rho, double small_prs);
 
 #ifdef __cplusplus
 }

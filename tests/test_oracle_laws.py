@@ -168,3 +168,69 @@ def test_sod_against_exact_riemann_solution():
     e1, e2 = sod_error(200), sod_error(800)
     assert e1 < 6e-3 and e2 < 2.5e-3
     assert e2 < 0.6 * e1          # first-order convergence at discontinuities
+
+
+# ---- relativistic MHD ------------------------------------------------------
+
+def test_rmhd_conservation_and_divb():
+    """D, m, E - D of the periodic box change only by round-off, and the CT
+    face field stays divergence-free, through RK2 steps that include
+    superluminal-reconstruction flattening."""
+    import oracle
+    n = (12, 14, 10)
+    vc, vs = ic.rmhd_turbulence(n, 4, lorentz_max=2.0, sigma=0.5)
+    o = oracle.Oracle(phys="rmhd", n=n, gamma=4 / 3, solver="hll", time_stepping=2)
+    o.set_state(vc, vs)
+    _, _, uc0 = o.get_state()
+    err, _, _ = o.run(6, 1e-3)
+    assert err == 0
+    _, vs1, uc1 = o.get_state()
+    t0, t1 = totals(uc0), totals(uc1)
+    scale = np.array([np.sum(np.abs(uc0[v])) for v in (0, 1, 2, 3, 7)])
+    assert np.all(np.abs(t1 - t0) <= 1e-13 * scale), (t1 - t0) / scale
+    assert ic.divb_max(vs1) <= 1e-12 * np.max(np.abs(vs1)) * 16
+
+
+def test_rmhd_uniform_moving_state():
+    """A uniform relativistic state (Lorentz factor ~2, oblique field) stays
+    uniform: the HLL flux is the physical flux and con2prim inverts
+    PrimToCons to within the Newton tolerance."""
+    import oracle
+    n = (6, 6, 6)
+    vc = np.zeros((8,) + n)
+    vc[0] = 1.7; vc[1] = 0.5; vc[2] = -0.6; vc[3] = 0.3; vc[7] = 2.3
+    vs = np.zeros((3,) + n)
+    vs[0] = 0.8; vs[1] = -0.5; vs[2] = 1.1
+    vc[4], vc[5], vc[6] = vs[0], vs[1], vs[2]
+    o = oracle.Oracle(phys="rmhd", n=n, gamma=4 / 3, solver="hll", time_stepping=2)
+    o.set_state(vc, vs)
+    err, _, _ = o.run(4, 1e-3)
+    v1, vs1, _ = o.get_state()
+    assert err == 0
+    for q in range(8):
+        assert np.max(np.abs(v1[q] - vc[q])) <= 1e-11 * max(1.0, abs(vc[q].flat[0])), q
+    assert np.array_equal(vs1, vs)
+
+
+def test_rmhd_nonrelativistic_limit_tracks_mhd():
+    """At v << 1, p << rho and B^2 << rho the relativistic solution tends to the
+    Newtonian MHD one (HLL both): the two oracles agree to the order of the
+    relativistic corrections, and far better than the flow's own change."""
+    import oracle
+    n = (12, 12, 12)
+    vc, vs = ic.mhd_turbulence(n, 8, beta=10.0, mach=0.1)
+    eps = 1e-3                       # velocity scale: v ~ 1e-4, c_s ~ 1.3e-3
+    vcr = vc.copy()
+    vcr[1:4] *= eps; vcr[4:7] *= eps; vcr[7] *= eps * eps
+    vsr = vs * eps
+    outs = {}
+    for phys in ("mhd", "rmhd"):
+        o = oracle.Oracle(phys=phys, n=n, gamma=5 / 3, solver="hll", time_stepping=2)
+        o.set_state(vcr, vsr)
+        err, _, _ = o.run(5, 2e-3 / eps)
+        assert err == 0
+        outs[phys] = o.get_state()[0]
+    for q in (1, 2, 3):
+        dv = np.max(np.abs(outs["rmhd"][q] - outs["mhd"][q]))
+        change = np.max(np.abs(outs["mhd"][q] - vcr[q]))
+        assert dv <= 1e-2 * change, (q, dv, change)
