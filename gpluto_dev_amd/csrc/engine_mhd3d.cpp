@@ -91,7 +91,8 @@ public:
     if (march != 8 && march != 16 && march != 32) march = 16;
     emf_march = fp64 ? 16 : 0;
     if (const char *m = getenv("GPL_EMF_MARCH")) emf_march = atoi(m);
-    if (emf_march != 0 && emf_march != 4 && emf_march != 16) emf_march = 16;
+    if (emf_march != 0 && emf_march != 4 && emf_march != 8 && emf_march != 16 && emf_march != 32)
+      emf_march = 16;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);

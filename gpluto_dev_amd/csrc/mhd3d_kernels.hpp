@@ -788,6 +788,12 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     } else if (b.emf_march == 4) {
       const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 3)/4);
       hipLaunchKernelGGL((k_emf_march<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+    } else if (b.emf_march == 8) {
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 7)/8);
+      hipLaunchKernelGGL((k_emf_march<R, LIM, 8>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+    } else if (b.emf_march == 32) {
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 31)/32);
+      hipLaunchKernelGGL((k_emf_march<R, LIM, 32>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
     } else {
       const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 15)/16);
       hipLaunchKernelGGL((k_emf_march<R, LIM, 16>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
