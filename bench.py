@@ -39,24 +39,26 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # Algorithmic bytes per cell of one launch: each kernel's compulsory HBM
 # traffic if every element it touches were read once and written once
 # (w = bytes of the state type R, which also types the within-stage sums dU
-# and C_dt; the conserved state U, U0 and the faces are FP64).  A launch
-# covers the whole box; the figure is the mean over the two stages of an RK2
-# step, because the kernels' average launch time is too.
+# and C_dt; the conserved state U and the faces are FP64).  A launch covers
+# the whole box; the figure is the mean over the two stages of an RK2 step,
+# because the kernels' average launch time is too.  Stage 1 writes its state
+# to a second buffer and stage 2 reads both, so no stage copies U^n or the
+# faces of U^n aside.
 #   k_sweepx_fused : V 7w + face 8 -> FD 5w + dU 5w (+ C_dt w, stage 1)
 #                    = 18w + 8 | 17w + 8                           -> 17.5w + 8
 #   k_sweepy_fused : V 7w + face 8 + dU 5w (+ C_dt w) -> FD 5w + dU 5w (+ C_dt w)
 #                    = 24w + 8 | 22w + 8                           -> 23w + 8
-#   k_sweepz_fused : V 7w + face 8 + dU 5w + U 40 (+ C_dt w | + U0 40)
-#                    -> FD 5w + U 40 (+ U0 40, stage 1)
-#                    = 18w + 128 | 17w + 128                       -> 17.5w + 128
+#   k_sweepz_fused : V 7w + face 8 + dU 5w + U 40 (+ C_dt w | + U^n 40)
+#                    -> FD 5w + U 40
+#                    = 18w + 88 | 17w + 128                        -> 17.5w + 108
 #   k_emf          : FD 15w + faces 24 -> E 3w                     =  18w + 24
-#   k_finish       : E 3w + faces 24 + U 40 (+ Bs0 24, stage 2) -> faces 24 + V 8w
-#                    (+ Bs0 24, stage 1)                           =  11w + 112
+#   k_finish       : E 3w + faces 24 + U 40 (+ faces of U^n 24, stage 2)
+#                    -> faces 24 + V 8w                            =  11w + 100
 #   k_sweep_* (split, rows > 512): V 8w + face 8 -> F 5w + CM w + FD 5w = 19w + 8
-#   k_rhs_update   : F 15w + CM 3w + U 40 + U0 40 -> U 40          =  18w + 120
+#   k_rhs_update   : F 15w + CM 3w + U 40 (+ U^n 40, stage 2) -> U 40 = 18w + 100
 ALG_BYTES = {"k_sweepx_fused": (17.5, 8), "k_sweepy_fused": (23, 8),
-             "k_sweepz_fused": (17.5, 128), "k_emf": (18, 24), "k_finish": (11, 112),
-             "k_rhs_update": (18, 120)}
+             "k_sweepz_fused": (17.5, 108), "k_emf": (18, 24), "k_finish": (11, 100),
+             "k_rhs_update": (18, 100)}
 
 
 def alg_bytes_per_cell(kernel: str, w: int) -> float:

@@ -49,25 +49,30 @@ public:
   bool rmhd = false;
   size_t n = 0;
   void *V = nullptr;          // [8][n] R
-  double *U = nullptr, *U0 = nullptr;
-  double *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
+  // Conserved state: U holds U^n between steps; the first stage writes its
+  // state into Us, later stages read Us and U^n and the last writes U.
+  double *U = nullptr, *Us = nullptr;
+  // Staggered faces rotate over three buffers: the step starts from vs[cur],
+  // stage 1 writes vs[cur+1], stage 2 vs[cur+2] and RK3's stage 3 vs[cur+1].
+  double *vs[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
   void *F = nullptr, *CM = nullptr, *FD = nullptr, *E = nullptr;
   void *dU = nullptr, *Cdt = nullptr;   // R: the within-stage sums (FP64 in the drop-in)
   bool fused = true;          // fused sweep kernels (nx <= 512) or the split path
   int march = 16;             // cells per thread in the y/z marching sweeps
-  // edges per thread along z in the EMF kernel: 16 (z-march) for FP64, one
-  // edge per thread (0) for the mixed form, where more waves in flight pay
-  // more than the register-carried z-stencil
-  int emf_march = 16;
+  // EMF kernel: n > 0 register z-march of n edges per thread, -n LDS-staged
+  // 64 x 4 tiles marching n planes, 0 one edge per thread.  FP64 runs the
+  // LDS tiles (32 planes); the mixed form one edge per thread, where more
+  // waves in flight pay more than the staged plane
+  int emf_march = -32;
   double *cdt_part = nullptr;
   int nparts = 0;
-  double *cur_vs = nullptr;   // VsA or VsB: faces of the current state
   float *net = nullptr;       // RMHD learned con2prim guess, when enabled
   bool net_on = false;
 
   ~Mhd3dEngine() override
   {
-    void *ptrs[] = {V, U, U0, VsA, VsB, Bs0, F, CM, FD, E, dU, Cdt, cdt_part, net};
+    void *ptrs[] = {V, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net};
     for (void *p : ptrs) if (p) (void)hipFree(p);
   }
 
@@ -89,18 +94,18 @@ public:
     // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
     if (const char *m = getenv("GPL_MARCH")) march = atoi(m);
     if (march != 8 && march != 16 && march != 32) march = 16;
-    emf_march = fp64 ? 16 : 0;
+    emf_march = fp64 ? -32 : 0;
     if (const char *m = getenv("GPL_EMF_MARCH")) emf_march = atoi(m);
-    if (emf_march != 0 && emf_march != 4 && emf_march != 8 && emf_march != 16 && emf_march != 32)
-      emf_march = 16;
+    {
+      const int l = emf_march < 0 ? -emf_march : emf_march;
+      if (emf_march != 0 && l != 4 && l != 8 && l != 16 && l != 32) emf_march = fp64 ? -32 : 0;
+    }
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);
     rc |= dev_alloc(bytes, (void **)&U, 5*n*sizeof(double));
-    rc |= dev_alloc(bytes, (void **)&U0, 5*n*sizeof(double));
-    rc |= dev_alloc(bytes, (void **)&VsA, 3*n*sizeof(double));
-    rc |= dev_alloc(bytes, (void **)&VsB, 3*n*sizeof(double));
-    rc |= dev_alloc(bytes, (void **)&Bs0, 3*n*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&Us, 5*n*sizeof(double));
+    for (int q = 0; q < 3; q++) rc |= dev_alloc(bytes, (void **)&vs[q], 3*n*sizeof(double));
     if (fused) {
       rc |= dev_alloc(bytes, &dU, 5*n*rs);
       rc |= dev_alloc(bytes, &Cdt, n*rs);
@@ -111,13 +116,13 @@ public:
     rc |= dev_alloc(bytes, &FD, 15*n*rs);
     rc |= dev_alloc(bytes, &E, 3*n*rs);
     rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
-    cur_vs = VsA;
+    cur = 0;
     return rc ? GPL_ENOMEM : 0;
   }
 
-  int upload(const double *vc, const double *const *vs, int layout) override
+  int upload(const double *vc, const double *const *vsh, int layout) override
   {
-    if (!vs || !vs[0] || !vs[1] || !vs[2]) return GPL_EINVAL;
+    if (!vsh || !vsh[0] || !vsh[1] || !vsh[2]) return GPL_EINVAL;
     const Layout L = make_layout(cfg, layout);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     std::vector<double> hv(8*n), hu(5*n), hs(3*n);
@@ -125,7 +130,7 @@ public:
       for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
           for (int i = 0; i < nx; i++)
-            hs[d*n + ((size_t)k*ny + j)*nx + i] = vs[d][L.face(d, i, j, k)];
+            hs[d*n + ((size_t)k*ny + j)*nx + i] = vsh[d][L.face(d, i, j, k)];
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
@@ -147,13 +152,13 @@ public:
     int rc = fp64 ? to_device<double>(V, hv) : to_device<float>(V, hv);
     if (rc) return rc;
     if (hipMemcpy(U, hu.data(), 5*n*sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(VsA, hs.data(), 3*n*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(vs[0], hs.data(), 3*n*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return GPL_EHIP;
-    cur_vs = VsA;
+    cur = 0;
     return 0;
   }
 
-  int download(double *vc, double *const *vs, double *uc, int layout) override
+  int download(double *vc, double *const *vsh, double *uc, int layout) override
   {
     const Layout L = make_layout(cfg, layout);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -166,23 +171,23 @@ public:
             for (int i = 0; i < nx; i++)
               vc[L.cell(nv, i, j, k)] = hv[nv*n + ((size_t)k*ny + j)*nx + i];
     }
-    if (vs) {
+    if (vsh) {
       std::vector<double> hs(3*n);
-      if (hipMemcpy(hs.data(), cur_vs, 3*n*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      if (hipMemcpy(hs.data(), vs[cur], 3*n*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return GPL_EHIP;
       for (int d = 0; d < 3; d++) {
-        if (!vs[d]) continue;
+        if (!vsh[d]) continue;
         for (int k = 0; k < nz; k++)
           for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++)
-              vs[d][L.face(d, i, j, k)] = hs[d*n + ((size_t)k*ny + j)*nx + i];
+              vsh[d][L.face(d, i, j, k)] = hs[d*n + ((size_t)k*ny + j)*nx + i];
         if (L.ary) {   // the duplicate upper face equals the lower one (PeriodicBoundary)
           for (int k = 0; k < nz; k++)
             for (int j = 0; j < ny; j++)
               for (int i = 0; i < nx; i++) {
-                if (d == 0 && i == 0) vs[0][L.face(0, nx, j, k)] = hs[((size_t)k*ny + j)*nx];
-                if (d == 1 && j == 0) vs[1][L.face(1, i, ny, k)] = hs[n + ((size_t)k*ny)*nx + i];
-                if (d == 2 && k == 0) vs[2][L.face(2, i, j, nz)] = hs[2*n + (size_t)j*nx + i];
+                if (d == 0 && i == 0) vsh[0][L.face(0, nx, j, k)] = hs[((size_t)k*ny + j)*nx];
+                if (d == 1 && j == 0) vsh[1][L.face(1, i, ny, k)] = hs[n + ((size_t)k*ny)*nx + i];
+                if (d == 2 && k == 0) vsh[2][L.face(2, i, j, nz)] = hs[2*n + (size_t)j*nx + i];
               }
         }
       }
@@ -206,10 +211,12 @@ public:
   }
 
   template <typename R>
-  void enqueue_stage(double *vin, double *vout, RKW w, int first_stage, int want_dt, LaunchTimer *t)
+  void enqueue_stage(const double *uin, double *uout, const double *vin, double *vout,
+                     const double *vbase, RKW w, int first_stage, int want_dt, LaunchTimer *t)
   {
     Mhd3dBufs<R> b;
-    b.V = (R *)V; b.U = U; b.U0 = U0; b.Vin = vin; b.Vout = vout; b.Bs0 = Bs0;
+    b.V = (R *)V; b.Uin = uin; b.U = uout; b.U0 = U;
+    b.Vin = vin; b.Vout = vout; b.Bs0 = vbase;
     b.F = (R *)F; b.CM = (R *)CM; b.FD = (R *)FD; b.E = (R *)E;
     b.dU = (R *)dU; b.Cdt = (R *)Cdt;
     b.cdt_part = cdt_part; b.err = env.err;
@@ -235,23 +242,34 @@ public:
     const RKW w1 = {0.0, 0.0, 1.0};
     const RKW w2 = cfg.time_stepping == GPL_RK3 ? RKW{0.75, 0.25, 0.25} : RKW{0.5, 0.5, 0.5};
     const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
-    const int nst = cfg.time_stepping == GPL_RK3 ? 3 : 2;
-    double *in = cur_vs, *out = cur_vs == VsA ? VsB : VsA;
-    for (int st = 1; st <= nst; st++) {
-      const RKW w = st == 1 ? w1 : (st == 2 ? w2 : w3);
-      if (fp64) enqueue_stage<double>(in, out, w, st == 1, st == 1, t);
-      else      enqueue_stage<float>(in, out, w, st == 1, st == 1, t);
-      double *tmp = in; in = out; out = tmp;
+    const bool rk3 = cfg.time_stepping == GPL_RK3;
+    double *f0 = vs[cur], *f1 = vs[(cur + 1) % 3], *f2 = vs[(cur + 2) % 3];
+    // stage 1: U^n, faces f0 -> Us, f1
+    stage(U, Us, f0, f1, f0, w1, 1, t);
+    if (!rk3) {
+      // stage 2: 1/2 U^n + 1/2 (Us + dt L(Us)) -> U, f2
+      stage(Us, U, f1, f2, f0, w2, 0, t);
+      cur = (cur + 2) % 3;
+    } else {
+      stage(Us, Us, f1, f2, f0, w2, 0, t);     // -> Us, f2
+      stage(Us, U, f2, f1, f0, w3, 0, t);      // -> U, f1
+      cur = (cur + 1) % 3;
     }
     if (t) t->begin(K_DT);
     launch_dt(cdt_part, nparts, 3.0, env.ctl, cfg.cfl, cfg.cfl_max_var, advance_dt, env.stream);
     if (t) t->end(K_DT);
-    cur_vs = in;
   }
 
-  // RK2 maps the face buffers A -> B -> A within a step; RK3 flips them
-  int graph_period() const override { return cfg.time_stepping == GPL_RK3 ? 2 : 1; }
-  const void *state_key() const override { return cur_vs; }
+  void stage(const double *uin, double *uout, const double *vin, double *vout, const double *vbase,
+             RKW w, int first, LaunchTimer *t)
+  {
+    if (fp64) enqueue_stage<double>(uin, uout, vin, vout, vbase, w, first, first, t);
+    else      enqueue_stage<float>(uin, uout, vin, vout, vbase, w, first, first, t);
+  }
+
+  // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
+  int graph_period() const override { return 3; }
+  const void *state_key() const override { return vs[cur]; }
 
   int set_c2p_network(const float *w) override
   {

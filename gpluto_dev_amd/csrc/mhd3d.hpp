@@ -16,11 +16,17 @@ struct Box3 {
 template <typename R>
 struct Mhd3dBufs {
   R *V;            // [8][n]
+  // Conserved state: the stage reads Uin (and from its second stage on U0,
+  // the state at the start of the step) and writes U, cell by cell; U may be
+  // Uin or U0, never another cell's input.
+  const double *Uin; // [5][n]
   double *U;       // [5][n]
-  double *U0;      // [5][n]
+  const double *U0;  // [5][n]
+  // Staggered faces: three distinct buffers, so that no stage overwrites a
+  // face another thread still reads.
   const double *Vin; // [3][n] faces at stage start
   double *Vout;    // [3][n] faces after the stage
-  double *Bs0;     // [3][n]
+  const double *Bs0; // [3][n] faces at the start of the step (second stage on)
   R *F;            // [3][5][n]
   R *CM;           // [3][n]
   R *FD;           // [3][5][n]
@@ -29,7 +35,7 @@ struct Mhd3dBufs {
   R *Cdt;          // [n]    C_dt accumulator within a stage (fused path)
   double *cdt_part;
   uint32_t *err;
-  int emf_march;   // edges per thread along z in the EMF kernel (0: one edge per thread)
+  int emf_march;   // EMF kernel: 0 one edge per thread, n > 0 register z-march of n, -n LDS tiles marching n
   int phys;        // 0: ideal MHD, 1: relativistic MHD (FP64, fused kernels)
   const float *net; // RMHD: learned con2prim guess (nullptr: the reference's A27 guess)
 };

@@ -173,7 +173,7 @@ void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
 template <typename R>
 __global__ __launch_bounds__(256)
 void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
-                  double *__restrict__ U, double *__restrict__ U0,
+                  const double *Uin, double *U, const double *U0,
                   const StepCtl *__restrict__ ctl, RKW w, int first_stage,
                   double *__restrict__ cdt_part)
 {
@@ -198,13 +198,8 @@ void k_rhs_update(Box3 g, const R *__restrict__ F, const R *__restrict__ CM,
       double du = rx;
       du += ry;
       du += rz;
-      if (first_stage) {
-        const double u0 = U[o + c];
-        U0[o + c] = u0;
-        U[o + c] = u0 + w.wR*du;
-      } else {
-        U[o + c] = w.w0*U0[o + c] + w.w1*U[o + c] + w.wR*du;
-      }
+      if (first_stage) U[o + c] = Uin[o + c] + w.wR*du;
+      else             U[o + c] = w.w0*U0[o + c] + w.w1*Uin[o + c] + w.wR*du;
     }
     if (cdt_part) {
       cdt = 0.5*((double)CM[c] + (double)CM[ci])/g.dx;
@@ -303,10 +298,10 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
 // (getMax, update_stage.cpp:446-458).
 // ---------------------------------------------------------------------------
 template <typename R, int D, int LIM, int SOLVER, int L, int PH>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && D == 1 ? 4 : 1)))
 void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
                    R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
-                   double *__restrict__ U, double *__restrict__ U0,
+                   const double *Uin, double *U, const double *U0,
                    const StepCtl *__restrict__ ctl, RKW w, int first_stage, int want_cdt,
                    double *__restrict__ cdt_part, R gamma, uint32_t *__restrict__ err)
 {
@@ -323,6 +318,19 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
   uint32_t e = 0;
   if (t < g.nx*na*nseg) {
     const size_t n = (size_t)g.ncell;
+    // uniform array bases (scalar registers) + 32-bit cell offsets
+    const R *Vq[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) Vq[q] = V + VAR[q]*n;
+    const double *Bn = Vs + D*n;
+    R *dUq[5];
+    double *Uq[5];
+    const double *Uiq[5], *U0q[5];
+#pragma unroll
+    for (int nv = 0; nv < 5; nv++) {
+      dUq[nv] = dU + nv*n; Uq[nv] = U + nv*n; Uiq[nv] = Uin + nv*n; U0q[nv] = U0 + nv*n;
+    }
+    R *fd0 = FD + (size_t)D*5*n;
     const int i = t % g.nx;
     const int r = t / g.nx;
     const int a = r % na, seg = r / na;
@@ -333,12 +341,10 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
     const double ddx = D == 1 ? g.dy : g.dz;
     R Vm[7], Vc[7], Vp[7], vlc[7];
     {
-      const int cm2 = base + wrapm(s0 - 2, nd)*sd, cm1 = base + wrapm(s0 - 1, nd)*sd;
-      const int c0 = base + wrapm(s0, nd)*sd;
+      const unsigned cm2 = base + wrapm(s0 - 2, nd)*sd, cm1 = base + wrapm(s0 - 1, nd)*sd;
+      const unsigned c0 = base + wrapm(s0, nd)*sd;
 #pragma unroll
-      for (int q = 0; q < 7; q++) {
-        Vm[q] = V[VAR[q]*n + cm2]; Vc[q] = V[VAR[q]*n + cm1]; Vp[q] = V[VAR[q]*n + c0];
-      }
+      for (int q = 0; q < 7; q++) { Vm[q] = Vq[q][cm2]; Vc[q] = Vq[q][cm1]; Vp[q] = Vq[q][c0]; }
       R lo[7];
 #pragma unroll
       for (int q = 0; q < 7; q++) {
@@ -355,27 +361,27 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
     }
     double Fp[5] = {0, 0, 0, 0, 0}, cmp = 0.0;
     for (int f = s0; f <= s1; f++) {
-      const int cf = base + wrapm(f, nd)*sd;
-      const int cn = base + wrapm(f + 1, nd)*sd;
-      const int cc = base + wrapm(f - 1, nd)*sd;    // the cell completed by face f
+      const unsigned cf = base + wrapm(f, nd)*sd;
+      const unsigned cn = base + wrapm(f + 1, nd)*sd;
+      const unsigned cc = base + wrapm(f - 1, nd)*sd;    // the cell completed by face f
       // early loads for cell f-1: accumulator, C_dt, and for z the state
       double acc[5], u_old[5], u0_old[5], cacc = 0.0;
       if (f > s0) {
 #pragma unroll
-        for (int nv = 0; nv < 5; nv++) acc[nv] = (double)dU[nv*n + cc];
+        for (int nv = 0; nv < 5; nv++) acc[nv] = (double)dUq[nv][cc];
         if (want_cdt) cacc = (double)Cdt[cc];
         if (D == 2) {
 #pragma unroll
-          for (int nv = 0; nv < 5; nv++) u_old[nv] = U[nv*n + cc];
+          for (int nv = 0; nv < 5; nv++) u_old[nv] = Uiq[nv][cc];
           if (!first_stage) {
 #pragma unroll
-            for (int nv = 0; nv < 5; nv++) u0_old[nv] = U0[nv*n + cc];
+            for (int nv = 0; nv < 5; nv++) u0_old[nv] = U0q[nv][cc];
           }
         }
       }
       R vl[8], vr[8];
 #pragma unroll
-      for (int q = 0; q < 7; q++) Vp[q] = V[VAR[q]*n + cn];
+      for (int q = 0; q < 7; q++) Vp[q] = Vq[q][cn];
 #pragma unroll
       for (int q = 0; q < 7; q++) {
         const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
@@ -390,15 +396,14 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           for (int q = 0; q < 7; q++) { vr[VAR[q]] = Vc[q]; vlc[q] = Vc[q]; }
         }
       }
-      vl[BXn] = vr[BXn] = (R)Vs[D*n + cf];
+      vl[BXn] = vr[BXn] = (R)Bn[cf];
       FaceFlux<R> ff;
       riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff);
       e |= ff.err;
       if (f < s1) {
         R et, eb, aL, aR, dd;
         uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-        R *fd = FD + (size_t)D*5*n;
-        fd[0*n + cf] = et; fd[1*n + cf] = eb; fd[2*n + cf] = aL; fd[3*n + cf] = aR; fd[4*n + cf] = dd;
+        fd0[0*n + cf] = et; fd0[1*n + cf] = eb; fd0[2*n + cf] = aL; fd0[3*n + cf] = aR; fd0[4*n + cf] = dd;
       }
       const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
                            (double)ff.f[7]};
@@ -410,14 +415,12 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           if (PH == 0) rr += 0.0;
           double du = acc[nv];
           du += rr;
-          const size_t o = nv*n + cc;
           if (D == 1) {
-            dU[o] = (R)du;
+            dUq[nv][cc] = (R)du;
           } else if (first_stage) {
-            U0[o] = u_old[nv];
-            U[o] = u_old[nv] + w.wR*du;
+            Uq[nv][cc] = u_old[nv] + w.wR*du;
           } else {
-            U[o] = w.w0*u0_old[nv] + w.w1*u_old[nv] + w.wR*du;
+            Uq[nv][cc] = w.w0*u0_old[nv] + w.w1*u_old[nv] + w.wR*du;
           }
         }
         if (want_cdt) {
@@ -564,6 +567,140 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
   }
 }
 
+// The same EMFs with each plane of a 64 x 4 tile staged through LDS.  Wave
+// w owns row y0 + w; per plane every thread loads its own cell of the seven
+// in-plane stencil arrays and the nine face records once, three lanes per
+// row add the x halo (i-2, i-1, i+64) and waves 0-3 the y halo (j-2, j-1,
+// j+4 and the (i-1, j-1) corner), so each HBM value of a plane is loaded by
+// one thread instead of by every stencil that uses it.  The z-stencils and
+// the face records of plane k-1 stay in registers as in k_emf_march.
+template <typename R, int LIMV, int L>
+__global__ __launch_bounds__(256)
+void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
+{
+  constexpr int TX = 64, TY = 4;
+  __shared__ R xs[4][TY][TX + 3];        // By, e[K][J], Bz, e[J][K] at x0-2 .. x0+64
+  __shared__ R ys[4][TY + 3][TX];        // Bx, e[K][I], Bz, e[I][K] at y0-2 .. y0+4
+  __shared__ R cxs[3][TY + 1][TX];       // x-face records at y0-1 ..
+  __shared__ R cys[3][TY][TX + 1];       // y-face records at x0-1 ..
+  __shared__ R czs[3][TY + 1][TX + 1];   // z-face records at (x0-1 .., y0-1 ..)
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int ntx = (nx + TX - 1)/TX, nty = (ny + TY - 1)/TY;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int txb = t % ntx, rr = t / ntx, tyb = rr % nty, seg = rr / nty;
+  const int x0 = txb*TX, y0 = tyb*TY;
+  const int tw = nx - x0 < TX ? nx - x0 : TX, th = ny - y0 < TY ? ny - y0 : TY;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const bool act = tx < tw && ty < th;
+  const size_t n = (size_t)g.ncell;
+  const int pl = nx*ny;
+  const int k0 = seg*L, k1 = k0 + L < nz ? k0 + L : nz;
+  // uniform base of every array (scalar registers) + 32-bit cell offsets
+  const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
+  const R *ex0 = FD, *ex1 = FD + n, *fx2 = FD + 2*n, *fx3 = FD + 3*n, *fx4 = FD + 4*n;
+  const R *ey0 = FD + 5*n, *ey1 = FD + 6*n, *fy2 = FD + 7*n, *fy3 = FD + 8*n, *fy4 = FD + 9*n;
+  const R *ez0 = FD + 10*n, *ez1 = FD + 11*n, *fz2 = FD + 12*n, *fz3 = FD + 13*n, *fz4 = FD + 14*n;
+  R *Ex = E, *Ey = E + n, *Ez = E + 2*n;
+  const unsigned col = (x0 + (act ? tx : 0)) + nx*(y0 + (act ? ty : 0));
+  // x halo: lanes 0, 1, 2 of an active row load x0-2, x0-1, x0+tw
+  const bool hx = ty < th && tx < 3;
+  const int hxc = tx == 2 ? tw + 2 : tx;
+  const unsigned hxo = wrapm(tx == 2 ? x0 + tw : x0 - 2 + tx, nx) + nx*(y0 + (ty < th ? ty : 0));
+  // y halo: wave 0 row y0-2, wave 1 row y0-1, wave 2 row y0+th; wave 3 lane 0 the corner
+  const bool hy = ty < 3 && tx < tw;
+  const int hyr = ty == 2 ? th + 2 : ty;
+  const unsigned hyo = (x0 + (tx < tw ? tx : 0)) + nx*wrapm(ty == 2 ? y0 + th : y0 - 2 + ty, ny);
+  const bool hc = ty == 3 && tx == 0;
+  const unsigned hco = wrapm(x0 - 1, nx) + nx*wrapm(y0 - 1, ny);
+  R zbx[4], zex[4], zby[4], zey[4];   // z-stencils of X3a (Bx, e[J][I]) and X3b (By, e[I][J])
+  R cxm[3], cym[3];                   // x- and y-face records of plane k-1
+  if (act) {
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      const unsigned c = col + wrapm(k0 + o - 2, nz)*pl;
+      zbx[o] = (R)Bx[c]; zex[o] = ex1[c];
+      zby[o] = (R)By[c]; zey[o] = ey0[c];
+    }
+    const unsigned c = col + wrapm(k0 - 1, nz)*pl;
+    cxm[0] = fx2[c]; cxm[1] = fx3[c]; cxm[2] = fx4[c];
+    cym[0] = fy2[c]; cym[1] = fy3[c]; cym[2] = fy4[c];
+  }
+  for (int k = k0; k < k1; k++) {
+    const unsigned b = (unsigned)k*pl;
+    if (act) {
+      const unsigned c = col + b, cn = col + wrapm(k + 1, nz)*pl;
+      zbx[3] = (R)Bx[cn]; zex[3] = ex1[cn];
+      zby[3] = (R)By[cn]; zey[3] = ey0[cn];
+      const R bz = (R)Bz[c];
+      xs[0][ty][tx + 2] = zby[2];  xs[1][ty][tx + 2] = ey1[c];
+      xs[2][ty][tx + 2] = bz;      xs[3][ty][tx + 2] = ez0[c];
+      ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = ex0[c];
+      ys[2][ty + 2][tx] = bz;      ys[3][ty + 2][tx] = ez1[c];
+      cxs[0][ty + 1][tx] = fx2[c]; cxs[1][ty + 1][tx] = fx3[c]; cxs[2][ty + 1][tx] = fx4[c];
+      cys[0][ty][tx + 1] = fy2[c]; cys[1][ty][tx + 1] = fy3[c]; cys[2][ty][tx + 1] = fy4[c];
+      czs[0][ty + 1][tx + 1] = fz2[c]; czs[1][ty + 1][tx + 1] = fz3[c]; czs[2][ty + 1][tx + 1] = fz4[c];
+    }
+    if (hx) {
+      const unsigned c = hxo + b;
+      xs[0][ty][hxc] = (R)By[c];  xs[1][ty][hxc] = ey1[c];
+      xs[2][ty][hxc] = (R)Bz[c];  xs[3][ty][hxc] = ez0[c];
+      if (tx == 1) {
+        cys[0][ty][0] = fy2[c]; cys[1][ty][0] = fy3[c]; cys[2][ty][0] = fy4[c];
+        czs[0][ty + 1][0] = fz2[c]; czs[1][ty + 1][0] = fz3[c]; czs[2][ty + 1][0] = fz4[c];
+      }
+    }
+    if (hy) {
+      const unsigned c = hyo + b;
+      ys[0][hyr][tx] = (R)Bx[c];  ys[1][hyr][tx] = ex0[c];
+      ys[2][hyr][tx] = (R)Bz[c];  ys[3][hyr][tx] = ez1[c];
+      if (ty == 1) {
+        cxs[0][0][tx] = fx2[c]; cxs[1][0][tx] = fx3[c]; cxs[2][0][tx] = fx4[c];
+        czs[0][0][tx + 1] = fz2[c]; czs[1][0][tx + 1] = fz3[c]; czs[2][0][tx + 1] = fz4[c];
+      }
+    }
+    if (hc) {
+      const unsigned c = hco + b;
+      czs[0][0][0] = fz2[c]; czs[1][0][0] = fz3[c]; czs[2][0][0] = fz4[c];
+    }
+    __syncthreads();
+    if (act) {
+      R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        cx[q] = cxs[q][ty + 1][tx];  cxj[q] = cxs[q][ty][tx];
+        cy[q] = cys[q][ty][tx + 1];  cyi[q] = cys[q][ty][tx];
+        cz[q] = czs[q][ty + 1][tx + 1];
+        czi[q] = czs[q][ty + 1][tx];
+        czj[q] = czs[q][ty][tx + 1];
+      }
+      R q4[4], e4[4];
+#pragma unroll
+      for (int o = 0; o < 4; o++) { q4[o] = xs[0][ty][tx + o]; e4[o] = xs[1][ty][tx + o]; }
+      R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                      // X1a
+#pragma unroll
+      for (int o = 0; o < 4; o++) { q4[o] = ys[0][ty + o][tx]; e4[o] = ys[1][ty + o][tx]; }
+      ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                       // X2b
+#pragma unroll
+      for (int o = 0; o < 4; o++) { q4[o] = xs[2][ty][tx + o]; e4[o] = xs[3][ty][tx + o]; }
+      R ey = uct_val<R, LIMV>(q4, e4, cxm, cx, -1);                      // X1b
+      ey += uct_val<R, LIMV>(zbx, zex, czi, cz, +1);                     // X3a
+#pragma unroll
+      for (int o = 0; o < 4; o++) { q4[o] = ys[2][ty + o][tx]; e4[o] = ys[3][ty + o][tx]; }
+      R ex = uct_val<R, LIMV>(q4, e4, cym, cy, +1);                      // X2a
+      ex += uct_val<R, LIMV>(zby, zey, czj, cz, -1);                     // X3b
+      const unsigned c = col + b;
+      Ex[c] = ex;
+      Ey[c] = ey;
+      Ez[c] = ez;
+#pragma unroll
+      for (int o = 0; o < 3; o++) { zbx[o] = zbx[o + 1]; zex[o] = zex[o + 1]; zby[o] = zby[o + 1]; zey[o] = zey[o + 1]; }
+#pragma unroll
+      for (int q = 0; q < 3; q++) { cxm[q] = cx[q]; cym[q] = cy[q]; }
+    }
+    __syncthreads();                 // the next plane overwrites the tile
+  }
+}
+
 // The same edge EMFs with one thread per edge and every stencil value loaded
 // from memory (2D thread blocks over x-y; blockIdx.z is the plane).
 template <typename R, int LIMV>
@@ -658,7 +795,7 @@ __device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const
 template <typename R, int PH>
 __global__ __launch_bounds__(256)
 void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
-              double *__restrict__ Vout, double *__restrict__ Bs0,
+              double *__restrict__ Vout, const double *__restrict__ Bs0,
               double *__restrict__ U, R *__restrict__ V,
               const StepCtl *__restrict__ ctl, RKW w, int first_stage,
               double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err,
@@ -684,10 +821,8 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
   for (int d = 0; d < 3; d++) {
     const double *vin = Vin + d*n;
     if (first_stage) {
-      const double s0 = vin[c], s1 = vin[up[d]];
-      if (live) Bs0[d*n + c] = s0;
-      b_lo[d] = s0 + w.wR*rl[d];
-      b_hi[d] = s1 + w.wR*rh[d];
+      b_lo[d] = vin[c] + w.wR*rl[d];
+      b_hi[d] = vin[up[d]] + w.wR*rh[d];
     } else {
       const double *b0 = Bs0 + d*n;
       b_lo[d] = w.w0*b0[c] + w.w1*vin[c] + w.wR*rl[d];
@@ -758,7 +893,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   case K_RHS:
     hipLaunchKernelGGL((k_rhs_update<R>), dim3(nb), dim3(T), 0, s,
-                       g, b.F, b.CM, b.U, b.U0, ctl, w, first_stage,
+                       g, b.F, b.CM, b.Uin, b.U, b.U0, ctl, w, first_stage,
                        want_dt ? b.cdt_part : nullptr);
     break;
   case K_SWEEPX_F: {
@@ -770,14 +905,14 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
   case K_SWEEPY_F: {
     const int nt = g.nx*g.nz*((g.ny + L - 1)/L);
     hipLaunchKernelGGL((k_sweep_march<R, 1, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
-                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.Uin, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
     break;
   }
   case K_SWEEPZ_F: {
     const int nt = g.nx*g.ny*((g.nz + L - 1)/L);
     hipLaunchKernelGGL((k_sweep_march<R, 2, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
-                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.U, b.U0, ctl, w, first_stage, want_dt,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.Uin, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
     break;
   }
@@ -788,6 +923,17 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     } else if (b.emf_march == 4) {
       const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 3)/4);
       hipLaunchKernelGGL((k_emf_march<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+    } else if (b.emf_march < 0) {        // LDS-staged tiles marching -emf_march planes
+      const int lt = -b.emf_march;
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + lt - 1)/lt);
+      if (lt == 4)
+        hipLaunchKernelGGL((k_emf_tile<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+      else if (lt == 8)
+        hipLaunchKernelGGL((k_emf_tile<R, LIM, 8>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+      else if (lt == 16)
+        hipLaunchKernelGGL((k_emf_tile<R, LIM, 16>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
+      else
+        hipLaunchKernelGGL((k_emf_tile<R, LIM, 32>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
     } else if (b.emf_march == 8) {
       const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + 7)/8);
       hipLaunchKernelGGL((k_emf_march<R, LIM, 8>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);

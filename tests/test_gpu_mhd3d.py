@@ -87,10 +87,14 @@ def test_advance_step_seam_matches_oracle():
     g.close()
 
 
-def test_mixed_precision_l1_within_target():
+@pytest.mark.parametrize("emf", [None, "0", "-8"])
+def test_mixed_precision_l1_within_target(emf, monkeypatch):
     """Mixed form: FP32 V/PLM/HLLD, FP64 U, faces, CT and dt.  Relative L1 per
     variable against the FP64 oracle stays <= 1e-5; div B stays at FP64
-    round-off because the face update is FP64."""
+    round-off because the face update is FP64.  Default EMF kernel and the
+    LDS-staged tile kernel."""
+    if emf is not None:
+        monkeypatch.setenv("GPL_EMF_MARCH", emf)
     n, steps = (32, 32, 32), 10
     (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(
         n, 13, steps, 1e-3, precision="mixed")
@@ -105,16 +109,21 @@ def test_mixed_precision_l1_within_target():
     assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * n[0]
 
 
-@pytest.mark.parametrize("path", ["split", "march8", "march16", "march32"])
+@pytest.mark.parametrize("path", ["split", "march8", "march16", "march32",
+                                  "emf0", "emf8", "emf-4", "emf-8", "emf-16", "emf-32"])
 def test_kernel_paths_bitwise(path, monkeypatch):
     """Split (per-interface) and fused kernels (LDS x-row + register-window y/z
-    march, every march length), including segments that do not divide the
-    column (ny = 12, nz = 20, 17, 31)."""
+    march, every march length), and every EMF kernel (one edge per thread,
+    register march of 8, LDS-staged tiles marching 4 to 32 planes), including
+    segments and tiles that do not divide the box (ny = 12, nz = 20, 17, 31,
+    nx = 70)."""
     if path == "split":
         monkeypatch.setenv("GPL_SPLIT_KERNELS", "1")
     elif path.startswith("march"):
         monkeypatch.setenv("GPL_MARCH", path[5:])
-    for n in ((16, 12, 20), (20, 17, 31)):
+    elif path.startswith("emf"):
+        monkeypatch.setenv("GPL_EMF_MARCH", path[3:])
+    for n in ((16, 12, 20), (20, 17, 31), (70, 9, 6)):
         (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(n, 21, 5, 2e-3)
         assert gerr == oerr == 0
         assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
