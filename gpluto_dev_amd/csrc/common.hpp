@@ -20,7 +20,7 @@ struct RKW { double w0, w1, wR; };
 
 // Every kernel the library launches, for the per-kernel timing tables.
 enum KernelId { K_SWEEP_X = 0, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F,
-                K_EMF, K_FINISH, K_DT, K_HD_X, K_HD_Y, K_HD_Z, K_COUNT };
+                K_EMF, K_FINISH, K_DT, K_HD_X, K_HD_Y, K_HD_Z, K_SWEEPXY_F, K_COUNT };
 
 // invDt_hyp = max over the partial maxima / DIMENSIONS; with advance != 0
 // also NextTimeStep and time += dt on the device.

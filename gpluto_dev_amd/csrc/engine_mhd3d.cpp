@@ -60,6 +60,10 @@ public:
   void *dU = nullptr, *Cdt = nullptr;   // R: the within-stage sums (FP64 in the drop-in)
   bool fused = true;          // fused sweep kernels (nx <= 512) or the split path
   int march = 16;             // cells per thread in the y/z marching sweeps
+  // x and y sweeps in one row-marching kernel: the mixed form, where it saves
+  // HBM traffic (its three LDS rows must fit in 64 KB); the FP64 sweeps are
+  // VALU-bound and run faster as two kernels
+  bool fused_xy = false;
   // EMF kernel: n > 0 register z-march of n edges per thread, -n LDS-staged
   // 64 x 4 tiles marching n planes, 0 one edge per thread.  FP64 runs the
   // LDS tiles (32 planes); the mixed form one edge per thread, where more
@@ -100,6 +104,9 @@ public:
       const int l = emf_march < 0 ? -emf_march : emf_march;
       if (emf_march != 0 && l != 4 && l != 8 && l != 16 && l != 32) emf_march = fp64 ? -32 : 0;
     }
+    fused_xy = !fp64;
+    if (const char *m = getenv("GPL_FUSED_XY")) fused_xy = atoi(m) != 0;
+    fused_xy = fused_xy && fused && 20*(size_t)c.nx*rs <= 65536;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);
@@ -225,8 +232,9 @@ public:
     b.net = net_on ? net : nullptr;
     static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
     static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
-    const int *seq = fused ? fused_seq : split_seq;
-    const int nk = fused ? 5 : 6;
+    static const int xy_seq[] = {K_SWEEPXY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
+    const int *seq = fused ? (fused_xy ? xy_seq : fused_seq) : split_seq;
+    const int nk = fused ? (fused_xy ? 4 : 5) : 6;
     for (int q = 0; q < nk; q++) {
       const int k = seq[q];
       if (t) t->begin(k);

@@ -22,7 +22,8 @@ namespace {
 const char *kKernelNames[K_COUNT] = {"k_sweep_x", "k_sweep_y", "k_sweep_z", "k_rhs_update",
                                      "k_sweepx_fused", "k_sweepy_fused", "k_sweepz_fused",
                                      "k_emf", "k_finish", "k_dt",
-                                     "k_hd_sweep_x", "k_hd_sweep_y", "k_hd_sweep_z"};
+                                     "k_hd_sweep_x", "k_hd_sweep_y", "k_hd_sweep_z",
+                                     "k_sweepxy_fused"};
 
 // HIP events around each launch on the context stream; harvested after the
 // stream drains.

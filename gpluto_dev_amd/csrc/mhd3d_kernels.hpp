@@ -286,6 +286,181 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Fused x + y sweep: one block per (plane k, y-segment of L rows) with one
+// thread per x-cell (blockDim.x == nx).  At row f the block does
+//  * the x sweep of row f through LDS, as k_sweepx_fused: every slope once,
+//    the left state of face i+1 made by cell i, HLLD, UCT store;
+//  * the y face f from the register window, as k_sweep_march;
+// and completes cell (i, f-1) from rhs_x of row f-1, kept in registers:
+// dUc = rhs_x; dUc += rhs_y and C_dt likewise (update_stage.cpp:173-184,
+// 272-284), the reference's order.  dU and C_dt are written once per stage
+// instead of written, read and written again, and V is read once.  The
+// three LDS rows are disjoint, so a row needs three barriers.
+// ---------------------------------------------------------------------------
+template <typename R, int LIM, int SOLVER, int L, int PH>
+__global__ __launch_bounds__(512)
+void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
+                R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
+                const StepCtl *__restrict__ ctl, R gamma, int want_cdt,
+                uint32_t *__restrict__ err)
+{
+  extern __shared__ __align__(16) unsigned char gpl_smem[];
+  const int nx = g.nx, ny = g.ny, i = threadIdx.x;
+  R *sV = (R *)gpl_smem;           // [7][nx] primitives of row f
+  R *sD = sV + 7*nx;               // [7][nx] left state vL(i+1) made by cell i
+  R *sF = sD + 7*nx;               // [6][nx] x flux and cmax of face i
+  const size_t n = (size_t)g.ncell;
+  const int nseg = (ny + L - 1)/L;
+  const int k = blockIdx.x / nseg, seg = blockIdx.x - k*nseg;
+  const int s0 = seg*L, s1 = s0 + L < ny ? s0 + L : ny;
+  const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+  const unsigned base = i + (unsigned)nx*ny*k;
+  // y window order: B1 is a cell value, B2 comes from the face
+  constexpr int VY[7] = {0, 1, 2, 3, 4, 6, 7};
+  const R *Vq[7];
+#pragma unroll
+  for (int q = 0; q < 7; q++) Vq[q] = V + VY[q]*n;
+  const R *V5 = V + 5*n;
+  const double *Bxf = Vs, *Byf = Vs + n;
+  R *fdx = FD, *fdy = FD + 5*n;
+  R *dUq[5];
+#pragma unroll
+  for (int nv = 0; nv < 5; nv++) dUq[nv] = dU + nv*n;
+  const double dtdx = ctl->dt/g.dx, dtdy = ctl->dt/g.dy;
+  uint32_t e = 0;
+  R Vm[7], Vc[7], Vp[7], vlc[7];
+  {
+    const unsigned cm2 = base + wrapm(s0 - 2, ny)*nx, cm1 = base + wrapm(s0 - 1, ny)*nx;
+    const unsigned c0 = base + wrapm(s0, ny)*nx;
+#pragma unroll
+    for (int q = 0; q < 7; q++) { Vm[q] = Vq[q][cm2]; Vc[q] = Vq[q][cm1]; Vp[q] = Vq[q][c0]; }
+    R lo[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
+      lo[q] = Vc[q] - R(0.5)*dv;
+      vlc[q] = Vc[q] + R(0.5)*dv;
+    }
+    if (PH == 1 && superluminal(lo, vlc)) {
+#pragma unroll
+      for (int q = 0; q < 7; q++) vlc[q] = Vc[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) { Vm[q] = Vc[q]; Vc[q] = Vp[q]; }
+  }
+  double Fp[5] = {0, 0, 0, 0, 0}, cmp = 0.0;
+  R rx[5] = {0, 0, 0, 0, 0}, cx = 0;   // rhs_x and the x part of C_dt of row f-1
+  for (int f = s0; f <= s1; f++) {
+    const unsigned cf = base + wrapm(f, ny)*nx;
+    const unsigned cn = base + wrapm(f + 1, ny)*nx;
+    const unsigned cc = base + wrapm(f - 1, ny)*nx;    // the cell completed by face f
+    // the next row and the y face are loaded before the x sweep's barriers
+#pragma unroll
+    for (int q = 0; q < 7; q++) Vp[q] = Vq[q][cn];
+    const R byf = (R)Byf[cf];
+    R rxn[5], cxn = 0;
+    if (f < s1) {                      // block-uniform
+      // ---- x sweep of row f (Vc holds V(f)) ----
+      R v0[7];                         // x order: 0 1 2 3 5 6 7
+      v0[0] = Vc[0]; v0[1] = Vc[1]; v0[2] = Vc[2]; v0[3] = Vc[3];
+      v0[4] = V5[cf]; v0[5] = Vc[5]; v0[6] = Vc[6];
+      const R bn = (R)Bxf[cf];
+#pragma unroll
+      for (int q = 0; q < 7; q++) sV[q*nx + i] = v0[q];
+      __syncthreads();
+      R lo[7], hi[7];
+#pragma unroll
+      for (int q = 0; q < 7; q++) {
+        const R dv = plm_slope<R, LIM>(sV[q*nx + im], v0[q], sV[q*nx + ip]);
+        lo[q] = v0[q] - R(0.5)*dv;
+        hi[q] = v0[q] + R(0.5)*dv;
+      }
+      if (PH == 1 && superluminal(lo, hi)) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) lo[q] = hi[q] = v0[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; q++) sD[q*nx + i] = hi[q];
+      __syncthreads();
+      constexpr int VX[7] = {0, 1, 2, 3, 5, 6, 7};
+      R vl[8], vr[8];
+#pragma unroll
+      for (int q = 0; q < 7; q++) { vl[VX[q]] = sD[q*nx + im]; vr[VX[q]] = lo[q]; }
+      vl[4] = vr[4] = bn;
+      FaceFlux<R> ff;
+      riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
+      e |= ff.err;
+      {
+        R et, eb, aL, aR, dd;
+        uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+        fdx[0*n + cf] = et; fdx[1*n + cf] = eb; fdx[2*n + cf] = aL; fdx[3*n + cf] = aR; fdx[4*n + cf] = dd;
+      }
+      sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
+      sF[3*nx + i] = ff.f[3]; sF[4*nx + i] = ff.f[7];
+      sF[5*nx + i] = rmax(rabs(ff.Sl), rabs(ff.Sr));
+      __syncthreads();
+#pragma unroll
+      for (int nv = 0; nv < 5; nv++) {
+        double r = -dtdx*((double)sF[nv*nx + ip] - (double)sF[nv*nx + i]);
+        if (PH == 0) r += 0.0;         // MHD adds src[] = 0 (rhs.cpp:519); RMHD+CT has no source
+        rxn[nv] = (R)r;
+      }
+      if (want_cdt) cxn = (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx);
+    }
+    // ---- y face f ----
+    R vl[8], vr[8];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
+      vl[VY[q]] = vlc[q];
+      vr[VY[q]] = Vc[q] - R(0.5)*dv;
+      vlc[q] = Vc[q] + R(0.5)*dv;
+    }
+    if (PH == 1) {
+      const R lo[4] = {R(0), vr[1], vr[2], vr[3]};
+      if (superluminal(lo, vlc)) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) { vr[VY[q]] = Vc[q]; vlc[q] = Vc[q]; }
+      }
+    }
+    vl[5] = vr[5] = byf;
+    FaceFlux<R> ff;
+    riemann<R, 1, SOLVER, PH>(vl, vr, gamma, ff);
+    e |= ff.err;
+    if (f < s1) {
+      R et, eb, aL, aR, dd;
+      uct_hll_store<R, 1>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+      fdy[0*n + cf] = et; fdy[1*n + cf] = eb; fdy[2*n + cf] = aL; fdy[3*n + cf] = aR; fdy[4*n + cf] = dd;
+    }
+    const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
+                         (double)ff.f[7]};
+    const double cm = (double)rmax(rabs(ff.Sl), rabs(ff.Sr));
+    if (f > s0) {
+#pragma unroll
+      for (int nv = 0; nv < 5; nv++) {
+        double rr = -dtdy*(F[nv] - Fp[nv]);
+        if (PH == 0) rr += 0.0;
+        double du = (double)rx[nv];
+        du += rr;
+        dUq[nv][cc] = (R)du;
+      }
+      if (want_cdt) {
+        double cd = (double)cx;
+        cd += 0.5*(cmp + cm)/g.dy;
+        Cdt[cc] = (R)cd;
+      }
+    }
+#pragma unroll
+    for (int nv = 0; nv < 5; nv++) { Fp[nv] = F[nv]; rx[nv] = rxn[nv]; }
+    cx = cxn;
+    cmp = cm;
+#pragma unroll
+    for (int q = 0; q < 7; q++) { Vm[q] = Vc[q]; Vc[q] = Vp[q]; }
+  }
+  if (e) atomicOr(err, e);
+}
+
+// ---------------------------------------------------------------------------
 // Fused y / z sweep: each thread marches a segment of L cells along D with a
 // register window (V(f-1), V(f), left state of face f), computes faces
 // s0..s0+L (the last one is the first face of the next segment) and
@@ -899,6 +1074,13 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
   case K_SWEEPX_F: {
     const size_t shm = 14*(size_t)g.nx*sizeof(R);
     hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER, PH>), dim3(g.ny*g.nz), dim3(g.nx), shm, s,
+                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
+    break;
+  }
+  case K_SWEEPXY_F: {
+    const size_t shm = 20*(size_t)g.nx*sizeof(R);
+    const int nb = g.nz*((g.ny + L - 1)/L);
+    hipLaunchKernelGGL((k_sweep_xy<R, LIM, SOLVER, L, PH>), dim3(nb), dim3(g.nx), shm, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
     break;
   }

@@ -87,14 +87,14 @@ def test_advance_step_seam_matches_oracle():
     g.close()
 
 
-@pytest.mark.parametrize("emf", [None, "0", "-8"])
-def test_mixed_precision_l1_within_target(emf, monkeypatch):
+@pytest.mark.parametrize("env", ["", "GPL_EMF_MARCH=-8", "GPL_FUSED_XY=0"])
+def test_mixed_precision_l1_within_target(env, monkeypatch):
     """Mixed form: FP32 V/PLM/HLLD, FP64 U, faces, CT and dt.  Relative L1 per
     variable against the FP64 oracle stays <= 1e-5; div B stays at FP64
-    round-off because the face update is FP64.  Default EMF kernel and the
-    LDS-staged tile kernel."""
-    if emf is not None:
-        monkeypatch.setenv("GPL_EMF_MARCH", emf)
+    round-off because the face update is FP64.  Default kernels, the
+    LDS-staged EMF tiles, and separate x and y sweeps."""
+    if env:
+        monkeypatch.setenv(*env.split("="))
     n, steps = (32, 32, 32), 10
     (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(
         n, 13, steps, 1e-3, precision="mixed")
@@ -109,11 +109,12 @@ def test_mixed_precision_l1_within_target(emf, monkeypatch):
     assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * n[0]
 
 
-@pytest.mark.parametrize("path", ["split", "march8", "march16", "march32",
+@pytest.mark.parametrize("path", ["split", "march8", "march16", "march32", "xy1",
                                   "emf0", "emf8", "emf-4", "emf-8", "emf-16", "emf-32"])
 def test_kernel_paths_bitwise(path, monkeypatch):
-    """Split (per-interface) and fused kernels (LDS x-row + register-window y/z
-    march, every march length), and every EMF kernel (one edge per thread,
+    """Split (per-interface) and fused kernels (the LDS x-row kernel and a
+    register-window y march, or x and y rows in one marching kernel (xy1); the
+    z march; every march length), and every EMF kernel (one edge per thread,
     register march of 8, LDS-staged tiles marching 4 to 32 planes), including
     segments and tiles that do not divide the box (ny = 12, nz = 20, 17, 31,
     nx = 70)."""
@@ -123,6 +124,8 @@ def test_kernel_paths_bitwise(path, monkeypatch):
         monkeypatch.setenv("GPL_MARCH", path[5:])
     elif path.startswith("emf"):
         monkeypatch.setenv("GPL_EMF_MARCH", path[3:])
+    elif path == "xy1":
+        monkeypatch.setenv("GPL_FUSED_XY", "1")
     for n in ((16, 12, 20), (20, 17, 31), (70, 9, 6)):
         (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(n, 21, 5, 2e-3)
         assert gerr == oerr == 0
