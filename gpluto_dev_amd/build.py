@@ -24,14 +24,18 @@ PROBE_LIB = os.path.join(HERE, "libgpl_probe.so")
 ARCH = os.environ.get("GPL_OFFLOAD_ARCH", "gfx950")
 
 # (source, extra flags): the FP64 drop-in keeps IEEE division/sqrt and no
-# contraction; the mixed form's FP32 kernels may contract and use the fast
-# FP32 divide/sqrt (its accuracy is stated against the FP64 run).
+# contraction; the mixed form's FP32 kernels may contract, use the fast FP32
+# divide/sqrt and flush FP32 denormals, which drops the operand scaling
+# around every FP32 reciprocal (its accuracy is stated against the FP64 run;
+# FP64 arithmetic in the same files keeps denormals).
+MIXED = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+         "-fgpu-flush-denormals-to-zero"]
 SOURCES = [("mhd3d_fp64.hip", []),
-           ("mhd3d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+           ("mhd3d_mixed.hip", MIXED),
            ("mhd3d_common.hip", []),
            ("hd_kernels.hip", []),
            ("batch2d_fp64.hip", []),
-           ("batch2d_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+           ("batch2d_mixed.hip", MIXED),
            ("batch2d_common.hip", []),
            ("engine_batch2d.cpp", []),
            ("c2p_net.hip", []),

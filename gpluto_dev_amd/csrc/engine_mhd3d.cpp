@@ -306,6 +306,8 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
     return GPL_EUNSUPPORTED;
   // the PLM stencil wraps at most two cells: every direction needs >= 3 zones
   if (cfg.nx < 3 || cfg.ny < 3 || cfg.nz < 3) return GPL_EINVAL;
+  // the kernels address every array slot with 32-bit byte offsets (< 4 GiB)
+  if ((long long)cfg.nx*cfg.ny*cfg.nz >= (1LL << 29)) return GPL_EINVAL;
   Mhd3dEngine *e = new (std::nothrow) Mhd3dEngine();
   if (!e) return GPL_ENOMEM;
   const int rc = e->init(cfg, env);

@@ -35,6 +35,21 @@ namespace gpl {
 
 __device__ __forceinline__ int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// base[idx] with a 32-bit byte offset from a uniform base: the load / store
+// form with the array base in scalar registers and one 32-bit offset per
+// lane, instead of a 64-bit address per lane and array.  Every array slot of
+// a box is below 4 GiB (the engine refuses 2^29 cells or more).
+template <typename T>
+__device__ __forceinline__ T ldo(const T *base, unsigned idx)
+{
+  return *(const T *)((const char *)base + idx*(unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *base, unsigned idx, T v)
+{
+  *(T *)((char *)base + idx*(unsigned)sizeof(T)) = v;
+}
+
 struct Cell { int i, j, k; };
 
 __device__ __forceinline__ Cell cell_of(int c, const Box3 &g)
@@ -232,13 +247,13 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   R *sV = (R *)gpl_smem;           // [7][nx] primitives of the row
   R *sD = sV + 7*nx;               // [7][nx] left state vL(i+1) made by cell i
   const size_t n = (size_t)g.ncell;
-  const int c = i + nx*(int)blockIdx.x;
+  const unsigned c = i + nx*blockIdx.x;
   const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
   constexpr int VAR[7] = {0, 1, 2, 3, 5, 6, 7};   // B1 comes from the face
   R v0[7], dv[7];
 #pragma unroll
-  for (int q = 0; q < 7; q++) { v0[q] = V[VAR[q]*n + c]; sV[q*nx + i] = v0[q]; }
-  const R bn = (R)Vs[c];
+  for (int q = 0; q < 7; q++) { v0[q] = ldo(V + VAR[q]*n, c); sV[q*nx + i] = v0[q]; }
+  const R bn = (R)ldo(Vs, c);
   __syncthreads();
   R lo[7], hi[7];                  // this cell's vR(i) and vL(i+1)
 #pragma unroll
@@ -266,7 +281,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   {
     R et, eb, aL, aR, dd;
     uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-    FD[0*n + c] = et; FD[1*n + c] = eb; FD[2*n + c] = aL; FD[3*n + c] = aR; FD[4*n + c] = dd;
+    sto(FD, c, et); sto(FD + n, c, eb); sto(FD + 2*n, c, aL); sto(FD + 3*n, c, aR); sto(FD + 4*n, c, dd);
   }
   __syncthreads();                 // everyone is done with sV/sD
   R *sF = sV;                      // [6][nx]: hydro flux and cmax of face i
@@ -279,9 +294,9 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   for (int nv = 0; nv < 5; nv++) {
     double r = -dtdx*((double)sF[nv*nx + ip] - (double)sF[nv*nx + i]);
     if (PH == 0) r += 0.0;         // MHD adds src[] = 0 (rhs.cpp:519); RMHD+CT has no source
-    dU[nv*n + c] = (R)r;
+    sto(dU + nv*n, c, (R)r);
   }
-  if (want_cdt) Cdt[c] = (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx);
+  if (want_cdt) sto(Cdt, c, (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx));
   if (ff.err) atomicOr(err, ff.err);
 }
 
@@ -333,7 +348,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     const unsigned cm2 = base + wrapm(s0 - 2, ny)*nx, cm1 = base + wrapm(s0 - 1, ny)*nx;
     const unsigned c0 = base + wrapm(s0, ny)*nx;
 #pragma unroll
-    for (int q = 0; q < 7; q++) { Vm[q] = Vq[q][cm2]; Vc[q] = Vq[q][cm1]; Vp[q] = Vq[q][c0]; }
+    for (int q = 0; q < 7; q++) { Vm[q] = ldo(Vq[q], cm2); Vc[q] = ldo(Vq[q], cm1); Vp[q] = ldo(Vq[q], c0); }
     R lo[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) {
@@ -356,15 +371,15 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     const unsigned cc = base + wrapm(f - 1, ny)*nx;    // the cell completed by face f
     // the next row and the y face are loaded before the x sweep's barriers
 #pragma unroll
-    for (int q = 0; q < 7; q++) Vp[q] = Vq[q][cn];
-    const R byf = (R)Byf[cf];
+    for (int q = 0; q < 7; q++) Vp[q] = ldo(Vq[q], cn);
+    const R byf = (R)ldo(Byf, cf);
     R rxn[5], cxn = 0;
     if (f < s1) {                      // block-uniform
       // ---- x sweep of row f (Vc holds V(f)) ----
       R v0[7];                         // x order: 0 1 2 3 5 6 7
       v0[0] = Vc[0]; v0[1] = Vc[1]; v0[2] = Vc[2]; v0[3] = Vc[3];
-      v0[4] = V5[cf]; v0[5] = Vc[5]; v0[6] = Vc[6];
-      const R bn = (R)Bxf[cf];
+      v0[4] = ldo(V5, cf); v0[5] = Vc[5]; v0[6] = Vc[6];
+      const R bn = (R)ldo(Bxf, cf);
 #pragma unroll
       for (int q = 0; q < 7; q++) sV[q*nx + i] = v0[q];
       __syncthreads();
@@ -393,7 +408,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
       {
         R et, eb, aL, aR, dd;
         uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-        fdx[0*n + cf] = et; fdx[1*n + cf] = eb; fdx[2*n + cf] = aL; fdx[3*n + cf] = aR; fdx[4*n + cf] = dd;
+        sto(fdx, cf, et); sto(fdx + n, cf, eb); sto(fdx + 2*n, cf, aL); sto(fdx + 3*n, cf, aR); sto(fdx + 4*n, cf, dd);
       }
       sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
       sF[3*nx + i] = ff.f[3]; sF[4*nx + i] = ff.f[7];
@@ -430,7 +445,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     if (f < s1) {
       R et, eb, aL, aR, dd;
       uct_hll_store<R, 1>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-      fdy[0*n + cf] = et; fdy[1*n + cf] = eb; fdy[2*n + cf] = aL; fdy[3*n + cf] = aR; fdy[4*n + cf] = dd;
+      sto(fdy, cf, et); sto(fdy + n, cf, eb); sto(fdy + 2*n, cf, aL); sto(fdy + 3*n, cf, aR); sto(fdy + 4*n, cf, dd);
     }
     const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
                          (double)ff.f[7]};
@@ -442,12 +457,12 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
         if (PH == 0) rr += 0.0;
         double du = (double)rx[nv];
         du += rr;
-        dUq[nv][cc] = (R)du;
+        sto(dUq[nv], cc, (R)du);
       }
       if (want_cdt) {
         double cd = (double)cx;
         cd += 0.5*(cmp + cm)/g.dy;
-        Cdt[cc] = (R)cd;
+        sto(Cdt, cc, (R)cd);
       }
     }
 #pragma unroll
@@ -519,7 +534,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       const unsigned cm2 = base + wrapm(s0 - 2, nd)*sd, cm1 = base + wrapm(s0 - 1, nd)*sd;
       const unsigned c0 = base + wrapm(s0, nd)*sd;
 #pragma unroll
-      for (int q = 0; q < 7; q++) { Vm[q] = Vq[q][cm2]; Vc[q] = Vq[q][cm1]; Vp[q] = Vq[q][c0]; }
+      for (int q = 0; q < 7; q++) { Vm[q] = ldo(Vq[q], cm2); Vc[q] = ldo(Vq[q], cm1); Vp[q] = ldo(Vq[q], c0); }
       R lo[7];
 #pragma unroll
       for (int q = 0; q < 7; q++) {
@@ -543,20 +558,20 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       double acc[5], u_old[5], u0_old[5], cacc = 0.0;
       if (f > s0) {
 #pragma unroll
-        for (int nv = 0; nv < 5; nv++) acc[nv] = (double)dUq[nv][cc];
-        if (want_cdt) cacc = (double)Cdt[cc];
+        for (int nv = 0; nv < 5; nv++) acc[nv] = (double)ldo((const R *)dUq[nv], cc);
+        if (want_cdt) cacc = (double)ldo((const R *)Cdt, cc);
         if (D == 2) {
 #pragma unroll
-          for (int nv = 0; nv < 5; nv++) u_old[nv] = Uiq[nv][cc];
+          for (int nv = 0; nv < 5; nv++) u_old[nv] = ldo(Uiq[nv], cc);
           if (!first_stage) {
 #pragma unroll
-            for (int nv = 0; nv < 5; nv++) u0_old[nv] = U0q[nv][cc];
+            for (int nv = 0; nv < 5; nv++) u0_old[nv] = ldo(U0q[nv], cc);
           }
         }
       }
       R vl[8], vr[8];
 #pragma unroll
-      for (int q = 0; q < 7; q++) Vp[q] = Vq[q][cn];
+      for (int q = 0; q < 7; q++) Vp[q] = ldo(Vq[q], cn);
 #pragma unroll
       for (int q = 0; q < 7; q++) {
         const R dv = plm_slope<R, LIM>(Vm[q], Vc[q], Vp[q]);
@@ -571,14 +586,14 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           for (int q = 0; q < 7; q++) { vr[VAR[q]] = Vc[q]; vlc[q] = Vc[q]; }
         }
       }
-      vl[BXn] = vr[BXn] = (R)Bn[cf];
+      vl[BXn] = vr[BXn] = (R)ldo(Bn, cf);
       FaceFlux<R> ff;
       riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff);
       e |= ff.err;
       if (f < s1) {
         R et, eb, aL, aR, dd;
         uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-        fd0[0*n + cf] = et; fd0[1*n + cf] = eb; fd0[2*n + cf] = aL; fd0[3*n + cf] = aR; fd0[4*n + cf] = dd;
+        sto(fd0, cf, et); sto(fd0 + n, cf, eb); sto(fd0 + 2*n, cf, aL); sto(fd0 + 3*n, cf, aR); sto(fd0 + 4*n, cf, dd);
       }
       const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
                            (double)ff.f[7]};
@@ -591,17 +606,17 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           double du = acc[nv];
           du += rr;
           if (D == 1) {
-            dUq[nv][cc] = (R)du;
+            sto(dUq[nv], cc, (R)du);
           } else if (first_stage) {
-            Uq[nv][cc] = u_old[nv] + w.wR*du;
+            sto(Uq[nv], cc, u_old[nv] + w.wR*du);
           } else {
-            Uq[nv][cc] = w.w0*u0_old[nv] + w.w1*u_old[nv] + w.wR*du;
+            sto(Uq[nv], cc, w.w0*u0_old[nv] + w.w1*u_old[nv] + w.wR*du);
           }
         }
         if (want_cdt) {
           double cd = cacc;
           cd += 0.5*(cmp + cm)/ddx;
-          if (D == 1) Cdt[cc] = (R)cd;
+          if (D == 1) sto(Cdt, cc, (R)cd);
           else cdt_max = cdt_max >= cd ? cdt_max : cd;
         }
       }
@@ -793,49 +808,49 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
 #pragma unroll
     for (int o = 0; o < 3; o++) {
       const unsigned c = col + wrapm(k0 + o - 2, nz)*pl;
-      zbx[o] = (R)Bx[c]; zex[o] = ex1[c];
-      zby[o] = (R)By[c]; zey[o] = ey0[c];
+      zbx[o] = (R)ldo(Bx, c); zex[o] = ldo(ex1, c);
+      zby[o] = (R)ldo(By, c); zey[o] = ldo(ey0, c);
     }
     const unsigned c = col + wrapm(k0 - 1, nz)*pl;
-    cxm[0] = fx2[c]; cxm[1] = fx3[c]; cxm[2] = fx4[c];
-    cym[0] = fy2[c]; cym[1] = fy3[c]; cym[2] = fy4[c];
+    cxm[0] = ldo(fx2, c); cxm[1] = ldo(fx3, c); cxm[2] = ldo(fx4, c);
+    cym[0] = ldo(fy2, c); cym[1] = ldo(fy3, c); cym[2] = ldo(fy4, c);
   }
   for (int k = k0; k < k1; k++) {
     const unsigned b = (unsigned)k*pl;
     if (act) {
       const unsigned c = col + b, cn = col + wrapm(k + 1, nz)*pl;
-      zbx[3] = (R)Bx[cn]; zex[3] = ex1[cn];
-      zby[3] = (R)By[cn]; zey[3] = ey0[cn];
-      const R bz = (R)Bz[c];
-      xs[0][ty][tx + 2] = zby[2];  xs[1][ty][tx + 2] = ey1[c];
-      xs[2][ty][tx + 2] = bz;      xs[3][ty][tx + 2] = ez0[c];
-      ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = ex0[c];
-      ys[2][ty + 2][tx] = bz;      ys[3][ty + 2][tx] = ez1[c];
-      cxs[0][ty + 1][tx] = fx2[c]; cxs[1][ty + 1][tx] = fx3[c]; cxs[2][ty + 1][tx] = fx4[c];
-      cys[0][ty][tx + 1] = fy2[c]; cys[1][ty][tx + 1] = fy3[c]; cys[2][ty][tx + 1] = fy4[c];
-      czs[0][ty + 1][tx + 1] = fz2[c]; czs[1][ty + 1][tx + 1] = fz3[c]; czs[2][ty + 1][tx + 1] = fz4[c];
+      zbx[3] = (R)ldo(Bx, cn); zex[3] = ldo(ex1, cn);
+      zby[3] = (R)ldo(By, cn); zey[3] = ldo(ey0, cn);
+      const R bz = (R)ldo(Bz, c);
+      xs[0][ty][tx + 2] = zby[2];  xs[1][ty][tx + 2] = ldo(ey1, c);
+      xs[2][ty][tx + 2] = bz;      xs[3][ty][tx + 2] = ldo(ez0, c);
+      ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = ldo(ex0, c);
+      ys[2][ty + 2][tx] = bz;      ys[3][ty + 2][tx] = ldo(ez1, c);
+      cxs[0][ty + 1][tx] = ldo(fx2, c); cxs[1][ty + 1][tx] = ldo(fx3, c); cxs[2][ty + 1][tx] = ldo(fx4, c);
+      cys[0][ty][tx + 1] = ldo(fy2, c); cys[1][ty][tx + 1] = ldo(fy3, c); cys[2][ty][tx + 1] = ldo(fy4, c);
+      czs[0][ty + 1][tx + 1] = ldo(fz2, c); czs[1][ty + 1][tx + 1] = ldo(fz3, c); czs[2][ty + 1][tx + 1] = ldo(fz4, c);
     }
     if (hx) {
       const unsigned c = hxo + b;
-      xs[0][ty][hxc] = (R)By[c];  xs[1][ty][hxc] = ey1[c];
-      xs[2][ty][hxc] = (R)Bz[c];  xs[3][ty][hxc] = ez0[c];
+      xs[0][ty][hxc] = (R)ldo(By, c);  xs[1][ty][hxc] = ldo(ey1, c);
+      xs[2][ty][hxc] = (R)ldo(Bz, c);  xs[3][ty][hxc] = ldo(ez0, c);
       if (tx == 1) {
-        cys[0][ty][0] = fy2[c]; cys[1][ty][0] = fy3[c]; cys[2][ty][0] = fy4[c];
-        czs[0][ty + 1][0] = fz2[c]; czs[1][ty + 1][0] = fz3[c]; czs[2][ty + 1][0] = fz4[c];
+        cys[0][ty][0] = ldo(fy2, c); cys[1][ty][0] = ldo(fy3, c); cys[2][ty][0] = ldo(fy4, c);
+        czs[0][ty + 1][0] = ldo(fz2, c); czs[1][ty + 1][0] = ldo(fz3, c); czs[2][ty + 1][0] = ldo(fz4, c);
       }
     }
     if (hy) {
       const unsigned c = hyo + b;
-      ys[0][hyr][tx] = (R)Bx[c];  ys[1][hyr][tx] = ex0[c];
-      ys[2][hyr][tx] = (R)Bz[c];  ys[3][hyr][tx] = ez1[c];
+      ys[0][hyr][tx] = (R)ldo(Bx, c);  ys[1][hyr][tx] = ldo(ex0, c);
+      ys[2][hyr][tx] = (R)ldo(Bz, c);  ys[3][hyr][tx] = ldo(ez1, c);
       if (ty == 1) {
-        cxs[0][0][tx] = fx2[c]; cxs[1][0][tx] = fx3[c]; cxs[2][0][tx] = fx4[c];
-        czs[0][0][tx + 1] = fz2[c]; czs[1][0][tx + 1] = fz3[c]; czs[2][0][tx + 1] = fz4[c];
+        cxs[0][0][tx] = ldo(fx2, c); cxs[1][0][tx] = ldo(fx3, c); cxs[2][0][tx] = ldo(fx4, c);
+        czs[0][0][tx + 1] = ldo(fz2, c); czs[1][0][tx + 1] = ldo(fz3, c); czs[2][0][tx + 1] = ldo(fz4, c);
       }
     }
     if (hc) {
       const unsigned c = hco + b;
-      czs[0][0][0] = fz2[c]; czs[1][0][0] = fz3[c]; czs[2][0][0] = fz4[c];
+      czs[0][0][0] = ldo(fz2, c); czs[1][0][0] = ldo(fz3, c); czs[2][0][0] = ldo(fz4, c);
     }
     __syncthreads();
     if (act) {
@@ -864,9 +879,9 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       R ex = uct_val<R, LIMV>(q4, e4, cym, cy, +1);                      // X2a
       ex += uct_val<R, LIMV>(zby, zey, czj, cz, -1);                     // X3b
       const unsigned c = col + b;
-      Ex[c] = ex;
-      Ey[c] = ey;
-      Ez[c] = ez;
+      sto(Ex, c, ex);
+      sto(Ey, c, ey);
+      sto(Ez, c, ez);
 #pragma unroll
       for (int o = 0; o < 3; o++) { zbx[o] = zbx[o + 1]; zex[o] = zex[o + 1]; zby[o] = zby[o + 1]; zey[o] = zey[o + 1]; }
 #pragma unroll
@@ -887,10 +902,14 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   if (i >= g.nx || j >= g.ny) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)g.ncell;
-  const int pl = nx*ny, base = k*pl, c = i + nx*j + base;
+  const unsigned pl = nx*ny, base = k*pl, c = i + nx*j + base;
+  // uniform array bases (scalar registers) + 32-bit cell offsets
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
-  const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
-  int xo[4], yo[4], zo[4];
+  const R *ex0 = FD, *ex1 = FD + n, *ey0 = FD + 5*n, *ey1 = FD + 6*n, *ez0 = FD + 10*n, *ez1 = FD + 11*n;
+  const R *fx[3] = {FD + 2*n, FD + 3*n, FD + 4*n};
+  const R *fy[3] = {FD + 7*n, FD + 8*n, FD + 9*n};
+  const R *fz[3] = {FD + 12*n, FD + 13*n, FD + 14*n};
+  unsigned xo[4], yo[4], zo[4];
 #pragma unroll
   for (int o = 0; o < 4; o++) {
     xo[o] = wrapm(i + o - 2, nx) + nx*j + base;
@@ -900,36 +919,36 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3], cxk[3], cyk[3];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    cx[q] = FDx[(2 + q)*n + c];  cy[q] = FDy[(2 + q)*n + c];  cz[q] = FDz[(2 + q)*n + c];
-    cxj[q] = FDx[(2 + q)*n + yo[1]];
-    cyi[q] = FDy[(2 + q)*n + xo[1]];
-    czi[q] = FDz[(2 + q)*n + xo[1]];
-    czj[q] = FDz[(2 + q)*n + yo[1]];
-    cxk[q] = FDx[(2 + q)*n + zo[1]];
-    cyk[q] = FDy[(2 + q)*n + zo[1]];
+    cx[q] = ldo(fx[q], c);  cy[q] = ldo(fy[q], c);  cz[q] = ldo(fz[q], c);
+    cxj[q] = ldo(fx[q], yo[1]);
+    cyi[q] = ldo(fy[q], xo[1]);
+    czi[q] = ldo(fz[q], xo[1]);
+    czj[q] = ldo(fz[q], yo[1]);
+    cxk[q] = ldo(fx[q], zo[1]);
+    cyk[q] = ldo(fy[q], zo[1]);
   }
   R q4[4], e4[4];
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)By[xo[o]]; e4[o] = FDy[1*n + xo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(By, xo[o]); e4[o] = ldo(ey1, xo[o]); }
   R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                      // X1a
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[yo[o]]; e4[o] = FDx[0*n + yo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bx, yo[o]); e4[o] = ldo(ex0, yo[o]); }
   ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                       // X2b
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[xo[o]]; e4[o] = FDz[0*n + xo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bz, xo[o]); e4[o] = ldo(ez0, xo[o]); }
   R ey = uct_val<R, LIMV>(q4, e4, cxk, cx, -1);                      // X1b
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[zo[o]]; e4[o] = FDx[1*n + zo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bx, zo[o]); e4[o] = ldo(ex1, zo[o]); }
   ey += uct_val<R, LIMV>(q4, e4, czi, cz, +1);                       // X3a
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)Bz[yo[o]]; e4[o] = FDz[1*n + yo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bz, yo[o]); e4[o] = ldo(ez1, yo[o]); }
   R ex = uct_val<R, LIMV>(q4, e4, cyk, cy, +1);                      // X2a
 #pragma unroll
-  for (int o = 0; o < 4; o++) { q4[o] = (R)By[zo[o]]; e4[o] = FDy[0*n + zo[o]]; }
+  for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(By, zo[o]); e4[o] = ldo(ey0, zo[o]); }
   ex += uct_val<R, LIMV>(q4, e4, czj, cz, -1);                       // X3b
-  E[c] = ex;
-  E[n + c] = ey;
-  E[2*n + c] = ez;
+  sto(E, c, ex);
+  sto(E + n, c, ey);
+  sto(E + 2*n, c, ez);
 }
 
 // Rs = -dt curl E on the three lower faces of cell (i,j,k), Cartesian
@@ -943,9 +962,9 @@ __device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const
   const size_t n = (size_t)g.ncell;
   const int ip = wrapm(i + 1, g.nx), jp = wrapm(j + 1, g.ny), kp = wrapm(k + 1, g.nz);
   const R *Ex = E, *Ey = E + n, *Ez = E + 2*n;
-  const double ex0 = Ex[cid(i, j, k, g)], exj = Ex[cid(i, jp, k, g)], exk = Ex[cid(i, j, kp, g)];
-  const double ey0 = Ey[cid(i, j, k, g)], eyi = Ey[cid(ip, j, k, g)], eyk = Ey[cid(i, j, kp, g)];
-  const double ez0 = Ez[cid(i, j, k, g)], ezi = Ez[cid(ip, j, k, g)], ezj = Ez[cid(i, jp, k, g)];
+  const double ex0 = ldo(Ex, cid(i, j, k, g)), exj = ldo(Ex, cid(i, jp, k, g)), exk = ldo(Ex, cid(i, j, kp, g));
+  const double ey0 = ldo(Ey, cid(i, j, k, g)), eyi = ldo(Ey, cid(ip, j, k, g)), eyk = ldo(Ey, cid(i, j, kp, g));
+  const double ez0 = ldo(Ez, cid(i, j, k, g)), ezi = ldo(Ez, cid(ip, j, k, g)), ezj = ldo(Ez, cid(i, jp, k, g));
   const double dtdx1 = dt/g.dx, dtdx2 = dt/g.dy, dtdx3 = dt/g.dz;
   FaceR r;
   double curlE = 0.0 + dtdx2*(ezj - ez0);
@@ -996,20 +1015,20 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
   for (int d = 0; d < 3; d++) {
     const double *vin = Vin + d*n;
     if (first_stage) {
-      b_lo[d] = vin[c] + w.wR*rl[d];
-      b_hi[d] = vin[up[d]] + w.wR*rh[d];
+      b_lo[d] = ldo(vin, c) + w.wR*rl[d];
+      b_hi[d] = ldo(vin, up[d]) + w.wR*rh[d];
     } else {
       const double *b0 = Bs0 + d*n;
-      b_lo[d] = w.w0*b0[c] + w.w1*vin[c] + w.wR*rl[d];
-      b_hi[d] = w.w0*b0[up[d]] + w.w1*vin[up[d]] + w.wR*rh[d];
+      b_lo[d] = w.w0*ldo(b0, c) + w.w1*ldo(vin, c) + w.wR*rl[d];
+      b_hi[d] = w.w0*ldo(b0, up[d]) + w.w1*ldo(vin, up[d]) + w.wR*rh[d];
     }
-    if (live) Vout[d*n + c] = b_lo[d];
+    if (live) sto(Vout + d*n, c, b_lo[d]);
   }
   double b[3], u[5], v[8];
 #pragma unroll
   for (int d = 0; d < 3; d++) b[d] = 0.5*(b_lo[d] + b_hi[d]);
 #pragma unroll
-  for (int nv = 0; nv < 5; nv++) u[nv] = U[nv*n + c];
+  for (int nv = 0; nv < 5; nv++) u[nv] = ldo((const double *)U + nv*n, c);
   if constexpr (PH == 1) {
     // ConsToPrim RMHD (mappers.cpp:33-157); written back when the mapper reports
     // anything (mappers3D.cpp:182-190).  gmm1 carries gamma here.
@@ -1025,8 +1044,8 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
                                          scale);
     if (!live) return;
     if (local) {
-      U[0*n + c] = u8[0]; U[1*n + c] = u8[1]; U[2*n + c] = u8[2]; U[3*n + c] = u8[3];
-      U[4*n + c] = u8[7];
+      sto(U, c, u8[0]); sto(U + n, c, u8[1]); sto(U + 2*n, c, u8[2]); sto(U + 3*n, c, u8[3]);
+      sto(U + 4*n, c, u8[7]);
     }
     if (e) atomicOr(err, e);
   } else {
@@ -1034,12 +1053,12 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
     if (!live) return;
     if (e) {
 #pragma unroll
-      for (int nv = 0; nv < 5; nv++) U[nv*n + c] = u[nv];
+      for (int nv = 0; nv < 5; nv++) sto(U + nv*n, c, u[nv]);
       atomicOr(err, e);
     }
   }
 #pragma unroll
-  for (int nv = 0; nv < 8; nv++) V[nv*n + c] = (R)v[nv];
+  for (int nv = 0; nv < 8; nv++) sto(V + nv*n, c, (R)v[nv]);
 }
 
 // ---------------------------------------------------------------------------
