@@ -51,12 +51,15 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 #   k_sweepz_fused : V 7w + face 8 + dU 5w + U 40 (+ C_dt w | + U^n 40)
 #                    -> FD 5w + U 40
 #                    = 18w + 88 | 17w + 128                        -> 17.5w + 108
+#   k_sweepxy_fused (mixed form: the x and y sweeps in one row march):
+#                    V 8w + faces 16 -> FD 10w + dU 5w (+ C_dt w, stage 1)
+#                    = 24w + 16 | 23w + 16                         -> 23.5w + 16
 #   k_emf          : FD 15w + faces 24 -> E 3w                     =  18w + 24
 #   k_finish       : E 3w + faces 24 + U 40 (+ faces of U^n 24, stage 2)
 #                    -> faces 24 + V 8w                            =  11w + 100
 #   k_sweep_* (split, rows > 512): V 8w + face 8 -> F 5w + CM w + FD 5w = 19w + 8
 #   k_rhs_update   : F 15w + CM 3w + U 40 (+ U^n 40, stage 2) -> U 40 = 18w + 100
-ALG_BYTES = {"k_sweepx_fused": (17.5, 8), "k_sweepy_fused": (23, 8),
+ALG_BYTES = {"k_sweepx_fused": (17.5, 8), "k_sweepy_fused": (23, 8), "k_sweepxy_fused": (23.5, 16),
              "k_sweepz_fused": (17.5, 108), "k_emf": (18, 24), "k_finish": (11, 100),
              "k_rhs_update": (18, 100)}
 

@@ -65,9 +65,7 @@ public:
   // VALU-bound and run faster as two kernels
   bool fused_xy = false;
   // EMF kernel: n > 0 register z-march of n edges per thread, -n LDS-staged
-  // 64 x 4 tiles marching n planes, 0 one edge per thread.  FP64 runs the
-  // LDS tiles (32 planes); the mixed form one edge per thread, where more
-  // waves in flight pay more than the staged plane
+  // 64 x 4 tiles marching n planes (the default, 32), 0 one edge per thread
   int emf_march = -32;
   double *cdt_part = nullptr;
   int nparts = 0;
@@ -98,11 +96,11 @@ public:
     // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
     if (const char *m = getenv("GPL_MARCH")) march = atoi(m);
     if (march != 8 && march != 16 && march != 32) march = 16;
-    emf_march = fp64 ? -32 : 0;
+    emf_march = -32;
     if (const char *m = getenv("GPL_EMF_MARCH")) emf_march = atoi(m);
     {
       const int l = emf_march < 0 ? -emf_march : emf_march;
-      if (emf_march != 0 && l != 4 && l != 8 && l != 16 && l != 32) emf_march = fp64 ? -32 : 0;
+      if (emf_march != 0 && l != 4 && l != 8 && l != 16 && l != 32) emf_march = -32;
     }
     fused_xy = !fp64;
     if (const char *m = getenv("GPL_FUSED_XY")) fused_xy = atoi(m) != 0;

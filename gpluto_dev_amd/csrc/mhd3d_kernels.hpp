@@ -758,12 +758,12 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
 }
 
 // The same EMFs with each plane of a 64 x 4 tile staged through LDS.  Wave
-// w owns row y0 + w; per plane every thread loads its own cell of the seven
-// in-plane stencil arrays and the nine face records once, three lanes per
-// row add the x halo (i-2, i-1, i+64) and waves 0-3 the y halo (j-2, j-1,
-// j+4 and the (i-1, j-1) corner), so each HBM value of a plane is loaded by
-// one thread instead of by every stencil that uses it.  The z-stencils and
-// the face records of plane k-1 stay in registers as in k_emf_march.
+// w owns row y0 + w: per plane every thread loads its own cell of the seven
+// in-plane stencil arrays and the nine face records once; the halo (rows
+// y0-2, y0-1, y0+4 with their face records, columns x0-2, x0-1, x0+64 and
+// the (x0-1, y0-1) corner) is one task per lane.  The loads of plane k+1
+// are issued before plane k is computed, so their latency hides behind it.
+// The z-stencils and the face records of plane k-1 stay in registers.
 template <typename R, int LIMV, int L>
 __global__ __launch_bounds__(256)
 void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
@@ -783,76 +783,120 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const bool act = tx < tw && ty < th;
   const size_t n = (size_t)g.ncell;
-  const int pl = nx*ny;
+  const unsigned pl = nx*ny;
   const int k0 = seg*L, k1 = k0 + L < nz ? k0 + L : nz;
-  // uniform base of every array (scalar registers) + 32-bit cell offsets
+  // uniform array bases (scalar registers) + 32-bit cell offsets
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
   const R *ex0 = FD, *ex1 = FD + n, *fx2 = FD + 2*n, *fx3 = FD + 3*n, *fx4 = FD + 4*n;
   const R *ey0 = FD + 5*n, *ey1 = FD + 6*n, *fy2 = FD + 7*n, *fy3 = FD + 8*n, *fy4 = FD + 9*n;
   const R *ez0 = FD + 10*n, *ez1 = FD + 11*n, *fz2 = FD + 12*n, *fz3 = FD + 13*n, *fz4 = FD + 14*n;
   R *Ex = E, *Ey = E + n, *Ez = E + 2*n;
   const unsigned col = (x0 + (act ? tx : 0)) + nx*(y0 + (act ? ty : 0));
-  // x halo: lanes 0, 1, 2 of an active row load x0-2, x0-1, x0+tw
-  const bool hx = ty < th && tx < 3;
-  const int hxc = tx == 2 ? tw + 2 : tx;
-  const unsigned hxo = wrapm(tx == 2 ? x0 + tw : x0 - 2 + tx, nx) + nx*(y0 + (ty < th ? ty : 0));
-  // y halo: wave 0 row y0-2, wave 1 row y0-1, wave 2 row y0+th; wave 3 lane 0 the corner
-  const bool hy = ty < 3 && tx < tw;
-  const int hyr = ty == 2 ? th + 2 : ty;
-  const unsigned hyo = (x0 + (tx < tw ? tx : 0)) + nx*wrapm(ty == 2 ? y0 + th : y0 - 2 + ty, ny);
-  const bool hc = ty == 3 && tx == 0;
-  const unsigned hco = wrapm(x0 - 1, nx) + nx*wrapm(y0 - 1, ny);
+  // halo task of this lane: 1 = y-halo row (waves 0-2: y0-2, y0-1, y0+th),
+  // 2 = x-halo cell (wave 3, lane 3r+s: row y0+r, column x0-2, x0-1, x0+tw),
+  // 3 = the (x0-1, y0-1) corner (wave 3, lane 12)
+  int task = 0, hr = 0, hc = 0;
+  bool hf = false;                 // the task also carries face records
+  unsigned ho = 0;
+  if (ty < 3) {
+    if (tx < tw) {
+      task = 1; hr = ty == 2 ? th + 2 : ty; hc = tx; hf = ty == 1;
+      ho = (x0 + tx) + nx*wrapm(ty == 2 ? y0 + th : y0 - 2 + ty, ny);
+    }
+  } else if (tx < 3*th) {
+    const int r = tx/3, sl = tx - 3*r;
+    task = 2; hr = r; hc = sl == 2 ? tw + 2 : sl; hf = sl == 1;
+    ho = wrapm(sl == 2 ? x0 + tw : x0 - 2 + sl, nx) + nx*(y0 + r);
+  } else if (tx == 12) {
+    task = 3;
+    ho = wrapm(x0 - 1, nx) + nx*wrapm(y0 - 1, ny);
+  }
   R zbx[4], zex[4], zby[4], zey[4];   // z-stencils of X3a (Bx, e[J][I]) and X3b (By, e[I][J])
+  R zn[4] = {0, 0, 0, 0};             // their next plane
   R cxm[3], cym[3];                   // x- and y-face records of plane k-1
+  R o[14];                            // own cell of the staged plane
+  R ha[4], hq[6];                     // halo stencil values and face records
+  auto load_plane = [&](int k) {
+    const unsigned b = (unsigned)k*pl;
+    if (act) {
+      const unsigned c = col + b;
+      o[0] = ldo(ey1, c); o[1] = (R)ldo(Bz, c); o[2] = ldo(ez0, c);
+      o[3] = ldo(ex0, c); o[4] = ldo(ez1, c);
+      o[5] = ldo(fx2, c); o[6] = ldo(fx3, c); o[7] = ldo(fx4, c);
+      o[8] = ldo(fy2, c); o[9] = ldo(fy3, c); o[10] = ldo(fy4, c);
+      o[11] = ldo(fz2, c); o[12] = ldo(fz3, c); o[13] = ldo(fz4, c);
+    }
+    const unsigned c = ho + b;
+    if (task == 1) {
+      ha[0] = (R)ldo(Bx, c); ha[1] = ldo(ex0, c); ha[2] = (R)ldo(Bz, c); ha[3] = ldo(ez1, c);
+      if (hf) {
+        hq[0] = ldo(fx2, c); hq[1] = ldo(fx3, c); hq[2] = ldo(fx4, c);
+        hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+      }
+    } else if (task == 2) {
+      ha[0] = (R)ldo(By, c); ha[1] = ldo(ey1, c); ha[2] = (R)ldo(Bz, c); ha[3] = ldo(ez0, c);
+      if (hf) {
+        hq[0] = ldo(fy2, c); hq[1] = ldo(fy3, c); hq[2] = ldo(fy4, c);
+        hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+      }
+    } else if (task == 3) {
+      hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+    }
+  };
   if (act) {
 #pragma unroll
-    for (int o = 0; o < 3; o++) {
-      const unsigned c = col + wrapm(k0 + o - 2, nz)*pl;
-      zbx[o] = (R)ldo(Bx, c); zex[o] = ldo(ex1, c);
-      zby[o] = (R)ldo(By, c); zey[o] = ldo(ey0, c);
+    for (int q = 0; q < 4; q++) {
+      const unsigned c = col + wrapm(k0 + q - 2, nz)*pl;
+      zbx[q] = (R)ldo(Bx, c); zex[q] = ldo(ex1, c);
+      zby[q] = (R)ldo(By, c); zey[q] = ldo(ey0, c);
     }
     const unsigned c = col + wrapm(k0 - 1, nz)*pl;
     cxm[0] = ldo(fx2, c); cxm[1] = ldo(fx3, c); cxm[2] = ldo(fx4, c);
     cym[0] = ldo(fy2, c); cym[1] = ldo(fy3, c); cym[2] = ldo(fy4, c);
   }
+  load_plane(k0);
   for (int k = k0; k < k1; k++) {
-    const unsigned b = (unsigned)k*pl;
+    // stage plane k
     if (act) {
-      const unsigned c = col + b, cn = col + wrapm(k + 1, nz)*pl;
-      zbx[3] = (R)ldo(Bx, cn); zex[3] = ldo(ex1, cn);
-      zby[3] = (R)ldo(By, cn); zey[3] = ldo(ey0, cn);
-      const R bz = (R)ldo(Bz, c);
-      xs[0][ty][tx + 2] = zby[2];  xs[1][ty][tx + 2] = ldo(ey1, c);
-      xs[2][ty][tx + 2] = bz;      xs[3][ty][tx + 2] = ldo(ez0, c);
-      ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = ldo(ex0, c);
-      ys[2][ty + 2][tx] = bz;      ys[3][ty + 2][tx] = ldo(ez1, c);
-      cxs[0][ty + 1][tx] = ldo(fx2, c); cxs[1][ty + 1][tx] = ldo(fx3, c); cxs[2][ty + 1][tx] = ldo(fx4, c);
-      cys[0][ty][tx + 1] = ldo(fy2, c); cys[1][ty][tx + 1] = ldo(fy3, c); cys[2][ty][tx + 1] = ldo(fy4, c);
-      czs[0][ty + 1][tx + 1] = ldo(fz2, c); czs[1][ty + 1][tx + 1] = ldo(fz3, c); czs[2][ty + 1][tx + 1] = ldo(fz4, c);
-    }
-    if (hx) {
-      const unsigned c = hxo + b;
-      xs[0][ty][hxc] = (R)ldo(By, c);  xs[1][ty][hxc] = ldo(ey1, c);
-      xs[2][ty][hxc] = (R)ldo(Bz, c);  xs[3][ty][hxc] = ldo(ez0, c);
-      if (tx == 1) {
-        cys[0][ty][0] = ldo(fy2, c); cys[1][ty][0] = ldo(fy3, c); cys[2][ty][0] = ldo(fy4, c);
-        czs[0][ty + 1][0] = ldo(fz2, c); czs[1][ty + 1][0] = ldo(fz3, c); czs[2][ty + 1][0] = ldo(fz4, c);
+      xs[0][ty][tx + 2] = zby[2];  xs[1][ty][tx + 2] = o[0];
+      xs[2][ty][tx + 2] = o[1];    xs[3][ty][tx + 2] = o[2];
+      ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = o[3];
+      ys[2][ty + 2][tx] = o[1];    ys[3][ty + 2][tx] = o[4];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        cxs[q][ty + 1][tx] = o[5 + q];
+        cys[q][ty][tx + 1] = o[8 + q];
+        czs[q][ty + 1][tx + 1] = o[11 + q];
       }
     }
-    if (hy) {
-      const unsigned c = hyo + b;
-      ys[0][hyr][tx] = (R)ldo(Bx, c);  ys[1][hyr][tx] = ldo(ex0, c);
-      ys[2][hyr][tx] = (R)ldo(Bz, c);  ys[3][hyr][tx] = ldo(ez1, c);
-      if (ty == 1) {
-        cxs[0][0][tx] = ldo(fx2, c); cxs[1][0][tx] = ldo(fx3, c); cxs[2][0][tx] = ldo(fx4, c);
-        czs[0][0][tx + 1] = ldo(fz2, c); czs[1][0][tx + 1] = ldo(fz3, c); czs[2][0][tx + 1] = ldo(fz4, c);
+    if (task == 1) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) ys[a][hr][hc] = ha[a];
+      if (hf) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) { cxs[q][0][hc] = hq[q]; czs[q][0][hc + 1] = hq[3 + q]; }
       }
-    }
-    if (hc) {
-      const unsigned c = hco + b;
-      czs[0][0][0] = ldo(fz2, c); czs[1][0][0] = ldo(fz3, c); czs[2][0][0] = ldo(fz4, c);
+    } else if (task == 2) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) xs[a][hr][hc] = ha[a];
+      if (hf) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) { cys[q][hr][0] = hq[q]; czs[q][hr + 1][0] = hq[3 + q]; }
+      }
+    } else if (task == 3) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) czs[q][0][0] = hq[3 + q];
     }
     __syncthreads();
+    // loads of plane k+1 (and the z-stencil's plane k+2) fly during plane k
+    if (k + 1 < k1) {
+      load_plane(k + 1);
+      if (act) {
+        const unsigned c = col + wrapm(k + 2, nz)*pl;
+        zn[0] = (R)ldo(Bx, c); zn[1] = ldo(ex1, c);
+        zn[2] = (R)ldo(By, c); zn[3] = ldo(ey0, c);
+      }
+    }
     if (act) {
       R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3];
 #pragma unroll
@@ -865,25 +909,26 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       }
       R q4[4], e4[4];
 #pragma unroll
-      for (int o = 0; o < 4; o++) { q4[o] = xs[0][ty][tx + o]; e4[o] = xs[1][ty][tx + o]; }
+      for (int a = 0; a < 4; a++) { q4[a] = xs[0][ty][tx + a]; e4[a] = xs[1][ty][tx + a]; }
       R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                      // X1a
 #pragma unroll
-      for (int o = 0; o < 4; o++) { q4[o] = ys[0][ty + o][tx]; e4[o] = ys[1][ty + o][tx]; }
+      for (int a = 0; a < 4; a++) { q4[a] = ys[0][ty + a][tx]; e4[a] = ys[1][ty + a][tx]; }
       ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                       // X2b
 #pragma unroll
-      for (int o = 0; o < 4; o++) { q4[o] = xs[2][ty][tx + o]; e4[o] = xs[3][ty][tx + o]; }
+      for (int a = 0; a < 4; a++) { q4[a] = xs[2][ty][tx + a]; e4[a] = xs[3][ty][tx + a]; }
       R ey = uct_val<R, LIMV>(q4, e4, cxm, cx, -1);                      // X1b
       ey += uct_val<R, LIMV>(zbx, zex, czi, cz, +1);                     // X3a
 #pragma unroll
-      for (int o = 0; o < 4; o++) { q4[o] = ys[2][ty + o][tx]; e4[o] = ys[3][ty + o][tx]; }
+      for (int a = 0; a < 4; a++) { q4[a] = ys[2][ty + a][tx]; e4[a] = ys[3][ty + a][tx]; }
       R ex = uct_val<R, LIMV>(q4, e4, cym, cy, +1);                      // X2a
       ex += uct_val<R, LIMV>(zby, zey, czj, cz, -1);                     // X3b
-      const unsigned c = col + b;
+      const unsigned c = col + (unsigned)k*pl;
       sto(Ex, c, ex);
       sto(Ey, c, ey);
       sto(Ez, c, ez);
 #pragma unroll
-      for (int o = 0; o < 3; o++) { zbx[o] = zbx[o + 1]; zex[o] = zex[o + 1]; zby[o] = zby[o + 1]; zey[o] = zey[o + 1]; }
+      for (int a = 0; a < 3; a++) { zbx[a] = zbx[a + 1]; zex[a] = zex[a + 1]; zby[a] = zby[a + 1]; zey[a] = zey[a + 1]; }
+      zbx[3] = zn[0]; zex[3] = zn[1]; zby[3] = zn[2]; zey[3] = zn[3];
 #pragma unroll
       for (int q = 0; q < 3; q++) { cxm[q] = cx[q]; cym[q] = cy[q]; }
     }
