@@ -78,6 +78,8 @@ def log(*a):
 
 
 def run_gpu(n, steps, warmup, precision, vc, vs, first_dt):
+    """Warm-up, then exactly ``steps`` steps timed on the host clock with HIP
+    events around every kernel on the library's stream; the final state."""
     from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
     g = StageUpdate(Definitions(), RuntimeSetup(n=(n, n, n), first_dt=first_dt), precision=precision)
     g.upload(vc, vs)
@@ -206,6 +208,78 @@ def run_batch2d(seed: int, steps: int = 100):
     return res
 
 
+def run_rmhd(seed: int, n: int = 192, steps: int = 10):
+    """The 3D RMHD workload: 192^3 periodic, gamma 4/3, PLM(MC) + HLL + UCT_HLL
+    + RK2, CFL 0.4, 10 steps, Lorentz factors up to 10 and magnetisation
+    sigma log-uniform in [0.1, 10] by seed.  The stage update runs with the
+    reference's Eq. (A27) con2prim start and with the learned start (network
+    trained here on the GPU); Newton passes and restart rate of both starts
+    are measured on the final conserved states of the plain run."""
+    from gpluto_dev_amd import c2p_net, ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    sigma = float(10.0 ** np.random.Generator(np.random.PCG64(seed + 1000)).uniform(-1.0, 1.0))
+    vc, vs = ic.rmhd_turbulence((n, n, n), seed, lorentz_max=10.0, sigma=sigma)
+    t0 = time.perf_counter()
+    w, loss = c2p_net.train(nsamples=1 << 18, steps=3000, batch=4096, lr=3e-3, seed=seed)
+    train_s = time.perf_counter() - t0
+    res = {"n": n, "zones": n ** 3, "steps": steps, "sigma": sigma, "lorentz_max": 10.0,
+           "train": {"seconds": train_s, "loss": loss, "samples": 1 << 18, "steps": 3000}}
+    finals = {}
+    for use in (False, True):
+        g = StageUpdate(Definitions(physics="RMHD"),
+                        RuntimeSetup(n=(n, n, n), solver="hll", gamma=4.0 / 3.0, cfl=0.4))
+        if use:
+            g.set_c2p_network(w)
+        g.upload(vc, vs)
+        g.run(1, 1e-4)                            # warm-up (graph capture)
+        g.upload(vc, vs)
+        t0 = time.perf_counter()
+        err, t, _ = g.run(steps, 1e-4)
+        wall = time.perf_counter() - t0
+        finals[use] = g.download()
+        g.close()
+        res["learned" if use else "a27"] = {"value": n ** 3 * steps / wall, "unit": "zone-updates/s",
+                                            "ms_per_step": wall / steps * 1e3, "err": err, "time": t}
+    v0, v1 = finals[False][0], finals[True][0]
+    res["learned"]["max_l1_rel_vs_a27"] = max(float(np.sum(np.abs(v1[q] - v0[q])) / np.sum(np.abs(v0[q])))
+                                              for q in range(8))
+    u = np.ascontiguousarray(finals[False][2].reshape(8, -1).T)
+    _, pa, ea, ms_a = c2p_net.solve(u)
+    _, pn, en, ms_n, rs = c2p_net.solve(u, w, restarts=True)
+    res["con2prim"] = {"states": int(u.shape[0]),
+                       "a27": {"mean_passes": float(pa.mean()), "failed": int(np.count_nonzero(ea)),
+                               "kernel_ms": ms_a},
+                       "learned": {"mean_passes": float(pn.mean()), "failed": int(np.count_nonzero(en)),
+                                   "restart_rate": float(rs.mean()), "kernel_ms": ms_n}}
+    return res
+
+
+def run_mixed512(seed: int, steps: int = 20):
+    """The 512^3 mixed-precision configuration: the same random-field MHD IC
+    at 512^3, 20 steps in FP64 and in the mixed form; per-variable relative
+    L1 of the mixed run against the FP64 run and max |div B|."""
+    from gpluto_dev_amd import ic
+    n = 512
+    first_dt = 5e-4 * 256 / n
+    log(f"bench: generating {n}^3 IC (seed {seed})")
+    vc, vs = ic.mhd_turbulence((n, n, n), seed)
+    res = {"n": n, "zones": n ** 3, "steps": steps}
+    out = {}
+    for prec in ("fp64", "mixed"):
+        log(f"bench: {n}^3 {prec} run")
+        wall, stats, vcf, vsf, t = run_gpu(n, steps, 2, prec, vc, vs, first_dt)
+        out[prec] = (vcf, vsf)
+        res[prec] = {"value": n ** 3 * steps / wall, "ms_per_step": wall / steps * 1e3, "time": t,
+                     "divb_max": ic.divb_max(vsf)}
+    names = ["rho", "vx1", "vx2", "vx3", "Bx1", "Bx2", "Bx3", "prs"]
+    (v64, s64), (vm, sm) = out["fp64"], out["mixed"]
+    l1 = {names[q]: float(np.sum(np.abs(vm[q] - v64[q])) / np.sum(np.abs(v64[q]))) for q in range(8)}
+    res["mixed"]["l1_rel_vs_fp64"] = l1
+    res["mixed"]["max_l1_rel"] = max(l1.values())
+    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    return res
+
+
 def run_hd1d(steps: int = 400):
     """The 1D HD shock tube (4096 zones, outflow, gamma 1.4, PLM(MC) + HLLC +
     RK2, CFL 0.8): the reference's CPU-runnable case, timed on the GPU."""
@@ -237,7 +311,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the batched 2D and 1D HD workloads reported beside the headline")
+                    help="skip the batched 2D, 1D HD and 3D RMHD workloads reported beside the headline")
+    ap.add_argument("--mixed512", action="store_true",
+                    help="also run the 512^3 FP64 vs mixed comparison (about 1.5 min of IC generation)")
     ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
                     help="run one step of the workload plus a 1 GiB device copy (k_copy_probe, the "
                          "HBM byte-counter calibration) for rocprofv3 --pmc, then exit")
@@ -298,6 +374,10 @@ def main():
         log("bench: batched 2D and 1D HD workloads")
         out["batch2d"] = run_batch2d(a.seed)
         out["hd1d"] = run_hd1d()
+        log("bench: 3D RMHD workload (A27 and learned con2prim start)")
+        out["rmhd"] = run_rmhd(a.seed)
+    if a.mixed512:
+        out["mixed512"] = run_mixed512(a.seed)
     if not a.no_cpu:
         log("bench: CPU baseline")
         try:

@@ -115,7 +115,7 @@ def load() -> C.CDLL:
     L.gpl_c2p_net_infer.argtypes = [ip, fp, fp, ip, fp]
     L.gpl_c2p_net_infer.restype = ip
     L.gpl_c2p_solve.argtypes = [ip, fp, dp, ip, C.c_double, dp, C.POINTER(C.c_int),
-                                C.POINTER(C.c_uint32), dp]
+                                C.POINTER(C.c_uint32), dp, C.POINTER(C.c_int)]
     L.gpl_c2p_solve.restype = ip
     L.gpl_set_c2p_network.argtypes = [vp, fp]
     L.gpl_set_c2p_network.restype = ip

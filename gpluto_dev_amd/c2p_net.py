@@ -62,17 +62,23 @@ def infer(w, x, device: int = 0):
     return out
 
 
-def solve(u, w=None, gamma: float = 4.0 / 3.0, device: int = 0):
-    """Energy solve of every state: (v[n, 8], passes[n], err[n], kernel ms)."""
+def solve(u, w=None, gamma: float = 4.0 / 3.0, device: int = 0, restarts: bool = False):
+    """Energy solve of every state: (v[n, 8], passes[n], err[n], kernel ms),
+    and with ``restarts`` also restart[n] (1 where the learned start failed
+    and the A27 restart ran)."""
     u = np.ascontiguousarray(u, dtype=np.float64)
     n = u.shape[0]
     v = np.empty((n, 8))
     passes = np.empty(n, dtype=np.int32)
     err = np.empty(n, dtype=np.uint32)
+    rs = np.empty(n, dtype=np.int32) if restarts else None
     ms = np.zeros(1)
     wp = None if w is None else _f(np.ascontiguousarray(w, dtype=np.float32))
     _lib.check(_lib.load().gpl_c2p_solve(device, wp, _d(u), n, gamma, _d(v),
                                          passes.ctypes.data_as(C.POINTER(C.c_int)),
-                                         err.ctypes.data_as(C.POINTER(C.c_uint32)), _d(ms)),
+                                         err.ctypes.data_as(C.POINTER(C.c_uint32)), _d(ms),
+                                         None if rs is None else rs.ctypes.data_as(C.POINTER(C.c_int))),
                "gpl_c2p_solve")
+    if restarts:
+        return v, passes, err, float(ms[0]), rs
     return v, passes, err, float(ms[0])

@@ -174,20 +174,21 @@ void k_net_infer(const float *__restrict__ net, const float *__restrict__ x, int
 // or from the learned guess with the A27 fallback
 __global__ __launch_bounds__(256)
 void k_net_solve(const float *__restrict__ net, const double *__restrict__ uin, int n, double gamma,
-                 double *__restrict__ vout, int *__restrict__ passes, uint32_t *__restrict__ errs)
+                 double *__restrict__ vout, int *__restrict__ passes, uint32_t *__restrict__ errs,
+                 int *__restrict__ restarts)
 {
   const int i = blockIdx.x*blockDim.x + threadIdx.x;
   const bool live = i < n;
   double u[8], v[8];
   for (int q = 0; q < 8; q++) u[q] = uin[(size_t)(live ? i : 0)*8 + q];
   for (int q = 0; q < 8; q++) v[q] = u[q];
-  int it = 0;
+  int it = 0, again = 0;
   uint32_t e;
   if (net) {
     float x[4];
     net_features(u, x);
     const float pred = net_eval64(net, x);          // every lane of the wavefront
-    e = rmhd_energy_solve_guided(u, v, gamma, pred, net[NET_YSC], &it);
+    e = rmhd_energy_solve_guided(u, v, gamma, pred, net[NET_YSC], &it, &again);
   } else {
     e = rmhd_energy_solve(u, v, gamma, &it);
   }
@@ -195,6 +196,7 @@ void k_net_solve(const float *__restrict__ net, const double *__restrict__ uin, 
   for (int q = 0; q < 8; q++) vout[(size_t)i*8 + q] = v[q];
   passes[i] = it;
   errs[i] = e;
+  if (restarts) restarts[i] = again;
 }
 
 struct DevBuf {
@@ -356,14 +358,15 @@ extern "C" int gpl_c2p_net_train(int device, uint64_t seed, int nsamples, int st
 }
 
 extern "C" int gpl_c2p_solve(int device, const float *weights, const double *u, int n, double gamma,
-                             double *v_out, int *passes_out, uint32_t *err_out, double *ms_out)
+                             double *v_out, int *passes_out, uint32_t *err_out, double *ms_out,
+                             int *restart_out)
 {
   if (!u || !v_out || !passes_out || !err_out || n < 1) return GPL_EINVAL;
   if (int rc = set_device(device)) return rc;
-  DevBuf w, du, dv, dp, de;
+  DevBuf w, du, dv, dp, de, dr;
   if ((weights && w.alloc(NET_SIZE*sizeof(float))) || du.alloc((size_t)n*8*sizeof(double)) ||
       dv.alloc((size_t)n*8*sizeof(double)) || dp.alloc((size_t)n*sizeof(int)) ||
-      de.alloc((size_t)n*sizeof(uint32_t)))
+      de.alloc((size_t)n*sizeof(uint32_t)) || (restart_out && dr.alloc((size_t)n*sizeof(int))))
     return GPL_ENOMEM;
   if ((weights && hipMemcpy(w.p, weights, NET_SIZE*sizeof(float), hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(du.p, u, (size_t)n*8*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
@@ -374,7 +377,7 @@ extern "C" int gpl_c2p_solve(int device, const float *weights, const double *u, 
   hipEventRecord(e0, 0);
   hipLaunchKernelGGL(k_net_solve, dim3((n + 255)/256), dim3(256), 0, 0,
                      weights ? w.as<float>() : nullptr, du.as<double>(), n, gamma, dv.as<double>(),
-                     dp.as<int>(), de.as<uint32_t>());
+                     dp.as<int>(), de.as<uint32_t>(), restart_out ? dr.as<int>() : nullptr);
   hipEventRecord(e1, 0);
   const bool ok = hipEventSynchronize(e1) == hipSuccess;
   float ms = 0.f;
@@ -385,6 +388,13 @@ extern "C" int gpl_c2p_solve(int device, const float *weights, const double *u, 
       hipMemcpy(passes_out, dp.p, (size_t)n*sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(err_out, de.p, (size_t)n*sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
     return GPL_EHIP;
+  if (restart_out) {
+    if (!weights) {
+      for (int i = 0; i < n; i++) restart_out[i] = 0;
+    } else if (hipMemcpy(restart_out, dr.p, (size_t)n*sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      return GPL_EHIP;
+    }
+  }
   if (ms_out) *ms_out = ms;
   return 0;
 }

@@ -108,14 +108,17 @@ __device__ __forceinline__ float net_eval64(const float *__restrict__ net, const
 // Newton from the learned guess, the A27 guess as the fallback and judge.
 // Returns the solve's error bits (0 on success); *passes counts every pass.
 __device__ __forceinline__ uint32_t rmhd_energy_solve_guided(const double *u, double *v, double gamma,
-                                                             float pred, float ysc, int *passes)
+                                                             float pred, float ysc, int *passes,
+                                                             int *restart = nullptr)
 {
   int it1 = 0, it2 = 0;
   const double wa = rmhd_guess_a27(u);
   const double w0 = wa*exp((double)(ysc*pred));
   uint32_t e = rmhd_energy_solve(u, v, gamma, &it1, w0);
-  if (e) e = rmhd_energy_solve(u, v, gamma, &it2);
+  const bool again = e != 0;
+  if (again) e = rmhd_energy_solve(u, v, gamma, &it2);
   if (passes) *passes = it1 + it2;
+  if (restart) *restart = again ? 1 : 0;
   return e;
 }
 

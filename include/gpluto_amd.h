@@ -199,10 +199,12 @@ int gpl_c2p_net_infer(int device, const float *weights, const float *x, int n, f
 int gpl_set_c2p_network(gpl_ctx *ctx, const float *weights);
 /* Batched RMHD energy solve of n states u[n][8] from the A27 guess
  * (weights == NULL, the reference method) or from the learned guess with
- * the A27 restart; v_out[n][8], Newton passes and error bits per state, and
- * the kernel time. */
+ * the A27 restart; v_out[n][8], Newton passes and error bits per state, the
+ * kernel time, and (restart_out, may be NULL) 1 where the learned start
+ * failed and the A27 restart ran. */
 int gpl_c2p_solve(int device, const float *weights, const double *u, int n, double gamma,
-                  double *v_out, int *passes_out, uint32_t *err_out, double *ms_out);
+                  double *v_out, int *passes_out, uint32_t *err_out, double *ms_out,
+                  int *restart_out);
 
 #ifdef __cplusplus
 }
