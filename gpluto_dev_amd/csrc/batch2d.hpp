@@ -27,11 +27,15 @@ struct Item2 { int box, start; };
 template <typename R>
 struct Batch2dBufs {
   R *V;              // [8][ncell] primitives (B1, B2 = face averages, B3 cell-centred)
-  double *U;         // [6][ncell] rho, m1, m2, m3, B3, E
-  double *U0;        // [6][ncell]
+  // conserved rho, m1, m2, m3, B3, E [6][ncell]: the stage reads Uin (and
+  // from its second stage on U0, the state at the start of the step) and
+  // writes U, cell by cell; U may be Uin or U0
+  const double *Uin;
+  double *U;
+  const double *U0;
   const double *Vin; // [2][ncell] lower x- and y-faces at stage start
-  double *Vout;      // [2][ncell] faces after the stage
-  double *Bs0;       // [2][ncell]
+  double *Vout;      // [2][ncell] faces after the stage (a third buffer)
+  const double *Bs0; // [2][ncell] faces at the start of the step (second stage on)
   R *FD;             // [2][5][ncell] UCT face data et, eb, aL, aR, d
   R *E;              // [ncell] Ez at the lower-left corner of every cell
   R *dU;             // [6][ncell] x-sweep flux divergence

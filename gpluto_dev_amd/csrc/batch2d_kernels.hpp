@@ -179,7 +179,7 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
         double acc[6], u_old[6], cacc = 0.0;
         if (f > s0) {
 #pragma unroll
-          for (int q = 0; q < 6; q++) { acc[q] = (double)b.dU[q*n + cc]; u_old[q] = b.U[q*n + cc]; }
+          for (int q = 0; q < 6; q++) { acc[q] = (double)b.dU[q*n + cc]; u_old[q] = b.Uin[q*n + cc]; }
           if (want_cdt) cacc = (double)b.Cdt[cc];
         }
         R vl[8], vr[8];
@@ -216,7 +216,6 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
             du += rr;
             const size_t o = q*n + cc;
             if (first_stage) {
-              b.U0[o] = u_old[q];
               b.U[o] = u_old[q] + w.wR*du;
             } else {
               b.U[o] = w.w0*b.U0[o] + w.w1*u_old[q] + w.wR*du;
@@ -320,10 +319,8 @@ void k2_finish(Batch2dBufs<R> b, RKW w, int first_stage, double gmm1, double sma
     for (int d = 0; d < 2; d++) {
       const double *vin = b.Vin + d*n;
       if (first_stage) {
-        const double s0 = vin[c], s1 = vin[up[d]];
-        b.Bs0[d*n + c] = s0;
-        b_lo[d] = s0 + w.wR*rl[d];
-        b_hi[d] = s1 + w.wR*rh[d];
+        b_lo[d] = vin[c] + w.wR*rl[d];
+        b_hi[d] = vin[up[d]] + w.wR*rh[d];
       } else {
         const double *bs0 = b.Bs0 + d*n;
         b_lo[d] = w.w0*bs0[c] + w.w1*vin[c] + w.wR*rl[d];

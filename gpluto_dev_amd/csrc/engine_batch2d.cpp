@@ -45,9 +45,12 @@ struct gpl_batch {
   size_t bytes = 0;
   // device state
   void *V = nullptr, *FD = nullptr, *E = nullptr, *dU = nullptr, *Cdt = nullptr;
-  double *U = nullptr, *U0 = nullptr, *VsA = nullptr, *VsB = nullptr, *Bs0 = nullptr;
+  // U holds U^n between steps; stage 1 writes its state to Us, later stages
+  // read Us and U^n and the last writes U.  The faces rotate over vs[0..2]:
+  // a step starts from vs[cur] (see enqueue_step).
+  double *U = nullptr, *Us = nullptr, *vs[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
   double *cdt_part = nullptr;
-  double *cur_vs = nullptr;
   uint32_t *err = nullptr;
   StepCtl *ctl = nullptr;
   // work lists
@@ -72,7 +75,7 @@ struct gpl_batch {
 
 static void batch_free(gpl_batch *b)
 {
-  void *ptrs[] = {b->V, b->FD, b->E, b->dU, b->Cdt, b->U, b->U0, b->VsA, b->VsB, b->Bs0,
+  void *ptrs[] = {b->V, b->FD, b->E, b->dU, b->Cdt, b->U, b->Us, b->vs[0], b->vs[1], b->vs[2],
                   b->cdt_part, b->err, b->ctl, b->box, b->xitems, b->tiles,
                   b->yitems[0], b->yitems[1], b->yitems[2], b->part_beg[0], b->part_beg[1],
                   b->part_beg[2]};
@@ -161,10 +164,8 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   }
   rc |= dev_alloc(b->bytes, &b->V, 8*nc*rs);
   rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, (void **)&b->U0, 6*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, (void **)&b->VsA, 2*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, (void **)&b->VsB, 2*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, (void **)&b->Bs0, 2*nc*sizeof(double));
+  rc |= dev_alloc(b->bytes, (void **)&b->Us, 6*nc*sizeof(double));
+  for (int q = 0; q < 3; q++) rc |= dev_alloc(b->bytes, (void **)&b->vs[q], 2*nc*sizeof(double));
   rc |= dev_alloc(b->bytes, &b->FD, 10*nc*rs);
   rc |= dev_alloc(b->bytes, &b->E, nc*rs);
   rc |= dev_alloc(b->bytes, &b->dU, 6*nc*rs);
@@ -184,7 +185,7 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
     delete b;
     return GPL_EHIP;
   }
-  b->cur_vs = b->VsA;
+  b->cur = 0;
   if (const char *e = getenv("GPL_BATCH_SHAPE")) {   // "march,bpc" fixes the launch shape
     int m = 16, p = 8;
     if (sscanf(e, "%d,%d", &m, &p) == 2) gpl_batch_set_tuning(b, m, p);
@@ -257,7 +258,7 @@ extern "C" int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const d
     if (hipMemcpy(b->U + s*nc + d.off, &hu[s*m], m*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return GPL_EHIP;
   for (int c = 0; c < 2; c++)
-    if (hipMemcpy(b->cur_vs + c*nc + d.off, &hs[c*m], m*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(b->vs[b->cur] + c*nc + d.off, &hs[c*m], m*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return GPL_EHIP;
   const StepCtl z = {0.0, 0.0, 0.0, 0};
   if (hipMemcpy(b->ctl + box, &z, sizeof(z), hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
@@ -291,7 +292,7 @@ extern "C" int gpl_batch_download(gpl_batch *b, int box, double *vc, double *con
     if (hipMemcpy(&hu[s*m], b->U + s*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return GPL_EHIP;
   for (int c = 0; c < 2; c++)
-    if (hipMemcpy(&hs[c*m], b->cur_vs + c*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(&hs[c*m], b->vs[b->cur] + c*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return GPL_EHIP;
   for (int j = 0; j < NY; j++)
     for (int i = 0; i < NX; i++) {
@@ -323,10 +324,12 @@ static int grid_for(const gpl_batch *b, int groups)
 }
 
 template <typename R>
-static void enqueue_stage(gpl_batch *b, double *vin, double *vout, RKW w, int first_stage)
+static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, const double *vin,
+                          double *vout, const double *vbase, RKW w, int first_stage)
 {
   Batch2dBufs<R> q;
-  q.V = (R *)b->V; q.U = b->U; q.U0 = b->U0; q.Vin = vin; q.Vout = vout; q.Bs0 = b->Bs0;
+  q.V = (R *)b->V; q.Uin = uin; q.U = uout; q.U0 = b->U;
+  q.Vin = vin; q.Vout = vout; q.Bs0 = vbase;
   q.FD = (R *)b->FD; q.E = (R *)b->E; q.dU = (R *)b->dU; q.Cdt = (R *)b->Cdt;
   q.cdt_part = b->cdt_part; q.err = b->err; q.ctl = b->ctl; q.box = b->box;
   q.xitems = b->xitems; q.yitems = b->yitems[b->march_idx]; q.tiles = b->tiles;
@@ -347,37 +350,46 @@ static void enqueue_stage(gpl_batch *b, double *vin, double *vout, RKW w, int fi
 }
 
 // one AdvanceStep of every box (Src/rk_step.cpp:88-720), then per-box dt
+static void enqueue_stage_r(gpl_batch *b, const double *uin, double *uout, const double *vin,
+                            double *vout, const double *vbase, RKW w, int first)
+{
+  if (b->fp64) enqueue_stage<double>(b, uin, uout, vin, vout, vbase, w, first);
+  else         enqueue_stage<float>(b, uin, uout, vin, vout, vbase, w, first);
+}
+
 static void enqueue_step(gpl_batch *b, int advance_dt)
 {
   const RKW w1 = {0.0, 0.0, 1.0};
   const RKW w2 = b->cfg.time_stepping == GPL_RK3 ? RKW{0.75, 0.25, 0.25} : RKW{0.5, 0.5, 0.5};
   const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
-  const int nst = b->cfg.time_stepping == GPL_RK3 ? 3 : 2;
-  double *in = b->cur_vs, *out = in == b->VsA ? b->VsB : b->VsA;
-  for (int st = 1; st <= nst; st++) {
-    const RKW w = st == 1 ? w1 : (st == 2 ? w2 : w3);
-    if (b->fp64) enqueue_stage<double>(b, in, out, w, st == 1);
-    else         enqueue_stage<float>(b, in, out, w, st == 1);
-    double *t = in; in = out; out = t;
+  double *f0 = b->vs[b->cur], *f1 = b->vs[(b->cur + 1) % 3], *f2 = b->vs[(b->cur + 2) % 3];
+  enqueue_stage_r(b, b->U, b->Us, f0, f1, f0, w1, 1);            // -> Us, f1
+  if (b->cfg.time_stepping != GPL_RK3) {
+    enqueue_stage_r(b, b->Us, b->U, f1, f2, f0, w2, 0);          // -> U, f2
+    b->cur = (b->cur + 2) % 3;
+  } else {
+    enqueue_stage_r(b, b->Us, b->Us, f1, f2, f0, w2, 0);         // -> Us, f2
+    enqueue_stage_r(b, b->Us, b->U, f2, f1, f0, w3, 0);          // -> U, f1
+    b->cur = (b->cur + 1) % 3;
   }
   batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
                     b->cfg.cfl_max_var, advance_dt, b->stream);
-  b->cur_vs = in;
 }
 
-static int graph_period(const gpl_batch *b) { return b->cfg.time_stepping == GPL_RK3 ? 2 : 1; }
+// the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
+static int graph_period(const gpl_batch *) { return 3; }
 
 static int build_graph(gpl_batch *b)
 {
   const int shape = b->march_idx*8 + b->bpc_idx;
-  if (b->graph_exec && b->graph_key == b->cur_vs && b->graph_shape == shape) return 0;
+  if (b->graph_exec && b->graph_key == b->vs[b->cur] && b->graph_shape == shape) return 0;
   if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
-  const void *key = b->cur_vs;
+  const void *key = b->vs[b->cur];
   hipGraph_t graph;
   if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return GPL_EHIP;
   for (int s = 0; s < graph_period(b); s++) enqueue_step(b, 1);
   if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) return GPL_EHIP;
-  if (b->cur_vs != key) { hipGraphDestroy(graph); return GPL_EHIP; }
+  if (b->vs[b->cur] != key) { hipGraphDestroy(graph); return GPL_EHIP; }
   const hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   if (e != hipSuccess) { b->graph_exec = nullptr; return GPL_EHIP; }
