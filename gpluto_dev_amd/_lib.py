@@ -30,7 +30,7 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_batch_run", "gpl_batch_set_tuning", "gpl_batch_get_tuning",
            "gpl_batch_last_run_ms", "gpl_batch_device_bytes",
            "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
-           "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network")
+           "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics")
 
 
 class GplConfig(C.Structure):
@@ -119,6 +119,8 @@ def load() -> C.CDLL:
     L.gpl_c2p_solve.restype = ip
     L.gpl_set_c2p_network.argtypes = [vp, fp]
     L.gpl_set_c2p_network.restype = ip
+    L.gpl_diagnostics.argtypes = [vp, dp, dp, dp, dp]
+    L.gpl_diagnostics.restype = ip
     _lib = L
     return L
 

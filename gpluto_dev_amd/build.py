@@ -33,6 +33,7 @@ MIXED = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
 SOURCES = [("mhd3d_fp64.hip", []),
            ("mhd3d_mixed.hip", MIXED),
            ("mhd3d_common.hip", []),
+           ("diagnostics.hip", []),
            ("hd_kernels.hip", []),
            ("batch2d_fp64.hip", []),
            ("batch2d_mixed.hip", MIXED),

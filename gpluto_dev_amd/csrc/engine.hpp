@@ -42,6 +42,10 @@ public:
   virtual const void *state_key() const = 0;
   // RMHD: learned con2prim guess weights (nullptr: the reference's guess).
   virtual int set_c2p_network(const float *) { return GPL_EUNSUPPORTED; }
+  // DataMinMax / CT_CheckDivB of the current state into out[2*nvar + 2]
+  // (min[nvar], max[nvar], max Mach, max |div B|), on the context stream.
+  virtual int diagnostics(double *out) = 0;
+  virtual int nvar() const = 0;
   size_t bytes = 0;   // device memory held
 };
 

@@ -1,6 +1,7 @@
 // HD engine (1D/2D/3D, periodic or outflow sides, HLLC/HLL, RK2/RK3, FP64):
 // device state and AdvanceStep orchestration (Src/rk_step.cpp:88-720) over
 // the kernels of hd_kernels.hip.
+#include "diagnostics.hpp"
 #include "engine.hpp"
 #include "hd.hpp"
 #include "physics.hpp"
@@ -22,10 +23,11 @@ public:
   double *U = nullptr, *U0 = nullptr, *dU = nullptr, *Cdt = nullptr, *cdt_part = nullptr;
   int nparts = 0;
   double *cur_v = nullptr;
+  double *diag = nullptr;     // diagnostics partials (DIAG_BLOCKS rows) and result row
 
   ~HdEngine() override
   {
-    void *ptrs[] = {VA, VB, U, U0, dU, Cdt, cdt_part};
+    void *ptrs[] = {VA, VB, U, U0, dU, Cdt, cdt_part, diag};
     for (void *p : ptrs) if (p) (void)hipFree(p);
   }
 
@@ -52,6 +54,7 @@ public:
       rc |= dev_alloc(bytes, (void **)&Cdt, n*sizeof(double));
     }
     rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&diag, (size_t)(DIAG_BLOCKS + 1)*DIAG_MAXQ*sizeof(double));
     cur_v = VA;
     return rc ? GPL_ENOMEM : 0;
   }
@@ -124,6 +127,21 @@ public:
 
   int graph_period() const override { return cfg.time_stepping == GPL_RK3 ? 2 : 1; }
   const void *state_key() const override { return cur_v; }
+
+  int nvar() const override { return 5; }
+
+  int diagnostics(double *out) override
+  {
+    DiagBox d;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 5;
+    d.ax = d.ay = d.az = 0.0;
+    double *res = diag + (size_t)DIAG_BLOCKS*DIAG_MAXQ;
+    launch_diagnostics<double>(d, cur_v, nullptr, diag, res, env.stream);
+    if (hipMemcpyAsync(out, res, 12*sizeof(double), hipMemcpyDeviceToHost, env.stream) != hipSuccess ||
+        hipStreamSynchronize(env.stream) != hipSuccess)
+      return GPL_EHIP;
+    return 0;
+  }
 };
 
 }  // namespace

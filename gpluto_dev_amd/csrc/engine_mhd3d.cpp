@@ -4,6 +4,7 @@
 #include "engine.hpp"
 #include "mhd3d.hpp"
 #include "c2p_net.hpp"
+#include "diagnostics.hpp"
 #include "physics.hpp"
 
 #include <cstdio>
@@ -71,10 +72,11 @@ public:
   int nparts = 0;
   float *net = nullptr;       // RMHD learned con2prim guess, when enabled
   bool net_on = false;
+  double *diag = nullptr;     // diagnostics partials (DIAG_BLOCKS rows) and result row
 
   ~Mhd3dEngine() override
   {
-    void *ptrs[] = {V, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net};
+    void *ptrs[] = {V, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net, diag};
     for (void *p : ptrs) if (p) (void)hipFree(p);
   }
 
@@ -121,6 +123,7 @@ public:
     rc |= dev_alloc(bytes, &FD, 15*n*rs);
     rc |= dev_alloc(bytes, &E, 3*n*rs);
     rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
+    rc |= dev_alloc(bytes, (void **)&diag, (size_t)(DIAG_BLOCKS + 1)*DIAG_MAXQ*sizeof(double));
     cur = 0;
     return rc ? GPL_ENOMEM : 0;
   }
@@ -276,6 +279,23 @@ public:
   // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
   int graph_period() const override { return 3; }
   const void *state_key() const override { return vs[cur]; }
+
+  int nvar() const override { return 8; }
+
+  int diagnostics(double *out) override
+  {
+    DiagBox d;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 8;
+    // face areas of the uniform Cartesian grid (Src/set_geometry.cpp:124,147,166)
+    d.ax = 1.0*g.dy*g.dz; d.ay = g.dx*1.0*g.dz; d.az = g.dx*g.dy*1.0;
+    double *res = diag + (size_t)DIAG_BLOCKS*DIAG_MAXQ;
+    if (fp64) launch_diagnostics<double>(d, (const double *)V, vs[cur], diag, res, env.stream);
+    else      launch_diagnostics<float>(d, (const float *)V, vs[cur], diag, res, env.stream);
+    if (hipMemcpyAsync(out, res, 18*sizeof(double), hipMemcpyDeviceToHost, env.stream) != hipSuccess ||
+        hipStreamSynchronize(env.stream) != hipSuccess)
+      return GPL_EHIP;
+    return 0;
+  }
 
   int set_c2p_network(const float *w) override
   {

@@ -5,6 +5,7 @@
 // device, so a run of many steps replays one captured HIP graph without
 // host round trips), the error bits, and the per-kernel timing.
 #include "../../include/gpluto_amd.h"
+#include "diagnostics.hpp"
 #include "engine.hpp"
 
 #include <hip/hip_runtime.h>
@@ -317,6 +318,23 @@ extern "C" int gpl_set_c2p_network(gpl_ctx *c, const float *weights)
     c->graph_exec = nullptr;
   }
   return rc;
+}
+
+extern "C" int gpl_diagnostics(gpl_ctx *c, double *vmin, double *vmax, double *mach_max,
+                               double *divb_max)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  double out[DIAG_MAXQ];
+  const int nv = c->engine->nvar();
+  if (const int rc = c->engine->diagnostics(out)) return rc;
+  for (int a = 0; a < nv; a++) {
+    if (vmin) vmin[a] = out[a];
+    if (vmax) vmax[a] = out[nv + a];
+  }
+  if (mach_max) *mach_max = out[2*nv];
+  if (divb_max) *divb_max = out[2*nv + 1];
+  return 0;
 }
 
 extern "C" int gpl_set_timing(gpl_ctx *c, int enable)

@@ -197,6 +197,17 @@ class StageUpdate:
         _lib.check(self._L.gpl_set_c2p_network(self._h, w.ctypes.data_as(C.POINTER(C.c_float))),
                    "gpl_set_c2p_network")
 
+    def diagnostics(self) -> dict:
+        """DataMinMax / CT_CheckDivB of the current state, reduced on the device:
+        {"min": [NVAR], "max": [NVAR], "mach_max": float, "divb_max": float}."""
+        vmin = np.empty(self.nvar)
+        vmax = np.empty(self.nvar)
+        mach, divb = C.c_double(0.0), C.c_double(0.0)
+        dp = C.POINTER(C.c_double)
+        _lib.check(self._L.gpl_diagnostics(self._h, vmin.ctypes.data_as(dp), vmax.ctypes.data_as(dp),
+                                           C.byref(mach), C.byref(divb)), "gpl_diagnostics")
+        return {"min": vmin, "max": vmax, "mach_max": mach.value, "divb_max": divb.value}
+
     # -- measurement ----------------------------------------------------------
     def set_timing(self, on: bool):
         _lib.check(self._L.gpl_set_timing(self._h, 1 if on else 0), "gpl_set_timing")

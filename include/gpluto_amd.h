@@ -121,6 +121,15 @@ double gpl_next_time_step(const gpl_ctx *ctx, double dt, double inv_dt_hyp);
  * current dt and time.  Returns the OR of the ERR_* bits. */
 uint32_t gpl_run(gpl_ctx *ctx, int nsteps, double first_dt, double *time_out, double *dt_out);
 
+/* DataMinMax (Src/main.cpp:752-860) and CT_CheckDivB
+ * (Src/MHD/CT/ct_update.cpp:580-635) of the current state, reduced on the
+ * device: vmin[NVAR] and vmax[NVAR] of the primitives over the active zones,
+ * the largest Mach number max(|v1|, |v2|, |v3|)/sqrt(p/rho), and for MHD and
+ * RMHD the largest |div B| with the reference's face-area weighting (0 for
+ * HD).  Any output pointer may be NULL.  The results are the reference's
+ * formulas bit for bit: min and max do not depend on reduction order. */
+int gpl_diagnostics(gpl_ctx *ctx, double *vmin, double *vmax, double *mach_max, double *divb_max);
+
 /* Kernel timing with HIP events on the library's stream: enable, then query
  * the average duration of each kernel family over the recorded launches. */
 int gpl_set_timing(gpl_ctx *ctx, int enable);
