@@ -16,7 +16,7 @@ GPL_SOLVER_HLL, GPL_SOLVER_HLLC, GPL_SOLVER_HLLD = 1, 2, 3
 GPL_VANLEER_LIM, GPL_MC_LIM = 26, 27
 GPL_UCT_HLL = 1
 GPL_RK2, GPL_RK3 = 2, 3
-GPL_PERIODIC, GPL_OUTFLOW = 1, 2
+GPL_PERIODIC, GPL_OUTFLOW, GPL_REFLECTIVE = 1, 2, 3
 GPL_FP64, GPL_MIXED = 0, 1
 GPL_LAYOUT_ACTIVE, GPL_LAYOUT_ARY4D = 0, 1
 GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED = -1, -2, -3, -4
@@ -41,7 +41,7 @@ class GplConfig(C.Structure):
                 ("solver", C.c_int), ("ct_emf_average", C.c_int), ("precision", C.c_int),
                 ("nghost", C.c_int), ("device", C.c_int),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("cfl_max_var", C.c_double),
-                ("small_rho", C.c_double), ("small_prs", C.c_double)]
+                ("small_rho", C.c_double), ("small_prs", C.c_double), ("side_bc", C.c_int * 6)]
 
 
 _lib = None

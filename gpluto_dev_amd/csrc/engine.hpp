@@ -49,6 +49,14 @@ public:
   size_t bytes = 0;   // device memory held
 };
 
+// pluto.ini [Boundary] type of side s (X1-beg, X1-end, X2-beg, ...)
+inline int side_type(const gpl_config &c, int s) { return c.side_bc[s] ? c.side_bc[s] : c.boundary; }
+inline bool all_periodic(const gpl_config &c)
+{
+  for (int s = 0; s < 2*c.dimensions; s++) if (side_type(c, s) != GPL_PERIODIC) return false;
+  return true;
+}
+
 int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
 int make_hd_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
 

@@ -93,7 +93,7 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   if (!ny) ny = nx;
   *out = nullptr;
   if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
-  if (cfg->boundary != GPL_PERIODIC || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (!all_periodic(*cfg) || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
   if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
   if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
   if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;

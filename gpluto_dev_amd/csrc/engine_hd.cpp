@@ -1,4 +1,4 @@
-// HD engine (1D/2D/3D, periodic or outflow sides, HLLC/HLL, RK2/RK3, FP64):
+// HD engine (1D/2D/3D, periodic, outflow or reflective sides, HLLC/HLL, RK2/RK3, FP64):
 // device state and AdvanceStep orchestration (Src/rk_step.cpp:88-720) over
 // the kernels of hd_kernels.hip.
 #include "diagnostics.hpp"
@@ -38,7 +38,10 @@ public:
     g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
     g.ncell = (int)n;
     g.dims = c.dimensions;
-    g.periodic = c.boundary == GPL_PERIODIC;
+    for (int d = 0; d < 3; d++) {
+      g.lo[d] = d < c.dimensions ? side_type(c, 2*d) : GPL_PERIODIC;
+      g.hi[d] = d < c.dimensions ? side_type(c, 2*d + 1) : GPL_PERIODIC;
+    }
     // dx = (xend - xbeg)/N (Src/set_grid.cpp:196); unused directions are never swept
     g.dx = (c.xend[0] - c.xbeg[0])/c.nx;
     g.dy = (c.xend[1] - c.xbeg[1])/c.ny;
@@ -150,7 +153,12 @@ int make_hd_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
 {
   *out = nullptr;
   if (cfg.dimensions < 1 || cfg.dimensions > 3) return GPL_EINVAL;
-  if (cfg.boundary != GPL_PERIODIC && cfg.boundary != GPL_OUTFLOW) return GPL_EUNSUPPORTED;
+  for (int d = 0; d < cfg.dimensions; d++) {
+    const int lo = side_type(cfg, 2*d), hi = side_type(cfg, 2*d + 1);
+    if (lo < GPL_PERIODIC || lo > GPL_REFLECTIVE || hi < GPL_PERIODIC || hi > GPL_REFLECTIVE)
+      return GPL_EUNSUPPORTED;
+    if ((lo == GPL_PERIODIC) != (hi == GPL_PERIODIC)) return GPL_EINVAL;
+  }
   if (cfg.solver != GPL_SOLVER_HLLC && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64) return GPL_EUNSUPPORTED;
   const int nn[3] = {cfg.nx, cfg.ny, cfg.nz};

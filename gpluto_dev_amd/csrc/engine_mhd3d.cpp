@@ -314,7 +314,7 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
 {
   *out = nullptr;
   if (cfg.dimensions != 3) return GPL_EUNSUPPORTED;
-  if (cfg.boundary != GPL_PERIODIC || cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (!all_periodic(cfg) || cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
   if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
   // RMHD: HLL, FP64 and the fused kernels (rows of at most 512 zones)

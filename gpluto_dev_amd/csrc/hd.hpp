@@ -1,9 +1,10 @@
 // Declarations of the hydrodynamics (HD) stage kernels: 1D, 2D or 3D boxes,
-// periodic or outflow sides, PLM + HLLC/HLL + RK2/RK3, FP64.
+// periodic, outflow or reflective sides, PLM + HLLC/HLL + RK2/RK3, FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/gpluto_amd.h"
 #include "common.hpp"
 
 namespace gpl {
@@ -12,7 +13,7 @@ struct HdBox {
   int nx, ny, nz;     // active zones (1 along unused directions)
   int ncell;
   int dims;           // DIMENSIONS
-  int periodic;       // 1: periodic sides, 0: outflow (zero-gradient) sides
+  int lo[3], hi[3];   // side types per direction: GPL_PERIODIC / GPL_OUTFLOW / GPL_REFLECTIVE
   double dx, dy, dz;
 };
 

@@ -2,10 +2,12 @@
 // or outflow sides, FP64, for gfx950.
 //
 // Device data model: V[5][n] (rho, v1..3, prs), U/U0[5][n] (rho, m1..3, E),
-// no ghost zones.  A neighbour outside the box is the wrapped cell (periodic)
-// or the nearest active cell (outflow: OutflowBoundary, boundary.cpp:677-760,
-// copies the edge cell into every ghost zone), which is bitwise what the
-// reference's ghost fill supplies to the sweep.
+// no ghost zones.  A neighbour outside the box is the wrapped cell
+// (periodic), the nearest active cell (outflow: OutflowBoundary,
+// boundary.cpp:677-760, copies the edge cell into every ghost zone) or the
+// mirrored cell with its normal velocity negated (reflective:
+// ReflectiveBoundary, boundary.cpp:931-1000, ghost i = vsign*Vc(i0 - i)),
+// which is bitwise what the reference's ghost fill supplies to the sweep.
 //
 // One kernel per direction and stage, one thread per cell: the thread
 // reconstructs cells -1, 0, +1 along the sweep, solves the Riemann problem on
@@ -59,18 +61,29 @@ void k_hd_sweep(HdBox g, const double *__restrict__ V, double *__restrict__ Vout
     const int p = D == 0 ? c % g.nx : (D == 1 ? (c/g.nx) % g.ny : c/(g.nx*g.ny));
     const int base = c - p*sd;
     int idx[5];
+    bool flip[5];                   // a mirrored ghost: its normal velocity changes sign
 #pragma unroll
     for (int o = 0; o < 5; o++) {
       int q = p + o - 2;
-      if (g.periodic) q = q < 0 ? q + nd : (q >= nd ? q - nd : q);
-      else            q = q < 0 ? 0 : (q >= nd ? nd - 1 : q);
+      flip[o] = false;
+      if (q < 0) {
+        const int t = g.lo[D];
+        flip[o] = t == GPL_REFLECTIVE;
+        q = t == GPL_PERIODIC ? q + nd : (t == GPL_OUTFLOW ? 0 : -1 - q);
+      } else if (q >= nd) {
+        const int t = g.hi[D];
+        flip[o] = t == GPL_REFLECTIVE;
+        q = t == GPL_PERIODIC ? q - nd : (t == GPL_OUTFLOW ? nd - 1 : 2*nd - 1 - q);
+      }
       idx[o] = base + q*sd;
     }
     double v[5][5];                 // [cell -2..+2][variable]
 #pragma unroll
-    for (int o = 0; o < 5; o++)
+    for (int o = 0; o < 5; o++) {
 #pragma unroll
       for (int nv = 0; nv < 5; nv++) v[o][nv] = V[nv*n + idx[o]];
+      if (flip[o]) v[o][1 + D] = -v[o][1 + D];     // vsign = -1 (FlipSign, boundary.cpp:840-925)
+    }
     // Reconstruct (plm_states.cpp:121-138): vL(i+1) = v + dv/2, vR(i) = v - dv/2
     double vlo_l[5], vlo_r[5], vhi_l[5], vhi_r[5];
 #pragma unroll

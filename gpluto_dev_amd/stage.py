@@ -33,6 +33,8 @@ _LIMITERS = {"MC_LIM": _lib.GPL_MC_LIM, "VANLEER_LIM": _lib.GPL_VANLEER_LIM,
              "DEFAULT": _lib.GPL_VANLEER_LIM}
 _SOLVERS = {"hlld": _lib.GPL_SOLVER_HLLD, "hll": _lib.GPL_SOLVER_HLL, "hllc": _lib.GPL_SOLVER_HLLC}
 _STEPPING = {"RK2": _lib.GPL_RK2, "RK3": _lib.GPL_RK3}
+_BOUNDARY = {"periodic": _lib.GPL_PERIODIC, "outflow": _lib.GPL_OUTFLOW,
+             "reflective": _lib.GPL_REFLECTIVE}
 
 
 @dataclass
@@ -57,6 +59,7 @@ class RuntimeSetup:
     first_dt: float = 1.e-4
     solver: str = "hlld"
     boundary: str = "periodic"
+    sides: tuple = ()        # X1-beg, X1-end, X2-beg, X2-end, X3-beg, X3-end when they differ
     gamma: float = 5.0 / 3.0
     extra: dict = field(default_factory=dict)
 
@@ -95,6 +98,8 @@ def parse_pluto_ini(text: str) -> RuntimeSetup:
     r.first_dt = float(kv.get("first_dt", [r.first_dt])[0])
     r.solver = kv.get("Solver", [r.solver])[0]
     r.boundary = kv.get("X1-beg", [r.boundary])[0]
+    r.sides = tuple(kv.get(f"{ax}-{e}", [r.boundary])[0]
+                    for ax in ("X1", "X2", "X3") for e in ("beg", "end"))
     return r
 
 
@@ -111,7 +116,9 @@ class StageUpdate:
         cfg.nx, cfg.ny, cfg.nz = ini.n
         for d in range(3):
             cfg.xbeg[d], cfg.xend[d] = ini.xbeg[d], ini.xend[d]
-        cfg.boundary = _lib.GPL_PERIODIC if ini.boundary == "periodic" else _lib.GPL_OUTFLOW
+        cfg.boundary = _BOUNDARY[ini.boundary]
+        for q, t in enumerate(ini.sides):
+            cfg.side_bc[q] = _BOUNDARY[t]
         cfg.time_stepping = _STEPPING[defs.time_stepping]
         cfg.limiter = _LIMITERS[defs.limiter]
         cfg.solver = _SOLVERS[ini.solver]

@@ -37,9 +37,10 @@ extern "C" {
 /* TIME_STEPPING (Src/gpluto.hpp:121-122) */
 #define GPL_RK2          2
 #define GPL_RK3          3
-/* pluto.ini [Boundary] (same type on every side) */
+/* pluto.ini [Boundary] side types */
 #define GPL_PERIODIC     1
 #define GPL_OUTFLOW      2
+#define GPL_REFLECTIVE   3
 /* arithmetic of the stage update */
 #define GPL_FP64         0   /* drop-in: FP64 everywhere, reference operation order */
 #define GPL_MIXED        1   /* FP32 state/PLM/Riemann; FP64 U, faces, CT, dt */
@@ -65,7 +66,8 @@ typedef struct gpl_config {
   int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
-  int    boundary;         /* GPL_PERIODIC (MHD, HD) or GPL_OUTFLOW (HD) on every side */
+  int    boundary;         /* type of every side whose side_bc entry is 0: GPL_PERIODIC
+                              (MHD, RMHD, HD), GPL_OUTFLOW or GPL_REFLECTIVE (HD) */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
   int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLL; RMHD: _HLL; HD: _HLLC / _HLL */
@@ -78,6 +80,9 @@ typedef struct gpl_config {
   double cfl_max_var;      /* [Time] CFL_max_var */
   double small_rho;        /* run_config.small_rho (default 1e-12) */
   double small_prs;        /* run_config.small_prs (default 1e-12) */
+  int    side_bc[6];       /* [Boundary] X1-beg, X1-end, X2-beg, X2-end, X3-beg, X3-end;
+                              0 takes `boundary`.  A direction is periodic on both
+                              sides or on neither (HD: outflow and reflective mix) */
 } gpl_config;
 
 typedef struct gpl_ctx gpl_ctx;

@@ -849,7 +849,25 @@ static void boundary(gplo_state *s)
     if (!upper) { lo[dir] = 0; hi[dir] = s->beg[dir] - 1; }
     else        { lo[dir] = s->end[dir] + 1; hi[dir] = s->nt[dir] - 1; }
     int nint = s->c.dims > dir ? (dir == 0 ? s->c.nx : dir == 1 ? s->c.ny : s->c.nz) : 1;
-    if (s->c.periodic) {
+    const int type = s->c.bc[side];
+    if (type == GPLO_BC_REFLECTIVE) {
+      /* ReflectiveBoundary, Src/boundary.cpp:931-1100, with FlipSign
+         (:840-925): ghost cell i = vsign*Vc(i0 - i), i0 = 2*beg - 1 on the
+         lower side and 2*end + 1 on the upper side; vsign flips the normal
+         velocity (HD) */
+      const int i0 = upper ? 2*s->end[dir] + 1 : 2*s->beg[dir] - 1;
+      int vsign[NVMAX];
+      for (nv = 0; nv < s->nv; nv++) vsign[nv] = 1;
+      vsign[1 + dir] = -1;
+      for (nv = 0; nv < s->nv; nv++)
+        for (k = lo[2]; k <= hi[2]; k++)
+        for (j = lo[1]; j <= hi[1]; j++)
+        for (i = lo[0]; i <= hi[0]; i++) {
+          int p[3] = {i, j, k};
+          p[dir] = i0 - p[dir];
+          VC(s,nv,i,j,k) = vsign[nv]*VC(s,nv,p[0],p[1],p[2]);
+        }
+    } else if (type == GPLO_BC_PERIODIC) {
       int sh = upper ? -nint : nint;
       int o[3] = {0, 0, 0}; o[dir] = sh;
       for (nv = 0; nv < s->nv; nv++)
@@ -1266,8 +1284,18 @@ uint32_t gplo_run(gplo_state *s, int nsteps, double first_dt, double *time_out, 
  * ====================================================================== */
 gplo_state *gplo_create(const gplo_config *cfg)
 {
+  /* boundary type of every side */
+  gplo_config c = *cfg;
+  for (int side = 0; side < 6; side++) {
+    if (c.bc[side] == 0) c.bc[side] = c.periodic ? GPLO_BC_PERIODIC : GPLO_BC_OUTFLOW;
+    if (c.bc[side] < GPLO_BC_PERIODIC || c.bc[side] > GPLO_BC_REFLECTIVE) return NULL;
+  }
+  for (int d = 0; d < c.dims; d++) {
+    if ((c.bc[2*d] == GPLO_BC_PERIODIC) != (c.bc[2*d + 1] == GPLO_BC_PERIODIC)) return NULL;
+    if (c.phys != GPLO_HD && c.bc[2*d] != GPLO_BC_PERIODIC) return NULL;
+  }
   gplo_state *s = (gplo_state *)calloc(1, sizeof(*s));
-  s->c = *cfg;
+  s->c = c;
   if (s->c.small_rho == 0.0) s->c.small_rho = 1.e-12;   /* Src/structs.hpp:407-408 */
   if (s->c.small_prs == 0.0) s->c.small_prs = 1.e-12;
   s->nv = cfg->phys == GPLO_HD ? 5 : 8;

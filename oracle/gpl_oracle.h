@@ -22,6 +22,7 @@ enum { GPLO_HD = 1, GPLO_MHD = 2, GPLO_RMHD = 3 };
 enum { GPLO_HLL = 1, GPLO_HLLC = 2, GPLO_HLLD = 3 };
 enum { GPLO_VANLEER_LIM = 26, GPLO_MC_LIM = 27 };    /* Src/gpluto.hpp:330-336 */
 enum { GPLO_UCT_HLL = 1 };
+enum { GPLO_BC_PERIODIC = 1, GPLO_BC_OUTFLOW = 2, GPLO_BC_REFLECTIVE = 3 };   /* pluto.ini [Boundary] */
 
 typedef struct gplo_config {
   int    phys;            /* GPLO_HD, GPLO_MHD or GPLO_RMHD */
@@ -34,10 +35,16 @@ typedef struct gplo_config {
   double gamma;
   double cfl, cfl_max_var;
   double small_rho, small_prs;
+  int    bc[6];           /* per side X1-beg, X1-end, X2-beg, ..., X3-end: GPLO_BC_*;
+                             0 takes periodic ? PERIODIC : OUTFLOW.  HD only for the
+                             non-periodic types (the staggered-field rules of MHD
+                             outflow/reflective sides are not restated) */
 } gplo_config;
 
 typedef struct gplo_state gplo_state;
 
+/* NULL for an unsupported configuration (a direction periodic on one side
+   only; MHD/RMHD with a non-periodic side) */
 gplo_state *gplo_create(const gplo_config *cfg);
 void        gplo_destroy(gplo_state *s);
 /* vc[nvar][nz][ny][nx]; vs[dims][nz][ny][nx] (lower faces, periodic) or NULL */

@@ -27,7 +27,9 @@ class Config(C.Structure):
                 ("nz", C.c_int), ("periodic", C.c_int), ("time_stepping", C.c_int),
                 ("limiter", C.c_int), ("solver", C.c_int), ("gamma", C.c_double),
                 ("cfl", C.c_double), ("cfl_max_var", C.c_double),
-                ("small_rho", C.c_double), ("small_prs", C.c_double)]
+                ("small_rho", C.c_double), ("small_prs", C.c_double), ("bc", C.c_int * 6)]
+
+BC = {"periodic": 1, "outflow": 2, "reflective": 3}
 
 
 _lib = None
@@ -63,15 +65,21 @@ class Oracle:
 
     def __init__(self, *, phys: str, n, gamma: float, solver: str = "hlld",
                  limiter: str = "MC_LIM", time_stepping: int = 2, periodic: bool = True,
-                 cfl: float = 0.4, cfl_max_var: float = 1.1):
+                 cfl: float = 0.4, cfl_max_var: float = 1.1, sides=None):
+        """sides: optional pluto.ini [Boundary] types of X1-beg, X1-end, X2-beg,
+        ..., X3-end ("periodic", "outflow", "reflective"); default from
+        ``periodic``."""
         nx, ny, nz = n
         dims = 3 if nz > 1 else (2 if ny > 1 else 1)
         self.n, self.dims = (nx, ny, nz), dims
         self.phys = PHYS[phys.lower()]
         self.nvar = 5 if self.phys == HD else 8
+        bc = (C.c_int * 6)(*([BC[t] for t in sides] if sides else [0] * 6))
         cfg = Config(self.phys, dims, nx, ny, nz, 1 if periodic else 0, time_stepping,
-                     LIMITERS[limiter], SOLVERS[solver], gamma, cfl, cfl_max_var, 1e-12, 1e-12)
+                     LIMITERS[limiter], SOLVERS[solver], gamma, cfl, cfl_max_var, 1e-12, 1e-12, bc)
         self._h = lib().gplo_create(C.byref(cfg))
+        if not self._h:
+            raise ValueError("oracle: unsupported configuration")
 
     def __del__(self):
         if getattr(self, "_h", None):

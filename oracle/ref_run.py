@@ -70,18 +70,19 @@ def _grid_line(name: str, n: int) -> str:
 
 def run_reference(binary: str, vc: np.ndarray, vs: np.ndarray | None, *, gamma: float,
                   steps: int, cfl: float, first_dt: float, solver: str,
-                  periodic: bool = True, cfl_max_var: float = 1.1,
+                  periodic: bool = True, cfl_max_var: float = 1.1, sides=None,
                   timeout: float = 3600.0, workdir: str | None = None) -> RefResult:
     if steps < 2:
         raise ValueError("the reference dumps its post-loop output only for >= 2 steps")
     nvar, nz, ny, nx = vc.shape
     bnd = "periodic" if periodic else "outflow"
+    sides = list(sides) if sides else [bnd] * 6
     ini = ["[Grid]\n\n", _grid_line("X1-grid", nx), _grid_line("X2-grid", ny),
            _grid_line("X3-grid", nz), "\n[Time]\n\n", f"CFL  {cfl!r}\n",
            f"CFL_max_var  {cfl_max_var!r}\n", "tstop  1.e30\n", f"first_dt  {first_dt!r}\n",
            "\n[Solver]\n\n", f"Solver  {solver}\n", "\n[Boundary]\n\n"]
-    for ax in ("X1", "X2", "X3"):
-        ini.append(f"{ax}-beg  {bnd}\n{ax}-end  {bnd}\n")
+    for d, ax in enumerate(("X1", "X2", "X3")):
+        ini.append(f"{ax}-beg  {sides[2*d]}\n{ax}-end  {sides[2*d + 1]}\n")
     ini += ["\n[Static Grid Output]\n\n", "uservar 0\n", f"dbl -1.0 {steps - 1} single_file\n",
             "flt -1.0 -1 single_file\n", "vtk -1.0 -1 single_file\n", "tab -1.0 -1\n",
             "ppm -1.0 -1\n", "png -1.0 -1\n", "log 1000000\n", "analysis -1.0 -1\n",

@@ -34,12 +34,20 @@ CASES = [
     # the CPU-runnable headline case: 1D Euler shock tube, 4096 cells, 400 steps
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),
+    ("hd1d_512_rk2_mc_hllc_walls", "hd1d", "RK2", "MC_LIM", (512, 1, 1), 5, 900, 1e-4, 0.8, "hllc", False, 1.4),
+    ("hd1d_400_rk3_vl_hll_wall_out", "hd1d", "RK3", "VANLEER_LIM", (400, 1, 1), 8, 600, 1e-4, 0.8, "hll", False, 1.4),
     # relativistic MHD (HLL with exact quartic speeds, energy-solve con2prim);
     # the first case drives cells into the pressure-fix branch
     ("rmhd3d_16_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (16, 16, 16), 3, 5, 1e-3, 0.4, "hll", True, 4 / 3),
     ("rmhd3d_12x14x10_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (12, 14, 10), 4, 6, 1e-3, 0.4, "hll", True, 4 / 3),
     ("rmhd3d_10x12x14_rk3_vl_hll", "rmhd3d", "RK3", "VANLEER_LIM", (10, 12, 14), 6, 4, 2e-3, 0.4, "hll", True, 4 / 3),
 ]
+
+# pluto.ini [Boundary] per side (X1-beg, X1-end, ...) where not all periodic /
+# all outflow: shock tubes between walls (ReflectiveBoundary), long enough for
+# the waves to reflect
+SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow",) * 4,
+         "hd1d_400_rk3_vl_hll_wall_out": ("reflective", "outflow") + ("outflow",) * 4}
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
@@ -57,9 +65,12 @@ def make(case):
         vc, vs = ic.mhd_turbulence(n, seed, gamma=gamma)
     else:
         vc, vs = ic.hd_riemann(n[0], seed), None
+    sides = SIDES.get(name)
+    if sides:
+        extra["sides"] = list(sides)
     binary = ref_run.ref_binary(cfg, ts, lim)
     r = ref_run.run_reference(binary, vc, vs, gamma=gamma, steps=steps, cfl=cfl,
-                              first_dt=first_dt, solver=solver, periodic=periodic)
+                              first_dt=first_dt, solver=solver, periodic=periodic, sides=sides)
     params = dict(name=name, cfg=cfg, time_stepping=ts, limiter=lim, n=list(n), seed=seed,
                   steps=steps, first_dt=first_dt, cfl=cfl, cfl_max_var=1.1, solver=solver,
                   periodic=periodic, gamma=gamma, time=r.time, err_log="cells reported errors" in r.log, **extra)

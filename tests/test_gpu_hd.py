@@ -1,5 +1,5 @@
-"""GPU parity of the HD stage update (PLM + HLLC/HLL + RK2/RK3, periodic or
-outflow sides, 1D/2D/3D), FP64, bit for bit:
+"""GPU parity of the HD stage update (PLM + HLLC/HLL + RK2/RK3, periodic,
+outflow or reflective sides, 1D/2D/3D), FP64, bit for bit:
 
 * against the reference CPU build's own outputs (tests/golden/hd1d_*.npz,
   among them the 4096-zone, 400-step shock tube);
@@ -16,11 +16,12 @@ from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
 pytestmark = pytest.mark.gpu
 
 
-def _stage(n, ts, lim, solver, periodic, cfl, gamma=1.4, first_dt=1e-3):
+def _stage(n, ts, lim, solver, periodic, cfl, gamma=1.4, first_dt=1e-3, sides=None):
     dims = 3 if n[2] > 1 else (2 if n[1] > 1 else 1)
     return StageUpdate(Definitions(physics="HD", dimensions=dims, time_stepping=ts, limiter=lim),
                        RuntimeSetup(n=n, solver=solver, cfl=cfl, first_dt=first_dt, gamma=gamma,
-                                    boundary="periodic" if periodic else "outflow"))
+                                    boundary="periodic" if periodic else "outflow",
+                                    sides=tuple(sides) if sides else ()))
 
 
 @pytest.mark.parametrize("name", cases("hd1d"))
@@ -28,7 +29,7 @@ def test_gpu_matches_reference_fixture(name):
     p, a = load(name)
     print(f"case {name}", flush=True)
     g = _stage(tuple(p["n"]), p["time_stepping"], p["limiter"], p["solver"], p["periodic"],
-               p["cfl"], p["gamma"], p["first_dt"])
+               p["cfl"], p["gamma"], p["first_dt"], p.get("sides"))
     g.upload(a["vc_in"])
     err, t, _ = g.run(p["steps"], p["first_dt"])
     vc, vs, _ = g.download()
@@ -48,6 +49,13 @@ CASES = [
     ((16, 12, 10), True, "hllc", "MC_LIM", "RK2", 10),
     ((12, 10, 14), False, "hllc", "VANLEER_LIM", "RK2", 10),
     ((300, 257, 1), True, "hllc", "MC_LIM", "RK2", 4),     # 256-thread blocks, ragged last block
+    # reflective walls alone and mixed with outflow and periodic directions
+    ((200, 1, 1), ("reflective", "reflective") + ("outflow",) * 4, "hllc", "MC_LIM", "RK2", 200),
+    ((40, 36, 1), ("reflective", "outflow", "outflow", "reflective", "outflow", "outflow"),
+     "hll", "VANLEER_LIM", "RK3", 40),
+    ((14, 12, 10), ("reflective",) * 6, "hllc", "MC_LIM", "RK2", 20),
+    ((20, 16, 12), ("periodic", "periodic", "reflective", "reflective", "outflow", "reflective"),
+     "hllc", "VANLEER_LIM", "RK2", 15),
 ]
 
 
@@ -55,13 +63,15 @@ CASES = [
 def test_gpu_matches_oracle(n, periodic, solver, lim, ts, steps):
     import oracle
     print(f"case n={n} periodic={periodic} {solver} {lim} {ts} steps={steps}", flush=True)
+    sides = periodic if isinstance(periodic, tuple) else None
+    periodic = periodic is True
     vc = ic.hd_blast(n, 5)
     o = oracle.Oracle(phys="hd", n=n, gamma=1.4, solver=solver, limiter=lim,
-                      time_stepping=int(ts[-1]), periodic=periodic, cfl=0.4)
+                      time_stepping=int(ts[-1]), periodic=periodic, cfl=0.4, sides=sides)
     o.set_state(vc)
     oerr, ot, odt = o.run(steps, 1e-3)
     ovc, _, ouc = o.get_state()
-    g = _stage(n, ts, lim, solver, periodic, 0.4)
+    g = _stage(n, ts, lim, solver, periodic, 0.4, sides=sides)
     g.upload(vc)
     gerr, gt, gdt = g.run(steps, 1e-3)
     gvc, _, guc = g.download()
@@ -107,4 +117,12 @@ def test_hd_rejects_unsupported():
     with pytest.raises(RuntimeError, match="invalid"):
         StageUpdate(Definitions(physics="HD", dimensions=1),
                     RuntimeSetup(n=(64, 2, 1), solver="hllc"))
+    # a direction periodic on one side only
+    with pytest.raises(RuntimeError, match="invalid"):
+        StageUpdate(Definitions(physics="HD", dimensions=1),
+                    RuntimeSetup(n=(64, 1, 1), solver="hllc",
+                                 sides=("periodic", "reflective") + ("outflow",) * 4))
+    # MHD takes periodic sides only
+    with pytest.raises(RuntimeError, match="not supported"):
+        StageUpdate(Definitions(), RuntimeSetup(n=(16, 16, 16), boundary="reflective"))
     assert _lib.GPL_EUNSUPPORTED == -4

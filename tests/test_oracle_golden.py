@@ -18,7 +18,8 @@ def test_oracle_matches_reference_bitwise(name):
     phys = re.match(r"([a-z]+?)\d", p["cfg"]).group(1)    # mhd3d -> mhd, rmhd3d -> rmhd
     o = oracle.Oracle(phys=phys, n=tuple(p["n"]), gamma=p["gamma"], solver=p["solver"],
                       limiter=p["limiter"], time_stepping=int(p["time_stepping"][-1]),
-                      periodic=p["periodic"], cfl=p["cfl"], cfl_max_var=p["cfl_max_var"])
+                      periodic=p["periodic"], cfl=p["cfl"], cfl_max_var=p["cfl_max_var"],
+                      sides=p.get("sides"))
     o.set_state(a["vc_in"], a.get("vs_in"))
     err, t, _ = o.run(p["steps"], p["first_dt"])
     vc, vs, _ = o.get_state()

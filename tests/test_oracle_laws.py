@@ -7,6 +7,9 @@ Each one fails on a plausible restatement mistake:
   axis permutation  -- a direction-specific component mix-up (t/b cycling)
   Sod shock tube    -- the HLLC solver against the exact Riemann solution
                        (Toro, "Riemann Solvers and Numerical Methods", ch. 4)
+  closed box        -- reflective walls: a wrong mirror index or a missing
+                       sign flip of the normal velocity lets mass through the
+                       wall; mirror symmetry of a mirrored IC is kept
 """
 import numpy as np
 import pytest
@@ -234,3 +237,48 @@ def test_rmhd_nonrelativistic_limit_tracks_mhd():
         dv = np.max(np.abs(outs["rmhd"][q] - outs["mhd"][q]))
         change = np.max(np.abs(outs["mhd"][q] - vcr[q]))
         assert dv <= 1e-2 * change, (q, dv, change)
+
+
+@pytest.mark.parametrize("solver", ["hllc", "hll"])
+def test_hd_reflective_box_conserves_mass(solver):
+    """Between reflective walls the mirrored ghost state is the wall state
+    with its normal velocity negated, so the HLL/HLLC mass flux through the
+    wall is zero: total mass stays at round-off while outflow sides change it."""
+    import oracle
+    n = (24, 20, 1)
+    vc = ic.hd_blast(n, 11)
+    totals = {}
+    for name, sides in (("walls", ("reflective",) * 4 + ("outflow",) * 2),
+                        ("open", ("outflow",) * 6)):
+        o = oracle.Oracle(phys="hd", n=n, gamma=1.4, solver=solver, periodic=False, sides=sides)
+        o.set_state(vc)
+        _, _, uc0 = o.get_state()
+        err, _, _ = o.run(30, 1e-3)
+        assert err == 0
+        _, _, uc1 = o.get_state()
+        totals[name] = (np.sum(uc0[0]), np.sum(uc1[0]))
+    m0, m1 = totals["walls"]
+    assert abs(m1 - m0) <= 1e-13 * m0
+    m0, m1 = totals["open"]
+    assert abs(m1 - m0) > 1e-8 * m0
+
+
+def test_hd_reflective_mirror_symmetry():
+    """A shock tube mirrored about x = 1/2 between two walls stays mirrored:
+    rho(x) = rho(1 - x), v(x) = -v(1 - x) to round-off."""
+    import oracle
+    n = 256
+    x = (np.arange(n) + 0.5) / n
+    vc = np.zeros((5, 1, 1, n))
+    inner = np.abs(x - 0.5) < 0.15
+    vc[0, 0, 0] = np.where(inner, 1.0, 0.125)
+    vc[4, 0, 0] = np.where(inner, 1.0, 0.1)
+    vc[1, 0, 0] = np.where(x < 0.5, -0.3, 0.3) * inner
+    o = oracle.Oracle(phys="hd", n=(n, 1, 1), gamma=1.4, solver="hllc", periodic=False,
+                      sides=("reflective",) * 2 + ("outflow",) * 4, cfl=0.8)
+    o.set_state(vc)
+    err, t, _ = o.run(300, 1e-4)
+    assert err == 0 and t > 0.2                  # the waves have hit the walls
+    v, _, _ = o.get_state()
+    assert np.max(np.abs(v[0, 0, 0] - v[0, 0, 0][::-1])) <= 1e-12
+    assert np.max(np.abs(v[1, 0, 0] + v[1, 0, 0][::-1])) <= 1e-12
