@@ -837,8 +837,57 @@ static void ghost_field_average(gplo_state *s, const int lo[3], const int hi[3])
   }
 }
 
+/* CT_FillStaggeredFields, Src/MHD/CT/ct_fill_mag_field.cpp:39-330: the
+   normal staggered component in the ghost layers of one side from
+   div B = 0, outward from the boundary face, over every transverse index
+   (ghosts included).  Face areas of the uniform Cartesian grid
+   (Src/set_geometry.cpp:124,147,166); the missing direction of a 2D box adds
+   its 0.0. */
+static void fill_staggered(gplo_state *s, int dir, int upper)
+{
+  const int dims = s->c.dims, t1 = (dir + 1)%3, t2 = (dir + 2)%3;
+  const double *dx = s->dx;
+  double A[3];
+  A[0] = dims == 3 ? 1.0*dx[1]*dx[2] : 1.0*dx[1];
+  A[1] = dims == 3 ? dx[0]*1.0*dx[2] : dx[0]*1.0;
+  A[2] = dx[0]*dx[1]*1.0;
+  int a, b, i;
+  for (b = 0; b < s->nt[t2]; b++)
+  for (a = 0; a < s->nt[t1]; a++) {
+    const int n = s->nt[dir];
+    const int i0 = upper ? s->end[dir] + 1 : s->beg[dir] - 1;
+    const int i1 = upper ? n - 1 : 0, st = upper ? 1 : -1;
+    for (i = i0; upper ? i <= i1 : i >= i1; i += st) {
+      int p[3];
+      p[dir] = i; p[t1] = a; p[t2] = b;
+      double Bp[3], Bm[3], dB[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < dims; d++) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[d] += 1;
+        Bp[d] = s->Vs[d][IS(s, q[0], q[1], q[2])];
+        Bm[d] = s->Vs[d][IS(s, p[0], p[1], p[2])];
+        dB[d] = A[d]*Bp[d] - A[d]*Bm[d];
+      }
+      /* the two transverse terms in the reference's order (x, y, z skipping dir) */
+      double ta = 0.0, tb = 0.0;
+      if (dir == 0)      { ta = dB[1]; tb = dB[2]; }
+      else if (dir == 1) { ta = dB[0]; tb = dB[2]; }
+      else               { ta = dB[0]; tb = dB[1]; }
+      if (!upper) {
+        s->Vs[dir][IS(s, p[0], p[1], p[2])] = (A[dir]*Bp[dir] + ta + tb)/A[dir];
+      } else {
+        int q[3] = {p[0], p[1], p[2]};
+        q[dir] += 1;
+        s->Vs[dir][IS(s, q[0], q[1], q[2])] = (A[dir]*Bm[dir] - (ta + tb))/A[dir];
+      }
+    }
+  }
+}
+
 /* Boundary(), Src/boundary.cpp:67-303: sides X1_BEG..X3_END in order;
-   PeriodicBoundary :502-669; OutflowBoundary :677-760 (cell data only). */
+   PeriodicBoundary :502-669; OutflowBoundary :677-760 with
+   CT_FillStaggeredFields for MHD/RMHD; ReflectiveBoundary :931 (HD); then
+   the ghost cells' field average (:275-303). */
 static void boundary(gplo_state *s)
 {
   int side, i, j, k, nv, d;
@@ -888,16 +937,33 @@ static void boundary(gplo_state *s)
         }
       }
     } else {
-      int i0[3] = {0, 0, 0};
+      /* OutflowBoundary, Src/boundary.cpp:677-760: every variable and the
+         tangential staggered components (BOX{J,K}_LOOP: one more face along
+         the component) copied from the edge layer; then the normal
+         component from div B = 0 (Src/boundary.cpp:193-199) */
       for (nv = 0; nv < s->nv; nv++)
         for (k = lo[2]; k <= hi[2]; k++)
         for (j = lo[1]; j <= hi[1]; j++)
         for (i = lo[0]; i <= hi[0]; i++) {
           int p[3] = {i, j, k};
           p[dir] = upper ? s->end[dir] : s->beg[dir];
-          (void)i0;
           VC(s,nv,i,j,k) = VC(s,nv,p[0],p[1],p[2]);
         }
+      if (is_mhd(s)) {
+        for (d = 0; d < s->c.dims; d++) {
+          if (d == dir) continue;
+          int ext[3] = {0, 0, 0};
+          ext[d] = 1;
+          for (k = lo[2]; k <= hi[2] + ext[2]; k++)
+          for (j = lo[1]; j <= hi[1] + ext[1]; j++)
+          for (i = lo[0]; i <= hi[0] + ext[0]; i++) {
+            int p[3] = {i, j, k};
+            p[dir] = upper ? s->end[dir] : s->beg[dir];
+            s->Vs[d][IS(s,i,j,k)] = s->Vs[d][IS(s,p[0],p[1],p[2])];
+          }
+        }
+        fill_staggered(s, dir, upper);
+      }
     }
     if (is_mhd(s)) ghost_field_average(s, lo, hi);
   }
@@ -1292,7 +1358,8 @@ gplo_state *gplo_create(const gplo_config *cfg)
   }
   for (int d = 0; d < c.dims; d++) {
     if ((c.bc[2*d] == GPLO_BC_PERIODIC) != (c.bc[2*d + 1] == GPLO_BC_PERIODIC)) return NULL;
-    if (c.phys != GPLO_HD && c.bc[2*d] != GPLO_BC_PERIODIC) return NULL;
+    if (c.phys != GPLO_HD && (c.bc[2*d] == GPLO_BC_REFLECTIVE || c.bc[2*d + 1] == GPLO_BC_REFLECTIVE))
+      return NULL;
   }
   gplo_state *s = (gplo_state *)calloc(1, sizeof(*s));
   s->c = c;

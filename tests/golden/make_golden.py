@@ -35,6 +35,12 @@ CASES = [
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_512_rk2_mc_hllc_walls", "hd1d", "RK2", "MC_LIM", (512, 1, 1), 5, 900, 1e-4, 0.8, "hllc", False, 1.4),
+    # MHD with outflow sides: tangential face copies, the normal ghost faces
+    # from div B = 0 (CT_FillStaggeredFields) and the ghost field average
+    ("mhd2d_32x24_rk2_mc_outflow", "mhd2d", "RK2", "MC_LIM", (32, 24, 1), 6, 12, 3e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd3d_12x14x10_rk2_mc_outflow", "mhd3d", "RK2", "MC_LIM", (12, 14, 10), 7, 8, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd3d_10x12x8_rk3_vl_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 8), 8, 6, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("rmhd3d_12x10x12_rk2_mc_hll_outflow", "rmhd3d", "RK2", "MC_LIM", (12, 10, 12), 5, 5, 1e-3, 0.4, "hll", False, 4 / 3),
     ("hd1d_400_rk3_vl_hll_wall_out", "hd1d", "RK3", "VANLEER_LIM", (400, 1, 1), 8, 600, 1e-4, 0.8, "hll", False, 1.4),
     # relativistic MHD (HLL with exact quartic speeds, energy-solve con2prim);
     # the first case drives cells into the pressure-fix branch
@@ -51,7 +57,7 @@ SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow"
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
-           "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0)}
+           "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0), "rmhd3d_12x10x12_rk2_mc_hll_outflow": (2.0, 1.0)}
 
 
 def make(case):
