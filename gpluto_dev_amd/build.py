@@ -34,6 +34,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("mhd3d_mixed.hip", MIXED),
            ("mhd3d_common.hip", []),
            ("diagnostics.hip", []),
+           ("mhd3d_boundary.hip", []),
            ("hd_kernels.hip", []),
            ("batch2d_fp64.hip", []),
            ("batch2d_mixed.hip", MIXED),

@@ -52,6 +52,11 @@ void k_diag(DiagBox b, const R *__restrict__ V, const double *__restrict__ Vs,
   q[2*NV] = 0.0;
   q[2*NV + 1] = 0.0;
   for (int c = blockIdx.x*blockDim.x + threadIdx.x; c < b.ncell; c += gridDim.x*blockDim.x) {
+    {
+      const int i = c % b.nx, t = c / b.nx, j = t % b.ny, k = t / b.ny;
+      if (i < b.lo[0] || i > b.hi[0] || j < b.lo[1] || j > b.hi[1] || k < b.lo[2] || k > b.hi[2])
+        continue;
+    }
     double v[NV];
 #pragma unroll
     for (int a = 0; a < NV; a++) {

@@ -73,6 +73,15 @@ public:
   float *net = nullptr;       // RMHD learned con2prim guess, when enabled
   bool net_on = false;
   double *diag = nullptr;     // diagnostics partials (DIAG_BLOCKS rows) and result row
+  // A box with an outflow side runs padded: the device box is the reference's
+  // array, two ghost layers + the active cells + two ghost layers + one spare
+  // layer (the outermost faces), refilled at every stage by mhd3d_boundary;
+  // the kernels treat it as periodic, and the cells within two of its edges,
+  // whose wrapped values never reach an active result, are refilled
+  // before they are read again.
+  bool padded = false;
+  int side[6] = {GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC};
+  int na[3] = {0, 0, 0};      // active cells per direction
 
   ~Mhd3dEngine() override
   {
@@ -85,8 +94,13 @@ public:
     cfg = c; env = e;
     fp64 = c.precision == GPL_FP64;
     rmhd = c.physics == GPL_RMHD;
-    n = (size_t)c.nx*c.ny*c.nz;
-    g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+    na[0] = c.nx; na[1] = c.ny; na[2] = c.nz;
+    padded = !all_periodic(c);
+    for (int q = 0; q < 6; q++) side[q] = side_type(c, q);
+    const int pad = padded ? 5 : 0, off = padded ? 2 : 0;
+    g.nx = c.nx + pad; g.ny = c.ny + pad; g.nz = c.nz + pad;
+    for (int d = 0; d < 3; d++) { g.lo[d] = off; g.hi[d] = off + na[d] - 1; }
+    n = (size_t)g.nx*g.ny*g.nz;
     g.ncell = (int)n;
     // dx = (xend - xbeg)/N for every cell (Src/set_grid.cpp:196)
     g.dx = (c.xend[0] - c.xbeg[0])/c.nx;
@@ -94,7 +108,8 @@ public:
     g.dz = (c.xend[2] - c.xbeg[2])/c.nz;
     const size_t rs = fp64 ? sizeof(double) : sizeof(float);
     // the fused x sweep holds one periodic x-row per block
-    fused = c.nx <= 512 && getenv("GPL_SPLIT_KERNELS") == nullptr;
+    fused = g.nx <= 512 && getenv("GPL_SPLIT_KERNELS") == nullptr;
+    if (padded && !fused) return GPL_EUNSUPPORTED;
     // GPL_MARCH = 8/16/32: cells per thread in the y/z marching sweeps
     if (const char *m = getenv("GPL_MARCH")) march = atoi(m);
     if (march != 8 && march != 16 && march != 32) march = 16;
@@ -106,7 +121,7 @@ public:
     }
     fused_xy = !fp64;
     if (const char *m = getenv("GPL_FUSED_XY")) fused_xy = atoi(m) != 0;
-    fused_xy = fused_xy && fused && 20*(size_t)c.nx*rs <= 65536;
+    fused_xy = fused_xy && fused && 20*(size_t)g.nx*rs <= 65536;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);
@@ -128,23 +143,43 @@ public:
     return rc ? GPL_ENOMEM : 0;
   }
 
+  // device index of active cell (i, j, k) and of the device cell that an
+  // index of the padded box reads at upload (image or clamped edge)
+  size_t dev_cell(int i, int j, int k) const
+  {
+    return (size_t)(i + g.lo[0]) + (size_t)g.nx*((size_t)(j + g.lo[1]) + (size_t)g.ny*(k + g.lo[2]));
+  }
+  int src_index(int d, int p, bool wrap) const
+  {
+    const int a = p - g.lo[d], m = na[d];
+    if (wrap || side[2*d] == GPL_PERIODIC) return ((a % m) + m) % m;
+    return a < 0 ? 0 : (a >= m ? m - 1 : a);
+  }
+
   int upload(const double *vc, const double *const *vsh, int layout) override
   {
     if (!vsh || !vsh[0] || !vsh[1] || !vsh[2]) return GPL_EINVAL;
     const Layout L = make_layout(cfg, layout);
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     std::vector<double> hv(8*n), hu(5*n), hs(3*n);
+    // the whole device box: active values, and in the ghost layers of a
+    // padded box the periodic image or the clamped edge value of a cell, and
+    // the periodic image of a face (the seeded IC's face field is the curl of
+    // a periodic potential; an outflow box's upper boundary face starts as
+    // the image of its lower one)
     for (int d = 0; d < 3; d++)
       for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
           for (int i = 0; i < nx; i++)
-            hs[d*n + ((size_t)k*ny + j)*nx + i] = vsh[d][L.face(d, i, j, k)];
+            hs[d*n + ((size_t)k*ny + j)*nx + i] =
+                vsh[d][L.face(d, src_index(0, i, true), src_index(1, j, true), src_index(2, k, true))];
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
           const size_t o = ((size_t)k*ny + j)*nx + i;
           double v[8], u[8];
-          for (int nv = 0; nv < 8; nv++) v[nv] = vc[L.cell(nv, i, j, k)];
+          for (int nv = 0; nv < 8; nv++)
+            v[nv] = vc[L.cell(nv, src_index(0, i, false), src_index(1, j, false), src_index(2, k, false))];
           // cell-centred field = face average (Src/startup.cpp:388 -> ct_field_average.cpp:182-184)
           const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1, kp = k + 1 == nz ? 0 : k + 1;
           v[4] = 0.5*(hs[0*n + o] + hs[0*n + ((size_t)k*ny + j)*nx + ip]);
@@ -170,14 +205,14 @@ public:
   {
     const Layout L = make_layout(cfg, layout);
     const int nx = L.nx, ny = L.ny, nz = L.nz;
+    std::vector<double> hv(8*n);
+    if ((vc || uc) && from_device_v(V, fp64, hv)) return GPL_EHIP;
     if (vc) {
-      std::vector<double> hv(8*n);
-      if (from_device_v(V, fp64, hv)) return GPL_EHIP;
       for (int nv = 0; nv < 8; nv++)
         for (int k = 0; k < nz; k++)
           for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++)
-              vc[L.cell(nv, i, j, k)] = hv[nv*n + ((size_t)k*ny + j)*nx + i];
+              vc[L.cell(nv, i, j, k)] = hv[nv*n + dev_cell(i, j, k)];
     }
     if (vsh) {
       std::vector<double> hs(3*n);
@@ -188,30 +223,32 @@ public:
         for (int k = 0; k < nz; k++)
           for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++)
-              vsh[d][L.face(d, i, j, k)] = hs[d*n + ((size_t)k*ny + j)*nx + i];
-        if (L.ary) {   // the duplicate upper face equals the lower one (PeriodicBoundary)
+              vsh[d][L.face(d, i, j, k)] = hs[d*n + dev_cell(i, j, k)];
+        if (L.ary) {
+          // the upper boundary face: the periodic image of the lower one
+          // (PeriodicBoundary), or its own value on an outflow side
           for (int k = 0; k < nz; k++)
             for (int j = 0; j < ny; j++)
               for (int i = 0; i < nx; i++) {
-                if (d == 0 && i == 0) vsh[0][L.face(0, nx, j, k)] = hs[((size_t)k*ny + j)*nx];
-                if (d == 1 && j == 0) vsh[1][L.face(1, i, ny, k)] = hs[n + ((size_t)k*ny)*nx + i];
-                if (d == 2 && k == 0) vsh[2][L.face(2, i, j, nz)] = hs[2*n + (size_t)j*nx + i];
+                const bool per = side[2*d] == GPL_PERIODIC;
+                if (d == 0 && i == 0) vsh[0][L.face(0, nx, j, k)] = hs[dev_cell(per ? 0 : nx, j, k)];
+                if (d == 1 && j == 0) vsh[1][L.face(1, i, ny, k)] = hs[n + dev_cell(i, per ? 0 : ny, k)];
+                if (d == 2 && k == 0) vsh[2][L.face(2, i, j, nz)] = hs[2*n + dev_cell(i, j, per ? 0 : nz)];
               }
         }
       }
     }
     if (uc) {
-      std::vector<double> hu(5*n), hv(8*n);
+      std::vector<double> hu(5*n);
       if (hipMemcpy(hu.data(), U, 5*n*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return GPL_EHIP;
       // the cell-centred field of Uc is the face average (= Vc's B1..3)
-      if (from_device_v(V, fp64, hv)) return GPL_EHIP;
       const int map[8] = {0, 1, 2, 3, -1, -1, -1, 4};
       for (int nv = 0; nv < 8; nv++)
         for (int k = 0; k < nz; k++)
           for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
-              const size_t o = ((size_t)k*ny + j)*nx + i;
+              const size_t o = dev_cell(i, j, k);
               uc[L.cell(nv, i, j, k)] = map[nv] >= 0 ? hu[map[nv]*n + o] : hv[nv*n + o];
             }
     }
@@ -219,9 +256,11 @@ public:
   }
 
   template <typename R>
-  void enqueue_stage(const double *uin, double *uout, const double *vin, double *vout,
+  void enqueue_stage(const double *uin, double *uout, double *vin, double *vout,
                      const double *vbase, RKW w, int first_stage, int want_dt, LaunchTimer *t)
   {
+    // Boundary (Src/rk_step.cpp: at the start of every stage) on a padded box
+    if (padded) mhd3d_boundary<R>(g, side, (R *)V, vin, env.stream);
     Mhd3dBufs<R> b;
     b.V = (R *)V; b.Uin = uin; b.U = uout; b.U0 = U;
     b.Vin = vin; b.Vout = vout; b.Bs0 = vbase;
@@ -269,7 +308,7 @@ public:
     if (t) t->end(K_DT);
   }
 
-  void stage(const double *uin, double *uout, const double *vin, double *vout, const double *vbase,
+  void stage(const double *uin, double *uout, double *vin, double *vout, const double *vbase,
              RKW w, int first, LaunchTimer *t)
   {
     if (fp64) enqueue_stage<double>(uin, uout, vin, vout, vbase, w, first, first, t);
@@ -286,6 +325,7 @@ public:
   {
     DiagBox d;
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 8;
+    for (int q = 0; q < 3; q++) { d.lo[q] = g.lo[q]; d.hi[q] = g.hi[q]; }
     // face areas of the uniform Cartesian grid (Src/set_geometry.cpp:124,147,166)
     d.ax = 1.0*g.dy*g.dz; d.ay = g.dx*1.0*g.dz; d.az = g.dx*g.dy*1.0;
     double *res = diag + (size_t)DIAG_BLOCKS*DIAG_MAXQ;
@@ -314,18 +354,27 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
 {
   *out = nullptr;
   if (cfg.dimensions != 3) return GPL_EUNSUPPORTED;
-  if (!all_periodic(cfg) || cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  // periodic or outflow sides (outflow: the padded box); a direction is
+  // periodic on both sides or on neither
+  for (int d = 0; d < 3; d++) {
+    const int lo = side_type(cfg, 2*d), hi = side_type(cfg, 2*d + 1);
+    if ((lo != GPL_PERIODIC && lo != GPL_OUTFLOW) || (hi != GPL_PERIODIC && hi != GPL_OUTFLOW))
+      return GPL_EUNSUPPORTED;
+    if ((lo == GPL_PERIODIC) != (hi == GPL_PERIODIC)) return GPL_EINVAL;
+  }
+  const int pad = all_periodic(cfg) ? 0 : 5;
   if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
   // RMHD: HLL, FP64 and the fused kernels (rows of at most 512 zones)
   if (cfg.physics == GPL_RMHD &&
-      (cfg.solver != GPL_SOLVER_HLL || cfg.precision != GPL_FP64 || cfg.nx > 512 ||
+      (cfg.solver != GPL_SOLVER_HLL || cfg.precision != GPL_FP64 || cfg.nx + pad > 512 ||
        getenv("GPL_SPLIT_KERNELS") != nullptr))
     return GPL_EUNSUPPORTED;
   // the PLM stencil wraps at most two cells: every direction needs >= 3 zones
   if (cfg.nx < 3 || cfg.ny < 3 || cfg.nz < 3) return GPL_EINVAL;
   // the kernels address every array slot with 32-bit byte offsets (< 4 GiB)
-  if ((long long)cfg.nx*cfg.ny*cfg.nz >= (1LL << 29)) return GPL_EINVAL;
+  if ((long long)(cfg.nx + pad)*(cfg.ny + pad)*(cfg.nz + pad) >= (1LL << 29)) return GPL_EINVAL;
   Mhd3dEngine *e = new (std::nothrow) Mhd3dEngine();
   if (!e) return GPL_ENOMEM;
   const int rc = e->init(cfg, env);

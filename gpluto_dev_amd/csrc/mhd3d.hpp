@@ -11,6 +11,10 @@ struct Box3 {
   int nx, ny, nz;
   int ncell;
   double dx, dy, dz;
+  // active cells lo..hi per direction: the whole box for a periodic box; a
+  // box padded with ghost layers (non-periodic sides) reduces C_dt and
+  // reports error bits from the reference's cells and faces only
+  int lo[3], hi[3];
 };
 
 template <typename R>
@@ -44,6 +48,11 @@ template <typename R>
 void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl *ctl, RKW w,
                   int first_stage, int want_dt, int limiter, int solver, int march, double gamma,
                   double small_rho, double small_prs, hipStream_t s);
+
+// Ghost-zone fill of a padded box (non-periodic sides), side by side;
+// type[s] = GPL_PERIODIC or GPL_OUTFLOW for X1-beg, X1-end, ..., X3-end.
+template <typename R>
+void mhd3d_boundary(const Box3 &g, const int type[6], R *V, double *Vs, hipStream_t s);
 
 int mhd3d_blocks(const Box3 &g);
 int mhd3d_march_blocks(const Box3 &g, int march);

@@ -52,6 +52,26 @@ __device__ __forceinline__ void sto(T *base, unsigned idx, T v)
 
 struct Cell { int i, j, k; };
 
+__device__ __forceinline__ bool is_active(const Box3 &g, int i, int j, int k)
+{
+  return i >= g.lo[0] && i <= g.hi[0] && j >= g.lo[1] && j <= g.hi[1] && k >= g.lo[2] && k <= g.hi[2];
+}
+
+// The reference solves the faces of the active cells and the upper boundary
+// face along a sweep, over boxes widened by two ghost layers across it
+// (update_stage.cpp:80-99); error bits count from those faces only.
+template <int D>
+__device__ __forceinline__ bool face_reported(const Box3 &g, int i, int j, int k)
+{
+  const int p[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int lo = d == D ? g.lo[d] : g.lo[d] - 2, hi = d == D ? g.hi[d] + 1 : g.hi[d] + 2;
+    if (p[d] < lo || p[d] > hi) return false;
+  }
+  return true;
+}
+
 __device__ __forceinline__ Cell cell_of(int c, const Box3 &g)
 {
   Cell r;
@@ -175,7 +195,7 @@ void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
   fd[2*n + c] = aL;
   fd[3*n + c] = aR;
   fd[4*n + c] = dd;
-  if (ff.err) atomicOr(err, ff.err);
+  if (ff.err && face_reported<D>(g, p.i, p.j, p.k)) atomicOr(err, ff.err);
 }
 
 // ---------------------------------------------------------------------------
@@ -297,7 +317,8 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
     sto(dU + nv*n, c, (R)r);
   }
   if (want_cdt) sto(Cdt, c, (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx));
-  if (ff.err) atomicOr(err, ff.err);
+  if (ff.err && face_reported<0>(g, i, (int)blockIdx.x % g.ny, (int)blockIdx.x / g.ny))
+    atomicOr(err, ff.err);
 }
 
 // ---------------------------------------------------------------------------
@@ -404,7 +425,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
       vl[4] = vr[4] = bn;
       FaceFlux<R> ff;
       riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
-      e |= ff.err;
+      if (face_reported<0>(g, i, f, k)) e |= ff.err;
       {
         R et, eb, aL, aR, dd;
         uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
@@ -441,7 +462,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     vl[5] = vr[5] = byf;
     FaceFlux<R> ff;
     riemann<R, 1, SOLVER, PH>(vl, vr, gamma, ff);
-    e |= ff.err;
+    if (face_reported<1>(g, i, wrapm(f, ny), k)) e |= ff.err;
     if (f < s1) {
       R et, eb, aL, aR, dd;
       uct_hll_store<R, 1>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
@@ -589,7 +610,10 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       vl[BXn] = vr[BXn] = (R)ldo(Bn, cf);
       FaceFlux<R> ff;
       riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff);
-      e |= ff.err;
+      {
+        const int ff_ = wrapm(f, nd);
+        if (D == 1 ? face_reported<1>(g, i, ff_, a) : face_reported<2>(g, i, a, ff_)) e |= ff.err;
+      }
       if (f < s1) {
         R et, eb, aL, aR, dd;
         uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
@@ -617,7 +641,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
           double cd = cacc;
           cd += 0.5*(cmp + cm)/ddx;
           if (D == 1) sto(Cdt, cc, (R)cd);
-          else cdt_max = cdt_max >= cd ? cdt_max : cd;
+          else if (is_active(g, i, a, wrapm(f - 1, nd))) cdt_max = cdt_max >= cd ? cdt_max : cd;
         }
       }
 #pragma unroll
@@ -1092,14 +1116,14 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
       sto(U, c, u8[0]); sto(U + n, c, u8[1]); sto(U + 2*n, c, u8[2]); sto(U + 3*n, c, u8[3]);
       sto(U + 4*n, c, u8[7]);
     }
-    if (e) atomicOr(err, e);
+    if (e && is_active(g, p.i, p.j, p.k)) atomicOr(err, e);
   } else {
     const uint32_t e = mhd_cons_to_prim(u, b, gmm1, small_rho, small_prs, v);
     if (!live) return;
     if (e) {
 #pragma unroll
       for (int nv = 0; nv < 5; nv++) sto(U + nv*n, c, u[nv]);
-      atomicOr(err, e);
+      if (is_active(g, p.i, p.j, p.k)) atomicOr(err, e);
     }
   }
 #pragma unroll

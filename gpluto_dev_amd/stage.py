@@ -180,6 +180,22 @@ class StageUpdate:
                                         _lib.GPL_LAYOUT_ACTIVE), "gpl_download")
         return vc, vs, uc
 
+    def upper_faces(self):
+        """The staggered field's upper boundary faces (index n along each
+        direction), slabs [nz, ny, 1], [nz, 1, nx], [1, ny, nx] as the
+        reference's Vs stores them: the periodic image of the lower face on a
+        periodic side, the evolved face on an outflow side."""
+        nx, ny, nz = self.n
+        g = 2                                   # the Ary4D ghost zones (gpl_config.nghost)
+        t = (nx + 2 * g, ny + 2 * g, nz + 2 * g)
+        arrs = [np.zeros((t[2] + (d == 2), t[1] + (d == 1), t[0] + (d == 0))) for d in range(3)]
+        ptrs = (C.c_void_p * 3)(*[a.ctypes.data for a in arrs])
+        _lib.check(self._L.gpl_download(self._h, None, ptrs, None, _lib.GPL_LAYOUT_ARY4D),
+                   "gpl_download")
+        return [arrs[0][g:g + nz, g:g + ny, g + nx:g + nx + 1],
+                arrs[1][g:g + nz, g + ny:g + ny + 1, g:g + nx],
+                arrs[2][g + nz:g + nz + 1, g:g + ny, g:g + nx]]
+
     # -- stepping -----------------------------------------------------------
     def advance_step(self, dt: float):
         inv = C.c_double(0.0)

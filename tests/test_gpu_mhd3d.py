@@ -14,18 +14,20 @@ from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
 pytestmark = pytest.mark.gpu
 
 
-def _pair(n, seed, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64"):
+def _pair(n, seed, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64",
+          sides=None):
     import oracle
     vc, vs = ic.mhd_turbulence(n, seed)
-    print(f"case n={n} seed={seed} steps={steps} ts={ts} lim={lim} solver={solver} {precision}",
-          flush=True)
+    print(f"case n={n} seed={seed} steps={steps} ts={ts} lim={lim} solver={solver} {precision} "
+          f"sides={sides}", flush=True)
     o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, solver=solver, limiter=lim,
-                      time_stepping=int(ts[-1]))
+                      time_stepping=int(ts[-1]), periodic=sides is None, sides=sides)
     o.set_state(vc, vs)
     oerr, ot, odt = o.run(steps, first_dt)
     ovc, ovs, ouc = o.get_state()
     g = StageUpdate(Definitions(time_stepping=ts, limiter=lim),
-                    RuntimeSetup(n=n, solver=solver, first_dt=first_dt), precision=precision)
+                    RuntimeSetup(n=n, solver=solver, first_dt=first_dt, sides=tuple(sides or ())),
+                    precision=precision)
     g.upload(vc, vs)
     gerr, gt, gdt = g.run(steps, first_dt)
     gvc, gvs, guc = g.download()
@@ -131,3 +133,38 @@ def test_kernel_paths_bitwise(path, monkeypatch):
         assert gerr == oerr == 0
         assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
         assert gt == ot and gdt == odt
+
+
+OUT = ("outflow",) * 6
+XPER = ("periodic", "periodic") + ("outflow",) * 4          # periodic in x, outflow in y, z
+
+
+@pytest.mark.parametrize("n,sides,ts,lim,solver,steps", [
+    ((16, 12, 10), OUT, "RK2", "MC_LIM", "hlld", 8),
+    ((12, 14, 16), XPER, "RK3", "VANLEER_LIM", "hlld", 6),
+    ((14, 10, 12), OUT, "RK2", "MC_LIM", "hll", 6),
+    ((20, 17, 31), ("outflow", "outflow", "periodic", "periodic", "outflow", "outflow"),
+     "RK2", "MC_LIM", "hlld", 5),
+])
+def test_outflow_sides_bitwise(n, sides, ts, lim, solver, steps):
+    """Outflow sides (OutflowBoundary + CT_FillStaggeredFields every stage) on
+    the padded device box, alone and mixed with periodic directions."""
+    (oerr, ot, odt, ovc, ovs, ouc), (gerr, gt, gdt, gvc, gvs, guc) = _pair(
+        n, 17, steps, 2e-3, ts=ts, lim=lim, solver=solver, sides=sides)
+    assert gerr == oerr == 0
+    for v in range(8):
+        d = np.max(np.abs(gvc[v] - ovc[v]))
+        assert np.array_equal(gvc[v], ovc[v]), f"Vc var {v} differs: max {d:.3e}"
+    assert np.array_equal(gvs, ovs)
+    for v in (0, 1, 2, 3, 7):
+        assert np.array_equal(guc[v], ouc[v]), f"Uc var {v} differs"
+    assert gt == ot and gdt == odt
+
+
+def test_outflow_sides_mixed_precision():
+    (oerr, _, _, ovc, _, _), (gerr, _, _, gvc, gvs, _) = _pair(
+        (24, 20, 16), 18, 8, 1e-3, precision="mixed", sides=OUT)
+    assert gerr == oerr == 0
+    for v in range(8):
+        l1 = np.sum(np.abs(gvc[v] - ovc[v])) / np.sum(np.abs(ovc[v]))
+        assert l1 <= 1e-5, (v, l1)

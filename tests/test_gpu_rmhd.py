@@ -24,9 +24,10 @@ pytestmark = pytest.mark.gpu
 TOL = 5e-12
 
 
-def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3):
+def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3, periodic=True):
     return StageUpdate(Definitions(physics="RMHD", time_stepping=ts, limiter=lim),
-                       RuntimeSetup(n=n, solver="hll", cfl=cfl, gamma=gamma))
+                       RuntimeSetup(n=n, solver="hll", cfl=cfl, gamma=gamma,
+                                    boundary="periodic" if periodic else "outflow"))
 
 
 def _compare(gv, gs, ov, osv, label):
@@ -48,7 +49,7 @@ def _compare(gv, gs, ov, osv, label):
 def test_gpu_matches_reference_fixture(name):
     p, a = load(name)
     print(f"case {name}", flush=True)
-    g = _stage(tuple(p["n"]), p["time_stepping"], p["limiter"], p["cfl"], p["gamma"])
+    g = _stage(tuple(p["n"]), p["time_stepping"], p["limiter"], p["cfl"], p["gamma"], p["periodic"])
     g.upload(a["vc_in"], a["vs_in"])
     err, t, _ = g.run(p["steps"], p["first_dt"])
     vc, vs, _ = g.download()
