@@ -886,7 +886,8 @@ static void fill_staggered(gplo_state *s, int dir, int upper)
 
 /* Boundary(), Src/boundary.cpp:67-303: sides X1_BEG..X3_END in order;
    PeriodicBoundary :502-669; OutflowBoundary :677-760 with
-   CT_FillStaggeredFields for MHD/RMHD; ReflectiveBoundary :931 (HD); then
+   CT_FillStaggeredFields for MHD/RMHD; ReflectiveBoundary :931-1100 with
+   FlipSign :840-925; then
    the ghost cells' field average (:275-303). */
 static void boundary(gplo_state *s)
 {
@@ -903,11 +904,12 @@ static void boundary(gplo_state *s)
       /* ReflectiveBoundary, Src/boundary.cpp:931-1100, with FlipSign
          (:840-925): ghost cell i = vsign*Vc(i0 - i), i0 = 2*beg - 1 on the
          lower side and 2*end + 1 on the upper side; vsign flips the normal
-         velocity (HD) */
+         velocity and the normal field */
       const int i0 = upper ? 2*s->end[dir] + 1 : 2*s->beg[dir] - 1;
       int vsign[NVMAX];
       for (nv = 0; nv < s->nv; nv++) vsign[nv] = 1;
       vsign[1 + dir] = -1;
+      if (is_mhd(s)) vsign[BX1 + dir] = -1;
       for (nv = 0; nv < s->nv; nv++)
         for (k = lo[2]; k <= hi[2]; k++)
         for (j = lo[1]; j <= hi[1]; j++)
@@ -916,6 +918,29 @@ static void boundary(gplo_state *s)
           p[dir] = i0 - p[dir];
           VC(s,nv,i,j,k) = vsign[nv]*VC(s,nv,p[0],p[1],p[2]);
         }
+      if (is_mhd(s)) {
+        /* staggered part (Src/boundary.cpp:955-985 and analogues): the
+           tangential components mirrored over BOX{J,K}_LOOP, the normal one
+           mirrored about the boundary face with its sign flipped */
+        for (d = 0; d < s->c.dims; d++) {
+          int ext[3] = {0, 0, 0};
+          if (d != dir) ext[d] = 1;
+          for (k = lo[2]; k <= hi[2] + ext[2]; k++)
+          for (j = lo[1]; j <= hi[1] + ext[1]; j++)
+          for (i = lo[0]; i <= hi[0] + ext[0]; i++) {
+            int p[3] = {i, j, k}, q[3] = {i, j, k};
+            if (d != dir) {
+              p[dir] = i0 - p[dir];
+            } else if (!upper) {
+              p[dir] = i0 - p[dir] + 1;
+            } else {
+              p[dir] = i0 - p[dir];
+              q[dir] += 1;
+            }
+            s->Vs[d][IS(s,q[0],q[1],q[2])] = vsign[BX1 + d]*s->Vs[d][IS(s,p[0],p[1],p[2])];
+          }
+        }
+      }
     } else if (type == GPLO_BC_PERIODIC) {
       int sh = upper ? -nint : nint;
       int o[3] = {0, 0, 0}; o[dir] = sh;
@@ -1358,8 +1383,7 @@ gplo_state *gplo_create(const gplo_config *cfg)
   }
   for (int d = 0; d < c.dims; d++) {
     if ((c.bc[2*d] == GPLO_BC_PERIODIC) != (c.bc[2*d + 1] == GPLO_BC_PERIODIC)) return NULL;
-    if (c.phys != GPLO_HD && (c.bc[2*d] == GPLO_BC_REFLECTIVE || c.bc[2*d + 1] == GPLO_BC_REFLECTIVE))
-      return NULL;
+
   }
   gplo_state *s = (gplo_state *)calloc(1, sizeof(*s));
   s->c = c;

@@ -36,15 +36,13 @@ typedef struct gplo_config {
   double cfl, cfl_max_var;
   double small_rho, small_prs;
   int    bc[6];           /* per side X1-beg, X1-end, X2-beg, ..., X3-end: GPLO_BC_*;
-                             0 takes periodic ? PERIODIC : OUTFLOW.  MHD/RMHD:
-                             periodic or outflow (reflective staggered rules are
-                             not restated) */
+                             0 takes periodic ? PERIODIC : OUTFLOW */
 } gplo_config;
 
 typedef struct gplo_state gplo_state;
 
 /* NULL for an unsupported configuration (a direction periodic on one side
-   only; MHD/RMHD with a reflective side) */
+   only) */
 gplo_state *gplo_create(const gplo_config *cfg);
 void        gplo_destroy(gplo_state *s);
 /* vc[nvar][nz][ny][nx]; vs[dims][nz][ny][nx] (lower faces, periodic) or NULL */

@@ -41,6 +41,11 @@ CASES = [
     ("mhd3d_12x14x10_rk2_mc_outflow", "mhd3d", "RK2", "MC_LIM", (12, 14, 10), 7, 8, 2e-3, 0.4, "hlld", False, 5 / 3),
     ("mhd3d_10x12x8_rk3_vl_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 8), 8, 6, 2e-3, 0.4, "hlld", False, 5 / 3),
     ("rmhd3d_12x10x12_rk2_mc_hll_outflow", "rmhd3d", "RK2", "MC_LIM", (12, 10, 12), 5, 5, 1e-3, 0.4, "hll", False, 4 / 3),
+    # MHD between reflective walls (ReflectiveBoundary with FlipSign on v and B),
+    # and walls mixed with outflow and periodic directions
+    ("mhd3d_12x10x14_rk2_mc_walls", "mhd3d", "RK2", "MC_LIM", (12, 10, 14), 9, 8, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd3d_10x12x12_rk3_vl_mixed_sides", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 12), 10, 6, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd2d_28x20_rk2_mc_walls", "mhd2d", "RK2", "MC_LIM", (28, 20, 1), 11, 10, 3e-3, 0.4, "hlld", False, 5 / 3),
     ("hd1d_400_rk3_vl_hll_wall_out", "hd1d", "RK3", "VANLEER_LIM", (400, 1, 1), 8, 600, 1e-4, 0.8, "hll", False, 1.4),
     # relativistic MHD (HLL with exact quartic speeds, energy-solve con2prim);
     # the first case drives cells into the pressure-fix branch
@@ -53,7 +58,11 @@ CASES = [
 # all outflow: shock tubes between walls (ReflectiveBoundary), long enough for
 # the waves to reflect
 SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow",) * 4,
-         "hd1d_400_rk3_vl_hll_wall_out": ("reflective", "outflow") + ("outflow",) * 4}
+         "hd1d_400_rk3_vl_hll_wall_out": ("reflective", "outflow") + ("outflow",) * 4,
+         "mhd3d_12x10x14_rk2_mc_walls": ("reflective",) * 6,
+         "mhd3d_10x12x12_rk3_vl_mixed_sides": ("reflective", "reflective", "outflow", "outflow",
+                                               "periodic", "periodic"),
+         "mhd2d_28x20_rk2_mc_walls": ("reflective",) * 4 + ("outflow",) * 2}
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
