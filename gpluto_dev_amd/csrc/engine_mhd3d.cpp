@@ -73,7 +73,7 @@ public:
   float *net = nullptr;       // RMHD learned con2prim guess, when enabled
   bool net_on = false;
   double *diag = nullptr;     // diagnostics partials (DIAG_BLOCKS rows) and result row
-  // A box with an outflow side runs padded: the device box is the reference's
+  // A box with an outflow or reflective side runs padded: the device box is the reference's
   // array, two ghost layers + the active cells + two ghost layers + one spare
   // layer (the outermost faces), refilled at every stage by mhd3d_boundary;
   // the kernels treat it as periodic, and the cells within two of its edges,
@@ -355,11 +355,11 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   *out = nullptr;
   if (cfg.dimensions != 3) return GPL_EUNSUPPORTED;
   if (cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
-  // periodic or outflow sides (outflow: the padded box); a direction is
-  // periodic on both sides or on neither
+  // periodic, outflow or reflective sides (the last two: the padded box); a
+  // direction is periodic on both sides or on neither
   for (int d = 0; d < 3; d++) {
     const int lo = side_type(cfg, 2*d), hi = side_type(cfg, 2*d + 1);
-    if ((lo != GPL_PERIODIC && lo != GPL_OUTFLOW) || (hi != GPL_PERIODIC && hi != GPL_OUTFLOW))
+    if (lo < GPL_PERIODIC || lo > GPL_REFLECTIVE || hi < GPL_PERIODIC || hi > GPL_REFLECTIVE)
       return GPL_EUNSUPPORTED;
     if ((lo == GPL_PERIODIC) != (hi == GPL_PERIODIC)) return GPL_EINVAL;
   }

@@ -50,7 +50,7 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
                   double small_rho, double small_prs, hipStream_t s);
 
 // Ghost-zone fill of a padded box (non-periodic sides), side by side;
-// type[s] = GPL_PERIODIC or GPL_OUTFLOW for X1-beg, X1-end, ..., X3-end.
+// type[s] = GPL_PERIODIC, GPL_OUTFLOW or GPL_REFLECTIVE for X1-beg, ..., X3-end.
 template <typename R>
 void mhd3d_boundary(const Box3 &g, const int type[6], R *V, double *Vs, hipStream_t s);
 

@@ -9,6 +9,8 @@
 //                  tangential faces, then CT_FillStaggeredFields
 //                  (Src/MHD/CT/ct_fill_mag_field.cpp:39-330) for the normal
 //                  faces from div B = 0;
+//   reflective side: ReflectiveBoundary (:931-1100), mirror images with the
+//                  normal velocity and normal field negated (FlipSign);
 // then the ghost cells' field average (:275-303 with
 // Src/MHD/CT/ct_field_average.cpp:182-184).
 #include "mhd3d.hpp"
@@ -38,15 +40,23 @@ void k_bnd_copy(Box3 g, int side, int type, R *__restrict__ V, double *__restric
   const int layer = t / (na*nb), r = t - layer*na*nb, b = r / na, a = r - b*na;
   const int n = g.hi[dir] - g.lo[dir] + 1;
   const int gi = upper ? g.hi[dir] + 1 + layer : layer;          // ghost index along dir
-  const int src = type == GPL_PERIODIC ? gi + (upper ? -n : n) : (upper ? g.hi[dir] : g.lo[dir]);
+  // reflective: the mirror image i0 - i, i0 = 2*lo - 1 or 2*hi + 1 (Src/boundary.cpp:939-945)
+  const int i0 = upper ? 2*g.hi[dir] + 1 : 2*g.lo[dir] - 1;
+  const int src = type == GPL_PERIODIC ? gi + (upper ? -n : n)
+                : type == GPL_REFLECTIVE ? i0 - gi : (upper ? g.hi[dir] : g.lo[dir]);
   int p[3], q[3];
   p[dir] = gi; p[t1] = a; p[t2] = b;
   q[dir] = src; q[t1] = a; q[t2] = b;
   const size_t N = (size_t)g.ncell;
   const size_t cp = pid(g, p[0], p[1], p[2]), cq = pid(g, q[0], q[1], q[2]);
   const bool cell = layer < 2 && a <= na - 2 && b <= nb - 2;    // BOX_LOOP over the ghost box
-  if (cell)
-    for (int nv = 0; nv < 8; nv++) V[nv*N + cp] = V[nv*N + cq];
+  if (cell) {
+    for (int nv = 0; nv < 8; nv++) {
+      const R v = V[nv*N + cq];
+      // FlipSign (Src/boundary.cpp:840-925): a wall negates the normal velocity and field
+      V[nv*N + cp] = type == GPL_REFLECTIVE && (nv == 1 + dir || nv == 4 + dir) ? -v : v;
+    }
+  }
   if (layer < 2) {
     // tangential components: one more face along the component (BOX{I,J,K}_LOOP)
     if (a <= na - 1 && b <= nb - 2) Vs[t1*N + cp] = Vs[t1*N + cq];
@@ -56,6 +66,15 @@ void k_bnd_copy(Box3 g, int side, int type, R *__restrict__ V, double *__restric
   // more face, the spare layer) on the upper side
   if (type == GPL_PERIODIC && a <= na - 2 && b <= nb - 2 && (layer < 2 || upper))
     Vs[dir*N + cp] = Vs[dir*N + cq];
+  // reflective normal component, mirrored about the boundary face and
+  // negated: face i from i0 - i + 1 on the lower side, face i + 1 from i0 - i
+  // on the upper side
+  if (type == GPL_REFLECTIVE && layer < 2 && a <= na - 2 && b <= nb - 2) {
+    int t[3] = {p[0], p[1], p[2]}, f[3] = {q[0], q[1], q[2]};
+    if (!upper) f[dir] += 1;
+    else        t[dir] += 1;
+    Vs[dir*N + pid(g, t[0], t[1], t[2])] = -Vs[dir*N + pid(g, f[0], f[1], f[2])];
+  }
 }
 
 // one thread per transverse (a, b) of the ghost box: the outflow side's

@@ -66,8 +66,8 @@ typedef struct gpl_config {
   int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
-  int    boundary;         /* type of every side whose side_bc entry is 0: GPL_PERIODIC
-                              (MHD, RMHD, HD), GPL_OUTFLOW or GPL_REFLECTIVE (HD) */
+  int    boundary;         /* type of every side whose side_bc entry is 0: GPL_PERIODIC,
+                              GPL_OUTFLOW or GPL_REFLECTIVE */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
   int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLL; RMHD: _HLL; HD: _HLLC / _HLL */
@@ -82,7 +82,7 @@ typedef struct gpl_config {
   double small_prs;        /* run_config.small_prs (default 1e-12) */
   int    side_bc[6];       /* [Boundary] X1-beg, X1-end, X2-beg, X2-end, X3-beg, X3-end;
                               0 takes `boundary`.  A direction is periodic on both
-                              sides or on neither (HD: outflow and reflective mix) */
+                              sides or on neither; outflow and reflective mix */
 } gpl_config;
 
 typedef struct gpl_ctx gpl_ctx;

@@ -17,7 +17,8 @@ def test_gpu_matches_reference_fixture(name):
     g = StageUpdate(Definitions(time_stepping=p["time_stepping"], limiter=p["limiter"]),
                     RuntimeSetup(n=n, solver=p["solver"], cfl=p["cfl"], first_dt=p["first_dt"],
                                  gamma=p["gamma"],
-                                 boundary="periodic" if p["periodic"] else "outflow"))
+                                 boundary="periodic" if p["periodic"] else "outflow",
+                                 sides=tuple(p.get("sides") or ())))
     g.upload(a["vc_in"], a["vs_in"])
     err, t, _ = g.run(p["steps"], p["first_dt"])
     vc, vs, _ = g.download()

@@ -122,7 +122,8 @@ def test_hd_rejects_unsupported():
         StageUpdate(Definitions(physics="HD", dimensions=1),
                     RuntimeSetup(n=(64, 1, 1), solver="hllc",
                                  sides=("periodic", "reflective") + ("outflow",) * 4))
-    # MHD takes periodic sides only
-    with pytest.raises(RuntimeError, match="not supported"):
-        StageUpdate(Definitions(), RuntimeSetup(n=(16, 16, 16), boundary="reflective"))
+    # MHD: a direction periodic on one side only
+    with pytest.raises(RuntimeError, match="invalid"):
+        StageUpdate(Definitions(), RuntimeSetup(n=(16, 16, 16),
+                                                sides=("periodic", "outflow") + ("periodic",) * 4))
     assert _lib.GPL_EUNSUPPORTED == -4

@@ -145,10 +145,14 @@ XPER = ("periodic", "periodic") + ("outflow",) * 4          # periodic in x, out
     ((14, 10, 12), OUT, "RK2", "MC_LIM", "hll", 6),
     ((20, 17, 31), ("outflow", "outflow", "periodic", "periodic", "outflow", "outflow"),
      "RK2", "MC_LIM", "hlld", 5),
+    ((14, 12, 10), ("reflective",) * 6, "RK2", "MC_LIM", "hlld", 8),
+    ((12, 16, 14), ("reflective", "outflow", "periodic", "periodic", "outflow", "reflective"),
+     "RK3", "VANLEER_LIM", "hll", 6),
 ])
 def test_outflow_sides_bitwise(n, sides, ts, lim, solver, steps):
-    """Outflow sides (OutflowBoundary + CT_FillStaggeredFields every stage) on
-    the padded device box, alone and mixed with periodic directions."""
+    """Outflow sides (OutflowBoundary + CT_FillStaggeredFields every stage) and
+    reflective walls (ReflectiveBoundary) on the padded device box, alone and
+    mixed with each other and with periodic directions."""
     (oerr, ot, odt, ovc, ovs, ouc), (gerr, gt, gdt, gvc, gvs, guc) = _pair(
         n, 17, steps, 2e-3, ts=ts, lim=lim, solver=solver, sides=sides)
     assert gerr == oerr == 0
