@@ -1442,6 +1442,16 @@ static int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
    Uc = PrimToCons(Vc) on the active box (:514-561). */
 int gplo_set_state(gplo_state *s, const double *vc, const double *vs)
 {
+  return gplo_set_state_faces(s, vc, vs, NULL);
+}
+
+/* As gplo_set_state; on a direction whose sides are not periodic the upper
+   boundary face (index n along d) comes from up, the dims slabs of
+   RestartFromFile's staggered read (Src/restart.cpp:145-172 with
+   ReadOutputBuffer, Src/write_data.cpp:685-723: faces 0..n along d), back
+   to back, slab d shaped as the active box with n_d = 1. */
+int gplo_set_state_faces(gplo_state *s, const double *vc, const double *vs, const double *up)
+{
   const int nx = s->c.nx, ny = s->c.ny, nz = s->c.nz;
   const size_t na = (size_t)nx*ny*nz;
   for (int k = 0; k < s->nt[2]; k++)
@@ -1464,7 +1474,16 @@ int gplo_set_state(gplo_state *s, const double *vc, const double *vs)
       for (int j = 0; j < s->ns[1]; j++)
       for (int i = 0; i < s->ns[0]; i++) {
         int a = wrap(i - s->beg[0], nx), b = wrap(j - s->beg[1], ny), c = wrap(k - s->beg[2], nz);
-        s->Vs[d][IS(s,i,j,k)] = vs[(size_t)d*na + ((size_t)c*ny + b)*nx + a];
+        const int p[3] = {i - s->beg[0], j - s->beg[1], k - s->beg[2]}, n[3] = {nx, ny, nz};
+        if (up && s->c.bc[2*d] != GPLO_BC_PERIODIC && p[d] == n[d]) {
+          size_t off = 0;
+          for (int e = 0; e < d; e++) off += na/n[e];
+          const int sx = d == 0 ? 1 : nx, sy = d == 1 ? 1 : ny;
+          const int q[3] = {d == 0 ? 0 : a, d == 1 ? 0 : b, d == 2 ? 0 : c};
+          s->Vs[d][IS(s,i,j,k)] = up[off + ((size_t)q[2]*sy + q[1])*sx + q[0]];
+        } else {
+          s->Vs[d][IS(s,i,j,k)] = vs[(size_t)d*na + ((size_t)c*ny + b)*nx + a];
+        }
       }
     int lo[3] = {0, 0, 0}, hi[3] = {s->nt[0] - 1, s->nt[1] - 1, s->nt[2] - 1};
     ghost_field_average(s, lo, hi);

@@ -47,6 +47,11 @@ gplo_state *gplo_create(const gplo_config *cfg);
 void        gplo_destroy(gplo_state *s);
 /* vc[nvar][nz][ny][nx]; vs[dims][nz][ny][nx] (lower faces, periodic) or NULL */
 int         gplo_set_state(gplo_state *s, const double *vc, const double *vs);
+/* as gplo_set_state, with the upper boundary faces of non-periodic
+   directions from up[] (the dims slabs of a restart dump, back to back;
+   slab d shaped [nz][ny][nx] with n_d = 1); up may be NULL */
+int         gplo_set_state_faces(gplo_state *s, const double *vc, const double *vs,
+                                 const double *up);
 /* uc may be NULL */
 int         gplo_get_state(const gplo_state *s, double *vc, double *vs, double *uc);
 /* One AdvanceStep with time step dt; returns the ERR_* bits; *inv_dt gets

@@ -46,6 +46,7 @@ def lib():
         L.gplo_create.argtypes = [C.POINTER(Config)]
         L.gplo_destroy.argtypes = [C.c_void_p]
         L.gplo_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.gplo_set_state_faces.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.gplo_get_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.gplo_advance_step.restype = C.c_uint32
         L.gplo_advance_step.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double)]
@@ -86,10 +87,17 @@ class Oracle:
             lib().gplo_destroy(self._h)
             self._h = None
 
-    def set_state(self, vc, vs=None):
+    def set_state(self, vc, vs=None, upper=None):
+        """upper: optional upper boundary face slabs per staggered direction
+        (a restart dump's face index n), used where the sides are not
+        periodic."""
         vc = np.ascontiguousarray(vc, dtype=np.float64)
         vs = None if vs is None else np.ascontiguousarray(vs, dtype=np.float64)
-        if lib().gplo_set_state(self._h, _ptr(vc), _ptr(vs)) != 0:
+        up = None
+        if upper is not None:
+            up = np.ascontiguousarray(np.concatenate([np.ravel(u) for u in upper[:self.dims]]),
+                                      dtype=np.float64)
+        if lib().gplo_set_state_faces(self._h, _ptr(vc), _ptr(vs), _ptr(up)) != 0:
             raise ValueError("oracle: bad state")
 
     def get_state(self):

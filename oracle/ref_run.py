@@ -62,60 +62,44 @@ class RefResult:
     wall_s: float
     sec_per_step: float | None
     log: str
+    restart: dict | None = None
 
 
 def _grid_line(name: str, n: int) -> str:
     return f"{name}  0.0  {n}  1.0\n"
 
 
-def run_reference(binary: str, vc: np.ndarray, vs: np.ndarray | None, *, gamma: float,
-                  steps: int, cfl: float, first_dt: float, solver: str,
-                  periodic: bool = True, cfl_max_var: float = 1.1, sides=None,
-                  timeout: float = 3600.0, workdir: str | None = None) -> RefResult:
-    if steps < 2:
-        raise ValueError("the reference dumps its post-loop output only for >= 2 steps")
-    nvar, nz, ny, nx = vc.shape
-    bnd = "periodic" if periodic else "outflow"
-    sides = list(sides) if sides else [bnd] * 6
+def _ini(nx, ny, nz, cfl, cfl_max_var, first_dt, solver, sides, dn):
     ini = ["[Grid]\n\n", _grid_line("X1-grid", nx), _grid_line("X2-grid", ny),
            _grid_line("X3-grid", nz), "\n[Time]\n\n", f"CFL  {cfl!r}\n",
            f"CFL_max_var  {cfl_max_var!r}\n", "tstop  1.e30\n", f"first_dt  {first_dt!r}\n",
            "\n[Solver]\n\n", f"Solver  {solver}\n", "\n[Boundary]\n\n"]
     for d, ax in enumerate(("X1", "X2", "X3")):
         ini.append(f"{ax}-beg  {sides[2*d]}\n{ax}-end  {sides[2*d + 1]}\n")
-    ini += ["\n[Static Grid Output]\n\n", "uservar 0\n", f"dbl -1.0 {steps - 1} single_file\n",
+    ini += ["\n[Static Grid Output]\n\n", "uservar 0\n", f"dbl -1.0 {dn} single_file\n",
             "flt -1.0 -1 single_file\n", "vtk -1.0 -1 single_file\n", "tab -1.0 -1\n",
             "ppm -1.0 -1\n", "png -1.0 -1\n", "log 1000000\n", "analysis -1.0 -1\n",
             "\n[Parameters]\n\n", "DUMMY_PAR  0.0\n"]
-    own = workdir is None
-    tmp = tempfile.mkdtemp(prefix="gpl_ref_") if own else workdir
-    try:
-        with open(os.path.join(tmp, "pluto.ini"), "w") as f:
-            f.writelines(ini)
-        write_ic(os.path.join(tmp, "gpl_ic.bin"), vc, vs, gamma, periodic)
-        env = dict(os.environ, GPLUTO_FTZ_DAZ="0")  # IEEE mode (Src/cpu_fp.hpp:10-17)
-        t0 = time.perf_counter()
-        p = subprocess.run([binary, "-i", "pluto.ini", "-maxsteps", str(steps - 1)], cwd=tmp,
-                           env=env, capture_output=True, text=True, timeout=timeout)
-        wall = time.perf_counter() - t0
-        if p.returncode != 0:
-            raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
-        with open(os.path.join(tmp, "dbl.out")) as f:
-            last = f.read().strip().splitlines()[-1].split()
-        nfile, t = int(last[0]), float(last[1])
-        names = last[6:]
-        raw = np.fromfile(os.path.join(tmp, f"data.{nfile:04d}.dbl"), dtype="<f8")
-        spt = None
-        for line in p.stdout.splitlines():
-            if "Average time/step" in line:
-                try:
-                    spt = float(line.split()[-2 if line.strip().endswith("(sec)") else -1])
-                except ValueError:
-                    pass
-    finally:
-        if own:
-            import shutil
-            shutil.rmtree(tmp, ignore_errors=True)
+    return ini
+
+
+def _run(binary, tmp, args, timeout):
+    env = dict(os.environ, GPLUTO_FTZ_DAZ="0")  # IEEE mode (Src/cpu_fp.hpp:10-17)
+    t0 = time.perf_counter()
+    p = subprocess.run([binary, "-i", "pluto.ini"] + args, cwd=tmp,
+                       env=env, capture_output=True, text=True, timeout=timeout)
+    wall = time.perf_counter() - t0
+    if p.returncode != 0:
+        raise RuntimeError(f"reference run failed ({p.returncode}):\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
+    return p, wall
+
+
+def _read_last_dump(tmp, nx, ny, nz):
+    with open(os.path.join(tmp, "dbl.out")) as f:
+        last = f.read().strip().splitlines()[-1].split()
+    nfile, t = int(last[0]), float(last[1])
+    names = last[6:]
+    raw = np.fromfile(os.path.join(tmp, f"data.{nfile:04d}.dbl"), dtype="<f8")
     ncell = nx * ny * nz
     off = 0
     cells = []
@@ -139,6 +123,63 @@ def run_reference(binary: str, vc: np.ndarray, vs: np.ndarray | None, *, gamma: 
             cells.append(raw[off:off + ncell].reshape(nz, ny, nx))
             off += ncell
     assert off == raw.size, (off, raw.size)
-    return RefResult(vc=np.stack(cells), vs=np.stack(stag) if stag else None,
-                     vs_upper=upper if stag else None, time=t, wall_s=wall,
-                     sec_per_step=spt, log=p.stdout)
+    return np.stack(cells), (np.stack(stag) if stag else None), (upper if stag else None), t
+
+
+def _sec_per_step(log):
+    for line in log.splitlines():
+        if "Average time/step" in line:
+            try:
+                return float(line.split()[-2 if line.strip().endswith("(sec)") else -1])
+            except ValueError:
+                pass
+    return None
+
+
+def run_reference(binary: str, vc: np.ndarray, vs: np.ndarray | None, *, gamma: float,
+                  steps: int, cfl: float, first_dt: float, solver: str,
+                  periodic: bool = True, cfl_max_var: float = 1.1, sides=None,
+                  timeout: float = 3600.0, workdir: str | None = None,
+                  restart_steps: int = 0) -> RefResult:
+    """Run ``steps`` steps; with ``restart_steps`` > 0, restart the reference
+    from its post-loop dump (``-restart 1``, Src/main.cpp:277-283) and run
+    that many more steps.  After the first leg the step counter reads
+    steps - 1 (Src/main.cpp:336, 482), which RestartGet restores
+    (Src/restart.cpp:392, main.cpp:293), so the second leg passes
+    ``-maxsteps steps + restart_steps - 2`` and dumps with that dn.  The
+    result then holds the second leg's dump, and ``restart`` the first leg's
+    files (dbl.out text, restart.out and data.0001.dbl bytes) and state."""
+    if steps < 2:
+        raise ValueError("the reference dumps its post-loop output only for >= 2 steps")
+    nvar, nz, ny, nx = vc.shape
+    bnd = "periodic" if periodic else "outflow"
+    sides = list(sides) if sides else [bnd] * 6
+    own = workdir is None
+    tmp = tempfile.mkdtemp(prefix="gpl_ref_") if own else workdir
+    leg1 = None
+    try:
+        with open(os.path.join(tmp, "pluto.ini"), "w") as f:
+            f.writelines(_ini(nx, ny, nz, cfl, cfl_max_var, first_dt, solver, sides, steps - 1))
+        write_ic(os.path.join(tmp, "gpl_ic.bin"), vc, vs, gamma, periodic)
+        p, wall = _run(binary, tmp, ["-maxsteps", str(steps - 1)], timeout)
+        if restart_steps > 0:
+            rvc, rvs, rup, rt = _read_last_dump(tmp, nx, ny, nz)
+            files = {}
+            for name in ("dbl.out", "restart.out", "data.0001.dbl"):
+                with open(os.path.join(tmp, name), "rb") as f:
+                    files[name] = f.read()
+            leg1 = dict(vc=rvc, vs=rvs, vs_upper=rup, time=rt, files=files)
+            last = steps + restart_steps - 2
+            with open(os.path.join(tmp, "pluto.ini"), "w") as f:
+                f.writelines(_ini(nx, ny, nz, cfl, cfl_max_var, first_dt, solver, sides, last))
+            p, wall = _run(binary, tmp, ["-restart", "1", "-maxsteps", str(last)], timeout)
+            for name in ("dbl.out", "restart.out"):
+                with open(os.path.join(tmp, name), "rb") as f:
+                    leg1["files_final_" + name] = f.read()
+        ovc, ovs, oup, t = _read_last_dump(tmp, nx, ny, nz)
+    finally:
+        if own:
+            import shutil
+            shutil.rmtree(tmp, ignore_errors=True)
+    return RefResult(vc=ovc, vs=ovs, vs_upper=oup, time=t, wall_s=wall,
+                     sec_per_step=_sec_per_step(p.stdout), log=p.stdout, restart=leg1)

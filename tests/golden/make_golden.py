@@ -62,11 +62,69 @@ SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow"
          "mhd3d_12x10x14_rk2_mc_walls": ("reflective",) * 6,
          "mhd3d_10x12x12_rk3_vl_mixed_sides": ("reflective", "reflective", "outflow", "outflow",
                                                "periodic", "periodic"),
-         "mhd2d_28x20_rk2_mc_walls": ("reflective",) * 4 + ("outflow",) * 2}
+         "mhd2d_28x20_rk2_mc_walls": ("reflective",) * 4 + ("outflow",) * 2,
+         "mhd2d_12x10_rk2_mc_walls_restart": ("reflective",) * 4 + ("outflow",) * 2,
+         "hd1d_256_rk2_mc_hllc_walls_restart": ("reflective", "reflective") + ("outflow",) * 4}
+
+# restart: the reference runs `steps`, dumps (data.0001.dbl, dbl.out,
+# restart.out), restarts from that dump with -restart 1 and runs
+# `restart_steps` more; the fixture keeps the first leg's files as bytes
+RESTART_CASES = [
+    # name, cfg, ts, lim, n, seed, steps, restart_steps, first_dt, cfl, solver, periodic, gamma
+    ("mhd3d_8x6x6_rk2_mc_restart", "mhd3d", "RK2", "MC_LIM", (8, 6, 6), 12, 4, 3, 2e-3, 0.4, "hlld", True, 5 / 3),
+    ("mhd3d_6x8x6_rk3_vl_outflow_restart", "mhd3d", "RK3", "VANLEER_LIM", (6, 8, 6), 13, 3, 3, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd2d_12x10_rk2_mc_walls_restart", "mhd2d", "RK2", "MC_LIM", (12, 10, 1), 16, 5, 4, 3e-3, 0.4, "hlld", False, 5 / 3),
+    ("rmhd3d_6x8x6_rk2_mc_hll_restart", "rmhd3d", "RK2", "MC_LIM", (6, 8, 6), 14, 3, 3, 1e-3, 0.4, "hll", True, 4 / 3),
+    ("hd1d_256_rk2_mc_hllc_walls_restart", "hd1d", "RK2", "MC_LIM", (256, 1, 1), 15, 60, 50, 1e-4, 0.8, "hllc", False, 1.4),
+]
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
-           "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0), "rmhd3d_12x10x12_rk2_mc_hll_outflow": (2.0, 1.0)}
+           "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0), "rmhd3d_12x10x12_rk2_mc_hll_outflow": (2.0, 1.0),
+           "rmhd3d_6x8x6_rk2_mc_hll_restart": (2.5, 1.0)}
+
+
+def _ic(name, cfg, n, seed, gamma):
+    extra = {}
+    if cfg.startswith("rmhd"):
+        lmax, sigma = RMHD_IC[name]
+        vc, vs = ic.rmhd_turbulence(n, seed, gamma=gamma, lorentz_max=lmax, sigma=sigma)
+        extra = dict(lorentz_max=lmax, sigma=sigma)
+    elif cfg.startswith("mhd"):
+        vc, vs = ic.mhd_turbulence(n, seed, gamma=gamma)
+    else:
+        vc, vs = ic.hd_riemann(n[0], seed), None
+    return vc, vs, extra
+
+
+def make_restart(case):
+    name, cfg, ts, lim, n, seed, steps, rsteps, first_dt, cfl, solver, periodic, gamma = case
+    vc, vs, extra = _ic(name, cfg, n, seed, gamma)
+    sides = SIDES.get(name)
+    if sides:
+        extra["sides"] = list(sides)
+    binary = ref_run.ref_binary(cfg, ts, lim)
+    r = ref_run.run_reference(binary, vc, vs, gamma=gamma, steps=steps, cfl=cfl,
+                              first_dt=first_dt, solver=solver, periodic=periodic, sides=sides,
+                              restart_steps=rsteps)
+    leg = r.restart
+    params = dict(name=name, cfg=cfg, time_stepping=ts, limiter=lim, n=list(n), seed=seed,
+                  steps=steps, restart_steps=rsteps, first_dt=first_dt, cfl=cfl, cfl_max_var=1.1,
+                  solver=solver, periodic=periodic, gamma=gamma, time_mid=leg["time"], time=r.time,
+                  **extra)
+    u8 = lambda b: np.frombuffer(b, dtype=np.uint8)
+    arrays = dict(vc_in=vc, vc_mid=leg["vc"], vc_out=r.vc,
+                  ref_dbl_out=u8(leg["files"]["dbl.out"]),
+                  ref_restart_out=u8(leg["files"]["restart.out"]),
+                  ref_data_mid=u8(leg["files"]["data.0001.dbl"]),
+                  ref_dbl_out_final=u8(leg["files_final_dbl.out"]),
+                  ref_restart_out_final=u8(leg["files_final_restart.out"]))
+    if vs is not None:
+        arrays.update(vs_in=vs, vs_mid=leg["vs"], vs_out=r.vs)
+        arrays.update({f"vs_upper_mid{d}": u for d, u in enumerate(leg["vs_upper"])})
+        arrays.update({f"vs_upper{d}": u for d, u in enumerate(r.vs_upper)})
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=json.dumps(params), **arrays)
+    print(f"{name}: t = {leg['time']:.9e} -> {r.time:.9e}")
 
 
 def make(case):
@@ -102,3 +160,6 @@ if __name__ == "__main__":
     for c in CASES:
         if not only or c[0] in only:
             make(c)
+    for c in RESTART_CASES:
+        if not only or c[0] in only:
+            make_restart(c)

@@ -7,8 +7,18 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def cases(prefix=""):
+def _all(prefix):
     return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith(prefix))
+
+
+def cases(prefix=""):
+    """Fixtures of a plain run from the initial condition."""
+    return [c for c in _all(prefix) if not c.endswith("_restart")]
+
+
+def restart_cases(prefix=""):
+    """Fixtures of a run, its dump and a restart from that dump."""
+    return [c for c in _all(prefix) if c.endswith("_restart")]
 
 
 def load(name):
