@@ -19,7 +19,7 @@ GPL_RK2, GPL_RK3 = 2, 3
 GPL_PERIODIC, GPL_OUTFLOW, GPL_REFLECTIVE = 1, 2, 3
 GPL_FP64, GPL_MIXED = 0, 1
 GPL_LAYOUT_ACTIVE, GPL_LAYOUT_ARY4D = 0, 1
-GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED = -1, -2, -3, -4
+GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED, GPL_EIO = -1, -2, -3, -4, -5
 
 # every symbol include/gpluto_amd.h declares
 EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl_download",
@@ -30,7 +30,8 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_batch_run", "gpl_batch_set_tuning", "gpl_batch_get_tuning",
            "gpl_batch_last_run_ms", "gpl_batch_device_bytes",
            "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
-           "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics")
+           "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics",
+           "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl")
 
 
 class GplConfig(C.Structure):
@@ -66,6 +67,14 @@ def load() -> C.CDLL:
     L.gpl_upload.restype = ip
     L.gpl_download.argtypes = [vp, vp, C.POINTER(vp), vp, ip]
     L.gpl_download.restype = ip
+    L.gpl_set_step.argtypes = [vp, C.c_double, C.c_double, C.c_longlong]
+    L.gpl_set_step.restype = ip
+    L.gpl_get_step.argtypes = [vp, dp, dp, C.POINTER(C.c_longlong)]
+    L.gpl_get_step.restype = ip
+    L.gpl_write_dbl.argtypes = [vp, C.c_char_p, ip, ip]
+    L.gpl_write_dbl.restype = ip
+    L.gpl_read_dbl.argtypes = [vp, C.c_char_p, ip]
+    L.gpl_read_dbl.restype = ip
     L.gpl_advance_step.argtypes = [vp, C.c_double, dp]
     L.gpl_advance_step.restype = C.c_uint32
     L.gpl_next_time_step.argtypes = [vp, C.c_double, C.c_double]

@@ -43,6 +43,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("c2p_net.hip", []),
            ("engine_mhd3d.cpp", []),
            ("engine_hd.cpp", []),
+           ("dbl_io.cpp", []),
            ("gpl_api.cpp", [])]
 
 

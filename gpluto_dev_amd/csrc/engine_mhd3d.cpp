@@ -156,6 +156,17 @@ public:
     return a < 0 ? 0 : (a >= m ? m - 1 : a);
   }
 
+  // host index along dd of the face that device face (component d, index p
+  // along dd) takes at upload: the periodic image, except the upper
+  // boundary face of a non-periodic direction, which the reference's array
+  // layout carries (a restart dump's face n, Src/restart.cpp:145-172)
+  int face_src(const Layout &L, int d, int dd, int p) const
+  {
+    if (L.ary && dd == d && d < cfg.dimensions && side[2*d] != GPL_PERIODIC && p - g.lo[d] == na[d])
+      return na[d];
+    return src_index(dd, p, true);
+  }
+
   int upload(const double *vc, const double *const *vsh, int layout) override
   {
     if (!vsh || !vsh[0] || !vsh[1] || !vsh[2]) return GPL_EINVAL;
@@ -165,14 +176,14 @@ public:
     // the whole device box: active values, and in the ghost layers of a
     // padded box the periodic image or the clamped edge value of a cell, and
     // the periodic image of a face (the seeded IC's face field is the curl of
-    // a periodic potential; an outflow box's upper boundary face starts as
-    // the image of its lower one)
+    // a periodic potential; with the active layout an outflow box's upper
+    // boundary face starts as the image of its lower one)
     for (int d = 0; d < 3; d++)
       for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
           for (int i = 0; i < nx; i++)
             hs[d*n + ((size_t)k*ny + j)*nx + i] =
-                vsh[d][L.face(d, src_index(0, i, true), src_index(1, j, true), src_index(2, k, true))];
+                vsh[d][L.face(d, face_src(L, d, 0, i), face_src(L, d, 1, j), face_src(L, d, 2, k))];
     for (int k = 0; k < nz; k++)
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {

@@ -5,6 +5,7 @@
 // device, so a run of many steps replays one captured HIP graph without
 // host round trips), the error bits, and the per-kernel timing.
 #include "../../include/gpluto_amd.h"
+#include "dbl_io.hpp"
 #include "diagnostics.hpp"
 #include "engine.hpp"
 
@@ -14,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace gpl;
@@ -212,6 +214,162 @@ extern "C" int gpl_download(gpl_ctx *c, double *vc, double *const *vs, double *u
   return c->engine->download(vc, vs, uc, layout);
 }
 
+extern "C" int gpl_set_step(gpl_ctx *c, double time, double dt, long long nstep)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  StepCtl h;
+  if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(&h, c->ctl, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_EHIP;
+  h.time = time;
+  h.dt = dt;
+  h.nstep = nstep;
+  return hipMemcpy(c->ctl, &h, sizeof(h), hipMemcpyHostToDevice) == hipSuccess ? 0 : GPL_EHIP;
+}
+
+extern "C" int gpl_get_step(gpl_ctx *c, double *time, double *dt, long long *nstep)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  StepCtl h;
+  if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(&h, c->ctl, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_EHIP;
+  if (time) *time = h.time;
+  if (dt) *dt = h.dt;
+  if (nstep) *nstep = h.nstep;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// dbl dumps and restarts
+// ---------------------------------------------------------------------------
+namespace {
+
+// dumped variables in file order (Src/set_output.cpp:129-193: NVAR, then the
+// staggered components of the DIMENSIONS directions)
+std::vector<std::string> dump_names(const gpl_config &g)
+{
+  std::vector<std::string> v = {"rho", "vx1", "vx2", "vx3"};
+  if (g.physics != GPL_HD) { v.push_back("Bx1"); v.push_back("Bx2"); v.push_back("Bx3"); }
+  v.push_back("prs");
+  if (g.physics != GPL_HD)
+    for (int d = 0; d < g.dimensions; d++) v.push_back("Bx" + std::to_string(d + 1) + "s");
+  return v;
+}
+
+// the dump's staggered component d: n + 1 faces along d
+void face_dims(const gpl_config &g, int d, int e[3])
+{
+  e[0] = g.nx + (d == 0); e[1] = g.ny + (d == 1); e[2] = g.nz + (d == 2);
+}
+
+size_t face_array_size(const Layout &A, int d)
+{
+  return (size_t)(A.tx + (d == 0))*(A.ty + (d == 1))*(A.tz + (d == 2));
+}
+
+}  // namespace
+
+extern "C" int gpl_write_dbl(gpl_ctx *c, const char *dir, int nfile, int nstep)
+{
+  if (!c || !dir || nfile < 0) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
+  const gpl_config &g = c->cfg;
+  const bool mhd = g.physics != GPL_HD;
+  const size_t na = (size_t)g.nx*g.ny*g.nz;
+  const int nv = c->engine->nvar();
+  size_t count = (size_t)nv*na;
+  if (mhd)
+    for (int d = 0; d < g.dimensions; d++) {
+      int e[3];
+      face_dims(g, d, e);
+      count += (size_t)e[0]*e[1]*e[2];
+    }
+  std::vector<double> buf(count);
+  // the cell-centred blocks are the active layout itself
+  if (int rc = c->engine->download(buf.data(), nullptr, nullptr, GPL_LAYOUT_ACTIVE)) return rc;
+  if (mhd) {
+    // faces 0..n along d: the reference's array layout holds the upper face
+    const Layout A = make_layout(g, GPL_LAYOUT_ARY4D);
+    std::vector<std::vector<double>> fa(3);
+    double *ptr[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < g.dimensions; d++) { fa[d].assign(face_array_size(A, d), 0.0); ptr[d] = fa[d].data(); }
+    if (int rc = c->engine->download(nullptr, ptr, nullptr, GPL_LAYOUT_ARY4D)) return rc;
+    size_t o = (size_t)nv*na;
+    for (int d = 0; d < g.dimensions; d++) {
+      int e[3];
+      face_dims(g, d, e);
+      for (int k = 0; k < e[2]; k++)
+        for (int j = 0; j < e[1]; j++)
+          for (int i = 0; i < e[0]; i++) buf[o++] = fa[d][A.face(d, i, j, k)];
+    }
+  }
+  StepCtl h;
+  if (hipMemcpy(&h, c->ctl, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return GPL_EHIP;
+  const DumpStep st = {h.time, h.dt, nstep >= 0 ? nstep : (int)h.nstep};
+  return dbl_write(dir, nfile, st, dump_names(g), buf.data(), count);
+}
+
+extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
+{
+  if (!c || !dir || nfile < 0) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  const gpl_config &g = c->cfg;
+  const bool mhd = g.physics != GPL_HD;
+  const size_t na = (size_t)g.nx*g.ny*g.nz;
+  const int nv = c->engine->nvar();
+  size_t count = (size_t)nv*na;
+  if (mhd)
+    for (int d = 0; d < g.dimensions; d++) {
+      int e[3];
+      face_dims(g, d, e);
+      count += (size_t)e[0]*e[1]*e[2];
+    }
+  DumpStep st;
+  std::vector<double> buf;
+  if (int rc = dbl_read(dir, nfile, dump_names(g), count, &st, buf)) return rc;
+  if (!mhd) {
+    if (int rc = gpl_upload(c, buf.data(), nullptr, GPL_LAYOUT_ACTIVE)) return rc;
+  } else {
+    // the reference's array layout carries every face of the dump
+    const Layout A = make_layout(g, GPL_LAYOUT_ARY4D);
+    std::vector<double> vc((size_t)nv*A.tx*A.ty*A.tz, 0.0);
+    for (int q = 0; q < nv; q++)
+      for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+          for (int i = 0; i < g.nx; i++)
+            vc[A.cell(q, i, j, k)] = buf[(size_t)q*na + ((size_t)k*g.ny + j)*g.nx + i];
+    std::vector<std::vector<double>> fa(3);
+    size_t o = (size_t)nv*na;
+    for (int d = 0; d < 3; d++) {
+      fa[d].assign(face_array_size(A, d), 0.0);
+      if (d < g.dimensions) {
+        int e[3];
+        face_dims(g, d, e);
+        for (int k = 0; k < e[2]; k++)
+          for (int j = 0; j < e[1]; j++)
+            for (int i = 0; i < e[0]; i++) fa[d][A.face(d, i, j, k)] = buf[o++];
+      } else {
+        // a direction outside DIMENSIONS has no staggered component: its
+        // slot holds the cell-centred field on both faces of the one zone
+        for (int k = 0; k <= g.nz - (d == 2 ? 0 : 1); k++)
+          for (int j = 0; j <= g.ny - (d == 1 ? 0 : 1); j++)
+            for (int i = 0; i <= g.nx - (d == 0 ? 0 : 1); i++) {
+              const int ic = i < g.nx ? i : g.nx - 1, jc = j < g.ny ? j : g.ny - 1,
+                        kc = k < g.nz ? k : g.nz - 1;
+              fa[d][A.face(d, i, j, k)] = buf[(size_t)(4 + d)*na + ((size_t)kc*g.ny + jc)*g.nx + ic];
+            }
+      }
+    }
+    const double *vs[3] = {fa[0].data(), fa[1].data(), fa[2].data()};
+    if (int rc = gpl_upload(c, vc.data(), vs, GPL_LAYOUT_ARY4D)) return rc;
+  }
+  return gpl_set_step(c, st.t, st.dt, st.nstep);
+}
+
 // ---------------------------------------------------------------------------
 // stepping
 // ---------------------------------------------------------------------------
@@ -370,6 +528,7 @@ extern "C" const char *gpl_strerror(int code)
   case GPL_EHIP: return "HIP runtime error";
   case GPL_ENOMEM: return "out of device memory";
   case GPL_EUNSUPPORTED: return "configuration not supported";
+  case GPL_EIO: return "dump or restart file error";
   default: return "unknown error";
   }
 }

@@ -15,6 +15,7 @@ returned as the reference's ERR_* bitmask (Src/gpluto.hpp:247-265).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from dataclasses import dataclass, field
 
@@ -186,15 +187,38 @@ class StageUpdate:
         reference's Vs stores them: the periodic image of the lower face on a
         periodic side, the evolved face on an outflow side."""
         nx, ny, nz = self.n
-        g = 2                                   # the Ary4D ghost zones (gpl_config.nghost)
-        t = (nx + 2 * g, ny + 2 * g, nz + 2 * g)
+        # the Ary4D ghost zones (gpl_config.nghost = 2) along the used directions
+        gx, gy, gz = (2 if self.defs.dimensions > d else 0 for d in range(3))
+        t = (nx + 2 * gx, ny + 2 * gy, nz + 2 * gz)
         arrs = [np.zeros((t[2] + (d == 2), t[1] + (d == 1), t[0] + (d == 0))) for d in range(3)]
         ptrs = (C.c_void_p * 3)(*[a.ctypes.data for a in arrs])
         _lib.check(self._L.gpl_download(self._h, None, ptrs, None, _lib.GPL_LAYOUT_ARY4D),
                    "gpl_download")
-        return [arrs[0][g:g + nz, g:g + ny, g + nx:g + nx + 1],
-                arrs[1][g:g + nz, g + ny:g + ny + 1, g:g + nx],
-                arrs[2][g + nz:g + nz + 1, g:g + ny, g:g + nx]]
+        return [arrs[0][gz:gz + nz, gy:gy + ny, gx + nx:gx + nx + 1],
+                arrs[1][gz:gz + nz, gy + ny:gy + ny + 1, gx:gx + nx],
+                arrs[2][gz + nz:gz + nz + 1, gy:gy + ny, gx:gx + nx]]
+
+    # -- step state, dumps and restarts --------------------------------------
+    def set_step(self, time: float, dt: float, nstep: int):
+        """timeStep time / dt and g_stepNumber of the device loop."""
+        _lib.check(self._L.gpl_set_step(self._h, time, dt, nstep), "gpl_set_step")
+
+    def get_step(self):
+        """(time, dt, nstep) of the device loop."""
+        t, dt, n = C.c_double(0.0), C.c_double(0.0), C.c_longlong(0)
+        _lib.check(self._L.gpl_get_step(self._h, C.byref(t), C.byref(dt), C.byref(n)), "gpl_get_step")
+        return t.value, dt.value, n.value
+
+    def write_dbl(self, directory: str, nfile: int, nstep: int | None = None):
+        """WriteData (dbl, single_file) + RestartDump: data.<nfile>.dbl, line
+        nfile of dbl.out, record nfile of restart.out in `directory`; the
+        step counter written is `nstep`, or the device's own."""
+        _lib.check(self._L.gpl_write_dbl(self._h, os.fsencode(directory), nfile,
+                                         -1 if nstep is None else nstep), "gpl_write_dbl")
+
+    def read_dbl(self, directory: str, nfile: int):
+        """RestartFromFile: the state, time, dt and step counter of dump nfile."""
+        _lib.check(self._L.gpl_read_dbl(self._h, os.fsencode(directory), nfile), "gpl_read_dbl")
 
     # -- stepping -----------------------------------------------------------
     def advance_step(self, dt: float):

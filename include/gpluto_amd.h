@@ -58,6 +58,7 @@ extern "C" {
 #define GPL_EHIP         (-2)
 #define GPL_ENOMEM       (-3)
 #define GPL_EUNSUPPORTED (-4)
+#define GPL_EIO          (-5)   /* a dump or restart file could not be opened, read or written */
 
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
@@ -106,8 +107,37 @@ int gpl_destroy(gpl_ctx *ctx);
 int gpl_upload(gpl_ctx *ctx, const double *vc, const double *const *vs, int layout);
 
 /* Device -> host copy of Vc, Vs (MHD) and (if uc != NULL) the conserved Uc;
- * a NULL vc or vs is skipped. */
+ * a NULL vc or vs is skipped.  ARY4D: Vs[d] carries the upper boundary face
+ * (index n along d): the periodic image of face 0, else its own value. */
 int gpl_download(gpl_ctx *ctx, double *vc, double *const *vs, double *uc, int layout);
+
+/* The step state the main loop keeps in timeStep and g_stepNumber
+ * (Src/main.cpp:207-209, 336, 458-463): time, the next dt and the step
+ * counter, device-resident.  gpl_upload resets them to 0; gpl_run with
+ * first_dt <= 0 continues from them and advances all three. */
+int gpl_set_step(gpl_ctx *ctx, double time, double dt, long long nstep);
+int gpl_get_step(gpl_ctx *ctx, double *time, double *dt, long long *nstep);
+
+/* WriteData for the dbl output in single_file mode followed by RestartDump
+ * (Src/main.cpp:673-687; Src/write_data.cpp:86-159, 476-505;
+ * Src/restart.cpp:417-453): writes <dir>/data.<nfile>.dbl (the active zones
+ * of rho vx1 vx2 vx3 [Bx1 Bx2 Bx3] prs, then MHD/RMHD's Bx1s..Bx<D>s with
+ * faces 0..n along their direction), line nfile of <dir>/dbl.out (created
+ * when nfile is 0) and record nfile of <dir>/restart.out, with the device's
+ * time and dt.  nstep >= 0 is written as the step counter, nstep < 0 writes
+ * the context's own count (the reference's post-loop dump carries one less,
+ * Src/main.cpp:336,482,497).  Returns GPL_EIO on a file error. */
+int gpl_write_dbl(gpl_ctx *ctx, const char *dir, int nfile, int nstep);
+
+/* RestartFromFile for the dbl output (Src/restart.cpp:34-274, RestartGet
+ * :342-406): reads the restart.out record of dump nfile (time, dt, step
+ * counter), checks line nfile of dbl.out (single_file, endianness, the
+ * variables this context dumps), reads data.<nfile>.dbl (byte-swapped if
+ * needed) and uploads it as gpl_upload does: Uc = PrimToCons(Vc) and every
+ * face of the dump, the upper boundary face of a non-periodic direction
+ * included.  Returns GPL_EINVAL when the files do not hold dump nfile of
+ * this configuration, GPL_EIO when one cannot be read. */
+int gpl_read_dbl(gpl_ctx *ctx, const char *dir, int nfile);
 
 /* AdvanceStep(data, grid, Dts) with Dts->dt = dt (Src/rk_step.cpp:88):
  * one full RK step on the device.  Returns the OR of the ERR_* bits raised;
