@@ -104,8 +104,9 @@ void k2_sweepx(Batch2dBufs<R> b, R gamma, int want_cdt)
     }
     vl[4] = vr[4] = (R)b.Vin[c0];
     FaceFlux<R> ff;
-    if (SOLVER == 3) mhd_hlld<R, 0>(vl, vr, gamma, ff);
-    else             mhd_hll<R, 0>(vl, vr, gamma, ff);
+    if (SOLVER == 3)      mhd_hlld<R, 0>(vl, vr, gamma, ff);
+    else if (SOLVER == 2) mhd_hllc<R, 0>(vl, vr, gamma, ff);
+    else                  mhd_hll<R, 0>(vl, vr, gamma, ff);
     e |= ff.err;
     if (lane < nc) {
       R et, eb, aL, aR, dd;
@@ -194,8 +195,9 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
         }
         vl[5] = vr[5] = (R)b.Vin[n + cf];
         FaceFlux<R> ff;
-        if (SOLVER == 3) mhd_hlld<R, 1>(vl, vr, gamma, ff);
-        else             mhd_hll<R, 1>(vl, vr, gamma, ff);
+        if (SOLVER == 3)      mhd_hlld<R, 1>(vl, vr, gamma, ff);
+        else if (SOLVER == 2) mhd_hllc<R, 1>(vl, vr, gamma, ff);
+        else                  mhd_hll<R, 1>(vl, vr, gamma, ff);
         e |= ff.err;
         if (f < s1) {
           R et, eb, aL, aR, dd;
@@ -389,11 +391,13 @@ void batch2d_launch(int phase, const Batch2dBufs<R> &b, RKW w, int first_stage, 
                     double small_rho, double small_prs, hipStream_t s)
 {
   if (limiter == MC_LIM) {
-    if (solver == 3) b2_launch_l<R, MC_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
-    else             b2_launch_l<R, MC_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    if (solver == 3)      b2_launch_l<R, MC_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else if (solver == 2) b2_launch_l<R, MC_LIM, 2>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else                  b2_launch_l<R, MC_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
   } else {
-    if (solver == 3) b2_launch_l<R, VANLEER_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
-    else             b2_launch_l<R, VANLEER_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    if (solver == 3)      b2_launch_l<R, VANLEER_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else if (solver == 2) b2_launch_l<R, VANLEER_LIM, 2>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else                  b2_launch_l<R, VANLEER_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
   }
 }
 

@@ -94,7 +94,8 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   *out = nullptr;
   if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
   if (!all_periodic(*cfg) || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
-  if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
+  if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLLC && cfg->solver != GPL_SOLVER_HLL)
+    return GPL_EUNSUPPORTED;
   if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
   if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
   if (cfg->precision != GPL_FP64 && cfg->precision != GPL_MIXED) return GPL_EINVAL;

@@ -375,7 +375,8 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
     if ((lo == GPL_PERIODIC) != (hi == GPL_PERIODIC)) return GPL_EINVAL;
   }
   const int pad = all_periodic(cfg) ? 0 : 5;
-  if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLL) return GPL_EUNSUPPORTED;
+  if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLLC && cfg.solver != GPL_SOLVER_HLL)
+    return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
   // RMHD: HLL, FP64 and the fused kernels (rows of at most 512 zones)
   if (cfg.physics == GPL_RMHD &&

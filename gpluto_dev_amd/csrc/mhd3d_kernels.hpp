@@ -117,13 +117,14 @@ __device__ __forceinline__ T block_max(T v)
   return red[0];
 }
 
-// Riemann solver of the physics module: PH 0 = MHD (HLLD or HLL), PH 1 =
-// RMHD (HLL with the exact quartic fast speeds; FP64 only).
+// Riemann solver of the physics module: PH 0 = MHD (SOLVER 3 HLLD, 2 HLLC,
+// 1 HLL), PH 1 = RMHD (HLL with the exact quartic fast speeds; FP64 only).
 template <typename R, int N, int SOLVER, int PH>
 __device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
 {
   if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 3) mhd_hlld<R, N>(vl, vr, gamma, ff);
+  else if constexpr (SOLVER == 2) mhd_hllc<R, N>(vl, vr, gamma, ff);
   else mhd_hll<R, N>(vl, vr, gamma, ff);
 }
 
@@ -178,8 +179,7 @@ void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
   // normal field from the face (plm_states.cpp:184-189)
   vl[4 + D] = vr[4 + D] = (R)Vs[D*n + c];
   FaceFlux<R> ff;
-  if (SOLVER == 3) mhd_hlld<R, D>(vl, vr, gamma, ff);
-  else             mhd_hll<R, D>(vl, vr, gamma, ff);
+  riemann<R, D, SOLVER, 0>(vl, vr, gamma, ff);
   R *Fd = F + (size_t)D*5*n;
   Fd[0*n + c] = ff.f[0];
   Fd[1*n + c] = ff.f[1];
@@ -1251,11 +1251,13 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
     }
   }
   if (limiter == MC_LIM) {
-    if (solver == 3) launch_l<R, MC_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
-    else             launch_l<R, MC_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    if (solver == 3)      launch_l<R, MC_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else if (solver == 2) launch_l<R, MC_LIM, 2, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else                  launch_l<R, MC_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   } else {
-    if (solver == 3) launch_l<R, VANLEER_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
-    else             launch_l<R, VANLEER_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    if (solver == 3)      launch_l<R, VANLEER_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else if (solver == 2) launch_l<R, VANLEER_LIM, 2, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else                  launch_l<R, VANLEER_LIM, 1, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
   }
 }
 

@@ -297,6 +297,85 @@ GPL_HD void mhd_hll(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
   out.err = err;
 }
 
+// HLLC of Li (2005), Src/MHD/hllc.cpp:130-335 (ideal EOS, no GLM, no
+// background field; HLL_SWITCH is off in the default configuration,
+// Src/gpluto.hpp:877-878, so the star states are always used)
+template <typename R, int N>
+GPL_HD void mhd_hllc(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
+{
+  constexpr int VXn = 1 + N;
+  constexpr int MXn = VXn, MXt = 1 + (N + 1)%3, MXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
+  constexpr int ENG = 7;
+  const R gmm1 = gamma - R(1);
+  R ul[8], ur[8], fl[8], fr[8], pl, pr, sl_min, sl_max, sr_min, sr_max;
+  mhd_prim_to_cons(vl, ul, gmm1);
+  mhd_prim_to_cons(vr, ur, gmm1);
+  mhd_max_signal_speed<R, N>(vl, gamma, sl_min, sl_max);    // HLL_Speed, mhd.hpp:296
+  mhd_max_signal_speed<R, N>(vr, gamma, sr_min, sr_max);
+  const R Sl = rmin(sl_min, sr_min);
+  const R Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  mhd_flux<R, N>(vl, ul, fl, pl);     // pl, pr: total pressure
+  mhd_flux<R, N>(vr, ur, fr, pr);
+  R *flux = out.f;
+  if (Sl >= R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= R(0)) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
+  } else {
+    // the HLL average state and flux
+    R Uhll[8], Fhll[8], usl[8], usr[8];
+    const R scrh = R(1)/(Sr - Sl);
+    for (int nv = 0; nv < 8; nv++) {
+      Uhll[nv]  = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+      Uhll[nv] *= scrh;
+      Fhll[nv]  =   Sl*Sr*(ur[nv] - ul[nv])
+                  + Sr*fl[nv] - Sl*fr[nv];
+      Fhll[nv] *= scrh;
+    }
+    const R Bxs = Uhll[BXn], Bys = Uhll[BXt], Bzs = Uhll[BXb];
+    const R vxL = vl[VXn], vxR = vr[VXn];
+    const R vxs = Uhll[MXn]/Uhll[0];
+    {
+      const R vBL = vl[1]*vl[4] + vl[2]*vl[5] + vl[3]*vl[6];
+      const R vBR = vr[1]*vr[4] + vr[2]*vr[5] + vr[3]*vr[6];
+      R vBs = Uhll[4]*Uhll[1]
+            + Uhll[5]*Uhll[2]
+            + Uhll[6]*Uhll[3];
+      vBs /= Uhll[0];
+      usl[0] = ul[0]*(Sl - vxL)/(Sl - vxs);
+      usr[0] = ur[0]*(Sr - vxR)/(Sr - vxs);
+      const R ps = Fhll[MXn] + Bxs*Bxs - Fhll[0]*vxs;
+      usl[ENG] = (ul[ENG]*(Sl - vxL) +
+                  ps*vxs - pl*vxL - Bxs*vBs + vl[BXn]*vBL)/(Sl - vxs);
+      usr[ENG] = (ur[ENG]*(Sr - vxR) +
+                  ps*vxs - pr*vxR - Bxs*vBs + vr[BXn]*vBR)/(Sr - vxs);
+    }
+    usl[MXn] = usl[0]*vxs;
+    usr[MXn] = usr[0]*vxs;
+    usl[MXt] =   (ul[MXt]*(Sl - vxL)
+               - (Bxs*Bys - vl[BXn]*vl[BXt]))/(Sl - vxs);
+    usr[MXt] =   (ur[MXt]*(Sr - vxR)
+               - (Bxs*Bys - vr[BXn]*vr[BXt]))/(Sr - vxs);
+    usl[MXb] =   (ul[MXb]*(Sl - vxL)
+               - (Bxs*Bzs - vl[BXn]*vl[BXb]))/(Sl - vxs);
+    usr[MXb] =   (ur[MXb]*(Sr - vxR)
+               - (Bxs*Bzs - vr[BXn]*vr[BXb]))/(Sr - vxs);
+    usl[BXn] = usr[BXn] = Bxs;
+    usl[BXt] = usr[BXt] = Bys;
+    usl[BXb] = usr[BXb] = Bzs;
+    if (vxs > R(0)) {
+      for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv] + Sl*(usl[nv] - ul[nv]);
+    } else {
+      for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv] + Sr*(usr[nv] - ur[nv]);
+    }
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
 // CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177.  Per face:
 //   et = e[DIRb][DIRn] (upwinded transverse velocity v_t),
 //   eb = e[DIRt][DIRn] (minus upwinded v_b),

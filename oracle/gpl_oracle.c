@@ -320,6 +320,91 @@ static uint32_t mhd_hlld(const double *vl, const double *vr, int n, double gamma
   return err;
 }
 
+/* MHD HLLC of Li (2005), Src/MHD/hllc.cpp:130-335 (ideal EOS, no GLM, no
+   background field; HLL_SWITCH is off in the default configuration,
+   Src/gpluto.hpp:877-878) */
+static uint32_t mhd_hllc(const double *vl, const double *vr, int n, double gamma,
+                         double *flux, double *pSl, double *pSr)
+{
+  const int VXn = 1 + n;
+  const int MXn = VXn, MXt = 1 + (n + 1)%3, MXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3;
+  const int NFLX = 8, ENG = 7;
+  int nv;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], usl[NVMAX], usr[NVMAX];
+  double Uhll[NVMAX], Fhll[NVMAX], pl, pr, Sl, Sr, sl_min, sl_max, sr_min, sr_max;
+
+  mhd_prim_to_cons(vl, ul, gamma);
+  mhd_prim_to_cons(vr, ur, gamma);
+  double a2L = sound_speed2(vl, gamma, 7);
+  double a2R = sound_speed2(vr, gamma, 7);
+  mhd_max_signal_speed(vl, a2L, n, &sl_min, &sl_max);   /* HLL_Speed, mhd.hpp:296 */
+  mhd_max_signal_speed(vr, a2R, n, &sr_min, &sr_max);
+  Sl = MIN(sl_min, sr_min);
+  Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+
+  mhd_flux(vl, ul, n, fl, &pl);      /* pl, pr: total pressure */
+  mhd_flux(vr, ur, n, fr, &pr);
+
+  if (Sl >= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv];
+  } else {
+    double scrh = 1.0/(Sr - Sl);
+    for (nv = 0; nv < NFLX; nv++) {
+      Uhll[nv]  = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+      Uhll[nv] *= scrh;
+      Fhll[nv]  =   Sl*Sr*(ur[nv] - ul[nv])
+                  + Sr*fl[nv] - Sl*fr[nv];
+      Fhll[nv] *= scrh;
+    }
+    double Bxs = Uhll[BXn];
+    double Bys = Uhll[BXt];
+    double Bzs = Uhll[BXb];
+    double vxL = vl[VXn];
+    double vxR = vr[VXn];
+    double vxs = Uhll[MXn]/Uhll[RHO];
+    {
+      double vBL = vl[1]*vl[4] + vl[2]*vl[5] + vl[3]*vl[6];
+      double vBR = vr[1]*vr[4] + vr[2]*vr[5] + vr[3]*vr[6];
+      double vBs = Uhll[4]*Uhll[1]
+                 + Uhll[5]*Uhll[2]
+                 + Uhll[6]*Uhll[3];
+      vBs /= Uhll[RHO];
+      usl[RHO] = ul[RHO]*(Sl - vxL)/(Sl - vxs);
+      usr[RHO] = ur[RHO]*(Sr - vxR)/(Sr - vxs);
+      double ps = Fhll[MXn] + Bxs*Bxs - Fhll[RHO]*vxs;
+      usl[ENG] = (ul[ENG]*(Sl - vxL) +
+                  ps*vxs - pl*vxL - Bxs*vBs + vl[BXn]*vBL)/(Sl - vxs);
+      usr[ENG] = (ur[ENG]*(Sr - vxR) +
+                  ps*vxs - pr*vxR - Bxs*vBs + vr[BXn]*vBR)/(Sr - vxs);
+    }
+    usl[MXn] = usl[RHO]*vxs;
+    usr[MXn] = usr[RHO]*vxs;
+    usl[MXt] =   (ul[MXt]*(Sl - vxL)
+               - (Bxs*Bys - vl[BXn]*vl[BXt]))/(Sl - vxs);
+    usr[MXt] =   (ur[MXt]*(Sr - vxR)
+               - (Bxs*Bys - vr[BXn]*vr[BXt]))/(Sr - vxs);
+    usl[MXb] =   (ul[MXb]*(Sl - vxL)
+               - (Bxs*Bzs - vl[BXn]*vl[BXb]))/(Sl - vxs);
+    usr[MXb] =   (ur[MXb]*(Sr - vxR)
+               - (Bxs*Bzs - vr[BXn]*vr[BXb]))/(Sr - vxs);
+    usl[BXn] = usr[BXn] = Bxs;
+    usl[BXt] = usr[BXt] = Bys;
+    usl[BXb] = usr[BXb] = Bzs;
+    if (vxs > 0.0) {
+      for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv] + Sl*(usl[nv] - ul[nv]);
+    } else {
+      for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv] + Sr*(usr[nv] - ur[nv]);
+    }
+  }
+  uint32_t err = 0;
+  for (nv = 0; nv < NFLX; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
 /* HD HLLC, Src/HD/hllc.cpp:54-250 (ideal EOS) */
 static uint32_t hd_hllc(const double *vl, const double *vr, int n, double gamma,
                         double *flux, double *pSl, double *pSr)
@@ -1055,7 +1140,8 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
       if (is_rmhd(s)) {
         err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
       } else if (mhd) {
-        if (s->c.solver == GPLO_HLLD) err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        if (s->c.solver == GPLO_HLLD)      err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        else if (s->c.solver == GPLO_HLLC) err |= mhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
         else err |= hll(vl, vr, dir, s->c.gamma, 1, flux + i*NVMAX, &Sl, &Sr);
       } else {
         if (s->c.solver == GPLO_HLLC) err |= hd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);

@@ -31,6 +31,9 @@ CASES = [
     ("mhd3d_12x16x10_rk3_vl", "mhd3d", "RK3", "VANLEER_LIM", (12, 16, 10), 9, 5, 3e-3, 0.4, "hlld", True, 5 / 3),
     ("mhd3d_16_rk2_mc_hll", "mhd3d", "RK2", "MC_LIM", (16, 16, 16), 5, 4, 2e-3, 0.4, "hll", True, 5 / 3),
     ("mhd2d_32x24_rk2_mc", "mhd2d", "RK2", "MC_LIM", (32, 24, 1), 5, 8, 3e-3, 0.4, "hlld", True, 5 / 3),
+    # MHD HLLC (Li 2005, Src/MHD/hllc.cpp) with UCT_HLL
+    ("mhd3d_12x10x14_rk2_mc_hllc", "mhd3d", "RK2", "MC_LIM", (12, 10, 14), 17, 6, 2e-3, 0.4, "hllc", True, 5 / 3),
+    ("mhd3d_10x14x8_rk3_vl_hllc_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 14, 8), 18, 5, 2e-3, 0.4, "hllc", False, 5 / 3),
     # the CPU-runnable headline case: 1D Euler shock tube, 4096 cells, 400 steps
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),

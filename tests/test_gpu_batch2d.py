@@ -42,7 +42,8 @@ SIZES = [(5, 5), (33, 33), (63, 64), (64, 63), (65, 65), (100, 37), (130, 130), 
 
 
 @pytest.mark.parametrize("ts,lim,solver", [("RK2", "MC_LIM", "hlld"),
-                                           ("RK3", "VANLEER_LIM", "hll")])
+                                           ("RK3", "VANLEER_LIM", "hll"),
+                                           ("RK2", "VANLEER_LIM", "hllc")])
 def test_every_box_matches_its_standalone_oracle_run(ts, lim, solver):
     steps, first_dt = 6, 1e-3
     print(f"case {ts} {lim} {solver} sizes={SIZES}", flush=True)

@@ -67,6 +67,32 @@ def test_fp64_bitwise_hll():
     assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
 
 
+@pytest.mark.parametrize("path", ["fused", "split", "emf0"])
+def test_fp64_bitwise_hllc(path, monkeypatch):
+    """MHD HLLC (Src/MHD/hllc.cpp) with UCT_HLL, on the fused sweeps, the
+    per-interface split kernels and the one-edge-per-thread EMF kernel."""
+    if path == "split":
+        monkeypatch.setenv("GPL_SPLIT_KERNELS", "1")
+    elif path == "emf0":
+        monkeypatch.setenv("GPL_EMF_MARCH", "0")
+    for n, ts, lim in (((16, 12, 20), "RK2", "MC_LIM"), ((20, 17, 12), "RK3", "VANLEER_LIM")):
+        (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(
+            n, 23, 5, 2e-3, ts=ts, lim=lim, solver="hllc")
+        assert gerr == oerr == 0
+        assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
+        assert gt == ot and gdt == odt
+
+
+def test_mixed_precision_hllc():
+    (_, _, _, ovc, _, _), (gerr, _, _, gvc, gvs, _) = _pair(
+        (32, 24, 20), 24, 8, 1e-3, solver="hllc", precision="mixed")
+    assert gerr == 0
+    for v in range(8):
+        l1 = np.sum(np.abs(gvc[v] - ovc[v])) / np.sum(np.abs(ovc[v]))
+        assert l1 <= 1e-5, (v, l1)
+    assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * 32
+
+
 def test_advance_step_seam_matches_oracle():
     """AdvanceStep + host NextTimeStep, step by step (Src/main.cpp:446-463)."""
     import oracle
@@ -148,6 +174,9 @@ XPER = ("periodic", "periodic") + ("outflow",) * 4          # periodic in x, out
     ((14, 12, 10), ("reflective",) * 6, "RK2", "MC_LIM", "hlld", 8),
     ((12, 16, 14), ("reflective", "outflow", "periodic", "periodic", "outflow", "reflective"),
      "RK3", "VANLEER_LIM", "hll", 6),
+    ((12, 10, 14), OUT, "RK3", "VANLEER_LIM", "hllc", 6),
+    ((10, 14, 12), ("reflective",) * 2 + ("periodic",) * 2 + ("outflow",) * 2,
+     "RK2", "MC_LIM", "hllc", 6),
 ])
 def test_outflow_sides_bitwise(n, sides, ts, lim, solver, steps):
     """Outflow sides (OutflowBoundary + CT_FillStaggeredFields every stage) and
