@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libgpluto_amd.so")
 GPL_HD, GPL_MHD, GPL_RMHD = 1, 2, 3
 GPL_SOLVER_HLL, GPL_SOLVER_HLLC, GPL_SOLVER_HLLD = 1, 2, 3
 GPL_VANLEER_LIM, GPL_MC_LIM = 26, 27
-GPL_UCT_HLL = 1
+GPL_UCT_HLL, GPL_UCT_HLLD = 1, 2
 GPL_RK2, GPL_RK3 = 2, 3
 GPL_PERIODIC, GPL_OUTFLOW, GPL_REFLECTIVE = 1, 2, 3
 GPL_FP64, GPL_MIXED = 0, 1

@@ -135,7 +135,8 @@ public:
       rc |= dev_alloc(bytes, &F, 15*n*rs);
       rc |= dev_alloc(bytes, &CM, 3*n*rs);
     }
-    rc |= dev_alloc(bytes, &FD, 15*n*rs);
+    // UCT face records: 5 per direction, 6 with UCT_HLLD (dL and dR apart)
+    rc |= dev_alloc(bytes, &FD, (c.ct_emf_average == GPL_UCT_HLLD ? 18 : 15)*n*rs);
     rc |= dev_alloc(bytes, &E, 3*n*rs);
     rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
     rc |= dev_alloc(bytes, (void **)&diag, (size_t)(DIAG_BLOCKS + 1)*DIAG_MAXQ*sizeof(double));
@@ -289,7 +290,9 @@ public:
     for (int q = 0; q < nk; q++) {
       const int k = seq[q];
       if (t) t->begin(k);
-      mhd3d_launch<R>(k, b, g, env.ctl, w, first_stage, want_dt, cfg.limiter, cfg.solver,
+      // solver 4: HLLD with the UCT_HLLD edge EMF (CT_Store_UCTHlld)
+      const int solver = cfg.ct_emf_average == GPL_UCT_HLLD ? 4 : cfg.solver;
+      mhd3d_launch<R>(k, b, g, env.ctl, w, first_stage, want_dt, cfg.limiter, solver,
                       march, cfg.gamma, cfg.small_rho, cfg.small_prs, env.stream);
       if (t) t->end(k);
     }
@@ -365,7 +368,11 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
 {
   *out = nullptr;
   if (cfg.dimensions != 3) return GPL_EUNSUPPORTED;
-  if (cfg.ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  // UCT_HLL, or UCT_HLLD with the MHD HLLD solver (the reference's HLL and
+  // HLLC quit with it: Src/MHD/hll.cpp:75-76, Src/MHD/hllc.cpp:81-83)
+  if (cfg.ct_emf_average != GPL_UCT_HLL && cfg.ct_emf_average != GPL_UCT_HLLD) return GPL_EUNSUPPORTED;
+  if (cfg.ct_emf_average == GPL_UCT_HLLD && (cfg.physics != GPL_MHD || cfg.solver != GPL_SOLVER_HLLD))
+    return GPL_EUNSUPPORTED;
   // periodic, outflow or reflective sides (the last two: the padded box); a
   // direction is periodic on both sides or on neither
   for (int d = 0; d < 3; d++) {

@@ -11,6 +11,7 @@
 //   Vs[3][ncell]   face-normal field of the lower face of every cell (FP64)
 //   F [3][5][ncell], CM[3][ncell]   interface flux / max speed  (type R)
 //   FD[3][5][ncell] UCT face data et, eb, aL, aR, d             (type R)
+//                   (UCT_HLLD: FD[3][6][ncell], et, eb, aL, aR, dL, dR)
 //   E [3][ncell]   edge EMFs Ex(i,j-1/2,k-1/2), Ey(i-1/2,j,k-1/2),
 //                  Ez(i-1/2,j-1/2,k)                            (type R)
 // Cell index c = i + nx*(j + ny*k) (reference Ary4D order, i fastest).
@@ -123,9 +124,30 @@ template <typename R, int N, int SOLVER, int PH>
 __device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
 {
   if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
-  else if constexpr (SOLVER == 3) mhd_hlld<R, N>(vl, vr, gamma, ff);
+  else if constexpr (SOLVER == 3 || SOLVER == 4) mhd_hlld<R, N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 2) mhd_hllc<R, N>(vl, vr, gamma, ff);
   else mhd_hll<R, N>(vl, vr, gamma, ff);
+}
+
+// Face records of the UCT EMF average per direction: et, eb, aL, aR, d for
+// UCT_HLL (dL = dR), et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with
+// CT_Store_UCTHlld).
+template <int SOLVER> __host__ __device__ constexpr int uct_nrec() { return SOLVER == 4 ? 6 : 5; }
+
+template <typename R, int N, int SOLVER>
+__device__ __forceinline__ void uct_store(const R *vl, const R *vr, R gamma, const FaceFlux<R> &ff,
+                                          R *fd, unsigned c, size_t n)
+{
+  if constexpr (SOLVER == 4) {
+    R et, eb, aL, aR, dL, dR;
+    uct_hlld_store<R, N>(vl, vr, gamma, ff.Sl, ff.Sr, et, eb, aL, aR, dL, dR);
+    sto(fd, c, et); sto(fd + n, c, eb); sto(fd + 2*n, c, aL); sto(fd + 3*n, c, aR);
+    sto(fd + 4*n, c, dL); sto(fd + 5*n, c, dR);
+  } else {
+    R et, eb, aL, aR, dd;
+    uct_hll_store<R, N>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+    sto(fd, c, et); sto(fd + n, c, eb); sto(fd + 2*n, c, aL); sto(fd + 3*n, c, aR); sto(fd + 4*n, c, dd);
+  }
 }
 
 // VelocityLimiter (Src/vel_limiter.hpp:14-31, applied in plm_states.cpp:158-
@@ -187,14 +209,7 @@ void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
   Fd[3*n + c] = ff.f[3];
   Fd[4*n + c] = ff.f[7];
   CM[(size_t)D*n + c] = rmax(rabs(ff.Sl), rabs(ff.Sr));
-  R et, eb, aL, aR, dd;
-  uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-  R *fd = FD + (size_t)D*5*n;
-  fd[0*n + c] = et;
-  fd[1*n + c] = eb;
-  fd[2*n + c] = aL;
-  fd[3*n + c] = aR;
-  fd[4*n + c] = dd;
+  uct_store<R, D, SOLVER>(vl, vr, gamma, ff, FD + (size_t)D*uct_nrec<SOLVER>()*n, c, n);
   if (ff.err && face_reported<D>(g, p.i, p.j, p.k)) atomicOr(err, ff.err);
 }
 
@@ -298,11 +313,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   vl[4] = vr[4] = bn;
   FaceFlux<R> ff;
   riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
-  {
-    R et, eb, aL, aR, dd;
-    uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-    sto(FD, c, et); sto(FD + n, c, eb); sto(FD + 2*n, c, aL); sto(FD + 3*n, c, aR); sto(FD + 4*n, c, dd);
-  }
+  uct_store<R, 0, SOLVER>(vl, vr, gamma, ff, FD, c, n);
   __syncthreads();                 // everyone is done with sV/sD
   R *sF = sV;                      // [6][nx]: hydro flux and cmax of face i
   sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
@@ -358,7 +369,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
   for (int q = 0; q < 7; q++) Vq[q] = V + VY[q]*n;
   const R *V5 = V + 5*n;
   const double *Bxf = Vs, *Byf = Vs + n;
-  R *fdx = FD, *fdy = FD + 5*n;
+  R *fdx = FD, *fdy = FD + uct_nrec<SOLVER>()*n;
   R *dUq[5];
 #pragma unroll
   for (int nv = 0; nv < 5; nv++) dUq[nv] = dU + nv*n;
@@ -426,11 +437,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
       FaceFlux<R> ff;
       riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
       if (face_reported<0>(g, i, f, k)) e |= ff.err;
-      {
-        R et, eb, aL, aR, dd;
-        uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-        sto(fdx, cf, et); sto(fdx + n, cf, eb); sto(fdx + 2*n, cf, aL); sto(fdx + 3*n, cf, aR); sto(fdx + 4*n, cf, dd);
-      }
+      uct_store<R, 0, SOLVER>(vl, vr, gamma, ff, fdx, cf, n);
       sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
       sF[3*nx + i] = ff.f[3]; sF[4*nx + i] = ff.f[7];
       sF[5*nx + i] = rmax(rabs(ff.Sl), rabs(ff.Sr));
@@ -463,11 +470,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     FaceFlux<R> ff;
     riemann<R, 1, SOLVER, PH>(vl, vr, gamma, ff);
     if (face_reported<1>(g, i, wrapm(f, ny), k)) e |= ff.err;
-    if (f < s1) {
-      R et, eb, aL, aR, dd;
-      uct_hll_store<R, 1>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-      sto(fdy, cf, et); sto(fdy + n, cf, eb); sto(fdy + 2*n, cf, aL); sto(fdy + 3*n, cf, aR); sto(fdy + 4*n, cf, dd);
-    }
+    if (f < s1) uct_store<R, 1, SOLVER>(vl, vr, gamma, ff, fdy, cf, n);
     const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
                          (double)ff.f[7]};
     const double cm = (double)rmax(rabs(ff.Sl), rabs(ff.Sr));
@@ -541,7 +544,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
     for (int nv = 0; nv < 5; nv++) {
       dUq[nv] = dU + nv*n; Uq[nv] = U + nv*n; Uiq[nv] = Uin + nv*n; U0q[nv] = U0 + nv*n;
     }
-    R *fd0 = FD + (size_t)D*5*n;
+    R *fd0 = FD + (size_t)D*uct_nrec<SOLVER>()*n;
     const int i = t % g.nx;
     const int r = t / g.nx;
     const int a = r % na, seg = r / na;
@@ -614,11 +617,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
         const int ff_ = wrapm(f, nd);
         if (D == 1 ? face_reported<1>(g, i, ff_, a) : face_reported<2>(g, i, a, ff_)) e |= ff.err;
       }
-      if (f < s1) {
-        R et, eb, aL, aR, dd;
-        uct_hll_store<R, D>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-        sto(fd0, cf, et); sto(fd0 + n, cf, eb); sto(fd0 + 2*n, cf, aL); sto(fd0 + 3*n, cf, aR); sto(fd0 + 4*n, cf, dd);
-      }
+      if (f < s1) uct_store<R, D, SOLVER>(vl, vr, gamma, ff, fd0, cf, n);
       const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
                            (double)ff.f[7]};
       const double cm = (double)rmax(rabs(ff.Sl), rabs(ff.Sr));
@@ -671,9 +670,9 @@ __device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
 // one UCT contribution (ct_emf_average.cpp:636-643):
 // (aL vL bL + aR vR bR) +/- (d bR - d bL), from preloaded values: face-field stencil q[0..3] and
 // transverse-velocity stencil e[0..3] along the reconstruction direction, and
-// the (aL, aR, d) records of the two faces the edge averages over (fa is the
-// displaced face, fb the face of the edge's own cell).
-template <typename R, int LIMV>
+// the (aL, aR, d) or (aL, aR, dL, dR) records of the two faces the edge
+// averages over (fa is the displaced face, fb the face of the edge's own cell).
+template <typename R, int LIMV, int NQ = 3>
 __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const R *fb, int sgn)
 {
   R bL, bR, vL, vR;
@@ -682,7 +681,9 @@ __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const 
   const R aL = R(0.5)*(fa[0] + fb[0]);
   const R aR = R(0.5)*(fa[1] + fb[1]);
   const R dL = R(0.5)*(fa[2] + fb[2]);
-  const R dR = dL;
+  // UCT_HLLD (NQ = 4 records aL, aR, dL, dR) keeps dR apart; UCT_HLL has dR = dL
+  R dR = dL;
+  if constexpr (NQ == 4) dR = R(0.5)*(fa[3] + fb[3]);
   const R phi = dR*bR - dL*bL;
   const R s = aL*vL*bL + aR*vR*bR;
   return sgn > 0 ? s + phi : s - phi;
@@ -962,10 +963,11 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
 
 // The same edge EMFs with one thread per edge and every stencil value loaded
 // from memory (2D thread blocks over x-y; blockIdx.z is the plane).
-template <typename R, int LIMV>
+template <typename R, int LIMV, int NQ = 3>
 __global__ __launch_bounds__(256)
 void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
 {
+  constexpr int NR = NQ + 2;       // records per face: et, eb and NQ coefficients
   const int i = blockIdx.x*64 + (threadIdx.x & 63), j = blockIdx.y*4 + (threadIdx.x >> 6);
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
@@ -974,10 +976,13 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   const unsigned pl = nx*ny, base = k*pl, c = i + nx*j + base;
   // uniform array bases (scalar registers) + 32-bit cell offsets
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
-  const R *ex0 = FD, *ex1 = FD + n, *ey0 = FD + 5*n, *ey1 = FD + 6*n, *ez0 = FD + 10*n, *ez1 = FD + 11*n;
-  const R *fx[3] = {FD + 2*n, FD + 3*n, FD + 4*n};
-  const R *fy[3] = {FD + 7*n, FD + 8*n, FD + 9*n};
-  const R *fz[3] = {FD + 12*n, FD + 13*n, FD + 14*n};
+  const R *ex0 = FD, *ex1 = FD + n, *ey0 = FD + NR*n, *ey1 = FD + (NR + 1)*n;
+  const R *ez0 = FD + 2*NR*n, *ez1 = FD + (2*NR + 1)*n;
+  const R *fx[NQ], *fy[NQ], *fz[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    fx[q] = FD + (2 + q)*n; fy[q] = FD + (NR + 2 + q)*n; fz[q] = FD + (2*NR + 2 + q)*n;
+  }
   unsigned xo[4], yo[4], zo[4];
 #pragma unroll
   for (int o = 0; o < 4; o++) {
@@ -985,9 +990,9 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
     yo[o] = i + nx*wrapm(j + o - 2, ny) + base;
     zo[o] = i + nx*j + wrapm(k + o - 2, nz)*pl;
   }
-  R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3], cxk[3], cyk[3];
+  R cx[NQ], cy[NQ], cz[NQ], cxj[NQ], cyi[NQ], czi[NQ], czj[NQ], cxk[NQ], cyk[NQ];
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
+  for (int q = 0; q < NQ; q++) {
     cx[q] = ldo(fx[q], c);  cy[q] = ldo(fy[q], c);  cz[q] = ldo(fz[q], c);
     cxj[q] = ldo(fx[q], yo[1]);
     cyi[q] = ldo(fy[q], xo[1]);
@@ -999,22 +1004,22 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   R q4[4], e4[4];
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(By, xo[o]); e4[o] = ldo(ey1, xo[o]); }
-  R ez = uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                      // X1a
+  R ez = uct_val<R, LIMV, NQ>(q4, e4, cxj, cx, +1);                      // X1a
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bx, yo[o]); e4[o] = ldo(ex0, yo[o]); }
-  ez += uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                       // X2b
+  ez += uct_val<R, LIMV, NQ>(q4, e4, cyi, cy, -1);                       // X2b
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bz, xo[o]); e4[o] = ldo(ez0, xo[o]); }
-  R ey = uct_val<R, LIMV>(q4, e4, cxk, cx, -1);                      // X1b
+  R ey = uct_val<R, LIMV, NQ>(q4, e4, cxk, cx, -1);                      // X1b
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bx, zo[o]); e4[o] = ldo(ex1, zo[o]); }
-  ey += uct_val<R, LIMV>(q4, e4, czi, cz, +1);                       // X3a
+  ey += uct_val<R, LIMV, NQ>(q4, e4, czi, cz, +1);                       // X3a
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(Bz, yo[o]); e4[o] = ldo(ez1, yo[o]); }
-  R ex = uct_val<R, LIMV>(q4, e4, cyk, cy, +1);                      // X2a
+  R ex = uct_val<R, LIMV, NQ>(q4, e4, cyk, cy, +1);                      // X2a
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(By, zo[o]); e4[o] = ldo(ey0, zo[o]); }
-  ex += uct_val<R, LIMV>(q4, e4, czj, cz, -1);                       // X3b
+  ex += uct_val<R, LIMV, NQ>(q4, e4, czj, cz, -1);                       // X3b
   sto(E, c, ex);
   sto(E + n, c, ey);
   sto(E + 2*n, c, ez);
@@ -1187,7 +1192,10 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   }
   case K_EMF: {
-    if (b.emf_march == 0) {
+    if (SOLVER == 4) {                   // UCT_HLLD: six records per face, one edge per thread
+      hipLaunchKernelGGL((k_emf_edge<R, LIM, 4>), dim3((g.nx + 63)/64, (g.ny + 3)/4, g.nz), dim3(256), 0, s,
+                         g, b.Vin, b.FD, b.E);
+    } else if (b.emf_march == 0) {
       hipLaunchKernelGGL((k_emf_edge<R, LIM>), dim3((g.nx + 63)/64, (g.ny + 3)/4, g.nz), dim3(256), 0, s,
                          g, b.Vin, b.FD, b.E);
     } else if (b.emf_march == 4) {
@@ -1249,6 +1257,11 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
       else                   launch_l<R, VANLEER_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
       return;
     }
+  }
+  if (solver == 4) {               // HLLD with the UCT_HLLD EMF
+    if (limiter == MC_LIM) launch_l<R, MC_LIM, 4, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    else                   launch_l<R, VANLEER_LIM, 4, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    return;
   }
   if (limiter == MC_LIM) {
     if (solver == 3)      launch_l<R, MC_LIM, 3, 0>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);

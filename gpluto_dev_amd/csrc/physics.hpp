@@ -376,6 +376,63 @@ GPL_HD void mhd_hllc(const R *vl, const R *vr, R gamma, FaceFlux<R> &out)
   out.err = err;
 }
 
+// CT_Store_UCTHlld<dir>, Src/MHD/CT/ct_store.hpp:254-354 (ideal EOS): the
+// UCT weights and diffusion coefficients from the HLLD fan, with the face's
+// own normal field Bn = vl[BXn] = vr[BXn] (after the PLM override), replaced
+// by the HLL values where a star density is negative.
+template <typename R, int N>
+GPL_HD void uct_hlld_store(const R *vl, const R *vr, R gamma, R Sl, R Sr, R &et, R &eb, R &aL,
+                           R &aR, R &dL, R &dR)
+{
+  constexpr int VXn = 1 + N, VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3, MXn = VXn;
+  R ul[8], ur[8];
+  mhd_prim_to_cons(vl, ul, gamma - R(1));
+  mhd_prim_to_cons(vr, ur, gamma - R(1));
+  const R aR_1 = rmax(R(0), Sr);
+  const R aL_1 = rmin(R(0), Sl);
+  const R Bn = vl[4 + N];
+  const R dul = Sl - vl[VXn];
+  const R dur = Sr - vr[VXn];
+  const R pL = vl[7] + R(0.5)*(vl[4]*vl[4] + vl[5]*vl[5] + vl[6]*vl[6]);
+  const R pR = vr[7] + R(0.5)*(vr[4]*vr[4] + vr[5]*vr[5] + vr[6]*vr[6]);
+  R scrh = R(1)/(dur*ur[0] - dul*ul[0]);
+  const R Sc = (dur*ur[MXn] - dul*ul[MXn] - pR + pL)*scrh;
+  const R usl_rho = ul[0]*dul/(Sl - Sc);
+  const R usr_rho = ur[0]*dur/(Sr - Sc);
+  const bool switch_to_hll = usl_rho < R(0) || usr_rho < R(0);
+  const R sqrL = rsqrt_(usl_rho);
+  const R sqrR = rsqrt_(usr_rho);
+  const R Sal = Sc - rabs(Bn)/sqrL;
+  const R Sar = Sc + rabs(Bn)/sqrR;
+  const R chiL = (vl[VXn] - Sc)*(Sl - Sc)/(Sal + Sl - R(2)*Sc);
+  const R chiR = (vr[VXn] - Sc)*(Sr - Sc)/(Sar + Sr - R(2)*Sc);
+  const R nuL  = (Sal + Sl)/(rabs(Sal) + rabs(Sl));
+  const R nuR  = (Sar + Sr)/(rabs(Sar) + rabs(Sr));
+  const R nuLR = (Sal + Sar)/(rabs(Sal) + rabs(Sar));
+  if (rabs(Sar - Sal) > R(1.e-9)*rabs(Sr - Sl)) {
+    dL =   R(0.5)*(chiL*nuL - chiL*nuLR)
+         + R(0.5)*(rabs(Sal) - nuLR*Sal);
+    dR =   R(0.5)*(chiR*nuR - chiR*nuLR)
+         + R(0.5)*(rabs(Sar) - nuLR*Sar);
+    aL = R(0.5)*(R(1) + nuLR);
+    aR = R(0.5)*(R(1) - nuLR);
+  } else {             // HLLC, degenerate limit Bn -> 0
+    dL = R(0.5)*chiL*nuL + R(0.5)*rabs(Sal);
+    dR = R(0.5)*chiR*nuR + R(0.5)*rabs(Sar);
+    aL = R(0.5);
+    aR = R(0.5);
+  }
+  scrh = R(1)/(aR_1 - aL_1);
+  if (switch_to_hll) {
+    aL =  aR_1*scrh;
+    aR = -aL_1*scrh;
+    dR = -aL_1*aR_1*scrh;
+    dL = -aL_1*aR_1*scrh;
+  }
+  et =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
+  eb = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
+}
+
 // CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177.  Per face:
 //   et = e[DIRb][DIRn] (upwinded transverse velocity v_t),
 //   eb = e[DIRt][DIRn] (minus upwinded v_b),

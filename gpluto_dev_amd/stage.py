@@ -34,6 +34,7 @@ _LIMITERS = {"MC_LIM": _lib.GPL_MC_LIM, "VANLEER_LIM": _lib.GPL_VANLEER_LIM,
              "DEFAULT": _lib.GPL_VANLEER_LIM}
 _SOLVERS = {"hlld": _lib.GPL_SOLVER_HLLD, "hll": _lib.GPL_SOLVER_HLL, "hllc": _lib.GPL_SOLVER_HLLC}
 _STEPPING = {"RK2": _lib.GPL_RK2, "RK3": _lib.GPL_RK3}
+_EMF = {"UCT_HLL": _lib.GPL_UCT_HLL, "UCT_HLLD": _lib.GPL_UCT_HLLD}
 _BOUNDARY = {"periodic": _lib.GPL_PERIODIC, "outflow": _lib.GPL_OUTFLOW,
              "reflective": _lib.GPL_REFLECTIVE}
 
@@ -123,7 +124,7 @@ class StageUpdate:
         cfg.time_stepping = _STEPPING[defs.time_stepping]
         cfg.limiter = _LIMITERS[defs.limiter]
         cfg.solver = _SOLVERS[ini.solver]
-        cfg.ct_emf_average = _lib.GPL_UCT_HLL
+        cfg.ct_emf_average = _EMF[defs.ct_emf_average]
         cfg.precision = _lib.GPL_FP64 if precision == "fp64" else _lib.GPL_MIXED
         cfg.device = device
         cfg.gamma = ini.gamma

@@ -32,8 +32,9 @@ extern "C" {
 /* LIMITER of definitions.hpp (Src/gpluto.hpp:330-336) */
 #define GPL_VANLEER_LIM  26
 #define GPL_MC_LIM       27
-/* CT_EMF_AVERAGE (Src/MHD/CT/ct.hpp:73-96) */
+/* CT_EMF_AVERAGE (Src/MHD/CT/ct.hpp:20-21, 73-96) */
 #define GPL_UCT_HLL      1
+#define GPL_UCT_HLLD     2   /* CT_Store_UCTHlld (Src/MHD/CT/ct_store.hpp:254); MHD with HLLD only */
 /* TIME_STEPPING (Src/gpluto.hpp:121-122) */
 #define GPL_RK2          2
 #define GPL_RK3          3
@@ -72,7 +73,7 @@ typedef struct gpl_config {
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
   int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLLC / _HLL; RMHD: _HLL; HD: _HLLC / _HLL */
-  int    ct_emf_average;   /* GPL_UCT_HLL (MHD, RMHD) */
+  int    ct_emf_average;   /* GPL_UCT_HLL (MHD, RMHD), or GPL_UCT_HLLD (MHD with HLLD) */
   int    precision;        /* GPL_FP64, or GPL_MIXED (MHD) */
   int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */
   int    device;           /* HIP device ordinal */

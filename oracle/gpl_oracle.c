@@ -901,6 +901,63 @@ static double plm_slope(double vm, double v0, double vp, int lim)
   return dv;
 }
 
+/* CT_Store_UCTHlld<dir>, Src/MHD/CT/ct_store.hpp:254-354 (ideal EOS): the
+   UCT weights and diffusion coefficients from the HLLD fan, with the face's
+   own normal field Bn (= vl[BXn] = vr[BXn] after the PLM override), the HLL
+   values where a star density is negative */
+static void uct_hlld_store(gplo_state *s, const double *vl, const double *vr, int dir,
+                           int t1, int t2, double Sl, double Sr, size_t id)
+{
+  const int VXn = 1 + dir, VXt = 1 + t1, VXb = 1 + t2, MXn = VXn;
+  double ul[NVMAX], ur[NVMAX], scrh;
+  int switch_to_hll = 0;
+  mhd_prim_to_cons(vl, ul, s->c.gamma);
+  mhd_prim_to_cons(vr, ur, s->c.gamma);
+  double aR_1 = MAX(0.0, Sr);
+  double aL_1 = MIN(0.0, Sl);
+  double Bn = vl[4 + dir];
+  double dul = Sl - vl[VXn];
+  double dur = Sr - vr[VXn];
+  double pL = vl[7] + 0.5*(vl[4]*vl[4] + vl[5]*vl[5] + vl[6]*vl[6]);
+  double pR = vr[7] + 0.5*(vr[4]*vr[4] + vr[5]*vr[5] + vr[6]*vr[6]);
+  scrh  = 1.0/(dur*ur[RHO] - dul*ul[RHO]);
+  double Sc = (dur*ur[MXn] - dul*ul[MXn] - pR + pL)*scrh;
+  double usl_RHO = ul[RHO]*dul/(Sl - Sc);
+  double usr_RHO = ur[RHO]*dur/(Sr - Sc);
+  if (usl_RHO < 0.0 || usr_RHO < 0.0) switch_to_hll = 1;
+  double sqrL = sqrt(usl_RHO);
+  double sqrR = sqrt(usr_RHO);
+  double Sal = Sc - fabs(Bn)/sqrL;
+  double Sar = Sc + fabs(Bn)/sqrR;
+  double chiL  = (vl[VXn] - Sc)*(Sl - Sc)/(Sal + Sl - 2.0*Sc);
+  double chiR  = (vr[VXn] - Sc)*(Sr - Sc)/(Sar + Sr - 2.0*Sc);
+  double nuL  = (Sal + Sl)/(fabs(Sal) + fabs(Sl));
+  double nuR  = (Sar + Sr)/(fabs(Sar) + fabs(Sr));
+  double nuLR = (Sal + Sar)/(fabs(Sal) + fabs(Sar));
+  if (fabs(Sar - Sal) > 1.e-9*fabs(Sr - Sl)) {
+    s->dL[dir][id] =   0.5*(chiL*nuL - chiL*nuLR)
+                     + 0.5*(fabs(Sal) - nuLR*Sal);
+    s->dR[dir][id] =   0.5*(chiR*nuR - chiR*nuLR)
+                     + 0.5*(fabs(Sar) - nuLR*Sar);
+    s->aL[dir][id] = 0.5*(1.0 + nuLR);
+    s->aR[dir][id] = 0.5*(1.0 - nuLR);
+  } else {   /* HLLC, degenerate limit Bn -> 0 */
+    s->dL[dir][id] = 0.5*chiL*nuL + 0.5*fabs(Sal);
+    s->dR[dir][id] = 0.5*chiR*nuR + 0.5*fabs(Sar);
+    s->aL[dir][id] = 0.5;
+    s->aR[dir][id] = 0.5;
+  }
+  scrh = 1.0/(aR_1 - aL_1);
+  if (switch_to_hll) {
+    s->aL[dir][id] =  aR_1*scrh;
+    s->aR[dir][id] = -aL_1*scrh;
+    s->dR[dir][id] = -aL_1*aR_1*scrh;
+    s->dL[dir][id] = -aL_1*aR_1*scrh;
+  }
+  s->e[t2][dir][id] =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
+  s->e[t1][dir][id] = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
+}
+
 /* ======================================================================
  * Boundary conditions
  * ====================================================================== */
@@ -1148,7 +1205,10 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
         else err |= hll(vl, vr, dir, s->c.gamma, 0, flux + i*NVMAX, &Sl, &Sr);
       }
       cmax[i] = MAX(fabs(Sl), fabs(Sr));
-      if (mhd) {
+      if (mhd && s->c.emf == GPLO_UCT_HLLD) {
+        p[dir] = i;
+        uct_hlld_store(s, vl, vr, dir, t1, t2, Sl, Sr, IC(s, p[0], p[1], p[2]));
+      } else if (mhd) {
         /* CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177 */
         const int VXt = 1 + t1, VXb = 1 + t2;
         p[dir] = i;
@@ -1469,8 +1529,12 @@ gplo_state *gplo_create(const gplo_config *cfg)
   }
   for (int d = 0; d < c.dims; d++) {
     if ((c.bc[2*d] == GPLO_BC_PERIODIC) != (c.bc[2*d + 1] == GPLO_BC_PERIODIC)) return NULL;
-
   }
+  /* UCT_HLLD: only with the MHD HLLD solver (Src/MHD/hll.cpp:75-76,
+     Src/MHD/hllc.cpp:81-83 quit otherwise) */
+  if (c.emf == 0) c.emf = GPLO_UCT_HLL;
+  if (c.emf != GPLO_UCT_HLL && c.emf != GPLO_UCT_HLLD) return NULL;
+  if (c.emf == GPLO_UCT_HLLD && (c.phys != GPLO_MHD || c.solver != GPLO_HLLD)) return NULL;
   gplo_state *s = (gplo_state *)calloc(1, sizeof(*s));
   s->c = c;
   if (s->c.small_rho == 0.0) s->c.small_rho = 1.e-12;   /* Src/structs.hpp:407-408 */

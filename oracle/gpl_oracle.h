@@ -21,7 +21,7 @@ extern "C" {
 enum { GPLO_HD = 1, GPLO_MHD = 2, GPLO_RMHD = 3 };
 enum { GPLO_HLL = 1, GPLO_HLLC = 2, GPLO_HLLD = 3 };
 enum { GPLO_VANLEER_LIM = 26, GPLO_MC_LIM = 27 };    /* Src/gpluto.hpp:330-336 */
-enum { GPLO_UCT_HLL = 1 };
+enum { GPLO_UCT_HLL = 1, GPLO_UCT_HLLD = 2 };   /* CT_EMF_AVERAGE (Src/MHD/CT/ct.hpp:20-21) */
 enum { GPLO_BC_PERIODIC = 1, GPLO_BC_OUTFLOW = 2, GPLO_BC_REFLECTIVE = 3 };   /* pluto.ini [Boundary] */
 
 typedef struct gplo_config {
@@ -37,6 +37,7 @@ typedef struct gplo_config {
   double small_rho, small_prs;
   int    bc[6];           /* per side X1-beg, X1-end, X2-beg, ..., X3-end: GPLO_BC_*;
                              0 takes periodic ? PERIODIC : OUTFLOW */
+  int    emf;             /* GPLO_UCT_HLL (0 too) or GPLO_UCT_HLLD (MHD with HLLD only) */
 } gplo_config;
 
 typedef struct gplo_state gplo_state;

@@ -27,7 +27,10 @@ class Config(C.Structure):
                 ("nz", C.c_int), ("periodic", C.c_int), ("time_stepping", C.c_int),
                 ("limiter", C.c_int), ("solver", C.c_int), ("gamma", C.c_double),
                 ("cfl", C.c_double), ("cfl_max_var", C.c_double),
-                ("small_rho", C.c_double), ("small_prs", C.c_double), ("bc", C.c_int * 6)]
+                ("small_rho", C.c_double), ("small_prs", C.c_double), ("bc", C.c_int * 6),
+                ("emf", C.c_int)]
+
+EMF = {"UCT_HLL": 1, "UCT_HLLD": 2}
 
 BC = {"periodic": 1, "outflow": 2, "reflective": 3}
 
@@ -66,7 +69,7 @@ class Oracle:
 
     def __init__(self, *, phys: str, n, gamma: float, solver: str = "hlld",
                  limiter: str = "MC_LIM", time_stepping: int = 2, periodic: bool = True,
-                 cfl: float = 0.4, cfl_max_var: float = 1.1, sides=None):
+                 cfl: float = 0.4, cfl_max_var: float = 1.1, sides=None, emf: str = "UCT_HLL"):
         """sides: optional pluto.ini [Boundary] types of X1-beg, X1-end, X2-beg,
         ..., X3-end ("periodic", "outflow", "reflective"); default from
         ``periodic``."""
@@ -77,7 +80,8 @@ class Oracle:
         self.nvar = 5 if self.phys == HD else 8
         bc = (C.c_int * 6)(*([BC[t] for t in sides] if sides else [0] * 6))
         cfg = Config(self.phys, dims, nx, ny, nz, 1 if periodic else 0, time_stepping,
-                     LIMITERS[limiter], SOLVERS[solver], gamma, cfl, cfl_max_var, 1e-12, 1e-12, bc)
+                     LIMITERS[limiter], SOLVERS[solver], gamma, cfl, cfl_max_var, 1e-12, 1e-12, bc,
+                     EMF[emf])
         self._h = lib().gplo_create(C.byref(cfg))
         if not self._h:
             raise ValueError("oracle: unsupported configuration")

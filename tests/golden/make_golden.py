@@ -34,6 +34,9 @@ CASES = [
     # MHD HLLC (Li 2005, Src/MHD/hllc.cpp) with UCT_HLL
     ("mhd3d_12x10x14_rk2_mc_hllc", "mhd3d", "RK2", "MC_LIM", (12, 10, 14), 17, 6, 2e-3, 0.4, "hllc", True, 5 / 3),
     ("mhd3d_10x14x8_rk3_vl_hllc_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 14, 8), 18, 5, 2e-3, 0.4, "hllc", False, 5 / 3),
+    # the UCT_HLLD edge EMF (CT_Store_UCTHlld, Src/MHD/CT/ct_store.hpp:254) with HLLD
+    ("mhd3d_12x14x10_rk2_mc_ucthlld", "mhd3d", "RK2", "MC_LIM", (12, 14, 10), 19, 6, 2e-3, 0.4, "hlld", True, 5 / 3),
+    ("mhd3d_10x12x14_rk3_vl_ucthlld_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 14), 20, 5, 2e-3, 0.4, "hlld", False, 5 / 3),
     # the CPU-runnable headline case: 1D Euler shock tube, 4096 cells, 400 steps
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),
@@ -80,6 +83,9 @@ RESTART_CASES = [
     ("rmhd3d_6x8x6_rk2_mc_hll_restart", "rmhd3d", "RK2", "MC_LIM", (6, 8, 6), 14, 3, 3, 1e-3, 0.4, "hll", True, 4 / 3),
     ("hd1d_256_rk2_mc_hllc_walls_restart", "hd1d", "RK2", "MC_LIM", (256, 1, 1), 15, 60, 50, 1e-4, 0.8, "hllc", False, 1.4),
 ]
+
+# CT_EMF_AVERAGE where it is not UCT_HLL
+EMF = {"mhd3d_12x14x10_rk2_mc_ucthlld": "UCT_HLLD", "mhd3d_10x12x14_rk3_vl_ucthlld_outflow": "UCT_HLLD"}
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
@@ -144,7 +150,10 @@ def make(case):
     sides = SIDES.get(name)
     if sides:
         extra["sides"] = list(sides)
-    binary = ref_run.ref_binary(cfg, ts, lim)
+    emf = EMF.get(name, "UCT_HLL")
+    if emf != "UCT_HLL":
+        extra["emf"] = emf
+    binary = ref_run.ref_binary(cfg, ts, lim, emf)
     r = ref_run.run_reference(binary, vc, vs, gamma=gamma, steps=steps, cfl=cfl,
                               first_dt=first_dt, solver=solver, periodic=periodic, sides=sides)
     params = dict(name=name, cfg=cfg, time_stepping=ts, limiter=lim, n=list(n), seed=seed,

@@ -14,7 +14,8 @@ def test_gpu_matches_reference_fixture(name):
     p, a = load(name)
     print(f"case {name}", flush=True)
     n = tuple(p["n"])
-    g = StageUpdate(Definitions(time_stepping=p["time_stepping"], limiter=p["limiter"]),
+    g = StageUpdate(Definitions(time_stepping=p["time_stepping"], limiter=p["limiter"],
+                                ct_emf_average=p.get("emf", "UCT_HLL")),
                     RuntimeSetup(n=n, solver=p["solver"], cfl=p["cfl"], first_dt=p["first_dt"],
                                  gamma=p["gamma"],
                                  boundary="periodic" if p["periodic"] else "outflow",

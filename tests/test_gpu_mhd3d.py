@@ -15,17 +15,17 @@ pytestmark = pytest.mark.gpu
 
 
 def _pair(n, seed, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64",
-          sides=None):
+          sides=None, emf="UCT_HLL"):
     import oracle
     vc, vs = ic.mhd_turbulence(n, seed)
     print(f"case n={n} seed={seed} steps={steps} ts={ts} lim={lim} solver={solver} {precision} "
-          f"sides={sides}", flush=True)
+          f"sides={sides} {emf}", flush=True)
     o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, solver=solver, limiter=lim,
-                      time_stepping=int(ts[-1]), periodic=sides is None, sides=sides)
+                      time_stepping=int(ts[-1]), periodic=sides is None, sides=sides, emf=emf)
     o.set_state(vc, vs)
     oerr, ot, odt = o.run(steps, first_dt)
     ovc, ovs, ouc = o.get_state()
-    g = StageUpdate(Definitions(time_stepping=ts, limiter=lim),
+    g = StageUpdate(Definitions(time_stepping=ts, limiter=lim, ct_emf_average=emf),
                     RuntimeSetup(n=n, solver=solver, first_dt=first_dt, sides=tuple(sides or ())),
                     precision=precision)
     g.upload(vc, vs)
@@ -81,6 +81,43 @@ def test_fp64_bitwise_hllc(path, monkeypatch):
         assert gerr == oerr == 0
         assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
         assert gt == ot and gdt == odt
+
+
+@pytest.mark.parametrize("path", ["fused", "split", "xy1"])
+def test_fp64_bitwise_uct_hlld(path, monkeypatch):
+    """HLLD with the UCT_HLLD edge EMF (CT_Store_UCTHlld: dL and dR apart, the
+    HLLC limit and the HLL fallback), on the fused and split sweeps."""
+    if path == "split":
+        monkeypatch.setenv("GPL_SPLIT_KERNELS", "1")
+    elif path == "xy1":
+        monkeypatch.setenv("GPL_FUSED_XY", "1")
+    for n, ts, lim in (((16, 12, 20), "RK2", "MC_LIM"), ((20, 17, 12), "RK3", "VANLEER_LIM")):
+        (oerr, ot, odt, ovc, ovs, _), (gerr, gt, gdt, gvc, gvs, _) = _pair(
+            n, 25, 5, 2e-3, ts=ts, lim=lim, emf="UCT_HLLD")
+        assert gerr == oerr == 0
+        assert np.array_equal(gvc, ovc) and np.array_equal(gvs, ovs)
+        assert gt == ot and gdt == odt
+
+
+def test_mixed_precision_uct_hlld():
+    (_, _, _, ovc, _, _), (gerr, _, _, gvc, gvs, _) = _pair(
+        (32, 24, 20), 26, 8, 1e-3, precision="mixed", emf="UCT_HLLD")
+    assert gerr == 0
+    for v in range(8):
+        l1 = np.sum(np.abs(gvc[v] - ovc[v])) / np.sum(np.abs(ovc[v]))
+        assert l1 <= 1e-5, (v, l1)
+    assert ic.divb_max(gvs) <= 1e-11 * np.max(np.abs(gvs)) * 32
+
+
+def test_uct_hlld_needs_hlld():
+    """The reference's HLL and HLLC solvers quit with UCT_HLLD
+    (Src/MHD/hll.cpp:75-76, Src/MHD/hllc.cpp:81-83)."""
+    for solver in ("hll", "hllc"):
+        with pytest.raises(RuntimeError, match="not supported"):
+            StageUpdate(Definitions(ct_emf_average="UCT_HLLD"), RuntimeSetup(n=(16, 16, 16), solver=solver))
+    with pytest.raises(RuntimeError, match="not supported"):
+        StageUpdate(Definitions(physics="RMHD", ct_emf_average="UCT_HLLD"),
+                    RuntimeSetup(n=(16, 16, 16), solver="hll"))
 
 
 def test_mixed_precision_hllc():
@@ -175,6 +212,8 @@ XPER = ("periodic", "periodic") + ("outflow",) * 4          # periodic in x, out
     ((12, 16, 14), ("reflective", "outflow", "periodic", "periodic", "outflow", "reflective"),
      "RK3", "VANLEER_LIM", "hll", 6),
     ((12, 10, 14), OUT, "RK3", "VANLEER_LIM", "hllc", 6),
+    ((14, 12, 10), ("reflective", "outflow", "outflow", "reflective", "periodic", "periodic"),
+     "RK2", "MC_LIM", "hlld+UCT_HLLD", 6),
     ((10, 14, 12), ("reflective",) * 2 + ("periodic",) * 2 + ("outflow",) * 2,
      "RK2", "MC_LIM", "hllc", 6),
 ])
@@ -182,8 +221,9 @@ def test_outflow_sides_bitwise(n, sides, ts, lim, solver, steps):
     """Outflow sides (OutflowBoundary + CT_FillStaggeredFields every stage) and
     reflective walls (ReflectiveBoundary) on the padded device box, alone and
     mixed with each other and with periodic directions."""
+    solver, _, emf = solver.partition("+")
     (oerr, ot, odt, ovc, ovs, ouc), (gerr, gt, gdt, gvc, gvs, guc) = _pair(
-        n, 17, steps, 2e-3, ts=ts, lim=lim, solver=solver, sides=sides)
+        n, 17, steps, 2e-3, ts=ts, lim=lim, solver=solver, sides=sides, emf=emf or "UCT_HLL")
     assert gerr == oerr == 0
     for v in range(8):
         d = np.max(np.abs(gvc[v] - ovc[v]))

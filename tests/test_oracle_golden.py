@@ -19,7 +19,7 @@ def test_oracle_matches_reference_bitwise(name):
     o = oracle.Oracle(phys=phys, n=tuple(p["n"]), gamma=p["gamma"], solver=p["solver"],
                       limiter=p["limiter"], time_stepping=int(p["time_stepping"][-1]),
                       periodic=p["periodic"], cfl=p["cfl"], cfl_max_var=p["cfl_max_var"],
-                      sides=p.get("sides"))
+                      sides=p.get("sides"), emf=p.get("emf", "UCT_HLL"))
     o.set_state(a["vc_in"], a.get("vs_in"))
     err, t, _ = o.run(p["steps"], p["first_dt"])
     vc, vs, _ = o.get_state()
