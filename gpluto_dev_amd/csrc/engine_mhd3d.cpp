@@ -385,9 +385,10 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLLC && cfg.solver != GPL_SOLVER_HLL)
     return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
-  // RMHD: HLL, FP64 and the fused kernels (rows of at most 512 zones)
+  // RMHD: HLL or HLLC, FP64 and the fused kernels (rows of at most 512 zones)
   if (cfg.physics == GPL_RMHD &&
-      (cfg.solver != GPL_SOLVER_HLL || cfg.precision != GPL_FP64 || cfg.nx + pad > 512 ||
+      ((cfg.solver != GPL_SOLVER_HLL && cfg.solver != GPL_SOLVER_HLLC) || cfg.precision != GPL_FP64 ||
+       cfg.nx + pad > 512 ||
        getenv("GPL_SPLIT_KERNELS") != nullptr))
     return GPL_EUNSUPPORTED;
   // the PLM stencil wraps at most two cells: every direction needs >= 3 zones

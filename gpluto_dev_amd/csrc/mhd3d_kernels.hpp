@@ -119,11 +119,13 @@ __device__ __forceinline__ T block_max(T v)
 }
 
 // Riemann solver of the physics module: PH 0 = MHD (SOLVER 3 HLLD, 2 HLLC,
-// 1 HLL), PH 1 = RMHD (HLL with the exact quartic fast speeds; FP64 only).
+// 1 HLL), PH 1 = RMHD (SOLVER 2 HLLC, 1 HLL; both with the exact quartic
+// fast speeds; FP64 only).
 template <typename R, int N, int SOLVER, int PH>
 __device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
 {
-  if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
+  if constexpr (PH == 1 && SOLVER == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
+  else if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 3 || SOLVER == 4) mhd_hlld<R, N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 2) mhd_hllc<R, N>(vl, vr, gamma, ff);
   else mhd_hll<R, N>(vl, vr, gamma, ff);
@@ -1252,9 +1254,14 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
                   double small_rho, double small_prs, hipStream_t s)
 {
   if constexpr (std::is_same<R, double>::value) {
-    if (b.phys == 1) {            // RMHD: HLL only, FP64 only
-      if (limiter == MC_LIM) launch_l<R, MC_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
-      else                   launch_l<R, VANLEER_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+    if (b.phys == 1) {            // RMHD: HLL or HLLC, FP64 only
+      if (solver == 2) {
+        if (limiter == MC_LIM) launch_l<R, MC_LIM, 2, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+        else                   launch_l<R, VANLEER_LIM, 2, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+      } else {
+        if (limiter == MC_LIM) launch_l<R, MC_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+        else                   launch_l<R, VANLEER_LIM, 1, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+      }
       return;
     }
   }

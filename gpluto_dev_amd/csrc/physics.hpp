@@ -33,6 +33,7 @@ template <typename R> GPL_HD R rsqrt_(R a);
 template <> GPL_HD double rsqrt_<double>(double a) { return sqrt(a); }
 template <> GPL_HD float rsqrt_<float>(float a) { return sqrtf(a); }
 template <typename R> GPL_HD bool risnan(R a) { return a != a; }
+template <typename R> GPL_HD bool risfinite(R a) { return __builtin_isfinite(a); }   // isfinite
 
 // PLM limited slope, Src/States/plm_states.cpp:121-133 with SET_MC_LIMITER /
 // SET_VL_LIMITER on a uniform Cartesian grid (Src/plm_coeffs.hpp:92-100).
@@ -684,9 +685,10 @@ GPL_HD void rmhd_prim_to_cons(const R *v, R *u, R gamma)
          + R(0.5)*(Bmag2 + e2);
 }
 
-// Flux<dir>, Src/RMHD/rmhd.hpp:315-366 (reduced energy)
+// Flux<dir>, Src/RMHD/rmhd.hpp:315-366 (reduced energy); pt_out receives the
+// total pressure the reference returns alongside
 template <typename R, int N>
-GPL_HD void rmhd_flux(const R *v, const R *u, R h, R *flux)
+GPL_HD void rmhd_flux(const R *v, const R *u, R h, R *flux, R &pt_out)
 {
   constexpr int VXn = 1 + N, VXt = 1 + (N + 1)%3, VXb = 1 + (N + 2)%3;
   constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
@@ -713,6 +715,14 @@ GPL_HD void rmhd_flux(const R *v, const R *u, R h, R *flux)
   flux[7] = u[1 + N];
   flux[7] -= flux[0];
   flux[1 + N] += pt;
+  pt_out = pt;
+}
+
+template <typename R, int N>
+GPL_HD void rmhd_flux(const R *v, const R *u, R h, R *flux)
+{
+  R pt;
+  rmhd_flux<R, N>(v, u, h, flux, pt);
 }
 
 // CubicSolve, Src/RMHD/rmhd.hpp:390-452 (three real roots, trigonometric form)
@@ -861,6 +871,161 @@ GPL_HD void rmhd_hll(const double *vl, const double *vr, double gamma, FaceFlux<
   }
   uint32_t err = 0;
   for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
+// HLLC of Mignone & Bodo (2006), Src/RMHD/hllc.cpp:150-470 (ideal EOS,
+// reduced energy, no GLM; HLL_SWITCH and RMHD_ENTROPY_FIX off by default,
+// Src/gpluto.hpp:877-878, Src/RMHD/rmhd.hpp:143-145).  Every failed star
+// construction (no real or subluminal contact speed, a superluminal or
+// non-positive star state, a non-finite value) falls back to the HLL flux.
+template <int N>
+GPL_HD void rmhd_hllc(const double *vl, const double *vr, double gamma, FaceFlux<double> &out)
+{
+  constexpr int VXn = 1 + N, MXn = 1 + N, MXt = 1 + (N + 1)%3, MXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3;
+  constexpr int ENG = 7;
+  constexpr double BX_MIN = 1.e-6;                      // Src/RMHD/hllc.cpp:32
+  double ul[8], ur[8], fl[8], fr[8], pl, pr, sl_min, sl_max, sr_min, sr_max;
+  rmhd_prim_to_cons<double>(vl, ul, gamma);
+  rmhd_prim_to_cons<double>(vr, ur, gamma);
+  const double hL = rmhd_enthalpy<double>(vl, gamma), hR = rmhd_enthalpy<double>(vr, gamma);
+  const double a2L = gamma*(vl[7]/vl[0])/hL, a2R = gamma*(vr[7]/vr[0])/hR;
+  rmhd_max_signal_speed<N>(vl, a2L, hL, sl_min, sl_max);
+  rmhd_max_signal_speed<N>(vr, a2R, hR, sr_min, sr_max);
+  const double Sl = rmin(sl_min, sr_min);
+  const double Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  rmhd_flux<double, N>(vl, ul, hL, fl, pl);
+  rmhd_flux<double, N>(vr, ur, hR, fr, pr);
+  double *flux = out.f;
+  double Uhll[8], Fhll[8];
+  double scrh = 1.0/(Sr - Sl);
+  for (int nv = 0; nv < 8; nv++) {
+    Uhll[nv] = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+    Uhll[nv] *= scrh;
+    Fhll[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+    Fhll[nv] *= scrh;
+  }
+  if (Sl >= 0.0) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
+  } else {
+    bool to_hll = false;
+    double usl[8], usr[8], vxs = 0.0;
+    const double vxl = vl[VXn], vxr = vr[VXn];
+    const double Bx = Uhll[BXn], Bys = Uhll[BXt], Bzs = Uhll[BXb];
+    Uhll[ENG] += Uhll[0];           // the star states are derived with the total energy
+    Fhll[ENG] += Fhll[0];
+    // coefficients of the quadratic for the contact speed v* (Bx -> 0: Eq. [47])
+    double a, b, c, BtFBt, FBt2;
+    const bool degenerate = rabs(Bx) < BX_MIN;
+    if (degenerate) {
+      BtFBt = FBt2 = 0.0;
+      a = Fhll[ENG];
+      b = - (Fhll[MXn] + Uhll[ENG]);
+      c = Uhll[MXn];
+    } else {
+      BtFBt = Uhll[BXt]*Fhll[BXt] + Uhll[BXb]*Fhll[BXb];
+      const double Bt2 = Uhll[BXt]*Uhll[BXt] + Uhll[BXb]*Uhll[BXb];
+      FBt2 = Fhll[BXt]*Fhll[BXt] + Fhll[BXb]*Fhll[BXb];
+      a = Fhll[ENG] - BtFBt;
+      b = Bt2 + FBt2 - (Fhll[MXn] + Uhll[ENG]);
+      c = Uhll[MXn] - BtFBt;
+    }
+    const double b2 = b*b;
+    if (!risfinite(a) || !risfinite(b) || !risfinite(c) || !risfinite(b2) || !(b2 > 0.0)) {
+      to_hll = true;
+    } else {
+      const double disc = 1.0 - 4.0*a*c/b2;
+      if (!risfinite(disc) || disc < 0.0) {
+        to_hll = true;
+      } else {
+        scrh = 1.0 + sqrt(disc);
+        const double den = b*scrh;
+        if (!risfinite(den) || den == 0.0) {
+          to_hll = true;
+        } else {
+          vxs = -2.0*c/den;
+          if (!risfinite(vxs) || !(rabs(vxs) < 1.0) || !(Sl < vxs && vxs < Sr)) to_hll = true;
+        }
+      }
+    }
+    if (!to_hll) {
+      if (degenerate) {              // (vy, vz) do not enter
+        const double alpha_l = (Sl - vxl)/(Sl - vxs);
+        const double alpha_r = (Sr - vxr)/(Sr - vxs);
+        usl[0] = ul[0]*alpha_l;
+        usr[0] = ur[0]*alpha_r;
+        const double ps = Fhll[MXn] - Fhll[ENG]*vxs;
+        usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs)/(Sl - vxs);
+        usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs)/(Sr - vxs);
+        usl[MXn] = (usl[ENG] + ps)*vxs;
+        usr[MXn] = (usr[ENG] + ps)*vxs;
+        usl[MXt] = ul[MXt]*alpha_l;
+        usr[MXt] = ur[MXt]*alpha_r;
+        usl[MXb] = ul[MXb]*alpha_l;
+        usr[MXb] = ur[MXb]*alpha_r;
+        usl[MXn] += usl[0]*vxs;      // reduced energy
+        usr[MXn] += usr[0]*vxs;
+        usl[BXn] = Bx;
+        usr[BXn] = Bx;
+        usl[BXt] = ul[BXt]*alpha_l;
+        usr[BXt] = ur[BXt]*alpha_r;
+        usl[BXb] = ul[BXb]*alpha_l;
+        usr[BXb] = ur[BXb]*alpha_r;
+      } else {
+        const double vys = (Bys*vxs - Fhll[BXt])/Bx;
+        const double vzs = (Bzs*vxs - Fhll[BXb])/Bx;
+        double gammas_2 = vxs*vxs + vys*vys + vzs*vzs;
+        gammas_2 = 1.0 - gammas_2;
+        if (!risfinite(gammas_2) || gammas_2 <= 0.0) to_hll = true;     // superluminal
+        if (!to_hll) {
+          const double vBs = vxs*Bx + vys*Bys + vzs*Bzs;
+          const double alpha_l = (Sl - vxl)/(Sl - vxs);
+          const double alpha_r = (Sr - vxr)/(Sr - vxs);
+          usl[0] = ul[0]*alpha_l;
+          usr[0] = ur[0]*alpha_r;
+          const double ps = (BtFBt - Fhll[ENG])*vxs + Bx*Bx + Fhll[MXn] - FBt2;
+          usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs - vBs*Bx)/(Sl - vxs);
+          usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs - vBs*Bx)/(Sr - vxs);
+          usl[MXn] = (usl[ENG] + ps)*vxs - vBs*Bx;
+          usr[MXn] = (usr[ENG] + ps)*vxs - vBs*Bx;
+          usl[MXt] = (Sl*ul[MXt] - fl[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sl - vxs);
+          usr[MXt] = (Sr*ur[MXt] - fr[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sr - vxs);
+          usl[MXb] = (Sl*ul[MXb] - fl[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sl - vxs);
+          usr[MXb] = (Sr*ur[MXb] - fr[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sr - vxs);
+          usl[MXn] += usl[0]*vxs;    // reduced energy
+          usr[MXn] += usr[0]*vxs;
+          usl[BXn] = usr[BXn] = Bx;
+          usl[BXt] = usr[BXt] = Bys;
+          usl[BXb] = usr[BXb] = Bzs;
+        }
+      }
+      if (!to_hll) {
+        if (!(usl[0] > 0.0) || !(usr[0] > 0.0)) to_hll = true;
+        for (int nv = 0; nv < 8; nv++)
+          if (!risfinite(usl[nv]) || !risfinite(usr[nv])) to_hll = true;
+      }
+    }
+    if (!to_hll) {
+      const bool left = vxs > 0.0;
+      for (int nv = 0; nv < 8; nv++)
+        flux[nv] = left ? fl[nv] + Sl*(usl[nv] - ul[nv]) : fr[nv] + Sr*(usr[nv] - ur[nv]);
+      for (int nv = 0; nv < 8; nv++) if (!risfinite(flux[nv])) to_hll = true;
+      if (!risfinite(left ? pl : pr)) to_hll = true;
+    }
+    if (to_hll) {
+      scrh = 1.0/(Sr - Sl);
+      for (int nv = 0; nv < 8; nv++) {
+        flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+        flux[nv] *= scrh;
+      }
+    }
+  }
+  uint32_t err = 0;
+  for (int nv = 0; nv < 8; nv++) if (!risfinite(flux[nv])) err |= ERR_NAN | ERR_QUIT;
   out.err = err;
 }
 

@@ -72,7 +72,7 @@ typedef struct gpl_config {
                               GPL_OUTFLOW or GPL_REFLECTIVE */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
-  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLLC / _HLL; RMHD: _HLL; HD: _HLLC / _HLL */
+  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLLC / _HLL; RMHD: _HLLC / _HLL; HD: _HLLC / _HLL */
   int    ct_emf_average;   /* GPL_UCT_HLL (MHD, RMHD), or GPL_UCT_HLLD (MHD with HLLD) */
   int    precision;        /* GPL_FP64, or GPL_MIXED (MHD) */
   int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */

@@ -740,6 +740,178 @@ static uint32_t rmhd_hll(const double *vl, const double *vr, int n, double gamma
   return err;
 }
 
+/* RMHD HLLC of Mignone & Bodo (2006), Src/RMHD/hllc.cpp:150-470 (ideal EOS,
+   reduced energy, no GLM; HLL_SWITCH and RMHD_ENTROPY_FIX off by default,
+   Src/gpluto.hpp:877-878, Src/RMHD/rmhd.hpp:143-145).  Every failed star
+   construction falls back to the HLL flux. */
+#define RMHD_BX_MIN 1.e-6                   /* BX_MIN, Src/RMHD/hllc.cpp:32 */
+static uint32_t rmhd_hllc(const double *vl, const double *vr, int n, double gamma,
+                          double *flux, double *pSl, double *pSr)
+{
+  const int VXn = 1 + n, MXn = 1 + n, MXt = 1 + (n + 1)%3, MXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3;
+  const int NFLX = 8, ENG = 7;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], pl, pr;
+  double a2L, a2R, hL, hR, sl_min, sl_max, sr_min, sr_max;
+  double Uhll[NVMAX], Fhll[NVMAX], usl[NVMAX], usr[NVMAX];
+  int nv, switch_to_hll = 0;
+  rmhd_prim_to_cons(vl, ul, gamma);
+  rmhd_prim_to_cons(vr, ur, gamma);
+  rmhd_sound_speed2(vl, gamma, &a2L, &hL);
+  rmhd_sound_speed2(vr, gamma, &a2R, &hR);
+  rmhd_max_signal_speed(vl, a2L, hL, n, &sl_min, &sl_max);
+  rmhd_max_signal_speed(vr, a2R, hR, n, &sr_min, &sr_max);
+  double Sl = MIN(sl_min, sr_min);
+  double Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+  rmhd_flux(vl, ul, hL, n, fl, &pl);
+  rmhd_flux(vr, ur, hR, n, fr, &pr);
+
+  double scrh = 1.0/(Sr - Sl);
+  for (nv = 0; nv < NFLX; nv++) {
+    Uhll[nv] = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+    Uhll[nv] *= scrh;
+    Fhll[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+    Fhll[nv] *= scrh;
+  }
+
+  if (Sl >= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv];
+  } else {
+    double vxs = 0.0;
+    double vxl = vl[VXn];
+    double vxr = vr[VXn];
+    double Bx  = Uhll[BXn];
+    double Bys = Uhll[BXt];
+    double Bzs = Uhll[BXb];
+    /* the derivation is done with the total energy */
+    Uhll[ENG] += Uhll[RHO];
+    Fhll[ENG] += Fhll[RHO];
+    /* coefficients of the quadratic for v* (Bx -> 0 limit: Eq. [47]) */
+    double a, b, c, BtFBt, Bt2, FBt2;
+    if (fabs(Bx) < RMHD_BX_MIN) {
+      BtFBt = Bt2 = FBt2 = 0.0;
+      a  = Fhll[ENG];
+      b  = - (Fhll[MXn] + Uhll[ENG]);
+      c  = Uhll[MXn];
+    } else {
+      BtFBt = Uhll[BXt]*Fhll[BXt] + Uhll[BXb]*Fhll[BXb];
+      Bt2   = Uhll[BXt]*Uhll[BXt] + Uhll[BXb]*Uhll[BXb];
+      FBt2  = Fhll[BXt]*Fhll[BXt] + Fhll[BXb]*Fhll[BXb];
+      a  = Fhll[ENG] - BtFBt;
+      b  = Bt2 + FBt2 - (Fhll[MXn] + Uhll[ENG]);
+      c  = Uhll[MXn] - BtFBt;
+    }
+    (void)Bt2;
+    const double b2 = b*b;
+    if (!isfinite(a) || !isfinite(b) || !isfinite(c) || !isfinite(b2) || !(b2 > 0.0)) {
+      switch_to_hll = 1;
+    } else {
+      const double discriminant = 1.0 - 4.0*a*c/b2;
+      if (!isfinite(discriminant) || discriminant < 0.0) {
+        switch_to_hll = 1;
+      } else {
+        scrh = 1.0 + sqrt(discriminant);
+        const double denominator = b*scrh;
+        if (!isfinite(denominator) || denominator == 0.0) {
+          switch_to_hll = 1;
+        } else {
+          vxs = -2.0*c/denominator;
+          if (!isfinite(vxs) || !(fabs(vxs) < 1.0) || !(Sl < vxs && vxs < Sr)) switch_to_hll = 1;
+        }
+      }
+    }
+    if (!switch_to_hll) {
+      if (fabs(Bx) < RMHD_BX_MIN) {
+        /* degenerate case: (vy, vz) do not enter */
+        double alpha_l = (Sl - vxl)/(Sl - vxs);
+        double alpha_r = (Sr - vxr)/(Sr - vxs);
+        double ps;
+        usl[RHO] = ul[RHO]*alpha_l;
+        usr[RHO] = ur[RHO]*alpha_r;
+        ps  = Fhll[MXn] - Fhll[ENG]*vxs;
+        usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs)/(Sl - vxs);
+        usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs)/(Sr - vxs);
+        usl[MXn] = (usl[ENG] + ps)*vxs;
+        usr[MXn] = (usr[ENG] + ps)*vxs;
+        usl[MXt] = ul[MXt]*alpha_l;
+        usr[MXt] = ur[MXt]*alpha_r;
+        usl[MXb] = ul[MXb]*alpha_l;
+        usr[MXb] = ur[MXb]*alpha_r;
+        usl[MXn] += usl[RHO]*vxs;         /* reduced energy */
+        usr[MXn] += usr[RHO]*vxs;
+        usl[BXn] = Bx;
+        usr[BXn] = Bx;
+        usl[BXt] = ul[BXt]*alpha_l;
+        usr[BXt] = ur[BXt]*alpha_r;
+        usl[BXb] = ul[BXb]*alpha_l;
+        usr[BXb] = ur[BXb]*alpha_r;
+      } else {
+        double vys = (Bys*vxs - Fhll[BXt])/Bx;
+        double vzs = (Bzs*vxs - Fhll[BXb])/Bx;
+        double gammas_2 = vxs*vxs + vys*vys + vzs*vzs;
+        gammas_2 = 1.0 - gammas_2;
+        if (!isfinite(gammas_2) || gammas_2 <= 0.0) switch_to_hll = 1;   /* superluminal */
+        if (!switch_to_hll) {
+          double vBs = vxs*Bx + vys*Bys + vzs*Bzs;
+          double alpha_l = (Sl - vxl)/(Sl - vxs);
+          double alpha_r = (Sr - vxr)/(Sr - vxs);
+          double ps;
+          usl[RHO] = ul[RHO]*alpha_l;
+          usr[RHO] = ur[RHO]*alpha_r;
+          ps = (BtFBt - Fhll[ENG])*vxs + Bx*Bx + Fhll[MXn] - FBt2;
+          usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs - vBs*Bx)/(Sl - vxs);
+          usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs - vBs*Bx)/(Sr - vxs);
+          usl[MXn] = (usl[ENG] + ps)*vxs - vBs*Bx;
+          usr[MXn] = (usr[ENG] + ps)*vxs - vBs*Bx;
+          usl[MXt] =  (Sl*ul[MXt] - fl[MXt] - Bx*(Bys*gammas_2 + vBs*vys))
+                      /(Sl - vxs);
+          usr[MXt] =  (Sr*ur[MXt] - fr[MXt] - Bx*(Bys*gammas_2 + vBs*vys))
+                      /(Sr - vxs);
+          usl[MXb] =  (Sl*ul[MXb] - fl[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))
+                      /(Sl - vxs);
+          usr[MXb] =  (Sr*ur[MXb] - fr[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))
+                      /(Sr - vxs);
+          usl[MXn] += usl[RHO]*vxs;       /* reduced energy */
+          usr[MXn] += usr[RHO]*vxs;
+          usl[BXn] = usr[BXn] = Bx;
+          usl[BXt] = usr[BXt] = Bys;
+          usl[BXb] = usr[BXb] = Bzs;
+        }
+      }
+      if (!switch_to_hll) {
+        if (!(usl[RHO] > 0.0) || !(usr[RHO] > 0.0)) switch_to_hll = 1;
+        for (nv = 0; nv < NFLX; nv++)
+          if (!isfinite(usl[nv]) || !isfinite(usr[nv])) switch_to_hll = 1;
+      }
+    }
+    if (!switch_to_hll) {
+      double press;
+      if (vxs > 0.0) {
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv] + Sl*(usl[nv] - ul[nv]);
+        press = pl;
+      } else {
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv] + Sr*(usr[nv] - ur[nv]);
+        press = pr;
+      }
+      for (nv = 0; nv < NFLX; nv++) if (!isfinite(flux[nv])) switch_to_hll = 1;
+      if (!isfinite(press)) switch_to_hll = 1;
+    }
+    if (switch_to_hll) {
+      scrh = 1.0/(Sr - Sl);
+      for (nv = 0; nv < NFLX; nv++) {
+        flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
+        flux[nv] *= scrh;
+      }
+    }
+  }
+  uint32_t err = 0;
+  for (nv = 0; nv < NFLX; nv++) if (!isfinite(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
 /* RMHD_EnergySolve, Src/RMHD/rmhd_energy_solve.cpp:35-209 (IDEAL, reduced
    energy, Newton-Raphson on W' = W - D, tolerance 1e-11, at most 50 passes) */
 static uint32_t rmhd_energy_solve(double *u, double *v, double gamma)
@@ -1195,7 +1367,8 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
       double Sl, Sr;
       const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
       if (is_rmhd(s)) {
-        err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        if (s->c.solver == GPLO_HLLC) err |= rmhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        else err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
       } else if (mhd) {
         if (s->c.solver == GPLO_HLLD)      err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
         else if (s->c.solver == GPLO_HLLC) err |= mhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
@@ -1679,6 +1852,17 @@ uint32_t gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamm
                         double *flux, double *sl, double *sr)
 {
   return rmhd_hll(vl, vr, dir, gamma, flux, sl, sr);
+}
+
+uint32_t gplo_face(int phys, int solver, const double *vl, const double *vr, int dir, double gamma,
+                   double *flux, double *sl, double *sr)
+{
+  if (phys == GPLO_RMHD)
+    return solver == GPLO_HLLC ? rmhd_hllc(vl, vr, dir, gamma, flux, sl, sr)
+                               : rmhd_hll(vl, vr, dir, gamma, flux, sl, sr);
+  if (solver == GPLO_HLLD) return mhd_hlld(vl, vr, dir, gamma, flux, sl, sr);
+  if (solver == GPLO_HLLC) return mhd_hllc(vl, vr, dir, gamma, flux, sl, sr);
+  return hll(vl, vr, dir, gamma, 1, flux, sl, sr);
 }
 
 uint32_t gplo_rmhd_c2p(double *u, double *v, double gamma, double small_rho, double small_prs)

@@ -67,6 +67,9 @@ int         gplo_nvar(const gplo_state *s);
    Davis speeds) and con2prim of u[8] (rewrites u like the reference) */
 uint32_t    gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamma,
                            double *flux, double *sl, double *sr);
+/* one MHD (GPLO_HLL/HLLC/HLLD) or RMHD (GPLO_HLL/HLLC) interface solve */
+uint32_t    gplo_face(int phys, int solver, const double *vl, const double *vr, int dir,
+                      double gamma, double *flux, double *sl, double *sr);
 uint32_t    gplo_rmhd_c2p(double *u, double *v, double gamma, double small_rho, double small_prs);
 
 #ifdef __cplusplus

@@ -58,6 +58,11 @@ CASES = [
     ("rmhd3d_16_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (16, 16, 16), 3, 5, 1e-3, 0.4, "hll", True, 4 / 3),
     ("rmhd3d_12x14x10_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (12, 14, 10), 4, 6, 1e-3, 0.4, "hll", True, 4 / 3),
     ("rmhd3d_10x12x14_rk3_vl_hll", "rmhd3d", "RK3", "VANLEER_LIM", (10, 12, 14), 6, 4, 2e-3, 0.4, "hll", True, 4 / 3),
+    # RMHD HLLC (Mignone & Bodo 2006, Src/RMHD/hllc.cpp); the second case is
+    # strongly magnetised and fast, so faces fall back to HLL
+    ("rmhd3d_12x10x14_rk2_mc_hllc", "rmhd3d", "RK2", "MC_LIM", (12, 10, 14), 21, 5, 1e-3, 0.4, "hllc", True, 4 / 3),
+    ("rmhd3d_14x12x10_rk3_vl_hllc", "rmhd3d", "RK3", "VANLEER_LIM", (14, 12, 10), 22, 4, 1e-3, 0.4, "hllc", True, 4 / 3),
+    ("rmhd3d_10x12x12_rk2_mc_hllc_outflow", "rmhd3d", "RK2", "MC_LIM", (10, 12, 12), 23, 5, 1e-3, 0.4, "hllc", False, 4 / 3),
 ]
 
 # pluto.ini [Boundary] per side (X1-beg, X1-end, ...) where not all periodic /
@@ -90,7 +95,9 @@ EMF = {"mhd3d_12x14x10_rk2_mc_ucthlld": "UCT_HLLD", "mhd3d_10x12x14_rk3_vl_ucthl
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),
            "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0), "rmhd3d_12x10x12_rk2_mc_hll_outflow": (2.0, 1.0),
-           "rmhd3d_6x8x6_rk2_mc_hll_restart": (2.5, 1.0)}
+           "rmhd3d_6x8x6_rk2_mc_hll_restart": (2.5, 1.0),
+           "rmhd3d_12x10x14_rk2_mc_hllc": (3.0, 1.0), "rmhd3d_14x12x10_rk3_vl_hllc": (5.0, 3.0),
+           "rmhd3d_10x12x12_rk2_mc_hllc_outflow": (2.0, 0.5)}
 
 
 def _ic(name, cfg, n, seed, gamma):

@@ -18,6 +18,34 @@ extern "C" uint32_t hp_rmhd_face(const double *vl, const double *vr, int dir, do
   return ff.err;
 }
 
+template <int N>
+static void face_n(int phys, int solver, const double *vl, const double *vr, double gamma,
+                   FaceFlux<double> &ff)
+{
+  if (phys == 3) {                       // RMHD
+    if (solver == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
+    else             rmhd_hll<N>(vl, vr, gamma, ff);
+  } else if (solver == 3) {
+    mhd_hlld<double, N>(vl, vr, gamma, ff);
+  } else if (solver == 2) {
+    mhd_hllc<double, N>(vl, vr, gamma, ff);
+  } else {
+    mhd_hll<double, N>(vl, vr, gamma, ff);
+  }
+}
+
+extern "C" uint32_t hp_face(int phys, int solver, const double *vl, const double *vr, int dir,
+                            double gamma, double *flux, double *sl, double *sr)
+{
+  FaceFlux<double> ff;
+  if (dir == 0) face_n<0>(phys, solver, vl, vr, gamma, ff);
+  else if (dir == 1) face_n<1>(phys, solver, vl, vr, gamma, ff);
+  else face_n<2>(phys, solver, vl, vr, gamma, ff);
+  for (int q = 0; q < 8; q++) flux[q] = ff.f[q];
+  *sl = ff.Sl; *sr = ff.Sr;
+  return ff.err;
+}
+
 extern "C" uint32_t hp_rmhd_c2p(double *u, double *v, double gamma, double small_rho,
                                 double small_prs)
 {

@@ -26,6 +26,9 @@ def libs():
     for f in (hp.hp_rmhd_face, orc.gplo_rmhd_face):
         f.argtypes = [dp, dp, C.c_int, C.c_double, dp, dp, dp]
         f.restype = C.c_uint32
+    for f in (hp.hp_face, orc.gplo_face):
+        f.argtypes = [C.c_int, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp, dp]
+        f.restype = C.c_uint32
     for f in (hp.hp_rmhd_c2p, orc.gplo_rmhd_c2p):
         f.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double]
         f.restype = C.c_uint32
@@ -62,6 +65,34 @@ def test_rmhd_interface_solve_bitwise(libs):
             s1, s2 = np.zeros(4), np.zeros(4)
             e1 = hp.hp_rmhd_face(_ptr(a), _ptr(b), dirn, 4 / 3, _ptr(f1), _ptr(s1[0:]), _ptr(s1[1:]))
             e2 = orc.gplo_rmhd_face(_ptr(a), _ptr(b), dirn, 4 / 3, _ptr(f2), _ptr(s2[0:]), _ptr(s2[1:]))
+            assert e1 == e2
+            assert np.array_equal(f1, f2, equal_nan=True) and np.array_equal(s1[:2], s2[:2]), k
+
+
+@pytest.mark.parametrize("phys,solver,gamma", [(3, 2, 4 / 3), (2, 3, 5 / 3), (2, 2, 5 / 3), (2, 1, 5 / 3)],
+                         ids=["rmhd-hllc", "mhd-hlld", "mhd-hllc", "mhd-hll"])
+def test_interface_solvers_bitwise(libs, phys, solver, gamma):
+    """Every MHD and RMHD interface solver of the device source against the
+    oracle's, on random states that cover the zero normal field (the RMHD
+    HLLC degenerate branch), fast strongly magnetised pairs whose star
+    states fail (the HLL fallbacks) and wide density jumps."""
+    hp, orc = libs
+    rng = np.random.default_rng(11 + 3 * solver + phys)
+    n = 3000
+    vl, vr = _states(rng, n), _states(rng, n)
+    if phys == 2:                                           # classical: |v| well below 1 is not needed
+        vl[:, 1:4] *= 3.0
+        vr[:, 1:4] *= 3.0
+    vr[:, 4:7] = vl[:, 4:7] * (1 + 0.1 * rng.normal(size=(n, 3)))
+    for dirn in range(3):
+        vr[:, 4 + dirn] = vl[:, 4 + dirn]                   # continuous normal field (CT)
+        for k in range(n):
+            a, b = vl[k].copy(), vr[k].copy()
+            f1, f2 = np.zeros(8), np.zeros(8)
+            s1, s2 = np.zeros(4), np.zeros(4)
+            e1 = hp.hp_face(phys, solver, _ptr(a), _ptr(b), dirn, gamma, _ptr(f1), _ptr(s1[0:]), _ptr(s1[1:]))
+            e2 = orc.gplo_face(phys, solver, _ptr(a), _ptr(b), dirn, gamma, _ptr(f2), _ptr(s2[0:]),
+                               _ptr(s2[1:]))
             assert e1 == e2
             assert np.array_equal(f1, f2, equal_nan=True) and np.array_equal(s1[:2], s2[:2]), k
 
