@@ -103,6 +103,8 @@ public:
     n = (size_t)g.nx*g.ny*g.nz;
     g.ncell = (int)n;
     // dx = (xend - xbeg)/N for every cell (Src/set_grid.cpp:196)
+    g.small_rho = c.small_rho;
+    g.small_prs = c.small_prs;
     g.dx = (c.xend[0] - c.xbeg[0])/c.nx;
     g.dy = (c.xend[1] - c.xbeg[1])/c.ny;
     g.dz = (c.xend[2] - c.xbeg[2])/c.nz;
@@ -385,9 +387,9 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   if (cfg.solver != GPL_SOLVER_HLLD && cfg.solver != GPL_SOLVER_HLLC && cfg.solver != GPL_SOLVER_HLL)
     return GPL_EUNSUPPORTED;
   if (cfg.precision != GPL_FP64 && cfg.precision != GPL_MIXED) return GPL_EINVAL;
-  // RMHD: HLL or HLLC, FP64 and the fused kernels (rows of at most 512 zones)
+  // RMHD: HLL, HLLC or HLLD, FP64 and the fused kernels (rows of at most 512 zones)
   if (cfg.physics == GPL_RMHD &&
-      ((cfg.solver != GPL_SOLVER_HLL && cfg.solver != GPL_SOLVER_HLLC) || cfg.precision != GPL_FP64 ||
+      (cfg.precision != GPL_FP64 ||
        cfg.nx + pad > 512 ||
        getenv("GPL_SPLIT_KERNELS") != nullptr))
     return GPL_EUNSUPPORTED;

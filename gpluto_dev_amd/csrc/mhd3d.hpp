@@ -15,6 +15,9 @@ struct Box3 {
   // box padded with ghost layers (non-periodic sides) reduces C_dt and
   // reports error bits from the reference's cells and faces only
   int lo[3], hi[3];
+  // density and pressure floors of ConsToPrim (run_config.small_rho /
+  // small_prs), read by the RMHD HLLD's con2prim of the HLL state
+  double small_rho, small_prs;
 };
 
 template <typename R>

@@ -119,12 +119,14 @@ __device__ __forceinline__ T block_max(T v)
 }
 
 // Riemann solver of the physics module: PH 0 = MHD (SOLVER 3 HLLD, 2 HLLC,
-// 1 HLL), PH 1 = RMHD (SOLVER 2 HLLC, 1 HLL; both with the exact quartic
-// fast speeds; FP64 only).
+// 1 HLL), PH 1 = RMHD (SOLVER 3 HLLD, 2 HLLC, 1 HLL, all with the exact
+// quartic fast speeds; FP64 only).
 template <typename R, int N, int SOLVER, int PH>
-__device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
+__device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff,
+                                        const Box3 &g)
 {
-  if constexpr (PH == 1 && SOLVER == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
+  if constexpr (PH == 1 && SOLVER == 3) rmhd_hlld<N>(vl, vr, gamma, g.small_rho, g.small_prs, ff);
+  else if constexpr (PH == 1 && SOLVER == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
   else if constexpr (PH == 1) rmhd_hll<N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 3 || SOLVER == 4) mhd_hlld<R, N>(vl, vr, gamma, ff);
   else if constexpr (SOLVER == 2) mhd_hllc<R, N>(vl, vr, gamma, ff);
@@ -203,7 +205,7 @@ void k_sweep(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
   // normal field from the face (plm_states.cpp:184-189)
   vl[4 + D] = vr[4 + D] = (R)Vs[D*n + c];
   FaceFlux<R> ff;
-  riemann<R, D, SOLVER, 0>(vl, vr, gamma, ff);
+  riemann<R, D, SOLVER, 0>(vl, vr, gamma, ff, g);
   R *Fd = F + (size_t)D*5*n;
   Fd[0*n + c] = ff.f[0];
   Fd[1*n + c] = ff.f[1];
@@ -314,7 +316,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   }
   vl[4] = vr[4] = bn;
   FaceFlux<R> ff;
-  riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
+  riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff, g);
   uct_store<R, 0, SOLVER>(vl, vr, gamma, ff, FD, c, n);
   __syncthreads();                 // everyone is done with sV/sD
   R *sF = sV;                      // [6][nx]: hydro flux and cmax of face i
@@ -437,7 +439,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
       for (int q = 0; q < 7; q++) { vl[VX[q]] = sD[q*nx + im]; vr[VX[q]] = lo[q]; }
       vl[4] = vr[4] = bn;
       FaceFlux<R> ff;
-      riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff);
+      riemann<R, 0, SOLVER, PH>(vl, vr, gamma, ff, g);
       if (face_reported<0>(g, i, f, k)) e |= ff.err;
       uct_store<R, 0, SOLVER>(vl, vr, gamma, ff, fdx, cf, n);
       sF[0*nx + i] = ff.f[0]; sF[1*nx + i] = ff.f[1]; sF[2*nx + i] = ff.f[2];
@@ -470,7 +472,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
     }
     vl[5] = vr[5] = byf;
     FaceFlux<R> ff;
-    riemann<R, 1, SOLVER, PH>(vl, vr, gamma, ff);
+    riemann<R, 1, SOLVER, PH>(vl, vr, gamma, ff, g);
     if (face_reported<1>(g, i, wrapm(f, ny), k)) e |= ff.err;
     if (f < s1) uct_store<R, 1, SOLVER>(vl, vr, gamma, ff, fdy, cf, n);
     const double F[5] = {(double)ff.f[0], (double)ff.f[1], (double)ff.f[2], (double)ff.f[3],
@@ -614,7 +616,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
       }
       vl[BXn] = vr[BXn] = (R)ldo(Bn, cf);
       FaceFlux<R> ff;
-      riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff);
+      riemann<R, D, SOLVER, PH>(vl, vr, gamma, ff, g);
       {
         const int ff_ = wrapm(f, nd);
         if (D == 1 ? face_reported<1>(g, i, ff_, a) : face_reported<2>(g, i, a, ff_)) e |= ff.err;
@@ -1254,8 +1256,11 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
                   double small_rho, double small_prs, hipStream_t s)
 {
   if constexpr (std::is_same<R, double>::value) {
-    if (b.phys == 1) {            // RMHD: HLL or HLLC, FP64 only
-      if (solver == 2) {
+    if (b.phys == 1) {            // RMHD: HLL, HLLC or HLLD, FP64 only
+      if (solver == 3) {
+        if (limiter == MC_LIM) launch_l<R, MC_LIM, 3, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+        else                   launch_l<R, VANLEER_LIM, 3, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
+      } else if (solver == 2) {
         if (limiter == MC_LIM) launch_l<R, MC_LIM, 2, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
         else                   launch_l<R, VANLEER_LIM, 2, 1>(which, b, g, ctl, w, first_stage, want_dt, march, gamma, small_rho, small_prs, s);
       } else {

@@ -1190,4 +1190,271 @@ GPL_HD uint32_t rmhd_cons_to_prim(double *u, double *v, double gamma, double sma
   return err;
 }
 
+// HLLD of Mignone, Ugliano & Bodo (2009), Src/RMHD/hlld.cpp:95-872 (ideal
+// EOS, reduced energy, no GLM; HLL_SWITCH and RMHD_ENTROPY_FIX off by
+// default).  The state behind each fast wave is a function of the total
+// pressure p; a secant iteration on p closes the normal-velocity jump across
+// the contact; any failure gives the HLL flux.
+struct RhlldState {          // Riemann_State (Src/structs.hpp)
+  double S, R[8], u[8];
+  double vx, vy, vz, Bx, By, Bz, w, sw, Kx, Ky, Kz, K2, Sa;
+  bool fail;
+};
+
+// HLLD_GetRiemannState, Src/RMHD/hlld.cpp:639-731: false when w < 0 (the
+// other members then keep their values)
+template <int N>
+GPL_HD bool rhlld_riemann_state(RhlldState &P, double p, double Bx, int side)
+{
+  constexpr int MXn = 1 + N, MXt = 1 + (N + 1)%3, MXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3, ENG = 7;
+  const double S = P.S;
+  const double *R = P.R;
+  const double A = R[MXn] + p*(1.0 - S*S) - S*R[ENG];           // Eq. [26]
+  const double G = R[BXt]*R[BXt] + R[BXb]*R[BXb];               // Eq. [27]
+  const double C = R[BXt]*R[MXt] + R[BXb]*R[MXb];               // Eq. [28]
+  const double X = Bx*(A*S*Bx + C) - (A + G)*(S*p + R[ENG]);    // Eq. [30]
+  const double vx = ( Bx*(A*Bx + C*S) - (R[MXn] + p)*(G + A) );
+  const double vy = ( - (A + G - Bx*Bx*(1.0 - S*S))*R[MXt]
+                      + R[BXt]*(C + Bx*(S*R[MXn] - R[ENG])) );
+  const double vz = ( - (A + G - Bx*Bx*(1.0 - S*S))*R[MXb]
+                      + R[BXb]*(C + Bx*(S*R[MXn] - R[ENG])) );
+  double scrh = vx*R[MXn] + vy*R[MXt] + vz*R[MXb];
+  scrh = X*R[ENG] - scrh;
+  P.w = p + scrh/(X*S - vx);                                    // Eq. [31]
+  if (P.w < 0.0) return false;
+  P.vx = vx/X;
+  P.vy = vy/X;
+  P.vz = vz/X;
+  P.Bx = Bx;
+  P.By = -(R[BXt]*(S*p + R[ENG]) - Bx*R[MXt])/A;
+  P.Bz = -(R[BXb]*(S*p + R[ENG]) - Bx*R[MXb])/A;
+  double s = Bx > 0.0 ? 1.0 : -1.0;
+  if (side < 0) s *= -1.0;
+  P.sw = s*sqrt(P.w);
+  scrh = 1.0/(S*p +  R[ENG] + Bx*P.sw);
+  P.Kx = scrh*(R[MXn] + p + R[BXn]*P.sw);
+  P.Ky = scrh*(R[MXt]     + R[BXt]*P.sw);
+  P.Kz = scrh*(R[MXb]     + R[BXb]*P.sw);
+  P.K2 = P.Kx*P.Kx + P.Ky*P.Ky + P.Kz*P.Kz;
+  return true;
+}
+
+// HLLD_Fstar, Src/RMHD/hlld.cpp:560-636: the normal velocity jump across
+// the contact at total pressure p (Eq. [48]); L.fail flags an unphysical fan
+template <int N>
+GPL_HD double rhlld_fstar(RhlldState &L, RhlldState &Rs, double p, double Bx, double &Sc)
+{
+  rhlld_riemann_state<N>(L, p, Bx, -1);     // (their own results are overwritten below,
+  rhlld_riemann_state<N>(Rs, p, Bx,  1);    //  as in the reference)
+  const double dK  = Rs.Kx - L.Kx + 1.e-12;
+  const double Bxc = Bx*dK;
+  const double Byc =   Rs.By*(Rs.Kx - Rs.vx)
+                     - L.By*(L.Kx - L.vx)
+                     + Bx*(Rs.vy - L.vy);
+  const double Bzc =   Rs.Bz*(Rs.Kx - Rs.vx)
+                     - L.Bz*(L.Kx - L.vx)
+                     + Bx*(Rs.vz - L.vz);
+  const double KLBc = L.Kx*Bxc + L.Ky*Byc + L.Kz*Bzc;
+  const double KRBc = Rs.Kx*Bxc + Rs.Ky*Byc + Rs.Kz*Bzc;
+  const double vxcL = L.Kx - dK*Bx*(1.0 - L.K2)/(L.sw*dK - KLBc);
+  const double vxcR = Rs.Kx - dK*Bx*(1.0 - Rs.K2)/(Rs.sw*dK - KRBc);
+  L.Sa  = L.Kx;
+  Rs.Sa = Rs.Kx;
+  Sc    = 0.5*(vxcL + vxcR);
+  bool ok = (vxcL - L.Kx) > -1.e-6;
+  ok = ok && (Rs.Kx - vxcR) > -1.e-6;
+  ok = ok && (L.S - L.vx) < 0.0;
+  ok = ok && (Rs.S - Rs.vx) > 0.0;
+  ok = ok && (Rs.w - p) > 0.0;
+  ok = ok && (L.w - p) > 0.0;
+  ok = ok && (L.Sa - L.S) > -1.e-6;
+  ok = ok && (Rs.S - Rs.Sa) > -1.e-6;
+  L.fail = !ok;
+  return vxcL - vxcR;
+}
+
+// HLLD_GetAState, Src/RMHD/hlld.cpp:734-768: the state behind a fast wave
+template <int N>
+GPL_HD void rhlld_astate(RhlldState &P, double p, double Bx)
+{
+  constexpr int MXn = 1 + N, MXt = 1 + (N + 1)%3, MXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3, ENG = 7;
+  double *ua = P.u;
+  const double *R = P.R;
+  const double scrh = 1.0/(P.S - P.vx);
+  ua[0]   =  R[0]*scrh;
+  ua[BXn] =  Bx;
+  ua[BXt] = (R[BXt] - Bx*P.vy)*scrh;
+  ua[BXb] = (R[BXb] - Bx*P.vz)*scrh;
+  const double vB = P.vx*ua[BXn] + P.vy*ua[BXt] + P.vz*ua[BXb];
+  ua[ENG] = (R[ENG] + p*P.vx - vB*Bx)*scrh;
+  ua[MXn] = (ua[ENG] + p)*P.vx - vB*ua[BXn];
+  ua[MXt] = (ua[ENG] + p)*P.vy - vB*ua[BXt];
+  ua[MXb] = (ua[ENG] + p)*P.vz - vB*ua[BXb];
+}
+
+// HLLD_GetCState, Src/RMHD/hlld.cpp:771-855: the state across the contact
+template <int N>
+GPL_HD void rhlld_cstate(RhlldState &L, RhlldState &Rs, double p, double Bx, double *Uc)
+{
+  constexpr int MXn = 1 + N, MXt = 1 + (N + 1)%3, MXb = 1 + (N + 2)%3;
+  constexpr int BXn = 4 + N, BXt = 4 + (N + 1)%3, BXb = 4 + (N + 2)%3, ENG = 7;
+  rhlld_astate<N>(L, p, Bx);
+  rhlld_astate<N>(Rs, p, Bx);
+  const double dK = (Rs.Kx - L.Kx) + 1.e-12;
+  double Byc =   Rs.By*(Rs.Kx - Rs.vx)                   // Eq. [45]
+               - L.By*(L.Kx - L.vx)
+               + Bx*(Rs.vy - L.vy);
+  double Bzc =   Rs.Bz*(Rs.Kx - Rs.vx)
+               - L.Bz*(L.Kx - L.vx)
+               + Bx*(Rs.vz - L.vz);
+  const double Bxc = Bx;
+  Byc /= dK;
+  Bzc /= dK;
+  Uc[BXn] = Bxc;
+  Uc[BXt] = Byc;
+  Uc[BXb] = Bzc;
+  const double KLBc = L.Kx*Bxc + L.Ky*Byc + L.Kz*Bzc;
+  const double KRBc = Rs.Kx*Bxc + Rs.Ky*Byc + Rs.Kz*Bzc;
+  const double scrhL = (1.0 - L.K2)/(L.sw - KLBc);
+  const double scrhR = (1.0 - Rs.K2)/(Rs.sw - KRBc);
+  const double vxcL = L.Kx - Uc[BXn]*scrhL;              // Eq. [47]
+  const double vxcR = Rs.Kx - Uc[BXn]*scrhR;
+  const double vycL = L.Ky - Uc[BXt]*scrhL;
+  const double vycR = Rs.Ky - Uc[BXt]*scrhR;
+  const double vzcL = L.Kz - Uc[BXb]*scrhL;
+  const double vzcR = Rs.Kz - Uc[BXb]*scrhR;
+  const double vxc = 0.5*(vxcL + vxcR);
+  const double vyc = 0.5*(vycL + vycR);
+  const double vzc = 0.5*(vzcL + vzcR);
+  RhlldState &A = vxc > 0.0 ? L : Rs;
+  rhlld_astate<N>(A, p, Bx);
+  const double *ua = A.u;
+  const double Sa = A.Sa, vxa = A.vx;
+  const double vBc = vxc*Uc[BXn] + vyc*Uc[BXt] + vzc*Uc[BXb];
+  Uc[0]   = ua[0]*(Sa - vxa)/(Sa - vxc);                        // Eq. [32]
+  Uc[ENG] = (Sa*ua[ENG] - ua[MXn] + p*vxc - vBc*Bx)/(Sa - vxc); // Eq. [33]
+  Uc[MXn] = (Uc[ENG] + p)*vxc - vBc*Bx;                         // Eq. [34]
+  Uc[MXt] = (Uc[ENG] + p)*vyc - vBc*Uc[BXt];
+  Uc[MXb] = (Uc[ENG] + p)*vzc - vBc*Uc[BXb];
+}
+
+template <int N>
+GPL_HD void rmhd_hlld(const double *vl, const double *vr, double gamma, double small_rho,
+                      double small_prs, FaceFlux<double> &out)
+{
+  constexpr int MXn = 1 + N, BXn = 4 + N, ENG = 7;
+  double ul[8], ur[8], fl[8], fr[8], pl, pr, sl_min, sl_max, sr_min, sr_max;
+  uint32_t err = 0;
+  rmhd_prim_to_cons<double>(vl, ul, gamma);
+  rmhd_prim_to_cons<double>(vr, ur, gamma);
+  const double hL = rmhd_enthalpy<double>(vl, gamma), hR = rmhd_enthalpy<double>(vr, gamma);
+  const double a2L = gamma*(vl[7]/vl[0])/hL, a2R = gamma*(vr[7]/vr[0])/hR;
+  rmhd_flux<double, N>(vl, ul, hL, fl, pl);
+  rmhd_flux<double, N>(vr, ur, hR, fr, pr);
+  rmhd_max_signal_speed<N>(vl, a2L, hL, sl_min, sl_max);
+  rmhd_max_signal_speed<N>(vr, a2R, hR, sr_min, sr_max);
+  const double Sl = rmin(sl_min, sr_min);
+  const double Sr = rmax(sl_max, sr_max);
+  out.Sl = Sl; out.Sr = Sr;
+  double *flux = out.f;
+  if (Sl >= 0.0) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
+  } else {
+    bool to_hll = false;
+    double Uhll[8], Fhll[8];
+    const double dS_1 = 1.0/(Sr - Sl);
+    for (int nv = 0; nv < 8; nv++) {
+      Uhll[nv]  = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+      Uhll[nv] *= dS_1;
+      Fhll[nv]  = Sr*fl[nv] - Sl*fr[nv] + Sl*Sr*(ur[nv] - ul[nv]);
+      Fhll[nv] *= dS_1;
+    }
+    RhlldState L = {}, Rs = {};
+    L.S = Sl;
+    Rs.S = Sr;
+    const double Bx = Uhll[BXn];
+    for (int nv = 0; nv < 8; nv++) {
+      L.R[nv]  = Sl*ul[nv] - fl[nv];
+      Rs.R[nv] = Sr*ur[nv] - fr[nv];
+    }
+    L.R[ENG]  += L.R[0];                 // reduced energy
+    Rs.R[ENG] += Rs.R[0];
+    // initial guess: the B -> 0 limit, or the total pressure of the HLL state
+    double p0, scrh = rmax(pl, pr);
+    if (Bx*Bx/scrh < 0.01) {
+      const double a = Sr - Sl;
+      const double b = Rs.R[ENG] - L.R[ENG] + Sr*L.R[MXn] - Sl*Rs.R[MXn];
+      const double c = L.R[MXn]*Rs.R[ENG] - Rs.R[MXn]*L.R[ENG];
+      scrh = b*b - 4.0*a*c;
+      scrh = rmax(scrh, 0.0);
+      p0 = 0.5*(- b + sqrt(scrh))*dS_1;
+    } else {
+      double Vhll[8];
+      uint32_t local = 0;
+      rmhd_cons_to_prim(Uhll, Vhll, gamma, small_rho, small_prs, local);   // HLLD_Solver :343-351
+      err |= local;
+      if (local) to_hll = true;
+      // HLLD_TotalPressure, Src/RMHD/hlld.cpp:857-872
+      const double Bmag2 = Vhll[4]*Vhll[4] + Vhll[5]*Vhll[5] + Vhll[6]*Vhll[6];
+      const double vel2  = Vhll[1]*Vhll[1] + Vhll[2]*Vhll[2] + Vhll[3]*Vhll[3];
+      const double vB    = Vhll[1]*Vhll[4] + Vhll[2]*Vhll[5] + Vhll[3]*Vhll[6];
+      p0 = Vhll[7] + 0.5*(Bmag2*(1.0 - vel2) + vB*vB);
+    }
+    double Sc = 0.0, p;
+    double f0 = rhlld_fstar<N>(L, Rs, p0, Bx, Sc);
+    if (risnan(f0) || L.fail) to_hll = true;
+    if (rabs(f0) > 1.e-12 && !to_hll) {
+      p = 1.025*p0;
+      for (int k1 = 1; k1 < 20; k1++) {           // MAX_ITER, Src/RMHD/hlld.cpp:45
+        const double f = rhlld_fstar<N>(L, Rs, p, Bx, Sc);
+        if (f != f || L.fail || (k1 > 7) || (rabs(f) > rabs(f0) && k1 > 4)) {
+          to_hll = true;
+          break;
+        }
+        const double dp = (p - p0)/(f - f0)*f;
+        p0 = p; f0 = f;
+        p -= dp;
+        if (p < 0.0) p = 1.e-6;
+        if (rabs(dp) < 1.e-5*p || rabs(f) < 1.e-6) break;
+      }
+    } else {
+      p = p0;
+    }
+    if (L.fail) to_hll = true;
+    if (!to_hll) {
+      if (L.Sa >= -1.e-6) {
+        rhlld_astate<N>(L, p, Bx);
+        L.u[ENG] -= L.u[0];
+        for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv] + Sl*(L.u[nv] - ul[nv]);
+      } else if (Rs.Sa <= 1.e-6) {
+        rhlld_astate<N>(Rs, p, Bx);
+        Rs.u[ENG] -= Rs.u[0];
+        for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv] + Sr*(Rs.u[nv] - ur[nv]);
+      } else {
+        double Uc[8];
+        rhlld_cstate<N>(L, Rs, p, Bx, Uc);
+        if (Sc > 0.0) {
+          L.u[ENG] -= L.u[0];
+          Uc[ENG]  -= Uc[0];
+          for (int nv = 0; nv < 8; nv++)
+            flux[nv] = fl[nv] + Sl*(L.u[nv] - ul[nv]) + L.Sa*(Uc[nv] - L.u[nv]);
+        } else {
+          Rs.u[ENG] -= Rs.u[0];
+          Uc[ENG]   -= Uc[0];
+          for (int nv = 0; nv < 8; nv++)
+            flux[nv] = fr[nv] + Sr*(Rs.u[nv] - ur[nv]) + Rs.Sa*(Uc[nv] - Rs.u[nv]);
+        }
+      }
+    } else {
+      for (int nv = 0; nv < 8; nv++) flux[nv] = Fhll[nv];
+    }
+  }
+  for (int nv = 0; nv < 8; nv++) if (risnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  out.err = err;
+}
+
 }  // namespace gpl

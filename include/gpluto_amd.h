@@ -64,7 +64,7 @@ extern "C" {
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
 typedef struct gpl_config {
-  int    physics;          /* GPL_MHD (3D), GPL_RMHD (3D, HLL, FP64) or GPL_HD (1D/2D/3D) */
+  int    physics;          /* GPL_MHD (3D), GPL_RMHD (3D, FP64) or GPL_HD (1D/2D/3D) */
   int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */
@@ -72,7 +72,7 @@ typedef struct gpl_config {
                               GPL_OUTFLOW or GPL_REFLECTIVE */
   int    time_stepping;    /* GPL_RK2 or GPL_RK3 */
   int    limiter;          /* GPL_MC_LIM or GPL_VANLEER_LIM */
-  int    solver;           /* MHD: GPL_SOLVER_HLLD / _HLLC / _HLL; RMHD: _HLLC / _HLL; HD: _HLLC / _HLL */
+  int    solver;           /* MHD and RMHD: GPL_SOLVER_HLLD / _HLLC / _HLL; HD: _HLLC / _HLL */
   int    ct_emf_average;   /* GPL_UCT_HLL (MHD, RMHD), or GPL_UCT_HLLD (MHD with HLLD) */
   int    precision;        /* GPL_FP64, or GPL_MIXED (MHD) */
   int    nghost;           /* ghost zones of the ARY4D layout (2 for PLM) */

@@ -912,6 +912,284 @@ static uint32_t rmhd_hllc(const double *vl, const double *vr, int n, double gamm
   return err;
 }
 
+/* RMHD HLLD of Mignone, Ugliano & Bodo (2009), Src/RMHD/hlld.cpp:95-872
+   (ideal EOS, reduced energy, no GLM; HLL_SWITCH and RMHD_ENTROPY_FIX off).
+   The state behind each fast wave is a function of the total pressure p
+   (HLLD_GetRiemannState); a secant iteration on p closes the jump of the
+   normal velocity across the contact (HLLD_Fstar); any failure gives the HLL
+   flux. */
+typedef struct {
+  double S, R[NVMAX], u[NVMAX];
+  double vx, vy, vz, Bx, By, Bz, w, sw, Kx, Ky, Kz, K2, Sa;
+  int fail;
+} rhlld_state;                                  /* Riemann_State, Src/structs.hpp */
+
+static uint32_t rmhd_cons_to_prim(double *u, double *v, const gplo_config *c);
+
+/* HLLD_GetRiemannState, Src/RMHD/hlld.cpp:639-731; 0 when w < 0 (the other
+   members then keep their previous values, as in the reference) */
+static int rhlld_riemann_state(rhlld_state *Pv, double p, double Bx, int side, int n)
+{
+  const int MXn = 1 + n, MXt = 1 + (n + 1)%3, MXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3, ENG = 7;
+  double A, C, G, X, s, vx, vy, vz, scrh, S = Pv->S, *R = Pv->R;
+  A = R[MXn] + p*(1.0 - S*S) - S*R[ENG];          /* Eq. [26] */
+  G = R[BXt]*R[BXt] + R[BXb]*R[BXb];              /* Eq. [27] */
+  C = R[BXt]*R[MXt] + R[BXb]*R[MXb];              /* Eq. [28] */
+  X = Bx*(A*S*Bx + C) - (A + G)*(S*p + R[ENG]);   /* Eq. [30] */
+  vx = ( Bx*(A*Bx + C*S) - (R[MXn] + p)*(G + A) );
+  vy = ( - (A + G - Bx*Bx*(1.0 - S*S))*R[MXt]
+         + R[BXt]*(C + Bx*(S*R[MXn] - R[ENG])) );
+  vz = ( - (A + G - Bx*Bx*(1.0 - S*S))*R[MXb]
+         + R[BXb]*(C + Bx*(S*R[MXn] - R[ENG])) );
+  scrh = vx*R[MXn] + vy*R[MXt] + vz*R[MXb];
+  scrh = X*R[ENG] - scrh;
+  Pv->w = p + scrh/(X*S - vx);                    /* Eq. [31] */
+  if (Pv->w < 0.0) return 0;
+  Pv->vx = vx/X;
+  Pv->vy = vy/X;
+  Pv->vz = vz/X;
+  Pv->Bx = Bx;
+  Pv->By = -(R[BXt]*(S*p + R[ENG]) - Bx*R[MXt])/A;
+  Pv->Bz = -(R[BXb]*(S*p + R[ENG]) - Bx*R[MXb])/A;
+  s  = Bx > 0.0 ? 1.0 : -1.0;
+  if (side < 0) s *= -1.0;
+  Pv->sw = s*sqrt(Pv->w);
+  scrh = 1.0/(S*p +  R[ENG] + Bx*Pv->sw);
+  Pv->Kx = scrh*(R[MXn] + p + R[BXn]*Pv->sw);
+  Pv->Ky = scrh*(R[MXt]     + R[BXt]*Pv->sw);
+  Pv->Kz = scrh*(R[MXb]     + R[BXb]*Pv->sw);
+  Pv->K2 = Pv->Kx*Pv->Kx + Pv->Ky*Pv->Ky + Pv->Kz*Pv->Kz;
+  return 1;
+}
+
+/* HLLD_Fstar, Src/RMHD/hlld.cpp:560-636: the normal velocity jump across
+   the contact at total pressure p (Eq. [48]) */
+static double rhlld_fstar(rhlld_state *PaL, rhlld_state *PaR, double p, double Bx, double *Sc, int n)
+{
+  int success = 1;
+  double dK, Bxc, Byc, Bzc, fun, vxcL, KLBc, vxcR, KRBc;
+  success *= rhlld_riemann_state(PaL, p, Bx, -1, n);
+  success *= rhlld_riemann_state(PaR, p, Bx,  1, n);
+  dK  = PaR->Kx - PaL->Kx + 1.e-12;
+  Bxc = Bx*dK;
+  Byc =   PaR->By*(PaR->Kx - PaR->vx)
+        - PaL->By*(PaL->Kx - PaL->vx)
+        + Bx*(PaR->vy - PaL->vy);
+  Bzc =   PaR->Bz*(PaR->Kx - PaR->vx)
+        - PaL->Bz*(PaL->Kx - PaL->vx)
+        + Bx*(PaR->vz - PaL->vz);
+  KLBc = PaL->Kx*Bxc + PaL->Ky*Byc + PaL->Kz*Bzc;
+  KRBc = PaR->Kx*Bxc + PaR->Ky*Byc + PaR->Kz*Bzc;
+  vxcL = PaL->Kx - dK*Bx*(1.0 - PaL->K2)/(PaL->sw*dK - KLBc);
+  vxcR = PaR->Kx - dK*Bx*(1.0 - PaR->K2)/(PaR->sw*dK - KRBc);
+  PaL->Sa = PaL->Kx;
+  PaR->Sa = PaR->Kx;
+  *Sc     = 0.5*(vxcL + vxcR);
+  fun     = vxcL - vxcR;
+  /* the reference overwrites the two state flags here */
+  success  = (vxcL - PaL->Kx) > -1.e-6;
+  success *= (PaR->Kx  - vxcR) > -1.e-6;
+  success *= (PaL->S - PaL->vx) < 0.0;
+  success *= (PaR->S - PaR->vx) > 0.0;
+  success *= (PaR->w - p) > 0.0;
+  success *= (PaL->w - p) > 0.0;
+  success *= (PaL->Sa - PaL->S)  > -1.e-6;
+  success *= (PaR->S  - PaR->Sa) > -1.e-6;
+  PaL->fail = !success;
+  return fun;
+}
+
+/* HLLD_GetAState, Src/RMHD/hlld.cpp:734-768: the state behind a fast wave */
+static void rhlld_astate(rhlld_state *Pa, double p, double Bx, int n)
+{
+  const int MXn = 1 + n, MXt = 1 + (n + 1)%3, MXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3, ENG = 7;
+  double vB, *ua = Pa->u, S = Pa->S, *R = Pa->R, scrh;
+  scrh = 1.0/(S - Pa->vx);
+  ua[RHO] =  R[RHO]*scrh;
+  ua[BXn] =  Bx;
+  ua[BXt] = (R[BXt] - Bx*Pa->vy)*scrh;
+  ua[BXb] = (R[BXb] - Bx*Pa->vz)*scrh;
+  vB     = Pa->vx*ua[BXn] + Pa->vy*ua[BXt] + Pa->vz*ua[BXb];
+  ua[ENG] = (R[ENG] + p*Pa->vx - vB*Bx)*scrh;
+  ua[MXn] = (ua[ENG] + p)*Pa->vx - vB*ua[BXn];
+  ua[MXt] = (ua[ENG] + p)*Pa->vy - vB*ua[BXt];
+  ua[MXb] = (ua[ENG] + p)*Pa->vz - vB*ua[BXb];
+}
+
+/* HLLD_GetCState, Src/RMHD/hlld.cpp:771-855: the state across the contact */
+static void rhlld_cstate(rhlld_state *PaL, rhlld_state *PaR, double p, double Bx, double *Uc, int n)
+{
+  const int MXn = 1 + n, MXt = 1 + (n + 1)%3, MXb = 1 + (n + 2)%3;
+  const int BXn = 4 + n, BXt = 4 + (n + 1)%3, BXb = 4 + (n + 2)%3, ENG = 7;
+  double dK, *ua, vxcL, vycL, vzcL, KLBc, vxcR, vycR, vzcR, KRBc;
+  double vxc, vyc, vzc, vBc, Bxc, Byc, Bzc, Sa, vxa, scrhL, scrhR;
+  rhlld_astate(PaL, p, Bx, n);
+  rhlld_astate(PaR, p, Bx, n);
+  dK = (PaR->Kx - PaL->Kx) + 1.e-12;
+  Bxc = Bx*dK;
+  Byc =   PaR->By*(PaR->Kx - PaR->vx)
+        - PaL->By*(PaL->Kx - PaL->vx)
+        + Bx*(PaR->vy - PaL->vy);
+  Bzc =   PaR->Bz*(PaR->Kx - PaR->vx)
+        - PaL->Bz*(PaL->Kx - PaL->vx)
+        + Bx*(PaR->vz - PaL->vz);
+  Bxc  = Bx;
+  Byc /= dK;
+  Bzc /= dK;
+  Uc[BXn] = Bxc;
+  Uc[BXt] = Byc;
+  Uc[BXb] = Bzc;
+  KLBc = PaL->Kx*Bxc + PaL->Ky*Byc + PaL->Kz*Bzc;
+  KRBc = PaR->Kx*Bxc + PaR->Ky*Byc + PaR->Kz*Bzc;
+  scrhL = (1.0 - PaL->K2)/(PaL->sw - KLBc);
+  scrhR = (1.0 - PaR->K2)/(PaR->sw - KRBc);
+  vxcL = PaL->Kx - Uc[BXn]*scrhL;                 /* Eq. [47] */
+  vxcR = PaR->Kx - Uc[BXn]*scrhR;
+  vycL = PaL->Ky - Uc[BXt]*scrhL;
+  vycR = PaR->Ky - Uc[BXt]*scrhR;
+  vzcL = PaL->Kz - Uc[BXb]*scrhL;
+  vzcR = PaR->Kz - Uc[BXb]*scrhR;
+  vxc = 0.5*(vxcL + vxcR);
+  vyc = 0.5*(vycL + vycR);
+  vzc = 0.5*(vzcL + vzcR);
+  if (vxc > 0.0) {
+    rhlld_astate(PaL, p, Bx, n);
+    ua = PaL->u; Sa = PaL->Sa; vxa = PaL->vx;
+  } else {
+    rhlld_astate(PaR, p, Bx, n);
+    ua = PaR->u; Sa = PaR->Sa; vxa = PaR->vx;
+  }
+  vBc = vxc*Uc[BXn] + vyc*Uc[BXt] + vzc*Uc[BXb];
+  Uc[RHO] = ua[RHO]*(Sa - vxa)/(Sa - vxc);                       /* Eq. [32] */
+  Uc[ENG] = (Sa*ua[ENG] - ua[MXn] + p*vxc - vBc*Bx)/(Sa - vxc);  /* Eq. [33] */
+  Uc[MXn] = (Uc[ENG] + p)*vxc - vBc*Bx;                          /* Eq. [34] */
+  Uc[MXt] = (Uc[ENG] + p)*vyc - vBc*Uc[BXt];
+  Uc[MXb] = (Uc[ENG] + p)*vzc - vBc*Uc[BXb];
+}
+
+/* HLLD_TotalPressure, Src/RMHD/hlld.cpp:857-872 */
+static double rhlld_total_pressure(const double *v)
+{
+  double Bmag2 = v[4]*v[4] + v[5]*v[5] + v[6]*v[6];
+  double vel2  = v[1]*v[1] + v[2]*v[2] + v[3]*v[3];
+  double vB    = v[1]*v[4] + v[2]*v[5] + v[3]*v[6];
+  return v[7] + 0.5*(Bmag2*(1.0 - vel2) + vB*vB);
+}
+
+static uint32_t rmhd_hlld(const double *vl, const double *vr, int n, const gplo_config *cfg,
+                          double *flux, double *pSl, double *pSr)
+{
+  const int MXn = 1 + n, BXn = 4 + n, NFLX = 8, ENG = 7;
+  const double gamma = cfg->gamma;
+  double ul[NVMAX], ur[NVMAX], fl[NVMAX], fr[NVMAX], pl, pr;
+  double a2L, a2R, hL, hR, sl_min, sl_max, sr_min, sr_max;
+  int nv;
+  uint32_t err = 0;
+  rmhd_prim_to_cons(vl, ul, gamma);
+  rmhd_prim_to_cons(vr, ur, gamma);
+  rmhd_sound_speed2(vl, gamma, &a2L, &hL);
+  rmhd_sound_speed2(vr, gamma, &a2R, &hR);
+  rmhd_flux(vl, ul, hL, n, fl, &pl);
+  rmhd_flux(vr, ur, hR, n, fr, &pr);
+  rmhd_max_signal_speed(vl, a2L, hL, n, &sl_min, &sl_max);
+  rmhd_max_signal_speed(vr, a2R, hR, n, &sr_min, &sr_max);
+  double Sl = MIN(sl_min, sr_min);
+  double Sr = MAX(sl_max, sr_max);
+  *pSl = Sl; *pSr = Sr;
+
+  if (Sl >= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv];
+  } else if (Sr <= 0.0) {
+    for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv];
+  } else {
+    int switch_to_hll = 0;
+    double Uhll[NVMAX], Fhll[NVMAX], Vhll[NVMAX], Uc[NVMAX];
+    double p0, f0, p, f, dp, Sc = 0.0, scrh;
+    double dS_1 = 1.0/(Sr - Sl);
+    for (nv = 0; nv < NFLX; nv++) {
+      Uhll[nv]  = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
+      Uhll[nv] *= dS_1;
+      Fhll[nv]  = Sr*fl[nv] - Sl*fr[nv] + Sl*Sr*(ur[nv] - ul[nv]);
+      Fhll[nv] *= dS_1;
+    }
+    rhlld_state PaL, PaR;
+    memset(&PaL, 0, sizeof(PaL));
+    memset(&PaR, 0, sizeof(PaR));
+    PaL.S = Sl;
+    PaR.S = Sr;
+    double Bx = Uhll[BXn];
+    for (nv = 0; nv < NFLX; nv++) {
+      PaL.R[nv] = Sl*ul[nv] - fl[nv];
+      PaR.R[nv] = Sr*ur[nv] - fr[nv];
+    }
+    PaL.R[ENG] += PaL.R[RHO];                     /* reduced energy */
+    PaR.R[ENG] += PaR.R[RHO];
+    /* initial guess: the B -> 0 limit, or the total pressure of the HLL state */
+    scrh = MAX(pl, pr);
+    if (Bx*Bx/scrh < 0.01) {
+      double a = Sr - Sl;
+      double b = PaR.R[ENG] - PaL.R[ENG] + Sr*PaL.R[MXn] - Sl*PaR.R[MXn];
+      double c = PaL.R[MXn]*PaR.R[ENG] - PaR.R[MXn]*PaL.R[ENG];
+      scrh = b*b - 4.0*a*c;
+      scrh = MAX(scrh, 0.0);
+      p0 = 0.5*(- b + sqrt(scrh))*dS_1;
+    } else {
+      uint32_t local_err = rmhd_cons_to_prim(Uhll, Vhll, cfg);
+      err |= local_err;
+      if (local_err) switch_to_hll = 1;
+      p0 = rhlld_total_pressure(Vhll);
+    }
+    f0 = rhlld_fstar(&PaL, &PaR, p0, Bx, &Sc, n);
+    if (isnan(f0) || PaL.fail) switch_to_hll = 1;
+    if (fabs(f0) > 1.e-12 && !switch_to_hll) {
+      p = 1.025*p0; f = f0;
+      for (int k1 = 1; k1 < 20; k1++) {           /* MAX_ITER, Src/RMHD/hlld.cpp:45 */
+        f = rhlld_fstar(&PaL, &PaR, p, Bx, &Sc, n);
+        if (f != f || PaL.fail || (k1 > 7) || (fabs(f) > fabs(f0) && k1 > 4)) {
+          switch_to_hll = 1;
+          break;
+        }
+        dp = (p - p0)/(f - f0)*f;
+        p0 = p; f0 = f;
+        p -= dp;
+        if (p < 0.0) p = 1.e-6;
+        if (fabs(dp) < 1.e-5*p || fabs(f) < 1.e-6) break;
+      }
+    } else p = p0;
+    if (PaL.fail) switch_to_hll = 1;
+    if (!switch_to_hll) {
+      if (PaL.Sa >= -1.e-6) {
+        rhlld_astate(&PaL, p, Bx, n);
+        PaL.u[ENG] -= PaL.u[RHO];
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fl[nv] + Sl*(PaL.u[nv] - ul[nv]);
+      } else if (PaR.Sa <= 1.e-6) {
+        rhlld_astate(&PaR, p, Bx, n);
+        PaR.u[ENG] -= PaR.u[RHO];
+        for (nv = 0; nv < NFLX; nv++) flux[nv] = fr[nv] + Sr*(PaR.u[nv] - ur[nv]);
+      } else {
+        rhlld_cstate(&PaL, &PaR, p, Bx, Uc, n);
+        if (Sc > 0.0) {
+          PaL.u[ENG] -= PaL.u[RHO];
+          Uc[ENG]    -= Uc[RHO];
+          for (nv = 0; nv < NFLX; nv++)
+            flux[nv] = fl[nv] + Sl*(PaL.u[nv] - ul[nv]) + PaL.Sa*(Uc[nv] - PaL.u[nv]);
+        } else {
+          PaR.u[ENG] -= PaR.u[RHO];
+          Uc[ENG]    -= Uc[RHO];
+          for (nv = 0; nv < NFLX; nv++)
+            flux[nv] = fr[nv] + Sr*(PaR.u[nv] - ur[nv]) + PaR.Sa*(Uc[nv] - PaR.u[nv]);
+        }
+      }
+    } else {
+      for (nv = 0; nv < NFLX; nv++) flux[nv] = Fhll[nv];
+    }
+  }
+  for (nv = 0; nv < NFLX; nv++) if (isnan(flux[nv])) err |= ERR_NAN | ERR_QUIT;
+  return err;
+}
+
 /* RMHD_EnergySolve, Src/RMHD/rmhd_energy_solve.cpp:35-209 (IDEAL, reduced
    energy, Newton-Raphson on W' = W - D, tolerance 1e-11, at most 50 passes) */
 static uint32_t rmhd_energy_solve(double *u, double *v, double gamma)
@@ -1367,7 +1645,8 @@ static uint32_t sweep(gplo_state *s, int dir, double dt)
       double Sl, Sr;
       const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
       if (is_rmhd(s)) {
-        if (s->c.solver == GPLO_HLLC) err |= rmhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        if (s->c.solver == GPLO_HLLC)      err |= rmhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+        else if (s->c.solver == GPLO_HLLD) err |= rmhd_hlld(vl, vr, dir, &s->c, flux + i*NVMAX, &Sl, &Sr);
         else err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
       } else if (mhd) {
         if (s->c.solver == GPLO_HLLD)      err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
@@ -1857,9 +2136,14 @@ uint32_t gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamm
 uint32_t gplo_face(int phys, int solver, const double *vl, const double *vr, int dir, double gamma,
                    double *flux, double *sl, double *sr)
 {
-  if (phys == GPLO_RMHD)
+  if (phys == GPLO_RMHD) {
+    gplo_config c;
+    memset(&c, 0, sizeof(c));
+    c.gamma = gamma; c.small_rho = 1.e-12; c.small_prs = 1.e-12;   /* Src/structs.hpp:407-408 */
+    if (solver == GPLO_HLLD) return rmhd_hlld(vl, vr, dir, &c, flux, sl, sr);
     return solver == GPLO_HLLC ? rmhd_hllc(vl, vr, dir, gamma, flux, sl, sr)
                                : rmhd_hll(vl, vr, dir, gamma, flux, sl, sr);
+  }
   if (solver == GPLO_HLLD) return mhd_hlld(vl, vr, dir, gamma, flux, sl, sr);
   if (solver == GPLO_HLLC) return mhd_hllc(vl, vr, dir, gamma, flux, sl, sr);
   return hll(vl, vr, dir, gamma, 1, flux, sl, sr);

@@ -63,6 +63,11 @@ CASES = [
     ("rmhd3d_12x10x14_rk2_mc_hllc", "rmhd3d", "RK2", "MC_LIM", (12, 10, 14), 21, 5, 1e-3, 0.4, "hllc", True, 4 / 3),
     ("rmhd3d_14x12x10_rk3_vl_hllc", "rmhd3d", "RK3", "VANLEER_LIM", (14, 12, 10), 22, 4, 1e-3, 0.4, "hllc", True, 4 / 3),
     ("rmhd3d_10x12x12_rk2_mc_hllc_outflow", "rmhd3d", "RK2", "MC_LIM", (10, 12, 12), 23, 5, 1e-3, 0.4, "hllc", False, 4 / 3),
+    # RMHD HLLD (Mignone, Ugliano & Bodo 2009, Src/RMHD/hlld.cpp): the secant
+    # iteration on the total pressure, the Alfven and contact states
+    ("rmhd3d_12x10x14_rk2_mc_hlld", "rmhd3d", "RK2", "MC_LIM", (12, 10, 14), 24, 5, 1e-3, 0.4, "hlld", True, 4 / 3),
+    ("rmhd3d_14x12x10_rk3_vl_hlld", "rmhd3d", "RK3", "VANLEER_LIM", (14, 12, 10), 25, 4, 1e-3, 0.4, "hlld", True, 4 / 3),
+    ("rmhd3d_10x12x12_rk2_mc_hlld_outflow", "rmhd3d", "RK2", "MC_LIM", (10, 12, 12), 26, 5, 1e-3, 0.4, "hlld", False, 4 / 3),
 ]
 
 # pluto.ini [Boundary] per side (X1-beg, X1-end, ...) where not all periodic /
@@ -97,7 +102,9 @@ RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.
            "rmhd3d_10x12x14_rk3_vl_hll": (5.0, 3.0), "rmhd3d_12x10x12_rk2_mc_hll_outflow": (2.0, 1.0),
            "rmhd3d_6x8x6_rk2_mc_hll_restart": (2.5, 1.0),
            "rmhd3d_12x10x14_rk2_mc_hllc": (3.0, 1.0), "rmhd3d_14x12x10_rk3_vl_hllc": (5.0, 3.0),
-           "rmhd3d_10x12x12_rk2_mc_hllc_outflow": (2.0, 0.5)}
+           "rmhd3d_10x12x12_rk2_mc_hllc_outflow": (2.0, 0.5),
+           "rmhd3d_12x10x14_rk2_mc_hlld": (3.0, 1.0), "rmhd3d_14x12x10_rk3_vl_hlld": (5.0, 3.0),
+           "rmhd3d_10x12x12_rk2_mc_hlld_outflow": (2.0, 0.5)}
 
 
 def _ic(name, cfg, n, seed, gamma):

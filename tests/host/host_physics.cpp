@@ -22,9 +22,10 @@ template <int N>
 static void face_n(int phys, int solver, const double *vl, const double *vr, double gamma,
                    FaceFlux<double> &ff)
 {
-  if (phys == 3) {                       // RMHD
-    if (solver == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
-    else             rmhd_hll<N>(vl, vr, gamma, ff);
+  if (phys == 3) {                       // RMHD (floors: the default 1e-12)
+    if (solver == 3)      rmhd_hlld<N>(vl, vr, gamma, 1.e-12, 1.e-12, ff);
+    else if (solver == 2) rmhd_hllc<N>(vl, vr, gamma, ff);
+    else                  rmhd_hll<N>(vl, vr, gamma, ff);
   } else if (solver == 3) {
     mhd_hlld<double, N>(vl, vr, gamma, ff);
   } else if (solver == 2) {

@@ -22,6 +22,15 @@ from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
 pytestmark = pytest.mark.gpu
 
 TOL = 5e-12
+# HLLD solves for the total pressure by a secant iteration whose stop test
+# (|dp| < 1e-5 p) and give-up tests see the quartic speeds: a last-bit
+# difference can end the iteration one pass earlier or later at a face, so the
+# differences grow further (measured 5.8e-12 on the Gamma_max 5, sigma 3
+# reference fixture and 4.0e-11, the largest variable, on the same IC as the
+# HLL case below, whose HLL and HLLC runs stay at <= 3e-14; the
+# host build of the same source is bitwise the oracle,
+# tests/test_host_physics.py)
+TOL_SOLVER = {"hll": TOL, "hllc": TOL, "hlld": 5e-11}
 
 
 def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3, periodic=True, solver="hll"):
@@ -30,16 +39,16 @@ def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3, periodic=True, solver="hll"):
                                     boundary="periodic" if periodic else "outflow"))
 
 
-def _compare(gv, gs, ov, osv, label):
+def _compare(gv, gs, ov, osv, label, tol=TOL):
     same = 0
     for v in range(8):
         den = np.sum(np.abs(ov[v]))
         l1 = np.sum(np.abs(gv[v] - ov[v])) / den if den > 0 else np.max(np.abs(gv[v] - ov[v]))
         same += int(np.count_nonzero(gv[v] == ov[v]))
-        assert l1 <= TOL, f"{label} var {v}: rel L1 {l1:.3e}"
+        assert l1 <= tol, f"{label} var {v}: rel L1 {l1:.3e}"
     for d in range(gs.shape[0]):
         l1 = np.sum(np.abs(gs[d] - osv[d])) / np.sum(np.abs(osv[d]))
-        assert l1 <= TOL, f"{label} face {d}: rel L1 {l1:.3e}"
+        assert l1 <= tol, f"{label} face {d}: rel L1 {l1:.3e}"
     worst = max(np.sum(np.abs(gv[v] - ov[v])) / max(np.sum(np.abs(ov[v])), 1e-300) for v in range(8))
     print(f"{label}: {same / (8 * ov[0].size):.4f} of primitive values bitwise, "
           f"max rel L1 {worst:.2e}", flush=True)
@@ -57,7 +66,7 @@ def test_gpu_matches_reference_fixture(name):
     g.close()
     assert err & (1 << 15) == 0 and (err == 0) == (not p["err_log"])
     assert abs(t - p["time"]) <= 1e-6 * abs(p["time"])
-    _compare(vc, vs, a["vc_out"], a["vs_out"], name)
+    _compare(vc, vs, a["vc_out"], a["vs_out"], name, TOL_SOLVER[p["solver"]])
 
 
 @pytest.mark.parametrize("n,seed,lmax,sigma,steps,ts,lim,solver", [
@@ -65,6 +74,8 @@ def test_gpu_matches_reference_fixture(name):
     ((20, 16, 28), 12, 5.0, 3.0, 6, "RK3", "VANLEER_LIM", "hll"),
     ((24, 20, 16), 13, 4.0, 2.0, 8, "RK2", "MC_LIM", "hllc"),
     ((20, 16, 28), 12, 5.0, 3.0, 6, "RK3", "VANLEER_LIM", "hllc"),   # the HLL case's IC
+    ((24, 24, 24), 11, 4.0, 2.0, 8, "RK2", "MC_LIM", "hlld"),
+    ((20, 16, 28), 12, 5.0, 3.0, 6, "RK3", "VANLEER_LIM", "hlld"),
 ])
 def test_gpu_matches_oracle(n, seed, lmax, sigma, steps, ts, lim, solver):
     import oracle
@@ -85,12 +96,10 @@ def test_gpu_matches_oracle(n, seed, lmax, sigma, steps, ts, lim, solver):
     # recovery branches a last-bit difference flips discrete choices
     assert oerr & (1 << 15) == 0
     assert abs(gt - ot) <= 1e-12 * ot and abs(gdt - odt) <= 1e-12 * odt
-    _compare(gv, gs, ov, osv, f"n={n}")
+    _compare(gv, gs, ov, osv, f"n={n} {solver}", TOL_SOLVER[solver])
 
 
 def test_rmhd_rejects_unsupported():
-    with pytest.raises(RuntimeError, match="not supported"):
-        StageUpdate(Definitions(physics="RMHD"), RuntimeSetup(n=(16, 16, 16), solver="hlld"))
     with pytest.raises(RuntimeError, match="not supported"):
         StageUpdate(Definitions(physics="RMHD"), RuntimeSetup(n=(16, 16, 16), solver="hll"),
                     precision="mixed")

@@ -69,13 +69,15 @@ def test_rmhd_interface_solve_bitwise(libs):
             assert np.array_equal(f1, f2, equal_nan=True) and np.array_equal(s1[:2], s2[:2]), k
 
 
-@pytest.mark.parametrize("phys,solver,gamma", [(3, 2, 4 / 3), (2, 3, 5 / 3), (2, 2, 5 / 3), (2, 1, 5 / 3)],
-                         ids=["rmhd-hllc", "mhd-hlld", "mhd-hllc", "mhd-hll"])
+@pytest.mark.parametrize("phys,solver,gamma", [(3, 3, 4 / 3), (3, 2, 4 / 3), (2, 3, 5 / 3), (2, 2, 5 / 3),
+                                                (2, 1, 5 / 3)],
+                         ids=["rmhd-hlld", "rmhd-hllc", "mhd-hlld", "mhd-hllc", "mhd-hll"])
 def test_interface_solvers_bitwise(libs, phys, solver, gamma):
     """Every MHD and RMHD interface solver of the device source against the
     oracle's, on random states that cover the zero normal field (the RMHD
-    HLLC degenerate branch), fast strongly magnetised pairs whose star
-    states fail (the HLL fallbacks) and wide density jumps."""
+    HLLC degenerate branch, the RMHD HLLD B -> 0 guess), fast strongly
+    magnetised pairs whose star states fail (the HLL fallbacks, the HLLD
+    secant iteration giving up) and wide density jumps."""
     hp, orc = libs
     rng = np.random.default_rng(11 + 3 * solver + phys)
     n = 3000
