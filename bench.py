@@ -280,6 +280,51 @@ def run_mixed512(seed: int, steps: int = 20):
     return res
 
 
+def run_dump(seed: int, n: int = 256, steps: int = 40):
+    """The cost of one dbl dump (data.NNNN.dbl, dbl.out, restart.out) in the
+    256^3 FP64 workload: `steps` steps with the dump after the first half,
+    written synchronously (gpl_write_dbl) or overlapped with the second half
+    (gpl_write_dbl_async, joined by gpl_wait_dbl at the end), against the
+    same steps without a dump (best of two runs each).  Files go to a
+    temporary directory."""
+    import shutil
+    import tempfile
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    vc, vs = ic.mhd_turbulence((n, n, n), seed)
+    g = StageUpdate(Definitions(), RuntimeSetup(n=(n, n, n), first_dt=5e-4))
+    g.upload(vc, vs)
+    g.run(3, 5e-4)                               # graph capture
+    tmp = tempfile.mkdtemp(prefix="gpl_dump_")
+    try:
+        g.write_dbl(tmp, 0)                      # staging buffers, first file
+        size = os.path.getsize(os.path.join(tmp, "data.0000.dbl"))
+        half = steps // 2
+
+        def timed(mode):
+            path = os.path.join(tmp, "data.0001.dbl")
+            if os.path.exists(path):
+                os.remove(path)                  # no writeback of an earlier dump in the timing
+            t0 = time.perf_counter()
+            g.run(half, 0.0)
+            if mode == "sync":
+                g.write_dbl(tmp, 1)
+            elif mode == "async":
+                g.write_dbl_async(tmp, 1)
+            g.run(steps - half, 0.0)
+            if mode == "async":
+                g.wait_dbl()
+            return (time.perf_counter() - t0) * 1e3
+
+        best = {m: min(timed(m) for _ in range(2)) for m in ("none", "sync", "async")}
+        base, sync, asyn = best["none"], best["sync"], best["async"]
+    finally:
+        g.close()
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"n": n, "steps": steps, "dump_bytes": size, "no_dump_ms": base, "sync_ms": sync,
+            "async_ms": asyn, "sync_exposed_ms": sync - base, "async_exposed_ms": asyn - base}
+
+
 def run_hd1d(steps: int = 400):
     """The 1D HD shock tube (4096 zones, outflow, gamma 1.4, PLM(MC) + HLLC +
     RK2, CFL 0.8): the reference's CPU-runnable case, timed on the GPU."""
@@ -314,6 +359,8 @@ def main():
                     help="skip the batched 2D, 1D HD and 3D RMHD workloads reported beside the headline")
     ap.add_argument("--mixed512", action="store_true",
                     help="also run the 512^3 FP64 vs mixed comparison (about 1.5 min of IC generation)")
+    ap.add_argument("--dump", action="store_true",
+                    help="also time one dbl dump in the 256^3 run, synchronous and overlapped")
     ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
                     help="run one step of the workload plus a 1 GiB device copy (k_copy_probe, the "
                          "HBM byte-counter calibration) for rocprofv3 --pmc, then exit")
@@ -378,6 +425,9 @@ def main():
         out["rmhd"] = run_rmhd(a.seed)
     if a.mixed512:
         out["mixed512"] = run_mixed512(a.seed)
+    if a.dump:
+        log("bench: dbl dump, synchronous and overlapped")
+        out["dump"] = run_dump(a.seed)
     if not a.no_cpu:
         log("bench: CPU baseline")
         try:

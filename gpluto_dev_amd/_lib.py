@@ -31,7 +31,8 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_batch_last_run_ms", "gpl_batch_device_bytes",
            "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
            "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics",
-           "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl")
+           "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl",
+           "gpl_write_dbl_async", "gpl_wait_dbl")
 
 
 class GplConfig(C.Structure):
@@ -75,6 +76,10 @@ def load() -> C.CDLL:
     L.gpl_write_dbl.restype = ip
     L.gpl_read_dbl.argtypes = [vp, C.c_char_p, ip]
     L.gpl_read_dbl.restype = ip
+    L.gpl_write_dbl_async.argtypes = [vp, C.c_char_p, ip, ip]
+    L.gpl_write_dbl_async.restype = ip
+    L.gpl_wait_dbl.argtypes = [vp]
+    L.gpl_wait_dbl.restype = ip
     L.gpl_advance_step.argtypes = [vp, C.c_double, dp]
     L.gpl_advance_step.restype = C.c_uint32
     L.gpl_next_time_step.argtypes = [vp, C.c_double, C.c_double]

@@ -35,6 +35,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("mhd3d_common.hip", []),
            ("diagnostics.hip", []),
            ("mhd3d_boundary.hip", []),
+           ("dump_gather.hip", []),
            ("hd_kernels.hip", []),
            ("batch2d_fp64.hip", []),
            ("batch2d_mixed.hip", MIXED),

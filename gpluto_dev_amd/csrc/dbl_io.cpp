@@ -112,8 +112,12 @@ int dbl_write(const char *dir, int nfile, const DumpStep &st, const std::vector<
 {
   if (!dir || nfile < 0 || (!data && count)) return GPL_EINVAL;
 
-  // 1. the dump: a fresh file per nfile (FileDelete, Src/write_data.cpp:141)
-  FILE *f = fopen(join(dir, dump_name(nfile)).c_str(), "wb");
+  // 1. the dump: a fresh file per nfile, the old one deleted first
+  //    (FileDelete, Src/write_data.cpp:141), which also spares the kernel
+  //    writing back the old file's pages
+  const std::string path = join(dir, dump_name(nfile));
+  std::remove(path.c_str());
+  FILE *f = fopen(path.c_str(), "wb");
   if (!f) return GPL_EIO;
   const size_t w = count ? fwrite(data, sizeof(double), count, f) : 0;
   if (fclose(f) != 0 || w != count) return GPL_EIO;

@@ -46,6 +46,10 @@ public:
   // (min[nvar], max[nvar], max Mach, max |div B|), on the context stream.
   virtual int diagnostics(double *out) = 0;
   virtual int nvar() const = 0;
+  // Enqueue on stream s the image of a dbl dump of the current state into
+  // the device buffer dst: the active zones of each primitive, then (MHD,
+  // RMHD) each staggered component with faces 0..n along its direction.
+  virtual int dump_gather(double *dst, hipStream_t s) = 0;
   size_t bytes = 0;   // device memory held
 };
 

@@ -83,6 +83,13 @@ public:
     return 0;
   }
 
+  // the primitives are stored active-only, i fastest: the dump image itself
+  int dump_gather(double *dst, hipStream_t s) override
+  {
+    return hipMemcpyAsync(dst, cur_v, 5*n*sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess
+               ? 0 : GPL_EHIP;
+  }
+
   int download(double *vc, double *const *, double *uc, int layout) override
   {
     const Layout L = make_layout(cfg, layout);

@@ -215,6 +215,15 @@ public:
     return 0;
   }
 
+  int dump_gather(double *dst, hipStream_t s) override
+  {
+    int per[3];
+    for (int d = 0; d < 3; d++) per[d] = side[2*d] == GPL_PERIODIC;
+    if (fp64) mhd3d_dump_gather<double>(g, na, per, (const double *)V, vs[cur], dst, s);
+    else      mhd3d_dump_gather<float>(g, na, per, (const float *)V, vs[cur], dst, s);
+    return hipGetLastError() == hipSuccess ? 0 : GPL_EHIP;
+  }
+
   int download(double *vc, double *const *vsh, double *uc, int layout) override
   {
     const Layout L = make_layout(cfg, layout);

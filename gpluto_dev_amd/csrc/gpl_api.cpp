@@ -16,6 +16,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace gpl;
@@ -91,6 +92,16 @@ struct gpl_ctx {
   Timing timing;
   hipEvent_t run_beg = nullptr, run_end = nullptr;
   double last_run_ms = 0.0;
+  // dbl dumps: the state's file image is gathered on the context stream into
+  // dump_dev (followed by the step-control block), copied to pinned host
+  // memory on dump_stream and written to disk by dump_thread
+  hipStream_t dump_stream = nullptr;
+  hipEvent_t dump_snap = nullptr, dump_copied = nullptr;
+  double *dump_dev = nullptr, *dump_host = nullptr;
+  size_t dump_cap = 0;              // doubles
+  std::thread dump_thread;
+  bool dump_inflight = false;
+  int dump_rc = 0;
 };
 
 extern "C" void gpl_default_config(gpl_config *cfg)
@@ -117,6 +128,12 @@ extern "C" void gpl_default_config(gpl_config *cfg)
 
 static void free_all(gpl_ctx *c)
 {
+  if (c->dump_inflight) { c->dump_thread.join(); c->dump_inflight = false; }
+  if (c->dump_dev) hipFree(c->dump_dev);
+  if (c->dump_host) hipHostFree(c->dump_host);
+  if (c->dump_snap) hipEventDestroy(c->dump_snap);
+  if (c->dump_copied) hipEventDestroy(c->dump_copied);
+  if (c->dump_stream) hipStreamDestroy(c->dump_stream);
   delete c->engine;
   c->engine = nullptr;
   if (c->err) hipFree(c->err);
@@ -270,64 +287,110 @@ size_t face_array_size(const Layout &A, int d)
   return (size_t)(A.tx + (d == 0))*(A.ty + (d == 1))*(A.tz + (d == 2));
 }
 
-}  // namespace
-
-extern "C" int gpl_write_dbl(gpl_ctx *c, const char *dir, int nfile, int nstep)
+// doubles in a dump of this configuration
+size_t dump_count(const gpl_config &g, int nv)
 {
-  if (!c || !dir || nfile < 0) return GPL_EINVAL;
-  hipSetDevice(c->dev);
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
-  const gpl_config &g = c->cfg;
-  const bool mhd = g.physics != GPL_HD;
-  const size_t na = (size_t)g.nx*g.ny*g.nz;
-  const int nv = c->engine->nvar();
-  size_t count = (size_t)nv*na;
-  if (mhd)
+  size_t count = (size_t)nv*g.nx*g.ny*g.nz;
+  if (g.physics != GPL_HD)
     for (int d = 0; d < g.dimensions; d++) {
       int e[3];
       face_dims(g, d, e);
       count += (size_t)e[0]*e[1]*e[2];
     }
-  std::vector<double> buf(count);
-  // the cell-centred blocks are the active layout itself
-  if (int rc = c->engine->download(buf.data(), nullptr, nullptr, GPL_LAYOUT_ACTIVE)) return rc;
-  if (mhd) {
-    // faces 0..n along d: the reference's array layout holds the upper face
-    const Layout A = make_layout(g, GPL_LAYOUT_ARY4D);
-    std::vector<std::vector<double>> fa(3);
-    double *ptr[3] = {nullptr, nullptr, nullptr};
-    for (int d = 0; d < g.dimensions; d++) { fa[d].assign(face_array_size(A, d), 0.0); ptr[d] = fa[d].data(); }
-    if (int rc = c->engine->download(nullptr, ptr, nullptr, GPL_LAYOUT_ARY4D)) return rc;
-    size_t o = (size_t)nv*na;
-    for (int d = 0; d < g.dimensions; d++) {
-      int e[3];
-      face_dims(g, d, e);
-      for (int k = 0; k < e[2]; k++)
-        for (int j = 0; j < e[1]; j++)
-          for (int i = 0; i < e[0]; i++) buf[o++] = fa[d][A.face(d, i, j, k)];
-    }
+  return count;
+}
+
+constexpr size_t kCtlDoubles = sizeof(StepCtl)/sizeof(double);
+static_assert(sizeof(StepCtl) % sizeof(double) == 0, "StepCtl packs into doubles");
+
+}  // namespace
+
+extern "C" int gpl_wait_dbl(gpl_ctx *c)
+{
+  if (!c) return GPL_EINVAL;
+  if (!c->dump_inflight) return 0;
+  c->dump_thread.join();
+  c->dump_inflight = false;
+  const int rc = c->dump_rc;
+  c->dump_rc = 0;
+  return rc;
+}
+
+extern "C" int gpl_write_dbl_async(gpl_ctx *c, const char *dir, int nfile, int nstep)
+{
+  if (!c || !dir || nfile < 0) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  // one dump in flight: the previous one finishes (and reports) first
+  if (int rc = gpl_wait_dbl(c)) return rc;
+  const size_t count = dump_count(c->cfg, c->engine->nvar());
+  const size_t need = count + kCtlDoubles;
+  if (!c->dump_stream) {
+    if (hipStreamCreateWithFlags(&c->dump_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->dump_snap, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->dump_copied, hipEventDisableTiming) != hipSuccess)
+      return GPL_EHIP;
   }
-  StepCtl h;
-  if (hipMemcpy(&h, c->ctl, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return GPL_EHIP;
-  const DumpStep st = {h.time, h.dt, nstep >= 0 ? nstep : (int)h.nstep};
-  return dbl_write(dir, nfile, st, dump_names(g), buf.data(), count);
+  if (need > c->dump_cap) {
+    if (c->dump_dev) { hipFree(c->dump_dev); c->bytes -= c->dump_cap*sizeof(double); }
+    if (c->dump_host) hipHostFree(c->dump_host);
+    c->dump_dev = c->dump_host = nullptr;
+    c->dump_cap = 0;
+    if (hipMalloc((void **)&c->dump_dev, need*sizeof(double)) != hipSuccess) {
+      c->dump_dev = nullptr;
+      return GPL_ENOMEM;
+    }
+    if (hipHostMalloc((void **)&c->dump_host, need*sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      c->dump_host = nullptr;
+      hipFree(c->dump_dev);
+      c->dump_dev = nullptr;
+      return GPL_ENOMEM;
+    }
+    c->dump_cap = need;
+    c->bytes += need*sizeof(double);
+  }
+  // snapshot on the context stream, behind the steps already enqueued: the
+  // file image, then the step-control block (time, dt, step counter)
+  if (int rc = c->engine->dump_gather(c->dump_dev, c->stream)) return rc;
+  if (hipMemcpyAsync(c->dump_dev + count, c->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice,
+                     c->stream) != hipSuccess ||
+      hipEventRecord(c->dump_snap, c->stream) != hipSuccess ||
+      hipStreamWaitEvent(c->dump_stream, c->dump_snap, 0) != hipSuccess ||
+      hipMemcpyAsync(c->dump_host, c->dump_dev, need*sizeof(double), hipMemcpyDeviceToHost,
+                     c->dump_stream) != hipSuccess ||
+      hipEventRecord(c->dump_copied, c->dump_stream) != hipSuccess)
+    return GPL_EHIP;
+  const std::string d(dir);
+  std::vector<std::string> names = dump_names(c->cfg);
+  const int dev = c->dev;
+  c->dump_rc = 0;
+  c->dump_inflight = true;
+  c->dump_thread = std::thread([c, d, nfile, nstep, count, dev, names]() {
+    hipSetDevice(dev);
+    if (hipEventSynchronize(c->dump_copied) != hipSuccess) { c->dump_rc = GPL_EHIP; return; }
+    StepCtl h;
+    memcpy(&h, c->dump_host + count, sizeof(h));
+    const DumpStep st = {h.time, h.dt, nstep >= 0 ? nstep : (int)h.nstep};
+    c->dump_rc = dbl_write(d.c_str(), nfile, st, names, c->dump_host, count);
+  });
+  return 0;
+}
+
+extern "C" int gpl_write_dbl(gpl_ctx *c, const char *dir, int nfile, int nstep)
+{
+  if (int rc = gpl_write_dbl_async(c, dir, nfile, nstep)) return rc;
+  return gpl_wait_dbl(c);
 }
 
 extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
 {
   if (!c || !dir || nfile < 0) return GPL_EINVAL;
   hipSetDevice(c->dev);
+  if (int rc = gpl_wait_dbl(c)) return rc;      // a dump in flight may be writing these files
   const gpl_config &g = c->cfg;
   const bool mhd = g.physics != GPL_HD;
   const size_t na = (size_t)g.nx*g.ny*g.nz;
   const int nv = c->engine->nvar();
-  size_t count = (size_t)nv*na;
-  if (mhd)
-    for (int d = 0; d < g.dimensions; d++) {
-      int e[3];
-      face_dims(g, d, e);
-      count += (size_t)e[0]*e[1]*e[2];
-    }
+  const size_t count = dump_count(g, nv);
   DumpStep st;
   std::vector<double> buf;
   if (int rc = dbl_read(dir, nfile, dump_names(g), count, &st, buf)) return rc;

@@ -54,6 +54,12 @@ void mhd3d_launch(int which, const Mhd3dBufs<R> &b, const Box3 &g, const StepCtl
 
 // Ghost-zone fill of a padded box (non-periodic sides), side by side;
 // type[s] = GPL_PERIODIC, GPL_OUTFLOW or GPL_REFLECTIVE for X1-beg, ..., X3-end.
+// dbl dump image of the box (active cells, faces 0..n along each direction;
+// the upper face of a periodic direction is the image of face 0) into dst
+template <typename R>
+void mhd3d_dump_gather(const Box3 &g, const int na[3], const int periodic[3], const R *V,
+                       const double *Vs, double *dst, hipStream_t s);
+
 template <typename R>
 void mhd3d_boundary(const Box3 &g, const int type[6], R *V, double *Vs, hipStream_t s);
 

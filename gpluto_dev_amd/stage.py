@@ -217,6 +217,15 @@ class StageUpdate:
         _lib.check(self._L.gpl_write_dbl(self._h, os.fsencode(directory), nfile,
                                          -1 if nstep is None else nstep), "gpl_write_dbl")
 
+    def write_dbl_async(self, directory: str, nfile: int, nstep: int | None = None):
+        """write_dbl overlapped with later calls: the state is snapshot on the
+        device now and written by a host thread; wait_dbl() joins it."""
+        _lib.check(self._L.gpl_write_dbl_async(self._h, os.fsencode(directory), nfile,
+                                               -1 if nstep is None else nstep), "gpl_write_dbl_async")
+
+    def wait_dbl(self):
+        _lib.check(self._L.gpl_wait_dbl(self._h), "gpl_wait_dbl")
+
     def read_dbl(self, directory: str, nfile: int):
         """RestartFromFile: the state, time, dt and step counter of dump nfile."""
         _lib.check(self._L.gpl_read_dbl(self._h, os.fsencode(directory), nfile), "gpl_read_dbl")

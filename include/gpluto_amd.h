@@ -130,6 +130,19 @@ int gpl_get_step(gpl_ctx *ctx, double *time, double *dt, long long *nstep);
  * Src/main.cpp:336,482,497).  Returns GPL_EIO on a file error. */
 int gpl_write_dbl(gpl_ctx *ctx, const char *dir, int nfile, int nstep);
 
+/* gpl_write_dbl without the wait, for dumps that overlap the computation
+ * (the "async download and overlap with compute" of the output path; the
+ * reference writes synchronously, Src/main.cpp:385-387): the state's file
+ * image and the step state are gathered on the device behind the work already
+ * enqueued, copied to pinned host memory on a second stream and written by a
+ * host thread, while later calls (gpl_run ...) proceed.  One dump is in flight
+ * per context: the next gpl_write_dbl_async, gpl_read_dbl or gpl_destroy waits
+ * for it.  The device and pinned staging buffers (one dump's size) stay
+ * allocated until gpl_destroy. */
+int gpl_write_dbl_async(gpl_ctx *ctx, const char *dir, int nfile, int nstep);
+/* Wait for the dump in flight, if any; returns its result (0 if none). */
+int gpl_wait_dbl(gpl_ctx *ctx);
+
 /* RestartFromFile for the dbl output (Src/restart.cpp:34-274, RestartGet
  * :342-406): reads the restart.out record of dump nfile (time, dt, step
  * counter), checks line nfile of dbl.out (single_file, endianness, the

@@ -260,3 +260,47 @@ def test_restart_rejects_foreign_or_missing_dumps(tmp_path):
         hd.read_dbl(str(tmp_path), 0)                # MHD variables in an HD context
     hd.close()
     g.close()
+
+
+@pytest.mark.parametrize("kind", ["mhd", "mhd-sides", "mhd-mixed", "hd"])
+def test_async_dump_snapshots_before_later_steps(kind, tmp_path):
+    """gpl_write_dbl_async takes its snapshot behind the work already
+    enqueued and writes while the next steps run: the files equal a
+    synchronous dump of the same state, not the state the run reaches."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    print(f"async dump {kind}", flush=True)
+    if kind == "hd":
+        n = (4096, 1, 1)
+        g = StageUpdate(Definitions(physics="HD", dimensions=1),
+                        RuntimeSetup(n=n, solver="hllc", cfl=0.8, gamma=1.4, boundary="outflow"))
+        g.upload(ic.hd_riemann(n[0], 3))
+        mhd, first_dt = False, 1e-4
+    else:
+        n = (48, 40, 36)
+        sides = ("outflow", "reflective", "periodic", "periodic", "reflective", "outflow") \
+            if kind == "mhd-sides" else ()
+        g = StageUpdate(Definitions(), RuntimeSetup(n=n, first_dt=1e-3, sides=sides),
+                        precision="mixed" if kind == "mhd-mixed" else "fp64")
+        g.upload(*ic.mhd_turbulence(n, 8))
+        mhd, first_dt = True, 1e-3
+    g.run(3, first_dt)
+    g.write_dbl(str(tmp_path), 0)
+    g.write_dbl_async(str(tmp_path), 1)
+    err, _, _ = g.run(5, 0.0)
+    g.wait_dbl()
+    assert err == 0
+    assert _read(tmp_path / "data.0001.dbl") == _read(tmp_path / "data.0000.dbl")
+    lines = _read(tmp_path / "dbl.out").decode().splitlines()
+    assert lines[1].split()[1:] == lines[0].split()[1:]          # same t, dt, step, names
+    g.write_dbl(str(tmp_path), 2)
+    vc, vs, _ = g.download()
+    up = g.upper_faces() if mhd else None
+    g.close()
+    fvc, faces = dbl_io.read_dbl(str(tmp_path), 2, n, mhd, 3 if mhd else 1)
+    assert np.array_equal(fvc, vc)
+    if mhd:
+        for d in range(3):
+            assert np.array_equal(np.take(faces[d], range(n[d]), axis=2 - d), vs[d])
+            assert np.array_equal(np.take(faces[d], [n[d]], axis=2 - d), up[d])
+    assert dbl_io.read_restart(str(tmp_path), 2)["nstep"] == 8
