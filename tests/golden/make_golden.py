@@ -53,6 +53,14 @@ CASES = [
     ("mhd3d_10x12x12_rk3_vl_mixed_sides", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 12), 10, 6, 2e-3, 0.4, "hlld", False, 5 / 3),
     ("mhd2d_28x20_rk2_mc_walls", "mhd2d", "RK2", "MC_LIM", (28, 20, 1), 11, 10, 3e-3, 0.4, "hlld", False, 5 / 3),
     ("hd1d_400_rk3_vl_hll_wall_out", "hd1d", "RK3", "VANLEER_LIM", (400, 1, 1), 8, 600, 1e-4, 0.8, "hll", False, 1.4),
+    # HD in 2D and 3D: the sweep's index cycling in y and z, blast boxes with
+    # periodic, outflow, reflective and mixed sides
+    ("hd3d_16x12x10_rk2_mc_hllc", "hd3d", "RK2", "MC_LIM", (16, 12, 10), 30, 12, 4e-3, 0.4, "hllc", True, 1.4),
+    ("hd3d_12x10x14_rk3_vl_hll_walls", "hd3d", "RK3", "VANLEER_LIM", (12, 10, 14), 31, 12, 4e-3, 0.4, "hll", False, 1.4),
+    ("hd3d_10x14x12_rk2_mc_hllc_mixed_sides", "hd3d", "RK2", "MC_LIM", (10, 14, 12), 32, 12, 4e-3, 0.4, "hllc", False, 1.4),
+    ("hd2d_40x32_rk2_mc_hllc_outflow", "hd2d", "RK2", "MC_LIM", (40, 32, 1), 33, 30, 1e-3, 0.4, "hllc", False, 1.4),
+    ("hd2d_36x28_rk3_vl_hllc_walls", "hd2d", "RK3", "VANLEER_LIM", (36, 28, 1), 34, 30, 1e-3, 0.4, "hllc", False, 1.4),
+    ("hd2d_33x20_rk2_mc_hll", "hd2d", "RK2", "MC_LIM", (33, 20, 1), 35, 25, 1e-3, 0.4, "hll", True, 1.4),
     # relativistic MHD (HLL with exact quartic speeds, energy-solve con2prim);
     # the first case drives cells into the pressure-fix branch
     ("rmhd3d_16_rk2_mc_hll", "rmhd3d", "RK2", "MC_LIM", (16, 16, 16), 3, 5, 1e-3, 0.4, "hll", True, 4 / 3),
@@ -79,6 +87,10 @@ SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow"
          "mhd3d_10x12x12_rk3_vl_mixed_sides": ("reflective", "reflective", "outflow", "outflow",
                                                "periodic", "periodic"),
          "mhd2d_28x20_rk2_mc_walls": ("reflective",) * 4 + ("outflow",) * 2,
+         "hd3d_12x10x14_rk3_vl_hll_walls": ("reflective",) * 6,
+         "hd3d_10x14x12_rk2_mc_hllc_mixed_sides": ("reflective", "outflow", "periodic", "periodic",
+                                                   "outflow", "reflective"),
+         "hd2d_36x28_rk3_vl_hllc_walls": ("reflective",) * 4 + ("outflow",) * 2,
          "mhd2d_12x10_rk2_mc_walls_restart": ("reflective",) * 4 + ("outflow",) * 2,
          "hd1d_256_rk2_mc_hllc_walls_restart": ("reflective", "reflective") + ("outflow",) * 4}
 
@@ -115,8 +127,10 @@ def _ic(name, cfg, n, seed, gamma):
         extra = dict(lorentz_max=lmax, sigma=sigma)
     elif cfg.startswith("mhd"):
         vc, vs = ic.mhd_turbulence(n, seed, gamma=gamma)
-    else:
+    elif cfg == "hd1d":
         vc, vs = ic.hd_riemann(n[0], seed), None
+    else:
+        vc, vs = ic.hd_blast(n, seed, gamma), None
     return vc, vs, extra
 
 
@@ -152,15 +166,7 @@ def make_restart(case):
 
 def make(case):
     name, cfg, ts, lim, n, seed, steps, first_dt, cfl, solver, periodic, gamma = case
-    extra = {}
-    if cfg.startswith("rmhd"):
-        lmax, sigma = RMHD_IC[name]
-        vc, vs = ic.rmhd_turbulence(n, seed, gamma=gamma, lorentz_max=lmax, sigma=sigma)
-        extra = dict(lorentz_max=lmax, sigma=sigma)
-    elif cfg.startswith("mhd"):
-        vc, vs = ic.mhd_turbulence(n, seed, gamma=gamma)
-    else:
-        vc, vs = ic.hd_riemann(n[0], seed), None
+    vc, vs, extra = _ic(name, cfg, n, seed, gamma)
     sides = SIDES.get(name)
     if sides:
         extra["sides"] = list(sides)

@@ -1,7 +1,7 @@
 """GPU parity of the HD stage update (PLM + HLLC/HLL + RK2/RK3, periodic,
 outflow or reflective sides, 1D/2D/3D), FP64, bit for bit:
 
-* against the reference CPU build's own outputs (tests/golden/hd1d_*.npz,
+* against the reference CPU build's own outputs (tests/golden/hd[123]d_*.npz,
   among them the 4096-zone, 400-step shock tube);
 * against the oracle (oracle/gpl_oracle.c, itself pinned bitwise to the
   reference build) on seeded blast boxes in every dimension and boundary.
@@ -24,7 +24,7 @@ def _stage(n, ts, lim, solver, periodic, cfl, gamma=1.4, first_dt=1e-3, sides=No
                                     sides=tuple(sides) if sides else ()))
 
 
-@pytest.mark.parametrize("name", cases("hd1d"))
+@pytest.mark.parametrize("name", cases("hd"))
 def test_gpu_matches_reference_fixture(name):
     p, a = load(name)
     print(f"case {name}", flush=True)
