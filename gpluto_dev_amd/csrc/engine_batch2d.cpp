@@ -5,6 +5,7 @@
 // that times the candidates on the run's own first steps: every candidate
 // computes the same bits, so tuning never changes a result.
 #include "../../include/gpluto_amd.h"
+#include "c_guard.hpp"
 #include "batch2d.hpp"
 #include "engine.hpp"
 #include "physics.hpp"
@@ -86,30 +87,11 @@ static void batch_free(gpl_batch *b)
   if (b->stream) hipStreamDestroy(b->stream);
 }
 
-extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny,
-                                gpl_batch **out)
+// Everything gpl_batch_create sets up after the validation; on failure the
+// caller frees what was allocated (batch_free takes a partly built batch).
+static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *nx, const int *ny,
+                      long long total)
 {
-  if (!cfg || !nx || !out || nbox < 1) return GPL_EINVAL;
-  if (!ny) ny = nx;
-  *out = nullptr;
-  if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
-  if (!all_periodic(*cfg) || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
-  if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLLC && cfg->solver != GPL_SOLVER_HLL)
-    return GPL_EUNSUPPORTED;
-  if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
-  if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
-  if (cfg->precision != GPL_FP64 && cfg->precision != GPL_MIXED) return GPL_EINVAL;
-  long long total = 0;
-  for (int q = 0; q < nbox; q++) {
-    if (nx[q] < 3 || nx[q] > 8192 || ny[q] < 3 || ny[q] > 8192) return GPL_EINVAL;
-    total += (long long)nx[q]*ny[q];
-  }
-  if (total > (1LL << 30)) return GPL_EINVAL;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return GPL_EHIP;
-
-  gpl_batch *b = new (std::nothrow) gpl_batch();
-  if (!b) return GPL_ENOMEM;
   b->cfg = *cfg;
   b->dev = cfg->device;
   b->fp64 = cfg->precision == GPL_FP64;
@@ -118,8 +100,6 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   if (hipSetDevice(b->dev) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->dev) != hipSuccess) {
-    batch_free(b);
-    delete b;
     return GPL_EHIP;
   }
   hipEventCreate(&b->ev0);
@@ -174,19 +154,45 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   rc |= dev_alloc(b->bytes, (void **)&b->cdt_part, (size_t)max_y*sizeof(double));
   rc |= dev_alloc(b->bytes, (void **)&b->err, sizeof(uint32_t));
   rc |= dev_alloc(b->bytes, (void **)&b->ctl, (size_t)nbox*sizeof(StepCtl));
+  if (rc) return rc == GPL_EHIP ? GPL_EHIP : GPL_ENOMEM;
+  std::vector<StepCtl> z(nbox, StepCtl{0.0, 0.0, 0.0, 0});
+  if (hipMemcpy(b->ctl, z.data(), nbox*sizeof(StepCtl), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->err, 0, sizeof(uint32_t)) != hipSuccess)
+    return GPL_EHIP;
+  b->cur = 0;
+  return 0;
+}
+
+extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny,
+                                gpl_batch **out)
+{
+  if (!cfg || !nx || !out || nbox < 1) return GPL_EINVAL;
+  if (!ny) ny = nx;
+  *out = nullptr;
+  if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
+  if (!all_periodic(*cfg) || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLLC && cfg->solver != GPL_SOLVER_HLL)
+    return GPL_EUNSUPPORTED;
+  if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
+  if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
+  if (cfg->precision != GPL_FP64 && cfg->precision != GPL_MIXED) return GPL_EINVAL;
+  long long total = 0;
+  for (int q = 0; q < nbox; q++) {
+    if (nx[q] < 3 || nx[q] > 8192 || ny[q] < 3 || ny[q] > 8192) return GPL_EINVAL;
+    total += (long long)nx[q]*ny[q];
+  }
+  if (total > (1LL << 30)) return GPL_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return GPL_EHIP;
+
+  gpl_batch *b = new (std::nothrow) gpl_batch();
+  if (!b) return GPL_ENOMEM;
+  const int rc = c_guard(GPL_ENOMEM, [&] { return batch_init(b, cfg, nbox, nx, ny, total); });
   if (rc) {
     batch_free(b);
     delete b;
-    return rc == GPL_EHIP ? GPL_EHIP : GPL_ENOMEM;
+    return rc;
   }
-  std::vector<StepCtl> z(nbox, StepCtl{0.0, 0.0, 0.0, 0});
-  if (hipMemcpy(b->ctl, z.data(), nbox*sizeof(StepCtl), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(b->err, 0, sizeof(uint32_t)) != hipSuccess) {
-    batch_free(b);
-    delete b;
-    return GPL_EHIP;
-  }
-  b->cur = 0;
   if (const char *e = getenv("GPL_BATCH_SHAPE")) {   // "march,bpc" fixes the launch shape
     int m = 16, p = 8;
     if (sscanf(e, "%d,%d", &m, &p) == 2) gpl_batch_set_tuning(b, m, p);
@@ -215,8 +221,8 @@ static Layout box_layout(const gpl_batch *b, int box, int layout)
   return make_layout(c, layout);
 }
 
-extern "C" int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const double *const *vs,
-                                int layout)
+static int gpl_batch_upload_impl(gpl_batch *b, int box, const double *vc, const double *const *vs,
+                                 int layout)
 {
   if (!b || box < 0 || box >= b->nbox || !vc || !vs || !vs[0] || !vs[1]) return GPL_EINVAL;
   if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
@@ -266,8 +272,14 @@ extern "C" int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const d
   return 0;
 }
 
-extern "C" int gpl_batch_download(gpl_batch *b, int box, double *vc, double *const *vs, double *uc,
-                                  int layout)
+extern "C" int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const double *const *vs,
+                                int layout)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_batch_upload_impl(b, box, vc, vs, layout); });
+}
+
+static int gpl_batch_download_impl(gpl_batch *b, int box, double *vc, double *const *vs, double *uc,
+                                   int layout)
 {
   if (!b || box < 0 || box >= b->nbox) return GPL_EINVAL;
   if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
@@ -312,6 +324,12 @@ extern "C" int gpl_batch_download(gpl_batch *b, int box, double *vc, double *con
     for (int i = 0; i < NX; i++) if (vs[1]) vs[1][L.face(1, i, NY, 0)] = hs[m + i];
   }
   return 0;
+}
+
+extern "C" int gpl_batch_download(gpl_batch *b, int box, double *vc, double *const *vs, double *uc,
+                                  int layout)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_batch_download_impl(b, box, vc, vs, uc, layout); });
 }
 
 // ---------------------------------------------------------------------------
@@ -424,8 +442,8 @@ static int autotune(gpl_batch *b, int *done)
 
 static constexpr int kTuneSteps = 12;
 
-extern "C" uint32_t gpl_batch_run(gpl_batch *b, int nsteps, const double *first_dt, double *time_out,
-                                  double *dt_out)
+static uint32_t gpl_batch_run_impl(gpl_batch *b, int nsteps, const double *first_dt, double *time_out,
+                                   double *dt_out)
 {
   if (!b || nsteps < 0) return GPL_STEP_API_ERROR;
   hipSetDevice(b->dev);
@@ -468,7 +486,14 @@ extern "C" uint32_t gpl_batch_run(gpl_batch *b, int nsteps, const double *first_
   return err;
 }
 
-extern "C" int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu)
+extern "C" uint32_t gpl_batch_run(gpl_batch *b, int nsteps, const double *first_dt, double *time_out,
+                                  double *dt_out)
+{
+  return c_guard<uint32_t>(GPL_STEP_API_ERROR, 
+                  [&] { return gpl_batch_run_impl(b, nsteps, first_dt, time_out, dt_out); });
+}
+
+static int gpl_batch_set_tuning_impl(gpl_batch *b, int march, int blocks_per_cu)
 {
   if (!b) return GPL_EINVAL;
   if (march == 0) { b->tune_pending = true; return 0; }
@@ -479,6 +504,11 @@ extern "C" int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu)
   b->march_idx = m; b->bpc_idx = p;
   b->tune_pending = false;
   return 0;
+}
+
+extern "C" int gpl_batch_set_tuning(gpl_batch *b, int march, int blocks_per_cu)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_batch_set_tuning_impl(b, march, blocks_per_cu); });
 }
 
 extern "C" int gpl_batch_get_tuning(const gpl_batch *b, int *march, int *blocks_per_cu, double *step_ms)

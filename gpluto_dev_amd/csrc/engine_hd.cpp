@@ -1,6 +1,7 @@
 // HD engine (1D/2D/3D, periodic, outflow or reflective sides, HLLC/HLL, RK2/RK3, FP64):
 // device state and AdvanceStep orchestration (Src/rk_step.cpp:88-720) over
 // the kernels of hd_kernels.hip.
+#include "c_guard.hpp"
 #include "diagnostics.hpp"
 #include "engine.hpp"
 #include "hd.hpp"
@@ -177,7 +178,7 @@ int make_hd_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   }
   HdEngine *e = new (std::nothrow) HdEngine();
   if (!e) return GPL_ENOMEM;
-  const int rc = e->init(cfg, env);
+  const int rc = c_guard(GPL_ENOMEM, [&] { return e->init(cfg, env); });
   if (rc) { delete e; return rc; }
   *out = e;
   return 0;

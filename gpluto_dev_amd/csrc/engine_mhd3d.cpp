@@ -1,6 +1,7 @@
 // 3D MHD engine (ideal MHD, or relativistic MHD with the same CT machinery):
 // device state and AdvanceStep orchestration (Src/rk_step.cpp:88-720) over
 // the kernels of mhd3d_kernels.hpp.
+#include "c_guard.hpp"
 #include "engine.hpp"
 #include "mhd3d.hpp"
 #include "c2p_net.hpp"
@@ -408,7 +409,7 @@ int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
   if ((long long)(cfg.nx + pad)*(cfg.ny + pad)*(cfg.nz + pad) >= (1LL << 29)) return GPL_EINVAL;
   Mhd3dEngine *e = new (std::nothrow) Mhd3dEngine();
   if (!e) return GPL_ENOMEM;
-  const int rc = e->init(cfg, env);
+  const int rc = c_guard(GPL_ENOMEM, [&] { return e->init(cfg, env); });
   if (rc) { delete e; return rc; }
   *out = e;
   return 0;

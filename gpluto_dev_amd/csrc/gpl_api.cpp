@@ -5,6 +5,7 @@
 // device, so a run of many steps replays one captured HIP graph without
 // host round trips), the error bits, and the per-kernel timing.
 #include "../../include/gpluto_amd.h"
+#include "c_guard.hpp"
 #include "dbl_io.hpp"
 #include "diagnostics.hpp"
 #include "engine.hpp"
@@ -209,7 +210,7 @@ extern "C" int gpl_destroy(gpl_ctx *c)
   return 0;
 }
 
-extern "C" int gpl_upload(gpl_ctx *c, const double *vc, const double *const *vs, int layout)
+static int gpl_upload_impl(gpl_ctx *c, const double *vc, const double *const *vs, int layout)
 {
   if (!c || !vc) return GPL_EINVAL;
   if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
@@ -222,13 +223,23 @@ extern "C" int gpl_upload(gpl_ctx *c, const double *vc, const double *const *vs,
   return 0;
 }
 
-extern "C" int gpl_download(gpl_ctx *c, double *vc, double *const *vs, double *uc, int layout)
+extern "C" int gpl_upload(gpl_ctx *c, const double *vc, const double *const *vs, int layout)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_upload_impl(c, vc, vs, layout); });
+}
+
+static int gpl_download_impl(gpl_ctx *c, double *vc, double *const *vs, double *uc, int layout)
 {
   if (!c) return GPL_EINVAL;
   if (layout != GPL_LAYOUT_ACTIVE && layout != GPL_LAYOUT_ARY4D) return GPL_EINVAL;
   hipSetDevice(c->dev);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
   return c->engine->download(vc, vs, uc, layout);
+}
+
+extern "C" int gpl_download(gpl_ctx *c, double *vc, double *const *vs, double *uc, int layout)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_download_impl(c, vc, vs, uc, layout); });
 }
 
 extern "C" int gpl_set_step(gpl_ctx *c, double time, double dt, long long nstep)
@@ -316,7 +327,7 @@ extern "C" int gpl_wait_dbl(gpl_ctx *c)
   return rc;
 }
 
-extern "C" int gpl_write_dbl_async(gpl_ctx *c, const char *dir, int nfile, int nstep)
+static int gpl_write_dbl_async_impl(gpl_ctx *c, const char *dir, int nfile, int nstep)
 {
   if (!c || !dir || nfile < 0) return GPL_EINVAL;
   hipSetDevice(c->dev);
@@ -375,13 +386,18 @@ extern "C" int gpl_write_dbl_async(gpl_ctx *c, const char *dir, int nfile, int n
   return 0;
 }
 
+extern "C" int gpl_write_dbl_async(gpl_ctx *c, const char *dir, int nfile, int nstep)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_write_dbl_async_impl(c, dir, nfile, nstep); });
+}
+
 extern "C" int gpl_write_dbl(gpl_ctx *c, const char *dir, int nfile, int nstep)
 {
   if (int rc = gpl_write_dbl_async(c, dir, nfile, nstep)) return rc;
   return gpl_wait_dbl(c);
 }
 
-extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
+static int gpl_read_dbl_impl(gpl_ctx *c, const char *dir, int nfile)
 {
   if (!c || !dir || nfile < 0) return GPL_EINVAL;
   hipSetDevice(c->dev);
@@ -433,10 +449,15 @@ extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
   return gpl_set_step(c, st.t, st.dt, st.nstep);
 }
 
+extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_read_dbl_impl(c, dir, nfile); });
+}
+
 // ---------------------------------------------------------------------------
 // stepping
 // ---------------------------------------------------------------------------
-extern "C" uint32_t gpl_advance_step(gpl_ctx *c, double dt, double *inv_dt_hyp)
+static uint32_t gpl_advance_step_impl(gpl_ctx *c, double dt, double *inv_dt_hyp)
 {
   if (!c) return GPL_STEP_API_ERROR;
   hipSetDevice(c->dev);
@@ -460,8 +481,14 @@ extern "C" uint32_t gpl_advance_step(gpl_ctx *c, double dt, double *inv_dt_hyp)
   return err;
 }
 
+extern "C" uint32_t gpl_advance_step(gpl_ctx *c, double dt, double *inv_dt_hyp)
+{
+  return c_guard<uint32_t>(GPL_STEP_API_ERROR, [&] { return gpl_advance_step_impl(c, dt, inv_dt_hyp); });
+}
+
 extern "C" double gpl_next_time_step(const gpl_ctx *c, double dt, double inv_dt_hyp)
 {
+  if (!c) return 0.0;
   const double dt_new = c->cfg.cfl/inv_dt_hyp;                       // next_time_step.cpp:80
   return dt_new <= c->cfg.cfl_max_var*dt ? dt_new : c->cfg.cfl_max_var*dt;   // :125 MIN()
 }
@@ -485,7 +512,7 @@ static int build_graph(gpl_ctx *c)
   return 0;
 }
 
-extern "C" uint32_t gpl_run(gpl_ctx *c, int nsteps, double first_dt, double *time_out, double *dt_out)
+static uint32_t gpl_run_impl(gpl_ctx *c, int nsteps, double first_dt, double *time_out, double *dt_out)
 {
   if (!c || nsteps < 0) return GPL_STEP_API_ERROR;
   hipSetDevice(c->dev);
@@ -528,7 +555,13 @@ extern "C" uint32_t gpl_run(gpl_ctx *c, int nsteps, double first_dt, double *tim
   return err;
 }
 
-extern "C" int gpl_set_c2p_network(gpl_ctx *c, const float *weights)
+extern "C" uint32_t gpl_run(gpl_ctx *c, int nsteps, double first_dt, double *time_out, double *dt_out)
+{
+  return c_guard<uint32_t>(GPL_STEP_API_ERROR, 
+                  [&] { return gpl_run_impl(c, nsteps, first_dt, time_out, dt_out); });
+}
+
+static int gpl_set_c2p_network_impl(gpl_ctx *c, const float *weights)
 {
   if (!c) return GPL_EINVAL;
   hipSetDevice(c->dev);
@@ -541,8 +574,13 @@ extern "C" int gpl_set_c2p_network(gpl_ctx *c, const float *weights)
   return rc;
 }
 
-extern "C" int gpl_diagnostics(gpl_ctx *c, double *vmin, double *vmax, double *mach_max,
-                               double *divb_max)
+extern "C" int gpl_set_c2p_network(gpl_ctx *c, const float *weights)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_set_c2p_network_impl(c, weights); });
+}
+
+static int gpl_diagnostics_impl(gpl_ctx *c, double *vmin, double *vmax, double *mach_max,
+                                double *divb_max)
 {
   if (!c) return GPL_EINVAL;
   hipSetDevice(c->dev);
@@ -558,7 +596,13 @@ extern "C" int gpl_diagnostics(gpl_ctx *c, double *vmin, double *vmax, double *m
   return 0;
 }
 
-extern "C" int gpl_set_timing(gpl_ctx *c, int enable)
+extern "C" int gpl_diagnostics(gpl_ctx *c, double *vmin, double *vmax, double *mach_max,
+                               double *divb_max)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_diagnostics_impl(c, vmin, vmax, mach_max, divb_max); });
+}
+
+static int gpl_set_timing_impl(gpl_ctx *c, int enable)
 {
   if (!c) return GPL_EINVAL;
   c->timing.on = enable != 0;
@@ -566,10 +610,15 @@ extern "C" int gpl_set_timing(gpl_ctx *c, int enable)
   return 0;
 }
 
+extern "C" int gpl_set_timing(gpl_ctx *c, int enable)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_set_timing_impl(c, enable); });
+}
+
 extern "C" int gpl_kernel_count(const gpl_ctx *) { return K_COUNT; }
 
-extern "C" int gpl_kernel_stats(gpl_ctx *c, int idx, const char **name, long long *launches,
-                                double *total_ms)
+static int gpl_kernel_stats_impl(gpl_ctx *c, int idx, const char **name, long long *launches,
+                                 double *total_ms)
 {
   if (!c || idx < 0 || idx >= K_COUNT) return GPL_EINVAL;
   c->timing.harvest();
@@ -577,6 +626,12 @@ extern "C" int gpl_kernel_stats(gpl_ctx *c, int idx, const char **name, long lon
   if (launches) *launches = c->timing.launches[idx];
   if (total_ms) *total_ms = c->timing.total_ms[idx];
   return 0;
+}
+
+extern "C" int gpl_kernel_stats(gpl_ctx *c, int idx, const char **name, long long *launches,
+                                double *total_ms)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_kernel_stats_impl(c, idx, name, launches, total_ms); });
 }
 
 extern "C" double gpl_last_run_ms(const gpl_ctx *c) { return c ? c->last_run_ms : 0.0; }
@@ -589,7 +644,7 @@ extern "C" const char *gpl_strerror(int code)
   case 0: return "ok";
   case GPL_EINVAL: return "invalid argument";
   case GPL_EHIP: return "HIP runtime error";
-  case GPL_ENOMEM: return "out of device memory";
+  case GPL_ENOMEM: return "out of device or host memory";
   case GPL_EUNSUPPORTED: return "configuration not supported";
   case GPL_EIO: return "dump or restart file error";
   default: return "unknown error";

@@ -57,7 +57,7 @@ extern "C" {
 /* API error codes */
 #define GPL_EINVAL       (-1)
 #define GPL_EHIP         (-2)
-#define GPL_ENOMEM       (-3)
+#define GPL_ENOMEM       (-3)   /* device memory, or host memory or threads for the host side */
 #define GPL_EUNSUPPORTED (-4)
 #define GPL_EIO          (-5)   /* a dump or restart file could not be opened, read or written */
 
@@ -160,7 +160,7 @@ int gpl_read_dbl(gpl_ctx *ctx, const char *dir, int nfile);
 uint32_t gpl_advance_step(gpl_ctx *ctx, double dt, double *inv_dt_hyp);
 
 /* NextTimeStep (Src/next_time_step.cpp:20-130) on one rank:
- * min(cfl/inv_dt_hyp, cfl_max_var*dt). */
+ * min(cfl/inv_dt_hyp, cfl_max_var*dt); 0 for a null context. */
 double gpl_next_time_step(const gpl_ctx *ctx, double dt, double inv_dt_hyp);
 
 /* The main-loop recurrence of Src/main.cpp:446-463 for nsteps steps with the

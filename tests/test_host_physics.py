@@ -17,8 +17,8 @@ LIB = os.path.join(ROOT, "tests", "_build", "libhost_physics.so")
 
 @pytest.fixture(scope="module")
 def libs():
-    if not os.path.exists(LIB):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "host")], check=True)
+    # make rebuilds the host library when physics.hpp is newer
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "host")], check=True)
     import oracle
     hp = C.CDLL(LIB)
     orc = oracle.lib()
