@@ -20,6 +20,7 @@ GPL_PERIODIC, GPL_OUTFLOW, GPL_REFLECTIVE = 1, 2, 3
 GPL_FP64, GPL_MIXED = 0, 1
 GPL_LAYOUT_ACTIVE, GPL_LAYOUT_ARY4D = 0, 1
 GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED, GPL_EIO = -1, -2, -3, -4, -5
+GPL_STEP_API_ERROR = 0x80000000     # stepping calls: the call itself failed
 
 # every symbol include/gpluto_amd.h declares
 EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl_download",
@@ -142,3 +143,13 @@ def load() -> C.CDLL:
 def check(rc: int, what: str):
     if rc != 0:
         raise RuntimeError(f"{what}: {load().gpl_strerror(rc).decode()} ({rc})")
+
+
+def check_step(err: int, what: str) -> int:
+    """Stepping calls return the OR of the ERR_* bits the numerics raised;
+    GPL_STEP_API_ERROR (bit 31, never a numerics bit) says the call itself
+    failed -- a HIP runtime error or host memory -- and raises here."""
+    err = int(err)
+    if err & GPL_STEP_API_ERROR:
+        raise RuntimeError(f"{what}: the call failed (HIP runtime error or out of host memory)")
+    return err

@@ -90,7 +90,7 @@ class BatchUpdate:
         err = self._L.gpl_batch_run(self._h, nsteps,
                                     None if fd is None else fd.ctypes.data_as(dp),
                                     t.ctypes.data_as(dp), dt.ctypes.data_as(dp))
-        return int(err), t, dt
+        return _lib.check_step(err, "gpl_batch_run"), t, dt
 
     def set_tuning(self, march: int, blocks_per_cu: int = 0):
         _lib.check(self._L.gpl_batch_set_tuning(self._h, march, blocks_per_cu),

@@ -233,16 +233,17 @@ class StageUpdate:
     # -- stepping -----------------------------------------------------------
     def advance_step(self, dt: float):
         inv = C.c_double(0.0)
-        err = self._L.gpl_advance_step(self._h, dt, C.byref(inv))
-        return int(err), inv.value
+        err = _lib.check_step(self._L.gpl_advance_step(self._h, dt, C.byref(inv)), "gpl_advance_step")
+        return err, inv.value
 
     def next_time_step(self, dt: float, inv_dt: float) -> float:
         return self._L.gpl_next_time_step(self._h, dt, inv_dt)
 
     def run(self, nsteps: int, first_dt: float):
         t, dt = C.c_double(0.0), C.c_double(0.0)
-        err = self._L.gpl_run(self._h, nsteps, first_dt, C.byref(t), C.byref(dt))
-        return int(err), t.value, dt.value
+        err = _lib.check_step(self._L.gpl_run(self._h, nsteps, first_dt, C.byref(t), C.byref(dt)),
+                              "gpl_run")
+        return err, t.value, dt.value
 
     def set_c2p_network(self, weights=None):
         """RMHD: learned con2prim guess (gpluto_dev_amd.c2p_net.train); None

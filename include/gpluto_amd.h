@@ -87,6 +87,11 @@ typedef struct gpl_config {
                               sides or on neither; outflow and reflective mix */
 } gpl_config;
 
+/* One domain on one device: its state, its stream and its step state.  Calls
+ * on one context come from one host thread at a time (they are not locked
+ * against each other); separate contexts, on one device or several, are
+ * independent.  The dump writer thread of gpl_write_dbl_async touches only
+ * its own staging buffer and files. */
 typedef struct gpl_ctx gpl_ctx;
 
 /* Fill *cfg with the reference defaults (3D MHD, HLLD, MC, RK2, UCT_HLL,

@@ -9,11 +9,12 @@ import subprocess
 
 import pytest
 
+from gpluto_dev_amd._lib import GPL_STEP_API_ERROR
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "tests", "_build", "libhost_guard.so")
 
 GPL_ENOMEM = -3
-GPL_STEP_API_ERROR = 0x80000000
 
 
 @pytest.fixture(scope="module")
@@ -53,3 +54,10 @@ def test_library_entry_points_reject_a_null_handle():
                L.gpl_write_dbl_async(None, b".", 0, -1), L.gpl_destroy(None)):
         assert rc == _lib.GPL_EINVAL
     assert L.gpl_strerror(GPL_ENOMEM) == b"out of device or host memory"
+
+
+def test_python_mirror_raises_on_a_failed_stepping_call():
+    from gpluto_dev_amd import _lib
+    assert _lib.check_step(0b101, "gpl_run") == 0b101          # numerics bits pass through
+    with pytest.raises(RuntimeError, match="gpl_run: the call failed"):
+        _lib.check_step(GPL_STEP_API_ERROR | 1, "gpl_run")
