@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .stage import _LIMITERS, _SOLVERS, _STEPPING, Definitions, RuntimeSetup
+from .stage import _EMF, _LIMITERS, _SOLVERS, _STEPPING, Definitions, RuntimeSetup
 
 
 class BatchUpdate:
@@ -32,7 +32,7 @@ class BatchUpdate:
         cfg.time_stepping = _STEPPING[defs.time_stepping]
         cfg.limiter = _LIMITERS[defs.limiter]
         cfg.solver = _SOLVERS[ini.solver]
-        cfg.ct_emf_average = _lib.GPL_UCT_HLL
+        cfg.ct_emf_average = _EMF[defs.ct_emf_average]
         cfg.precision = _lib.GPL_FP64 if precision == "fp64" else _lib.GPL_MIXED
         cfg.device = device
         cfg.gamma = ini.gamma

@@ -36,7 +36,8 @@ struct Batch2dBufs {
   const double *Vin; // [2][ncell] lower x- and y-faces at stage start
   double *Vout;      // [2][ncell] faces after the stage (a third buffer)
   const double *Bs0; // [2][ncell] faces at the start of the step (second stage on)
-  R *FD;             // [2][5][ncell] UCT face data et, eb, aL, aR, d
+  R *FD;             // [2][5][ncell] UCT face data et, eb, aL, aR, d; UCT_HLLD:
+                     // [2][6][ncell] et, eb, aL, aR, dL, dR
   R *E;              // [ncell] Ez at the lower-left corner of every cell
   R *dU;             // [6][ncell] x-sweep flux divergence
   R *Cdt;            // [ncell]    x-sweep C_dt
@@ -55,6 +56,7 @@ enum Batch2dPhase { B2_SWEEP_X = 0, B2_SWEEP_Y, B2_EMF, B2_FINISH };
 
 // Launch one phase as a persistent grid of 'blocks' workgroups of 256
 // threads; 'march' is the y-segment length the y items were built for.
+// solver: 1 HLL, 2 HLLC, 3 HLLD (UCT_HLL records), 4 HLLD with UCT_HLLD.
 template <typename R>
 void batch2d_launch(int phase, const Batch2dBufs<R> &b, RKW w, int first_stage, int want_dt,
                     int limiter, int solver, int march, int blocks, double gamma,

@@ -1,4 +1,5 @@
-// Batched 2D ideal-MHD stage update: PLM + HLLD/HLL + UCT_HLL CT + RK2/RK3
+// Batched 2D ideal-MHD stage update: PLM + HLLD/HLLC/HLL + UCT_HLL or UCT_HLLD CT
+// + RK2/RK3
 // on many independent periodic boxes of uneven sizes, for gfx950.
 //
 // Every phase is one launch over all boxes: a persistent grid whose
@@ -51,8 +52,38 @@ __device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
   Rr = q2 - R(0.5)*plm_slope<R, LIM>(q1, q2, q3);
 }
 
-// one UCT contribution (ct_emf_average.cpp:636-643), as in the 3D kernels
-template <typename R, int LIMV>
+// Face records per direction: et, eb, aL, aR, d for UCT_HLL (dL = dR), and
+// et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with CT_Store_UCTHlld,
+// Src/MHD/CT/ct_store.hpp:254-354).
+template <int SOLVER> __host__ __device__ constexpr int nrec() { return SOLVER == 4 ? 6 : 5; }
+
+template <typename R, int N, int SOLVER>
+__device__ __forceinline__ void store_rec(const R *vl, const R *vr, R gamma, const FaceFlux<R> &ff,
+                                          R *fd, int c, size_t n)
+{
+  if constexpr (SOLVER == 4) {
+    R et, eb, aL, aR, dL, dR;
+    uct_hlld_store<R, N>(vl, vr, gamma, ff.Sl, ff.Sr, et, eb, aL, aR, dL, dR);
+    fd[0*n + c] = et; fd[1*n + c] = eb; fd[2*n + c] = aL; fd[3*n + c] = aR;
+    fd[4*n + c] = dL; fd[5*n + c] = dR;
+  } else {
+    R et, eb, aL, aR, dd;
+    uct_hll_store<R, N>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
+    fd[0*n + c] = et; fd[1*n + c] = eb; fd[2*n + c] = aL; fd[3*n + c] = aR; fd[4*n + c] = dd;
+  }
+}
+
+template <typename R, int N, int SOLVER>
+__device__ __forceinline__ void solve(const R *vl, const R *vr, R gamma, FaceFlux<R> &ff)
+{
+  if constexpr (SOLVER == 3 || SOLVER == 4) mhd_hlld<R, N>(vl, vr, gamma, ff);
+  else if constexpr (SOLVER == 2) mhd_hllc<R, N>(vl, vr, gamma, ff);
+  else mhd_hll<R, N>(vl, vr, gamma, ff);
+}
+
+// one UCT contribution (ct_emf_average.cpp:636-643), as in the 3D kernels;
+// NQ = 4 coefficient records (aL, aR, dL, dR) for UCT_HLLD, 3 (dR = dL) for UCT_HLL
+template <typename R, int LIMV, int NQ>
 __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const R *fb, int sgn)
 {
   R bL, bR, vL, vR;
@@ -61,7 +92,8 @@ __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const 
   const R aL = R(0.5)*(fa[0] + fb[0]);
   const R aR = R(0.5)*(fa[1] + fb[1]);
   const R dL = R(0.5)*(fa[2] + fb[2]);
-  const R dR = dL;
+  R dR = dL;
+  if constexpr (NQ == 4) dR = R(0.5)*(fa[3] + fb[3]);
   const R phi = dR*bR - dL*bL;
   const R s = aL*vL*bL + aR*vR*bR;
   return sgn > 0 ? s + phi : s - phi;
@@ -104,16 +136,9 @@ void k2_sweepx(Batch2dBufs<R> b, R gamma, int want_cdt)
     }
     vl[4] = vr[4] = (R)b.Vin[c0];
     FaceFlux<R> ff;
-    if (SOLVER == 3)      mhd_hlld<R, 0>(vl, vr, gamma, ff);
-    else if (SOLVER == 2) mhd_hllc<R, 0>(vl, vr, gamma, ff);
-    else                  mhd_hll<R, 0>(vl, vr, gamma, ff);
+    b2::solve<R, 0, SOLVER>(vl, vr, gamma, ff);
     e |= ff.err;
-    if (lane < nc) {
-      R et, eb, aL, aR, dd;
-      uct_hll_store<R, 0>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-      R *fd = b.FD;
-      fd[0*n + c0] = et; fd[1*n + c0] = eb; fd[2*n + c0] = aL; fd[3*n + c0] = aR; fd[4*n + c0] = dd;
-    }
+    if (lane < nc) b2::store_rec<R, 0, SOLVER>(vl, vr, gamma, ff, b.FD, c0, n);
     R F[6], Fn[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) { F[q] = ff.f[b2::fidx(q)]; Fn[q] = __shfl_down(F[q], 1, 64); }
@@ -195,16 +220,9 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
         }
         vl[5] = vr[5] = (R)b.Vin[n + cf];
         FaceFlux<R> ff;
-        if (SOLVER == 3)      mhd_hlld<R, 1>(vl, vr, gamma, ff);
-        else if (SOLVER == 2) mhd_hllc<R, 1>(vl, vr, gamma, ff);
-        else                  mhd_hll<R, 1>(vl, vr, gamma, ff);
+        b2::solve<R, 1, SOLVER>(vl, vr, gamma, ff);
         e |= ff.err;
-        if (f < s1) {
-          R et, eb, aL, aR, dd;
-          uct_hll_store<R, 1>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-          R *fd = b.FD + 5*n;
-          fd[0*n + cf] = et; fd[1*n + cf] = eb; fd[2*n + cf] = aL; fd[3*n + cf] = aR; fd[4*n + cf] = dd;
-        }
+        if (f < s1) b2::store_rec<R, 1, SOLVER>(vl, vr, gamma, ff, b.FD + b2::nrec<SOLVER>()*n, cf, n);
         double F[6];
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] = (double)ff.f[b2::fidx(q)];
@@ -247,13 +265,13 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
 // ---------------------------------------------------------------------------
 // Ez at the lower-left corner of every cell: X1a + X2b
 // ---------------------------------------------------------------------------
-template <typename R, int LIMV>
+template <typename R, int LIMV, int NQ>
 __global__ __launch_bounds__(256)
 void k2_emf(Batch2dBufs<R> b)
 {
   const size_t n = (size_t)b.ncell;
   const double *Bx = b.Vin, *By = b.Vin + n;
-  const R *FDx = b.FD, *FDy = b.FD + 5*n;
+  const R *FDx = b.FD, *FDy = b.FD + (NQ + 2)*n;
   for (int gi = blockIdx.x; gi < b.ntiles; gi += gridDim.x) {
     const Item2 t2 = b.tiles[gi];
     const BoxDesc bx = b.box[t2.box];
@@ -266,9 +284,9 @@ void k2_emf(Batch2dBufs<R> b)
       xo[o] = bx.off + b2::wrapm(i + o - 2, NX) + NX*j;
       yo[o] = bx.off + i + NX*b2::wrapm(j + o - 2, NY);
     }
-    R cx[3], cy[3], cxj[3], cyi[3];
+    R cx[NQ], cy[NQ], cxj[NQ], cyi[NQ];
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
+    for (int s = 0; s < NQ; s++) {
       cx[s] = FDx[(2 + s)*n + c];  cy[s] = FDy[(2 + s)*n + c];
       cxj[s] = FDx[(2 + s)*n + yo[1]];
       cyi[s] = FDy[(2 + s)*n + xo[1]];
@@ -276,10 +294,10 @@ void k2_emf(Batch2dBufs<R> b)
     R q4[4], e4[4];
 #pragma unroll
     for (int o = 0; o < 4; o++) { q4[o] = (R)By[xo[o]]; e4[o] = FDy[1*n + xo[o]]; }
-    R ez = b2::uct_val<R, LIMV>(q4, e4, cxj, cx, +1);                 // X1a
+    R ez = b2::uct_val<R, LIMV, NQ>(q4, e4, cxj, cx, +1);             // X1a
 #pragma unroll
     for (int o = 0; o < 4; o++) { q4[o] = (R)Bx[yo[o]]; e4[o] = FDx[0*n + yo[o]]; }
-    ez += b2::uct_val<R, LIMV>(q4, e4, cyi, cy, -1);                  // X2b
+    ez += b2::uct_val<R, LIMV, NQ>(q4, e4, cyi, cy, -1);              // X2b
     b.E[c] = ez;
   }
 }
@@ -364,7 +382,7 @@ static void b2_launch_t(int phase, const Batch2dBufs<R> &b, RKW w, int first_sta
                        want_dt, (R)gamma);
     break;
   case B2_EMF:
-    hipLaunchKernelGGL((k2_emf<R, LIM>), dim3(blocks), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k2_emf<R, LIM, b2::nrec<SOLVER>() - 2>), dim3(blocks), dim3(256), 0, s, b);
     break;
   case B2_FINISH:
     hipLaunchKernelGGL((k2_finish<R>), dim3(blocks), dim3(256), 0, s, b, w, first_stage, gamma - 1.0,
@@ -391,11 +409,13 @@ void batch2d_launch(int phase, const Batch2dBufs<R> &b, RKW w, int first_stage, 
                     double small_rho, double small_prs, hipStream_t s)
 {
   if (limiter == MC_LIM) {
-    if (solver == 3)      b2_launch_l<R, MC_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    if (solver == 4)      b2_launch_l<R, MC_LIM, 4>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else if (solver == 3) b2_launch_l<R, MC_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
     else if (solver == 2) b2_launch_l<R, MC_LIM, 2>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
     else                  b2_launch_l<R, MC_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
   } else {
-    if (solver == 3)      b2_launch_l<R, VANLEER_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    if (solver == 4)      b2_launch_l<R, VANLEER_LIM, 4>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
+    else if (solver == 3) b2_launch_l<R, VANLEER_LIM, 3>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
     else if (solver == 2) b2_launch_l<R, VANLEER_LIM, 2>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
     else                  b2_launch_l<R, VANLEER_LIM, 1>(phase, b, w, first_stage, want_dt, march, blocks, gamma, small_rho, small_prs, s);
   }

@@ -42,6 +42,7 @@ struct gpl_batch {
   hipStream_t stream = nullptr;
   bool fp64 = true;
   int nbox = 0, ncell = 0, ncu = 0;
+  int ksolver = 0;      // kernel solver code: the Riemann solver, 4 = HLLD with UCT_HLLD
   std::vector<BoxDesc> hbox;
   size_t bytes = 0;
   // device state
@@ -97,6 +98,7 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   b->fp64 = cfg->precision == GPL_FP64;
   b->nbox = nbox;
   b->ncell = (int)total;
+  b->ksolver = cfg->ct_emf_average == GPL_UCT_HLLD ? 4 : cfg->solver;
   if (hipSetDevice(b->dev) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->dev) != hipSuccess) {
@@ -147,7 +149,7 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
   rc |= dev_alloc(b->bytes, (void **)&b->Us, 6*nc*sizeof(double));
   for (int q = 0; q < 3; q++) rc |= dev_alloc(b->bytes, (void **)&b->vs[q], 2*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, &b->FD, 10*nc*rs);
+  rc |= dev_alloc(b->bytes, &b->FD, 2*(b->ksolver == 4 ? 6 : 5)*nc*rs);
   rc |= dev_alloc(b->bytes, &b->E, nc*rs);
   rc |= dev_alloc(b->bytes, &b->dU, 6*nc*rs);
   rc |= dev_alloc(b->bytes, &b->Cdt, nc*rs);
@@ -170,7 +172,11 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   if (!ny) ny = nx;
   *out = nullptr;
   if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
-  if (!all_periodic(*cfg) || cfg->ct_emf_average != GPL_UCT_HLL) return GPL_EUNSUPPORTED;
+  if (!all_periodic(*cfg)) return GPL_EUNSUPPORTED;
+  // UCT_HLL, or UCT_HLLD with the HLLD solver (the reference's HLL and HLLC
+  // quit with it: Src/MHD/hll.cpp:75-76, Src/MHD/hllc.cpp:81-83)
+  if (cfg->ct_emf_average != GPL_UCT_HLL && cfg->ct_emf_average != GPL_UCT_HLLD) return GPL_EUNSUPPORTED;
+  if (cfg->ct_emf_average == GPL_UCT_HLLD && cfg->solver != GPL_SOLVER_HLLD) return GPL_EUNSUPPORTED;
   if (cfg->solver != GPL_SOLVER_HLLD && cfg->solver != GPL_SOLVER_HLLC && cfg->solver != GPL_SOLVER_HLL)
     return GPL_EUNSUPPORTED;
   if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
@@ -358,13 +364,13 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, const d
   const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
   const int gt = grid_for(b, q.ntiles);
   const gpl_config &c = b->cfg;
-  batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, c.solver, march, gx,
+  batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gx,
                     c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, c.solver, march, gy,
+  batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gy,
                     c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_EMF, q, w, first_stage, first_stage, c.limiter, c.solver, march, gt,
+  batch2d_launch<R>(B2_EMF, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
                     c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_FINISH, q, w, first_stage, first_stage, c.limiter, c.solver, march, gt,
+  batch2d_launch<R>(B2_FINISH, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
                     c.gamma, c.small_rho, c.small_prs, b->stream);
 }
 

@@ -207,9 +207,10 @@ const char *gpl_strerror(int code);
  * ------------------------------------------------------------------------- */
 typedef struct gpl_batch gpl_batch;
 
-/* cfg: physics GPL_MHD, dimensions 2, GPL_PERIODIC, GPL_UCT_HLL; cfg's
- * nx/ny/nz are ignored: box q is nx[q] x ny[q] zones on [xbeg, xend]
- * (3..8192 each; ny == NULL gives square boxes). */
+/* cfg: physics GPL_MHD, dimensions 2, GPL_PERIODIC; GPL_UCT_HLL, or
+ * GPL_UCT_HLLD with GPL_SOLVER_HLLD.  cfg's nx/ny/nz are ignored: box q is
+ * nx[q] x ny[q] zones on [xbeg, xend] (3..8192 each; ny == NULL gives square
+ * boxes). */
 int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny, gpl_batch **out);
 int gpl_batch_destroy(gpl_batch *b);
 

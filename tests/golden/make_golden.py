@@ -37,6 +37,8 @@ CASES = [
     # the UCT_HLLD edge EMF (CT_Store_UCTHlld, Src/MHD/CT/ct_store.hpp:254) with HLLD
     ("mhd3d_12x14x10_rk2_mc_ucthlld", "mhd3d", "RK2", "MC_LIM", (12, 14, 10), 19, 6, 2e-3, 0.4, "hlld", True, 5 / 3),
     ("mhd3d_10x12x14_rk3_vl_ucthlld_outflow", "mhd3d", "RK3", "VANLEER_LIM", (10, 12, 14), 20, 5, 2e-3, 0.4, "hlld", False, 5 / 3),
+    ("mhd2d_30x26_rk2_mc_ucthlld", "mhd2d", "RK2", "MC_LIM", (30, 26, 1), 27, 10, 3e-3, 0.4, "hlld", True, 5 / 3),
+    ("mhd2d_26x34_rk3_vl_ucthlld", "mhd2d", "RK3", "VANLEER_LIM", (26, 34, 1), 28, 8, 3e-3, 0.4, "hlld", True, 5 / 3),
     # the CPU-runnable headline case: 1D Euler shock tube, 4096 cells, 400 steps
     ("hd1d_4096_rk2_mc_hllc", "hd1d", "RK2", "MC_LIM", (4096, 1, 1), 0, 400, 1e-4, 0.8, "hllc", False, 1.4),
     ("hd1d_1024_rk3_vl_hllc", "hd1d", "RK3", "VANLEER_LIM", (1024, 1, 1), 3, 150, 1e-4, 0.8, "hllc", False, 1.4),
@@ -107,7 +109,8 @@ RESTART_CASES = [
 ]
 
 # CT_EMF_AVERAGE where it is not UCT_HLL
-EMF = {"mhd3d_12x14x10_rk2_mc_ucthlld": "UCT_HLLD", "mhd3d_10x12x14_rk3_vl_ucthlld_outflow": "UCT_HLLD"}
+EMF = {"mhd3d_12x14x10_rk2_mc_ucthlld": "UCT_HLLD", "mhd3d_10x12x14_rk3_vl_ucthlld_outflow": "UCT_HLLD",
+       "mhd2d_30x26_rk2_mc_ucthlld": "UCT_HLLD", "mhd2d_26x34_rk3_vl_ucthlld": "UCT_HLLD"}
 
 # relativistic IC parameters per case: (lorentz_max, sigma)
 RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.0, 0.5),

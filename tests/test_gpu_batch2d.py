@@ -3,7 +3,8 @@ sizes computes, bit for bit (FP64), what a standalone 2D run of that box
 computes -- against the oracle (itself pinned bitwise to the reference CPU
 build) and against the reference build's own 2D output fixture.  Launch
 shapes chosen by the autotuner never change a bit; the FP32 variant stays
-within 1e-5 relative L1 of the FP64 batch."""
+within 1e-5 relative L1 of the FP64 batch.  Both EMF averages: UCT_HLL, and
+UCT_HLLD with HLLD (against two more reference fixtures)."""
 import numpy as np
 import pytest
 
@@ -15,8 +16,8 @@ from gpluto_dev_amd.stage import Definitions, RuntimeSetup
 pytestmark = pytest.mark.gpu
 
 
-def _batch(sizes, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64"):
-    return BatchUpdate(Definitions(dimensions=2, time_stepping=ts, limiter=lim),
+def _batch(sizes, ts="RK2", lim="MC_LIM", solver="hlld", precision="fp64", emf="UCT_HLL"):
+    return BatchUpdate(Definitions(dimensions=2, time_stepping=ts, limiter=lim, ct_emf_average=emf),
                        RuntimeSetup(solver=solver, cfl=0.4), sizes, precision=precision)
 
 
@@ -28,10 +29,10 @@ def _ics(sizes, seed):
     return out
 
 
-def _oracle(n, vc, vs, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld"):
+def _oracle(n, vc, vs, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld", emf="UCT_HLL"):
     import oracle
     o = oracle.Oracle(phys="mhd", n=n, gamma=5 / 3, solver=solver, limiter=lim,
-                      time_stepping=int(ts[-1]), cfl=0.4)
+                      time_stepping=int(ts[-1]), cfl=0.4, emf=emf)
     o.set_state(vc, vs)
     err, t, dt = o.run(steps, first_dt)
     v, s, u = o.get_state()
@@ -41,21 +42,22 @@ def _oracle(n, vc, vs, steps, first_dt, ts="RK2", lim="MC_LIM", solver="hlld"):
 SIZES = [(5, 5), (33, 33), (63, 64), (64, 63), (65, 65), (100, 37), (130, 130), (32, 24)]
 
 
-@pytest.mark.parametrize("ts,lim,solver", [("RK2", "MC_LIM", "hlld"),
-                                           ("RK3", "VANLEER_LIM", "hll"),
-                                           ("RK2", "VANLEER_LIM", "hllc")])
-def test_every_box_matches_its_standalone_oracle_run(ts, lim, solver):
+@pytest.mark.parametrize("ts,lim,solver,emf", [("RK2", "MC_LIM", "hlld", "UCT_HLL"),
+                                               ("RK3", "VANLEER_LIM", "hll", "UCT_HLL"),
+                                               ("RK2", "VANLEER_LIM", "hllc", "UCT_HLL"),
+                                               ("RK3", "MC_LIM", "hlld", "UCT_HLLD")])
+def test_every_box_matches_its_standalone_oracle_run(ts, lim, solver, emf):
     steps, first_dt = 6, 1e-3
-    print(f"case {ts} {lim} {solver} sizes={SIZES}", flush=True)
+    print(f"case {ts} {lim} {solver} {emf} sizes={SIZES}", flush=True)
     ics = _ics(SIZES, 4)
-    g = _batch(SIZES, ts, lim, solver)
+    g = _batch(SIZES, ts, lim, solver, emf=emf)
     for b, (vc, vs) in enumerate(ics):
         g.upload(b, vc, vs)
     err, t, dt = g.run(steps, first_dt)
     assert err == 0
     for b, (nx, ny) in enumerate(SIZES):
         vc, vs = ics[b]
-        oerr, ot, odt, ov, osv, ou = _oracle((nx, ny, 1), vc, vs, steps, first_dt, ts, lim, solver)
+        oerr, ot, odt, ov, osv, ou = _oracle((nx, ny, 1), vc, vs, steps, first_dt, ts, lim, solver, emf)
         gv, gs, gu = g.download(b)
         assert oerr == 0
         assert t[b] == ot and dt[b] == odt, f"box {b} ({nx}x{ny}): time/dt differ"
@@ -68,13 +70,16 @@ def test_every_box_matches_its_standalone_oracle_run(ts, lim, solver):
     g.close()
 
 
-def test_reference_fixture_box_inside_a_batch():
+@pytest.mark.parametrize("name", ["mhd2d_32x24_rk2_mc", "mhd2d_30x26_rk2_mc_ucthlld",
+                                  "mhd2d_26x34_rk3_vl_ucthlld"])
+def test_reference_fixture_box_inside_a_batch(name):
     """The reference build's 2D output, reproduced by one box of a batch."""
-    p, a = load("mhd2d_32x24_rk2_mc")
+    p, a = load(name)
+    print(f"case {name}", flush=True)
     nx, ny, _ = p["n"]
     sizes = [(40, 40), (nx, ny), (70, 33)]
     ics = _ics(sizes, 9)
-    g = _batch(sizes, p["time_stepping"], p["limiter"], p["solver"])
+    g = _batch(sizes, p["time_stepping"], p["limiter"], p["solver"], emf=p.get("emf", "UCT_HLL"))
     for b, (vc, vs) in enumerate(ics):
         if b == 1:
             vc, vs = a["vc_in"], a["vs_in"]
@@ -113,12 +118,14 @@ def test_launch_shapes_and_autotuner_give_identical_bits():
             assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(c1, c2)
 
 
-def test_fp32_batch_within_1e5_of_fp64():
+@pytest.mark.parametrize("emf", ["UCT_HLL", "UCT_HLLD"])
+def test_fp32_batch_within_1e5_of_fp64(emf):
     sizes = [(64, 64), (150, 150), (37, 37), (256, 256)]
     ics = _ics(sizes, 6)
     out = {}
     for prec in ("fp64", "mixed"):
-        g = _batch(sizes, precision=prec)
+        print(f"case {emf} {prec}", flush=True)
+        g = _batch(sizes, precision=prec, emf=emf)
         for b, (vc, vs) in enumerate(ics):
             g.upload(b, vc, vs)
         err, t, _ = g.run(20, 5e-4)
@@ -132,3 +139,11 @@ def test_fp32_batch_within_1e5_of_fp64():
             l1 = np.sum(np.abs(v32[v] - v64[v])) / np.sum(np.abs(v64[v]))
             assert l1 <= 1e-5, f"box {b} var {v}: rel L1 {l1:.2e}"
         assert ic.divb_max(s32) < 1e-10
+
+
+def test_uct_hlld_needs_hlld():
+    """CT_Store_UCTHlld reads the HLLD star states: the reference's HLL and
+    HLLC solvers quit with it (Src/MHD/hll.cpp:75-76, Src/MHD/hllc.cpp:81-83)."""
+    for solver in ("hll", "hllc"):
+        with pytest.raises(RuntimeError, match="not supported"):
+            _batch([(16, 16)], solver=solver, emf="UCT_HLLD")
