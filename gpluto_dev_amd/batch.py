@@ -1,5 +1,6 @@
-"""Batched 2D MHD over the HIP library: many independent periodic boxes of
-uneven sizes advanced together, each with its own dt.
+"""Batched 2D MHD over the HIP library: many independent boxes of uneven
+sizes advanced together, each with its own dt; periodic, or with outflow or
+reflective sides (the same for every box of a batch).
 
 ``BatchUpdate`` mirrors ``StageUpdate`` box by box: ``upload(box, vc, vs)``,
 ``download(box)``, and ``run(nsteps, first_dt)`` -- the main loop of
@@ -14,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .stage import _EMF, _LIMITERS, _SOLVERS, _STEPPING, Definitions, RuntimeSetup
+from .stage import _BOUNDARY, _EMF, _LIMITERS, _SOLVERS, _STEPPING, Definitions, RuntimeSetup
 
 
 class BatchUpdate:
@@ -28,7 +29,11 @@ class BatchUpdate:
         cfg.dimensions = 2
         for d in range(3):
             cfg.xbeg[d], cfg.xend[d] = ini.xbeg[d], ini.xend[d]
-        cfg.boundary = _lib.GPL_PERIODIC
+        # periodic, or outflow / reflective per side (pluto.ini [Boundary]);
+        # every box of a batch has the same sides
+        cfg.boundary = _BOUNDARY[ini.boundary]
+        for q, t in enumerate(ini.sides):
+            cfg.side_bc[q] = _BOUNDARY[t]
         cfg.time_stepping = _STEPPING[defs.time_stepping]
         cfg.limiter = _LIMITERS[defs.limiter]
         cfg.solver = _SOLVERS[ini.solver]
@@ -77,6 +82,19 @@ class BatchUpdate:
         _lib.check(self._L.gpl_batch_download(self._h, box, vc.ctypes.data, arr, uc.ctypes.data,
                                               _lib.GPL_LAYOUT_ACTIVE), "gpl_batch_download")
         return vc, vs, uc
+
+    def upper_faces(self, box: int):
+        """The box's upper boundary faces (index n along x and y), slabs
+        [1, ny, 1] and [1, 1, nx] as the reference's Vs stores them: the
+        periodic image of the lower face on a periodic side, the evolved face
+        on an outflow or reflective side."""
+        nx, ny = self.sizes[box]
+        tx, ty = nx + 4, ny + 4          # Ary4D with gpl_config.nghost = 2 along x and y
+        arrs = [np.zeros((1, ty, tx + 1)), np.zeros((1, ty + 1, tx))]
+        ptrs = (C.c_void_p * 2)(*[a.ctypes.data for a in arrs])
+        _lib.check(self._L.gpl_batch_download(self._h, box, None, ptrs, None, _lib.GPL_LAYOUT_ARY4D),
+                   "gpl_batch_download")
+        return [arrs[0][:, 2:2 + ny, 2 + nx:3 + nx], arrs[1][:, 2 + ny:3 + ny, 2:2 + nx]]
 
     def run(self, nsteps: int, first_dt=None):
         """Returns (err, time[nbox], dt[nbox]); first_dt: scalar, per-box array or None."""

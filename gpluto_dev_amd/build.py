@@ -40,6 +40,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("batch2d_fp64.hip", []),
            ("batch2d_mixed.hip", MIXED),
            ("batch2d_common.hip", []),
+           ("batch2d_boundary.hip", []),
            ("engine_batch2d.cpp", []),
            ("c2p_net.hip", []),
            ("engine_mhd3d.cpp", []),

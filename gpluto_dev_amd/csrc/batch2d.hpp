@@ -1,6 +1,11 @@
-// Declarations of the batched 2D MHD stage kernels: many independent
-// periodic boxes of uneven sizes advanced together, one launch per phase,
-// each launch a persistent grid striding over a per-batch work list.
+// Declarations of the batched 2D MHD stage kernels: many independent boxes
+// of uneven sizes advanced together, one launch per phase, each launch a
+// persistent grid striding over a per-batch work list.  Boxes are periodic,
+// or, with an outflow or reflective side, stored as the reference's own array
+// (two ghost layers per side and one spare layer for the outermost face) and
+// computed on as a periodic box of that size, with the ghosts refilled at
+// every stage and C_dt and error bits taken from the reference's cells and
+// faces only.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -9,12 +14,14 @@
 
 namespace gpl {
 
-// One nx x ny periodic box: cells off .. off + nx*ny - 1 of every batch
-// array, c = off + i + nx*j.
+// One stored nx x ny box: cells off .. off + nx*ny - 1 of every batch
+// array, c = off + i + nx*j.  Active cells lo[d] .. hi[d]: the whole box when
+// periodic, else nx = active + 5 with lo = 2.
 struct BoxDesc {
   int nx, ny;
   int off;
   double dx, dy;
+  int lo[2], hi[2];
 };
 
 // Work items.  Sweep items are one wavefront each:
@@ -61,6 +68,13 @@ template <typename R>
 void batch2d_launch(int phase, const Batch2dBufs<R> &b, RKW w, int first_stage, int want_dt,
                     int limiter, int solver, int march, int blocks, double gamma,
                     double small_rho, double small_prs, hipStream_t s);
+
+// Boundary (Src/boundary.cpp:67-303) of every box, sides X1_BEG .. X2_END
+// in turn; type[4] per side (GPL_PERIODIC / GPL_OUTFLOW / GPL_REFLECTIVE),
+// maxp the largest stored box side.  Faces are FP64, cells R.
+template <typename R>
+void batch2d_boundary(const BoxDesc *box, int nbox, int maxp, const int type[4], R *V, double *Vs,
+                      size_t ncell, hipStream_t s);
 
 // Per-box dt: invDt_hyp = max over the box's y items / 2, then optionally
 // NextTimeStep and time += dt (one workgroup per box).

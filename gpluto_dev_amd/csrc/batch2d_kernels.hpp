@@ -1,6 +1,9 @@
 // Batched 2D ideal-MHD stage update: PLM + HLLD/HLLC/HLL + UCT_HLL or UCT_HLLD CT
-// + RK2/RK3
-// on many independent periodic boxes of uneven sizes, for gfx950.
+// + RK2/RK3 on many independent boxes of uneven sizes, for gfx950.  A box with
+// an outflow or reflective side is stored with the reference's ghost layers
+// and computed on as a periodic box of that size (batch2d_boundary.hip
+// refills the ghosts before every stage); C_dt and error bits come from the
+// reference's cells and faces only.
 //
 // Every phase is one launch over all boxes: a persistent grid whose
 // workgroups stride over a per-batch work list (sweep items are one
@@ -102,6 +105,29 @@ __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const 
 // flux-vector index of conserved slot q of the 2D batch (rho, m1, m2, m3, B3, E)
 __host__ __device__ constexpr int fidx(int q) { return q < 4 ? q : q + 2; }
 
+// The reference's active cells (the whole box when periodic).
+__device__ __forceinline__ bool active(const BoxDesc &bx, int i, int j)
+{
+  return i >= bx.lo[0] && i <= bx.hi[0] && j >= bx.lo[1] && j <= bx.hi[1];
+}
+
+// The reference solves the faces of the active cells and the upper boundary
+// face along a sweep, over boxes widened by two ghost layers across it
+// (update_stage.cpp:80-99); error bits count from those faces only.  (i, j):
+// the face's lower-face cell index, i along x; for a periodic box the face
+// index n along the sweep is the image of face 0.
+template <int D>
+__device__ __forceinline__ bool face_reported(const BoxDesc &bx, int i, int j)
+{
+  const int p[2] = {i, j};
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    const int lo = d == D ? bx.lo[d] : bx.lo[d] - 2, hi = d == D ? bx.hi[d] + 1 : bx.hi[d] + 2;
+    if (p[d] < lo || p[d] > hi) return false;
+  }
+  return true;
+}
+
 }  // namespace b2
 
 // ---------------------------------------------------------------------------
@@ -137,7 +163,7 @@ void k2_sweepx(Batch2dBufs<R> b, R gamma, int want_cdt)
     vl[4] = vr[4] = (R)b.Vin[c0];
     FaceFlux<R> ff;
     b2::solve<R, 0, SOLVER>(vl, vr, gamma, ff);
-    e |= ff.err;
+    if (b2::face_reported<0>(bx, f, w4.a)) e |= ff.err;
     if (lane < nc) b2::store_rec<R, 0, SOLVER>(vl, vr, gamma, ff, b.FD, c0, n);
     R F[6], Fn[6];
 #pragma unroll
@@ -221,7 +247,7 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
         vl[5] = vr[5] = (R)b.Vin[n + cf];
         FaceFlux<R> ff;
         b2::solve<R, 1, SOLVER>(vl, vr, gamma, ff);
-        e |= ff.err;
+        if (b2::face_reported<1>(bx, i, f)) e |= ff.err;
         if (f < s1) b2::store_rec<R, 1, SOLVER>(vl, vr, gamma, ff, b.FD + b2::nrec<SOLVER>()*n, cf, n);
         double F[6];
 #pragma unroll
@@ -241,7 +267,7 @@ void k2_sweepy(Batch2dBufs<R> b, RKW w, int first_stage, int want_cdt, R gamma)
               b.U[o] = w.w0*b.U0[o] + w.w1*u_old[q] + w.wR*du;
             }
           }
-          if (want_cdt) {
+          if (want_cdt && b2::active(bx, i, f - 1)) {
             double cd = cacc;
             cd += 0.5*(cmp + cm)/bx.dy;
             cdt_max = cdt_max >= cd ? cdt_max : cd;
@@ -358,7 +384,7 @@ void k2_finish(Batch2dBufs<R> b, RKW w, int first_stage, double gmm1, double sma
     if (e) {
       b.U[0*n + c] = u[0]; b.U[1*n + c] = u[1]; b.U[2*n + c] = u[2]; b.U[3*n + c] = u[3];
       b.U[5*n + c] = u[4];
-      eall |= e;
+      if (b2::active(bx, i, j)) eall |= e;
     }
 #pragma unroll
     for (int s = 0; s < 8; s++) b.V[s*n + c] = (R)v[s];

@@ -1,5 +1,6 @@
-// Batched 2D ideal MHD: many independent periodic boxes of uneven sizes in
-// one device state, advanced together (one launch per phase over all boxes,
+// Batched 2D ideal MHD: many independent boxes of uneven sizes (periodic, or
+// stored with the reference's ghost layers when a side is outflow or
+// reflective) in one device state, advanced together (one launch per phase over all boxes,
 // one HIP graph per step), each box with its own dt.  Launch shapes (the
 // y-march length and the persistent grid size) are chosen by an autotuner
 // that times the candidates on the run's own first steps: every candidate
@@ -43,6 +44,11 @@ struct gpl_batch {
   bool fp64 = true;
   int nbox = 0, ncell = 0, ncu = 0;
   int ksolver = 0;      // kernel solver code: the Riemann solver, 4 = HLLD with UCT_HLLD
+  // sides X1-beg, X1-end, X2-beg, X2-end; with an outflow or reflective side
+  // every box is stored padded (active + 5 per direction, active cells from 2)
+  int side[4] = {GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC};
+  bool padded = false;
+  int maxp = 0;         // largest stored box side
   std::vector<BoxDesc> hbox;
   size_t bytes = 0;
   // device state
@@ -110,16 +116,24 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   std::vector<Item4> xi;
   std::vector<Item2> ti;
   int off = 0;
+  for (int q = 0; q < 4; q++) b->side[q] = side_type(*cfg, q);
+  b->padded = !all_periodic(*cfg);
+  const int pad = b->padded ? 5 : 0, lo = b->padded ? 2 : 0;
   for (int q = 0; q < nbox; q++) {
     BoxDesc d;
-    d.nx = nx[q]; d.ny = ny[q]; d.off = off;
+    d.nx = nx[q] + pad; d.ny = ny[q] + pad; d.off = off;
     d.dx = (cfg->xend[0] - cfg->xbeg[0])/nx[q];   // Src/set_grid.cpp:196
     d.dy = (cfg->xend[1] - cfg->xbeg[1])/ny[q];
+    d.lo[0] = d.lo[1] = lo;
+    d.hi[0] = lo + nx[q] - 1;
+    d.hi[1] = lo + ny[q] - 1;
     b->hbox.push_back(d);
-    for (int j = 0; j < ny[q]; j++)
-      for (int i0 = 0; i0 < nx[q]; i0 += 63) xi.push_back({q, j, i0, nx[q] - i0 < 63 ? nx[q] - i0 : 63});
-    for (int s = 0; s < nx[q]*ny[q]; s += 256) ti.push_back({q, s});
-    off += nx[q]*ny[q];
+    if (d.nx > b->maxp) b->maxp = d.nx;
+    if (d.ny > b->maxp) b->maxp = d.ny;
+    for (int j = 0; j < d.ny; j++)
+      for (int i0 = 0; i0 < d.nx; i0 += 63) xi.push_back({q, j, i0, d.nx - i0 < 63 ? d.nx - i0 : 63});
+    for (int s = 0; s < d.nx*d.ny; s += 256) ti.push_back({q, s});
+    off += d.nx*d.ny;
   }
   const size_t nc = (size_t)b->ncell, rs = b->fp64 ? sizeof(double) : sizeof(float);
   int rc = 0;
@@ -135,9 +149,10 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
     std::vector<int> pb;
     for (int q = 0; q < nbox; q++) {
       pb.push_back((int)yi.size());
-      const int nseg = (ny[q] + L - 1)/L;
+      const int sx = b->hbox[q].nx, sy = b->hbox[q].ny;
+      const int nseg = (sy + L - 1)/L;
       for (int s = 0; s < nseg; s++)
-        for (int i0 = 0; i0 < nx[q]; i0 += 64) yi.push_back({q, i0, s, 0});
+        for (int i0 = 0; i0 < sx; i0 += 64) yi.push_back({q, i0, s, 0});
     }
     pb.push_back((int)yi.size());
     rc |= upload_vec(&b->yitems[m], yi, b->bytes);
@@ -172,7 +187,15 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   if (!ny) ny = nx;
   *out = nullptr;
   if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
-  if (!all_periodic(*cfg)) return GPL_EUNSUPPORTED;
+  // periodic, outflow or reflective sides (outflow and reflective: padded
+  // boxes); a direction is periodic on both sides or on neither
+  for (int d = 0; d < 2; d++) {
+    const int lo = side_type(*cfg, 2*d), hi = side_type(*cfg, 2*d + 1);
+    if (lo < GPL_PERIODIC || lo > GPL_REFLECTIVE || hi < GPL_PERIODIC || hi > GPL_REFLECTIVE)
+      return GPL_EUNSUPPORTED;
+    if ((lo == GPL_PERIODIC) != (hi == GPL_PERIODIC)) return GPL_EINVAL;
+  }
+  const int pad = all_periodic(*cfg) ? 0 : 5;
   // UCT_HLL, or UCT_HLLD with the HLLD solver (the reference's HLL and HLLC
   // quit with it: Src/MHD/hll.cpp:75-76, Src/MHD/hllc.cpp:81-83)
   if (cfg->ct_emf_average != GPL_UCT_HLL && cfg->ct_emf_average != GPL_UCT_HLLD) return GPL_EUNSUPPORTED;
@@ -185,7 +208,7 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   long long total = 0;
   for (int q = 0; q < nbox; q++) {
     if (nx[q] < 3 || nx[q] > 8192 || ny[q] < 3 || ny[q] > 8192) return GPL_EINVAL;
-    total += (long long)nx[q]*ny[q];
+    total += (long long)(nx[q] + pad)*(ny[q] + pad);
   }
   if (total > (1LL << 30)) return GPL_EINVAL;
   int ndev = 0;
@@ -217,14 +240,38 @@ extern "C" int gpl_batch_destroy(gpl_batch *b)
   return 0;
 }
 
+static int active_n(const BoxDesc &d, int dir) { return d.hi[dir] - d.lo[dir] + 1; }
+
+// the caller's layout of one box's active zones
 static Layout box_layout(const gpl_batch *b, int box, int layout)
 {
   gpl_config c = b->cfg;
-  c.nx = b->hbox[box].nx;
-  c.ny = b->hbox[box].ny;
+  c.nx = active_n(b->hbox[box], 0);
+  c.ny = active_n(b->hbox[box], 1);
   c.nz = 1;
   c.dimensions = 2;
   return make_layout(c, layout);
+}
+
+// Host index along dir that stored cell index p reads at upload: the active
+// cell, or in a ghost layer its periodic image or the clamped edge cell (the
+// boundary fill rewrites the ghosts before every stage).
+static int src_cell(const gpl_batch *b, const BoxDesc &d, int dir, int p)
+{
+  const int a = p - d.lo[dir], m = active_n(d, dir);
+  if (b->side[2*dir] == GPL_PERIODIC) return ((a % m) + m) % m;
+  return a < 0 ? 0 : (a >= m ? m - 1 : a);
+}
+
+// Host index along dir of the face that stored face (component comp, index p
+// along dir) takes at upload: the periodic image, except the upper boundary
+// face of a non-periodic direction, which the reference's array layout
+// carries (face n); with the active layout it starts as the image of face 0.
+static int src_face(const gpl_batch *b, const BoxDesc &d, const Layout &L, int comp, int dir, int p)
+{
+  const int a = p - d.lo[dir], m = active_n(d, dir);
+  if (L.ary && comp == dir && b->side[2*dir] != GPL_PERIODIC && a == m) return m;
+  return ((a % m) + m) % m;
 }
 
 static int gpl_batch_upload_impl(gpl_batch *b, int box, const double *vc, const double *const *vs,
@@ -241,12 +288,13 @@ static int gpl_batch_upload_impl(gpl_batch *b, int box, const double *vc, const 
   std::vector<double> hv(8*m), hu(6*m), hs(2*m);
   for (int c = 0; c < 2; c++)
     for (int j = 0; j < NY; j++)
-      for (int i = 0; i < NX; i++) hs[c*m + (size_t)j*NX + i] = vs[c][L.face(c, i, j, 0)];
+      for (int i = 0; i < NX; i++)
+        hs[c*m + (size_t)j*NX + i] = vs[c][L.face(c, src_face(b, d, L, c, 0, i), src_face(b, d, L, c, 1, j), 0)];
   for (int j = 0; j < NY; j++)
     for (int i = 0; i < NX; i++) {
       const size_t o = (size_t)j*NX + i;
       double v[8], u[8];
-      for (int s = 0; s < 8; s++) v[s] = vc[L.cell(s, i, j, 0)];
+      for (int s = 0; s < 8; s++) v[s] = vc[L.cell(s, src_cell(b, d, 0, i), src_cell(b, d, 1, j), 0)];
       // cell-centred B1, B2 = face averages (Src/startup.cpp:388, ct_field_average.cpp:182-184)
       const int ip = i + 1 == NX ? 0 : i + 1, jp = j + 1 == NY ? 0 : j + 1;
       v[4] = 0.5*(hs[o] + hs[(size_t)j*NX + ip]);
@@ -313,9 +361,10 @@ static int gpl_batch_download_impl(gpl_batch *b, int box, double *vc, double *co
   for (int c = 0; c < 2; c++)
     if (hipMemcpy(&hs[c*m], b->vs[b->cur] + c*nc + d.off, m*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return GPL_EHIP;
-  for (int j = 0; j < NY; j++)
-    for (int i = 0; i < NX; i++) {
-      const size_t o = (size_t)j*NX + i;
+  // the active zones of the stored box
+  for (int j = 0; j < L.ny; j++)
+    for (int i = 0; i < L.nx; i++) {
+      const size_t o = (size_t)(j + d.lo[1])*NX + (i + d.lo[0]);
       if (vc) for (int s = 0; s < 8; s++) vc[L.cell(s, i, j, 0)] = hv[s*m + o];
       if (uc) {
         // B1, B2 of Uc are the face averages (= Vc's)
@@ -325,9 +374,14 @@ static int gpl_batch_download_impl(gpl_batch *b, int box, double *vc, double *co
       }
       if (vs) for (int c = 0; c < 2; c++) if (vs[c]) vs[c][L.face(c, i, j, 0)] = hs[c*m + o];
     }
-  if (vs && L.ary) {   // the duplicate upper faces equal the lower ones (PeriodicBoundary)
-    for (int j = 0; j < NY; j++) if (vs[0]) vs[0][L.face(0, NX, j, 0)] = hs[(size_t)j*NX];
-    for (int i = 0; i < NX; i++) if (vs[1]) vs[1][L.face(1, i, NY, 0)] = hs[m + i];
+  if (vs && L.ary) {
+    // the upper boundary face: the periodic image of the lower one
+    // (PeriodicBoundary), or its own value on an outflow or reflective side
+    const bool px = b->side[0] == GPL_PERIODIC, py = b->side[2] == GPL_PERIODIC;
+    for (int j = 0; j < L.ny; j++)
+      if (vs[0]) vs[0][L.face(0, L.nx, j, 0)] = hs[(size_t)(j + d.lo[1])*NX + d.lo[0] + (px ? 0 : L.nx)];
+    for (int i = 0; i < L.nx; i++)
+      if (vs[1]) vs[1][L.face(1, i, L.ny, 0)] = hs[m + (size_t)(d.lo[1] + (py ? 0 : L.ny))*NX + d.lo[0] + i];
   }
   return 0;
 }
@@ -349,7 +403,7 @@ static int grid_for(const gpl_batch *b, int groups)
 }
 
 template <typename R>
-static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, const double *vin,
+static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, double *vin,
                           double *vout, const double *vbase, RKW w, int first_stage)
 {
   Batch2dBufs<R> q;
@@ -364,6 +418,10 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, const d
   const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
   const int gt = grid_for(b, q.ntiles);
   const gpl_config &c = b->cfg;
+  // Boundary (Src/rk_step.cpp: at the start of every stage) on padded boxes
+  if (b->padded)
+    batch2d_boundary<R>(b->box, b->nbox, b->maxp, b->side, (R *)b->V, vin, (size_t)b->ncell,
+                        b->stream);
   batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gx,
                     c.gamma, c.small_rho, c.small_prs, b->stream);
   batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gy,
@@ -375,7 +433,7 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, const d
 }
 
 // one AdvanceStep of every box (Src/rk_step.cpp:88-720), then per-box dt
-static void enqueue_stage_r(gpl_batch *b, const double *uin, double *uout, const double *vin,
+static void enqueue_stage_r(gpl_batch *b, const double *uin, double *uout, double *vin,
                             double *vout, const double *vbase, RKW w, int first)
 {
   if (b->fp64) enqueue_stage<double>(b, uin, uout, vin, vout, vbase, w, first);

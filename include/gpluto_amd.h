@@ -199,23 +199,27 @@ size_t gpl_device_bytes(const gpl_ctx *ctx);
 const char *gpl_strerror(int code);
 
 /* ---------------------------------------------------------------------------
- * Batched 2D MHD: many independent periodic N x N boxes of uneven sizes in
- * one device state, advanced together (one launch per phase over all boxes,
+ * Batched 2D MHD: many independent boxes of uneven sizes in one device
+ * state, advanced together (one launch per phase over all boxes,
  * one HIP graph per step), each box with its own dt.  No counterpart in the
  * reference, which runs one domain per process; each box computes exactly
  * what gpl_create/gpl_run would compute for it alone (FP64: bit for bit).
  * ------------------------------------------------------------------------- */
 typedef struct gpl_batch gpl_batch;
 
-/* cfg: physics GPL_MHD, dimensions 2, GPL_PERIODIC; GPL_UCT_HLL, or
- * GPL_UCT_HLLD with GPL_SOLVER_HLLD.  cfg's nx/ny/nz are ignored: box q is
- * nx[q] x ny[q] zones on [xbeg, xend] (3..8192 each; ny == NULL gives square
- * boxes). */
+/* cfg: physics GPL_MHD, dimensions 2; GPL_UCT_HLL, or GPL_UCT_HLLD with
+ * GPL_SOLVER_HLLD.  Sides X1-beg .. X2-end periodic, outflow or reflective
+ * (boundary / side_bc, the same for every box; a direction periodic on both
+ * sides or on neither): with an outflow or reflective side every box is
+ * stored with the reference's ghost layers, refilled at every stage.  cfg's
+ * nx/ny/nz are ignored: box q is nx[q] x ny[q] zones on [xbeg, xend]
+ * (3..8192 each; ny == NULL gives square boxes). */
 int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny, gpl_batch **out);
 int gpl_batch_destroy(gpl_batch *b);
 
 /* Per-box state transfer, as gpl_upload/gpl_download with dimensions 2:
- * vc holds 8 variables (B3 cell-centred), vs[0..1] the B1/B2 faces. */
+ * vc holds 8 variables (B3 cell-centred), vs[0..1] the B1/B2 faces; with
+ * GPL_LAYOUT_ARY4D the upper boundary faces too (index n along x and y). */
 int gpl_batch_upload(gpl_batch *b, int box, const double *vc, const double *const *vs, int layout);
 int gpl_batch_download(gpl_batch *b, int box, double *vc, double *const *vs, double *uc, int layout);
 
