@@ -76,6 +76,13 @@ template <typename R>
 void batch2d_boundary(const BoxDesc *box, int nbox, int maxp, const int type[4], R *V, double *Vs,
                       size_t ncell, hipStream_t s);
 
+// The dbl dump image of one box into dst (device): the eight primitives over
+// the active zones, then Bx1s (nx+1 faces along x) and Bx2s (ny+1 along y);
+// per[d]: periodic direction (face n = the image of face 0).
+template <typename R>
+void batch2d_dump_gather(const BoxDesc &d, const int per[2], const R *V, const double *Vs, size_t ncell,
+                         double *dst, hipStream_t s);
+
 // Per-box dt: invDt_hyp = max over the box's y items / 2, then optionally
 // NextTimeStep and time += dt (one workgroup per box).
 void batch2d_launch_dt(const double *cdt_part, const int *part_beg, int nbox, StepCtl *ctl,

@@ -79,8 +79,9 @@ void k_diag(DiagBox b, const R *__restrict__ V, const double *__restrict__ Vs,
       const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
       const double dbx = b.ax*Bx[ip + b.nx*(j + b.ny*k)] - b.ax*Bx[c];
       const double dby = b.ay*By[i + b.nx*(jp + b.ny*k)] - b.ay*By[c];
-      const double dbz = b.az*Bz[i + b.nx*(j + b.ny*kp)] - b.az*Bz[c];
-      const double divb = dbx + dby + dbz;
+      // DIM_EXPAND: no z term in 2D, whose faces hold two components
+      const double dbz = b.nb == 3 ? b.az*Bz[i + b.nx*(j + b.ny*kp)] - b.az*Bz[c] : 0.0;
+      const double divb = b.nb == 3 ? dbx + dby + dbz : dbx + dby;
       q[2*NV + 1] = dmax(q[2*NV + 1], fabs(divb));
     }
   }

@@ -13,6 +13,7 @@ struct DiagBox {
   int nv;          // primitives per cell (HD 5, MHD/RMHD 8); the pressure is the last
   double ax, ay, az;   // face areas of the uniform Cartesian grid
   int lo[3], hi[3];    // the active cells (a padded box's ghost layers are skipped)
+  int nb;              // staggered components: 3, or 2 in 2D (div B without a z term)
 };
 
 // Number of partial results the first pass writes (each 2*nv + 2 doubles).

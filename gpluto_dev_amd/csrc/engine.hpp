@@ -63,6 +63,7 @@ inline bool all_periodic(const gpl_config &c)
 
 int make_mhd3d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
 int make_hd_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);
+int make_batch2d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out);   // 2D MHD
 
 // Host-array layouts accepted by gpl_upload / gpl_download: the reference's
 // Ary4D with ghost zones along the used directions (Src/set_grid.cpp:57-58:

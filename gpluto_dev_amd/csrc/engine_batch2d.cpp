@@ -8,6 +8,7 @@
 #include "../../include/gpluto_amd.h"
 #include "c_guard.hpp"
 #include "batch2d.hpp"
+#include "diagnostics.hpp"
 #include "engine.hpp"
 #include "physics.hpp"
 
@@ -49,6 +50,9 @@ struct gpl_batch {
   int side[4] = {GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC};
   bool padded = false;
   int maxp = 0;         // largest stored box side
+  // a batch behind a gpl_ctx (one box) borrows the context's stream,
+  // step-control block and error bits instead of owning them
+  bool owns_env = true;
   std::vector<BoxDesc> hbox;
   size_t bytes = 0;
   // device state
@@ -84,20 +88,26 @@ struct gpl_batch {
 static void batch_free(gpl_batch *b)
 {
   void *ptrs[] = {b->V, b->FD, b->E, b->dU, b->Cdt, b->U, b->Us, b->vs[0], b->vs[1], b->vs[2],
-                  b->cdt_part, b->err, b->ctl, b->box, b->xitems, b->tiles,
+                  b->cdt_part, b->box, b->xitems, b->tiles,
                   b->yitems[0], b->yitems[1], b->yitems[2], b->part_beg[0], b->part_beg[1],
                   b->part_beg[2]};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->owns_env) {
+    if (b->err) hipFree(b->err);
+    if (b->ctl) hipFree(b->ctl);
+    if (b->stream) hipStreamDestroy(b->stream);
+  }
 }
 
 // Everything gpl_batch_create sets up after the validation; on failure the
 // caller frees what was allocated (batch_free takes a partly built batch).
+// env: the stream, step-control block and error bits of a gpl_ctx to borrow
+// (one box), or null for a batch of its own.
 static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *nx, const int *ny,
-                      long long total)
+                      long long total, const EngineEnv *env)
 {
   b->cfg = *cfg;
   b->dev = cfg->device;
@@ -105,8 +115,14 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   b->nbox = nbox;
   b->ncell = (int)total;
   b->ksolver = cfg->ct_emf_average == GPL_UCT_HLLD ? 4 : cfg->solver;
+  if (env) {
+    b->owns_env = false;
+    b->stream = env->stream;
+    b->ctl = env->ctl;
+    b->err = env->err;
+  }
   if (hipSetDevice(b->dev) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      (!env && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) ||
       hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->dev) != hipSuccess) {
     return GPL_EHIP;
   }
@@ -169,8 +185,10 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   rc |= dev_alloc(b->bytes, &b->dU, 6*nc*rs);
   rc |= dev_alloc(b->bytes, &b->Cdt, nc*rs);
   rc |= dev_alloc(b->bytes, (void **)&b->cdt_part, (size_t)max_y*sizeof(double));
-  rc |= dev_alloc(b->bytes, (void **)&b->err, sizeof(uint32_t));
-  rc |= dev_alloc(b->bytes, (void **)&b->ctl, (size_t)nbox*sizeof(StepCtl));
+  if (!env) {
+    rc |= dev_alloc(b->bytes, (void **)&b->err, sizeof(uint32_t));
+    rc |= dev_alloc(b->bytes, (void **)&b->ctl, (size_t)nbox*sizeof(StepCtl));
+  }
   if (rc) return rc == GPL_EHIP ? GPL_EHIP : GPL_ENOMEM;
   std::vector<StepCtl> z(nbox, StepCtl{0.0, 0.0, 0.0, 0});
   if (hipMemcpy(b->ctl, z.data(), nbox*sizeof(StepCtl), hipMemcpyHostToDevice) != hipSuccess ||
@@ -180,12 +198,9 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   return 0;
 }
 
-extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny,
-                                gpl_batch **out)
+// The configurations the batched kernels take; *total: stored cells.
+static int batch_validate(const gpl_config *cfg, int nbox, const int *nx, const int *ny, long long *total)
 {
-  if (!cfg || !nx || !out || nbox < 1) return GPL_EINVAL;
-  if (!ny) ny = nx;
-  *out = nullptr;
   if (cfg->physics != GPL_MHD || cfg->dimensions != 2) return GPL_EUNSUPPORTED;
   // periodic, outflow or reflective sides (outflow and reflective: padded
   // boxes); a direction is periodic on both sides or on neither
@@ -205,18 +220,29 @@ extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, 
   if (cfg->time_stepping != GPL_RK2 && cfg->time_stepping != GPL_RK3) return GPL_EINVAL;
   if (cfg->limiter != GPL_MC_LIM && cfg->limiter != GPL_VANLEER_LIM) return GPL_EINVAL;
   if (cfg->precision != GPL_FP64 && cfg->precision != GPL_MIXED) return GPL_EINVAL;
-  long long total = 0;
+  *total = 0;
   for (int q = 0; q < nbox; q++) {
     if (nx[q] < 3 || nx[q] > 8192 || ny[q] < 3 || ny[q] > 8192) return GPL_EINVAL;
-    total += (long long)(nx[q] + pad)*(ny[q] + pad);
+    *total += (long long)(nx[q] + pad)*(ny[q] + pad);
   }
-  if (total > (1LL << 30)) return GPL_EINVAL;
+  if (*total > (1LL << 30)) return GPL_EINVAL;
+  return 0;
+}
+
+extern "C" int gpl_batch_create(const gpl_config *cfg, int nbox, const int *nx, const int *ny,
+                                gpl_batch **out)
+{
+  if (!cfg || !nx || !out || nbox < 1) return GPL_EINVAL;
+  if (!ny) ny = nx;
+  *out = nullptr;
+  long long total = 0;
+  if (int rc = batch_validate(cfg, nbox, nx, ny, &total)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return GPL_EHIP;
 
   gpl_batch *b = new (std::nothrow) gpl_batch();
   if (!b) return GPL_ENOMEM;
-  const int rc = c_guard(GPL_ENOMEM, [&] { return batch_init(b, cfg, nbox, nx, ny, total); });
+  const int rc = c_guard(GPL_ENOMEM, [&] { return batch_init(b, cfg, nbox, nx, ny, total, nullptr); });
   if (rc) {
     batch_free(b);
     delete b;
@@ -587,3 +613,90 @@ extern "C" int gpl_batch_get_tuning(const gpl_batch *b, int *march, int *blocks_
 extern "C" double gpl_batch_last_run_ms(const gpl_batch *b) { return b ? b->last_run_ms : 0.0; }
 
 extern "C" size_t gpl_batch_device_bytes(const gpl_batch *b) { return b ? b->bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// 2D MHD behind gpl_create: one box of the batched kernels on the context's
+// stream, step-control block and error bits, so gpl_run, the dbl dumps and
+// restarts and the diagnostics work on a 2D domain as on a 3D one.
+// ---------------------------------------------------------------------------
+namespace gpl {
+
+namespace {
+
+class Batch2dEngine : public Engine {
+public:
+  gpl_batch *b = nullptr;
+  double *diag = nullptr;   // diagnostics scratch: partial results, then the result
+
+  ~Batch2dEngine() override
+  {
+    if (diag) hipFree(diag);
+    if (b) { batch_free(b); delete b; }
+  }
+
+  int upload(const double *vc, const double *const *vs, int layout) override
+  {
+    return gpl_batch_upload_impl(b, 0, vc, vs, layout);
+  }
+  int download(double *vc, double *const *vs, double *uc, int layout) override
+  {
+    return gpl_batch_download_impl(b, 0, vc, vs, uc, layout);
+  }
+  void enqueue_step(int advance_dt, LaunchTimer *) override { ::enqueue_step(b, advance_dt); }
+  int graph_period() const override { return ::graph_period(b); }
+  const void *state_key() const override { return b->vs[b->cur]; }
+  int nvar() const override { return 8; }
+
+  int diagnostics(double *out) override
+  {
+    const BoxDesc &x = b->hbox[0];
+    DiagBox d;
+    d.nx = x.nx; d.ny = x.ny; d.nz = 1; d.ncell = b->ncell; d.nv = 8; d.nb = 2;
+    d.lo[0] = x.lo[0]; d.hi[0] = x.hi[0];
+    d.lo[1] = x.lo[1]; d.hi[1] = x.hi[1];
+    d.lo[2] = d.hi[2] = 0;
+    // face areas of the uniform Cartesian grid in 2D (Src/set_geometry.cpp:124,147)
+    d.ax = 1.0*x.dy; d.ay = x.dx*1.0; d.az = 0.0;
+    double *res = diag + (size_t)DIAG_BLOCKS*DIAG_MAXQ;
+    if (b->fp64) launch_diagnostics<double>(d, (const double *)b->V, b->vs[b->cur], diag, res, b->stream);
+    else         launch_diagnostics<float>(d, (const float *)b->V, b->vs[b->cur], diag, res, b->stream);
+    if (hipMemcpyAsync(out, res, 18*sizeof(double), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return GPL_EHIP;
+    return 0;
+  }
+
+  int dump_gather(double *dst, hipStream_t s) override
+  {
+    const int per[2] = {b->side[0] == GPL_PERIODIC, b->side[2] == GPL_PERIODIC};
+    if (b->fp64) batch2d_dump_gather<double>(b->hbox[0], per, (const double *)b->V, b->vs[b->cur],
+                                             (size_t)b->ncell, dst, s);
+    else         batch2d_dump_gather<float>(b->hbox[0], per, (const float *)b->V, b->vs[b->cur],
+                                            (size_t)b->ncell, dst, s);
+    return hipGetLastError() == hipSuccess ? 0 : GPL_EHIP;
+  }
+};
+
+}  // namespace
+
+int make_batch2d_engine(const gpl_config &cfg, const EngineEnv &env, Engine **out)
+{
+  *out = nullptr;
+  if (cfg.nz != 1) return GPL_EINVAL;              // unused directions hold 1 zone
+  long long total = 0;
+  if (int rc = batch_validate(&cfg, 1, &cfg.nx, &cfg.ny, &total)) return rc;
+  Batch2dEngine *e = new (std::nothrow) Batch2dEngine();
+  if (!e) return GPL_ENOMEM;
+  e->b = new (std::nothrow) gpl_batch();
+  if (!e->b) { delete e; return GPL_ENOMEM; }
+  int rc = c_guard(GPL_ENOMEM, [&] { return batch_init(e->b, &cfg, 1, &cfg.nx, &cfg.ny, total, &env); });
+  // a fixed launch shape (the autotuner belongs to gpl_batch_run)
+  e->b->tune_pending = false;
+  if (!rc) rc = dev_alloc(e->b->bytes, (void **)&e->diag, (size_t)(DIAG_BLOCKS + 1)*DIAG_MAXQ*sizeof(double));
+  if (rc) { delete e; return rc; }
+  e->bytes = e->b->bytes;
+  *out = e;
+  return 0;
+}
+
+}  // namespace gpl

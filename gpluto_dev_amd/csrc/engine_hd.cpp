@@ -144,7 +144,7 @@ public:
   int diagnostics(double *out) override
   {
     DiagBox d;
-    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 5;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 5; d.nb = 0;
     d.lo[0] = d.lo[1] = d.lo[2] = 0;
     d.hi[0] = g.nx - 1; d.hi[1] = g.ny - 1; d.hi[2] = g.nz - 1;
     d.ax = d.ay = d.az = 0.0;

@@ -350,7 +350,7 @@ public:
   int diagnostics(double *out) override
   {
     DiagBox d;
-    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 8;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.ncell = g.ncell; d.nv = 8; d.nb = 3;
     for (int q = 0; q < 3; q++) { d.lo[q] = g.lo[q]; d.hi[q] = g.hi[q]; }
     // face areas of the uniform Cartesian grid (Src/set_geometry.cpp:124,147,166)
     d.ax = 1.0*g.dy*g.dz; d.ay = g.dx*1.0*g.dz; d.az = g.dx*g.dy*1.0;

@@ -180,7 +180,8 @@ extern "C" int gpl_create(const gpl_config *cfg, gpl_ctx **out)
   }
   const EngineEnv env = {c->dev, c->stream, c->ctl, c->err};
   rc = cfg->physics == GPL_HD ? make_hd_engine(*cfg, env, &c->engine)
-                              : make_mhd3d_engine(*cfg, env, &c->engine);
+     : cfg->physics == GPL_MHD && cfg->dimensions == 2 ? make_batch2d_engine(*cfg, env, &c->engine)
+                                                      : make_mhd3d_engine(*cfg, env, &c->engine);
   if (rc) {
     free_all(c);
     delete c;

@@ -138,6 +138,7 @@ class StageUpdate:
         self.n = tuple(ini.n)
         self.mhd = cfg.physics != _lib.GPL_HD   # staggered field + CT (MHD, RMHD)
         self.nvar = 8 if self.mhd else 5      # NVAR (Src/MHD/mhd.hpp:21-41, Src/HD/hd.hpp:21-34)
+        self._nfaces = defs.dimensions if self.mhd else 0   # staggered components
 
     def close(self):
         if getattr(self, "_h", None):
@@ -159,14 +160,15 @@ class StageUpdate:
     # -- state transfer -----------------------------------------------------
     def upload(self, vc: np.ndarray, vs: np.ndarray | None = None,
                layout: int = _lib.GPL_LAYOUT_ACTIVE):
-        """vc[NVAR, nz, ny, nx] primitives; MHD: vs[3, nz, ny, nx] lower faces."""
+        """vc[NVAR, nz, ny, nx] primitives; MHD: vs[DIMENSIONS, nz, ny, nx] lower
+        faces (2D: B1, B2 on faces, B3 cell-centred in vc)."""
         vc = np.ascontiguousarray(vc, dtype=np.float64)
         if vc.shape[0] != self.nvar:
             raise ValueError(f"vc holds {vc.shape[0]} variables, expected {self.nvar}")
         arr = None
         if self.mhd:
-            comps = [np.ascontiguousarray(vs[d], dtype=np.float64) for d in range(3)]
-            arr = (C.c_void_p * 3)(*[c.ctypes.data for c in comps])
+            comps = [np.ascontiguousarray(vs[d], dtype=np.float64) for d in range(self._nfaces)]
+            arr = (C.c_void_p * 3)(*[c.ctypes.data for c in comps], *[None] * (3 - self._nfaces))
         _lib.check(self._L.gpl_upload(self._h, vc.ctypes.data, arr, layout), "gpl_upload")
 
     def download(self):
@@ -176,8 +178,9 @@ class StageUpdate:
         uc = np.empty((self.nvar, nz, ny, nx))
         vs, arr = None, None
         if self.mhd:
-            vs = np.empty((3, nz, ny, nx))
-            arr = (C.c_void_p * 3)(*[vs[d].ctypes.data for d in range(3)])
+            vs = np.empty((self._nfaces, nz, ny, nx))
+            arr = (C.c_void_p * 3)(*[vs[d].ctypes.data for d in range(self._nfaces)],
+                                   *[None] * (3 - self._nfaces))
         _lib.check(self._L.gpl_download(self._h, vc.ctypes.data, arr, uc.ctypes.data,
                                         _lib.GPL_LAYOUT_ACTIVE), "gpl_download")
         return vc, vs, uc
@@ -195,9 +198,10 @@ class StageUpdate:
         ptrs = (C.c_void_p * 3)(*[a.ctypes.data for a in arrs])
         _lib.check(self._L.gpl_download(self._h, None, ptrs, None, _lib.GPL_LAYOUT_ARY4D),
                    "gpl_download")
-        return [arrs[0][gz:gz + nz, gy:gy + ny, gx + nx:gx + nx + 1],
-                arrs[1][gz:gz + nz, gy + ny:gy + ny + 1, gx:gx + nx],
-                arrs[2][gz + nz:gz + nz + 1, gy:gy + ny, gx:gx + nx]]
+        slabs = [arrs[0][gz:gz + nz, gy:gy + ny, gx + nx:gx + nx + 1],
+                 arrs[1][gz:gz + nz, gy + ny:gy + ny + 1, gx:gx + nx],
+                 arrs[2][gz + nz:gz + nz + 1, gy:gy + ny, gx:gx + nx]]
+        return slabs[:self._nfaces]          # 2D MHD: B1 and B2
 
     # -- step state, dumps and restarts --------------------------------------
     def set_step(self, time: float, dt: float, nstep: int):

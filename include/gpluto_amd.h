@@ -64,7 +64,7 @@ extern "C" {
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
 typedef struct gpl_config {
-  int    physics;          /* GPL_MHD (3D), GPL_RMHD (3D, FP64) or GPL_HD (1D/2D/3D) */
+  int    physics;          /* GPL_MHD (2D/3D), GPL_RMHD (3D, FP64) or GPL_HD (1D/2D/3D) */
   int    dimensions;       /* DIMENSIONS; unused directions hold 1 zone */
   int    nx, ny, nz;       /* [Grid] active zones per direction */
   double xbeg[3], xend[3]; /* [Grid] domain; dx = (xend - xbeg)/n (Src/set_grid.cpp:196) */

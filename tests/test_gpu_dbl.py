@@ -17,10 +17,9 @@ pytestmark = pytest.mark.gpu
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import dbl_io  # noqa: E402
 
-# the single-domain engine runs 3D MHD/RMHD and 1D-3D HD (2D MHD runs as
-# periodic batches, which have no dump files); the 2D MHD fixture pins the
-# oracle's restatement only
-CASES = [c for c in restart_cases() if not c.startswith("mhd2d")]
+# every restart fixture: 1D HD, 2D MHD (one box of the batched kernels behind
+# gpl_create), 3D MHD and RMHD
+CASES = restart_cases()
 
 
 def _setup(p):

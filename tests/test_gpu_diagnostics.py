@@ -61,6 +61,42 @@ def test_divb_of_a_planted_face_perturbation():
     assert abs(d["divb_max"] - ax * delta) <= 1e-10 * ax * delta + 2 * base
 
 
+def test_2d_mhd_diagnostics():
+    """2D MHD (one box of the batched kernels behind gpl_create): min/max/Mach
+    of the downloaded state, div B = dBx1 + dBx2 with the 2D face areas
+    (no z term), on a box with walls and outflow sides."""
+    n = (30, 22, 1)
+    print(f"case diagnostics mhd2d n={n}", flush=True)
+    vc, vs = ic.mhd_turbulence(n, 8)
+    sides = ("reflective", "outflow", "outflow", "reflective", "outflow", "outflow")
+    g = StageUpdate(Definitions(dimensions=2), RuntimeSetup(n=n, sides=sides))
+    g.upload(vc, vs)
+    err, _, _ = g.run(4, 1e-3)
+    assert err == 0
+    d = g.diagnostics()
+    v, s, _ = g.download()
+    for q in range(8):
+        assert d["min"][q] == v[q].min() and d["max"][q] == v[q].max(), q
+    assert d["mach_max"] == _mach(v, 7)
+    assert d["divb_max"] <= 1e-12 * np.max(np.abs(s))
+    # the initial field with one face raised (an active-layout upload starts
+    # the upper boundary faces as images of face 0, which the periodic
+    # initial field satisfies)
+    g.upload(vc, vs)
+    base = g.diagnostics()["divb_max"]
+    assert base <= 1e-12 * np.max(np.abs(vs))
+    delta = 0.25
+    vs2 = vs.copy()
+    i, j = 9, 13
+    vs2[1, 0, j, i] += delta               # lower y-face of cell (i, j)
+    g.upload(vc, vs2)
+    d2 = g.diagnostics()
+    g.close()
+    ay = (1.0 / n[0]) * 1.0                # dx*1.0 on the unit square
+    # cells (i, j-1) and (i, j) see -+ay*delta on top of round-off
+    assert abs(d2["divb_max"] - ay * delta) <= 1e-10 * ay * delta + 2 * base
+
+
 def test_hd_diagnostics():
     n = 4096
     print(f"case diagnostics hd n={n}", flush=True)
