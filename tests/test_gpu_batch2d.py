@@ -214,3 +214,26 @@ def test_sides_every_box_matches_its_standalone_oracle_run(sides, ts, lim, solve
 def test_one_sided_periodic_direction_is_rejected():
     with pytest.raises(RuntimeError, match="invalid argument"):
         _batch([(16, 16)], sides=("periodic", "outflow") + ("outflow",) * 4)
+
+
+def test_2d_context_mixed_within_1e5_of_fp64():
+    """A 2D MHD gpl_create context (one box of these kernels) in the mixed form,
+    with walls and outflow sides, stays within 1e-5 relative L1 of its FP64 run."""
+    from gpluto_dev_amd.stage import StageUpdate
+    n = (96, 80, 1)
+    vc, vs = ic.mhd_turbulence(n, 12)
+    sides = ("reflective", "outflow", "outflow", "reflective", "outflow", "outflow")
+    out = {}
+    for prec in ("fp64", "mixed"):
+        print(f"case 2D context {prec}", flush=True)
+        g = StageUpdate(Definitions(dimensions=2), RuntimeSetup(n=n, sides=sides), precision=prec)
+        g.upload(vc, vs)
+        err, _, _ = g.run(20, 5e-4)
+        assert err == 0
+        out[prec] = g.download()
+        g.close()
+    (v64, s64, _), (v32, s32, _) = out["fp64"], out["mixed"]
+    for v in range(8):
+        l1 = np.sum(np.abs(v32[v] - v64[v])) / np.sum(np.abs(v64[v]))
+        assert l1 <= 1e-5, f"var {v}: rel L1 {l1:.2e}"
+    assert _divb_interior(s32) < 1e-10
