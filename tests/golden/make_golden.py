@@ -78,6 +78,8 @@ CASES = [
     ("rmhd3d_12x10x14_rk2_mc_hlld", "rmhd3d", "RK2", "MC_LIM", (12, 10, 14), 24, 5, 1e-3, 0.4, "hlld", True, 4 / 3),
     ("rmhd3d_14x12x10_rk3_vl_hlld", "rmhd3d", "RK3", "VANLEER_LIM", (14, 12, 10), 25, 4, 1e-3, 0.4, "hlld", True, 4 / 3),
     ("rmhd3d_10x12x12_rk2_mc_hlld_outflow", "rmhd3d", "RK2", "MC_LIM", (10, 12, 12), 26, 5, 1e-3, 0.4, "hlld", False, 4 / 3),
+    # RMHD between reflective walls, mixed with outflow and periodic directions
+    ("rmhd3d_12x10x10_rk2_mc_hll_walls", "rmhd3d", "RK2", "MC_LIM", (12, 10, 10), 29, 5, 1e-3, 0.4, "hll", False, 4 / 3),
 ]
 
 # pluto.ini [Boundary] per side (X1-beg, X1-end, ...) where not all periodic /
@@ -93,6 +95,8 @@ SIDES = {"hd1d_512_rk2_mc_hllc_walls": ("reflective", "reflective") + ("outflow"
          "hd3d_10x14x12_rk2_mc_hllc_mixed_sides": ("reflective", "outflow", "periodic", "periodic",
                                                    "outflow", "reflective"),
          "hd2d_36x28_rk3_vl_hllc_walls": ("reflective",) * 4 + ("outflow",) * 2,
+         "rmhd3d_12x10x10_rk2_mc_hll_walls": ("reflective", "reflective", "outflow", "reflective",
+                                              "periodic", "periodic"),
          "mhd2d_12x10_rk2_mc_walls_restart": ("reflective",) * 4 + ("outflow",) * 2,
          "hd1d_256_rk2_mc_hllc_walls_restart": ("reflective", "reflective") + ("outflow",) * 4}
 
@@ -119,7 +123,8 @@ RMHD_IC = {"rmhd3d_16_rk2_mc_hll": (3.0, 1.0), "rmhd3d_12x14x10_rk2_mc_hll": (2.
            "rmhd3d_12x10x14_rk2_mc_hllc": (3.0, 1.0), "rmhd3d_14x12x10_rk3_vl_hllc": (5.0, 3.0),
            "rmhd3d_10x12x12_rk2_mc_hllc_outflow": (2.0, 0.5),
            "rmhd3d_12x10x14_rk2_mc_hlld": (3.0, 1.0), "rmhd3d_14x12x10_rk3_vl_hlld": (5.0, 3.0),
-           "rmhd3d_10x12x12_rk2_mc_hlld_outflow": (2.0, 0.5)}
+           "rmhd3d_10x12x12_rk2_mc_hlld_outflow": (2.0, 0.5),
+           "rmhd3d_12x10x10_rk2_mc_hll_walls": (3.0, 1.0)}
 
 
 def _ic(name, cfg, n, seed, gamma):

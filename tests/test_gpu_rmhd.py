@@ -33,10 +33,11 @@ TOL = 5e-12
 TOL_SOLVER = {"hll": TOL, "hllc": TOL, "hlld": 5e-11}
 
 
-def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3, periodic=True, solver="hll"):
+def _stage(n, ts, lim, cfl=0.4, gamma=4 / 3, periodic=True, solver="hll", sides=None):
     return StageUpdate(Definitions(physics="RMHD", time_stepping=ts, limiter=lim),
                        RuntimeSetup(n=n, solver=solver, cfl=cfl, gamma=gamma,
-                                    boundary="periodic" if periodic else "outflow"))
+                                    boundary="periodic" if periodic else "outflow",
+                                    sides=tuple(sides) if sides else ()))
 
 
 def _compare(gv, gs, ov, osv, label, tol=TOL):
@@ -59,7 +60,7 @@ def test_gpu_matches_reference_fixture(name):
     p, a = load(name)
     print(f"case {name}", flush=True)
     g = _stage(tuple(p["n"]), p["time_stepping"], p["limiter"], p["cfl"], p["gamma"], p["periodic"],
-               p["solver"])
+               p["solver"], p.get("sides"))
     g.upload(a["vc_in"], a["vs_in"])
     err, t, _ = g.run(p["steps"], p["first_dt"])
     vc, vs, _ = g.download()
