@@ -72,24 +72,30 @@ def _link(cc: str, objs: list[str], out: str) -> None:
     os.replace(tmp, out)
 
 
-def build(verbose: bool = False) -> str:
-    objs = []
+def _compile(cc: str, objdir: str, s: str, extra: list[str], verbose: bool) -> str:
+    src = os.path.join(CSRC, s)
+    obj = os.path.join(objdir, s + ".o")
+    cmd = [cc, *flags(), *extra, "-c", src, "-o", obj]
+    if s.endswith(".cpp"):
+        cmd.insert(1, "-xhip")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed on {s}")
+    return obj
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     cc = hipcc()
-    for s, extra in SOURCES + [("copy_probe.hip", [])]:
-        src = os.path.join(CSRC, s)
-        obj = os.path.join(objdir, s + ".o")
-        cmd = [cc, *flags(), *extra, "-c", src, "-o", obj]
-        if s.endswith(".cpp"):
-            cmd.insert(1, "-xhip")
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {s}")
-        objs.append(obj)
+    units = SOURCES + [("copy_probe.hip", [])]
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda u: _compile(cc, objdir, u[0], u[1], verbose), units))
     _link(cc, objs[:-1], LIB)
     _link(cc, objs[-1:], PROBE_LIB)
     return LIB

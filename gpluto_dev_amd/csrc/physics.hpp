@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "crmath.hpp"
+
 #define GPL_HD __host__ __device__ __forceinline__
 
 namespace gpl {
@@ -737,11 +739,19 @@ GPL_HD void rmhd_cubic_solve(double b, double c, double d, double z[3])
   double arg = -1.5*R/(Q*sQ);
   arg = rmax(-1.0, arg);
   arg = rmin( 1.0, arg);
+  // acos, cos and sin correctly rounded, as the host libm returns them for
+  // all but ~1e-3 of the arguments (the device library's are faithful only).
+  // GPL_CUBIC_HOST_LIBM (host test builds) calls the host libm itself, so the
+  // rest of the arithmetic can be compared with the oracle bit for bit.
+#if defined(GPL_CUBIC_HOST_LIBM)
   const double theta = acos(arg)*one_3;
-  // cos and sin of one argument: the reference's g++ build evaluates them with
-  // one sincos call, whose last bit can differ from separate cos/sin calls
   double sn1, cs;
-  sincos(theta, &sn1, &cs);
+  sincos(theta, &sn1, &cs);   // the reference's g++ build merges cos and sin
+#else
+  const double theta = cr::acos_cr(arg)*one_3;
+  double sn1, cs;
+  cr::sincos_cr(theta, &sn1, &cs);
+#endif
   const double sn = sqrt3*sn1;
   const double p  = -b*one_3;
   z[0] = -sQ*(cs + sn) + p;
