@@ -33,7 +33,7 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
            "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics",
            "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl",
-           "gpl_write_dbl_async", "gpl_wait_dbl")
+           "gpl_write_dbl_async", "gpl_wait_dbl", "gpl_set_zone_reports", "gpl_get_zone_reports")
 
 
 class GplConfig(C.Structure):
@@ -136,6 +136,10 @@ def load() -> C.CDLL:
     L.gpl_set_c2p_network.restype = ip
     L.gpl_diagnostics.argtypes = [vp, dp, dp, dp, dp]
     L.gpl_diagnostics.restype = ip
+    L.gpl_set_zone_reports.argtypes = [vp, ip]
+    L.gpl_set_zone_reports.restype = ip
+    L.gpl_get_zone_reports.argtypes = [vp, vp, vp, C.POINTER(C.c_ulonglong)]
+    L.gpl_get_zone_reports.restype = ip
     _lib = L
     return L
 

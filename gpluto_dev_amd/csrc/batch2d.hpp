@@ -57,6 +57,7 @@ struct Batch2dBufs {
   const Item2 *tiles;
   int nx_items, ny_items, ntiles;
   int ncell;
+  ZoneRep zr;        // this stage's zone-by-zone reports (zr.err null: off)
 };
 
 enum Batch2dPhase { B2_SWEEP_X = 0, B2_SWEEP_Y, B2_EMF, B2_FINISH };

@@ -386,6 +386,7 @@ void k2_finish(Batch2dBufs<R> b, RKW w, int first_stage, double gmm1, double sma
       b.U[5*n + c] = u[4];
       if (b2::active(bx, i, j)) eall |= e;
     }
+    if (b2::active(bx, i, j)) zone_report(b.zr, c, e);
 #pragma unroll
     for (int s = 0; s < 8; s++) b.V[s*n + c] = (R)v[s];
   }

@@ -29,7 +29,11 @@ struct EngineEnv {
 
 class Engine {
 public:
-  virtual ~Engine() {}
+  virtual ~Engine()
+  {
+    if (zerr) (void)hipFree(zerr);
+    if (zcnt) (void)hipFree(zcnt);
+  }
   virtual int upload(const double *vc, const double *const *vs, int layout) = 0;
   virtual int download(double *vc, double *const *vs, double *uc, int layout) = 0;
   // Enqueue one AdvanceStep; advance_dt != 0 also applies NextTimeStep and
@@ -51,6 +55,31 @@ public:
   // RMHD) each staggered component with faces 0..n along its direction.
   virtual int dump_gather(double *dst, hipStream_t s) = 0;
   size_t bytes = 0;   // device memory held
+
+  // Zone-by-zone error reports (off until set_zone_reports(1)): one uint32
+  // array of zone_cells() entries per RK stage and the counters [stage 0..2,
+  // total].  An engine that supports them returns its device cell count from
+  // zone_cells(), maps a stage array to the active zones in zone_extract(),
+  // passes zone_rep(stage) to its ConsToPrim kernel and resets the stage
+  // counters at the start of every step (launch_zone_reset).
+  uint32_t *zerr = nullptr;
+  unsigned long long *zcnt = nullptr;
+  virtual size_t zone_cells() const { return 0; }
+  virtual size_t zone_active() const { return 0; }   // active zones of the domain
+  virtual int nstages() const = 0;
+  // dst[(k*ny + j)*nx + i] for the active zones, from a stage array on the host
+  virtual void zone_extract(const uint32_t *src, uint32_t *dst) const { (void)src; (void)dst; }
+  virtual void zone_changed() {}
+  ZoneRep zone_rep(int stage) const
+  {
+    if (!zerr) return ZoneRep{nullptr, nullptr, nullptr};
+    return ZoneRep{zerr + (size_t)stage*zone_cells(), zcnt + stage, zcnt + 3};
+  }
+  // (gpl_api.cpp) allocate and clear, or free, the report buffers
+  int set_zone_reports(int on);
+  // (gpl_api.cpp) stage arrays of the last step (active zones), stage counts,
+  // total; returns the number of stages
+  int get_zone_reports(uint32_t *err, unsigned long long *counts, unsigned long long *total);
 };
 
 // pluto.ini [Boundary] type of side s (X1-beg, X1-end, X2-beg, ...)

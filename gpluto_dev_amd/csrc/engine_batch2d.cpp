@@ -65,6 +65,10 @@ struct gpl_batch {
   double *cdt_part = nullptr;
   uint32_t *err = nullptr;
   StepCtl *ctl = nullptr;
+  // zone-by-zone reports (a batch behind a gpl_ctx only): [stage][ncell] and
+  // the counters [stage 0..2, total], owned by the engine; null while off
+  uint32_t *zerr = nullptr;
+  unsigned long long *zcnt = nullptr;
   // work lists
   BoxDesc *box = nullptr;
   Item4 *xitems = nullptr;
@@ -430,9 +434,11 @@ static int grid_for(const gpl_batch *b, int groups)
 
 template <typename R>
 static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, double *vin,
-                          double *vout, const double *vbase, RKW w, int first_stage)
+                          double *vout, const double *vbase, RKW w, int first_stage, int stage_idx)
 {
   Batch2dBufs<R> q;
+  q.zr = b->zerr ? ZoneRep{b->zerr + (size_t)stage_idx*b->ncell, b->zcnt + stage_idx, b->zcnt + 3}
+                 : ZoneRep{nullptr, nullptr, nullptr};
   q.V = (R *)b->V; q.Uin = uin; q.U = uout; q.U0 = b->U;
   q.Vin = vin; q.Vout = vout; q.Bs0 = vbase;
   q.FD = (R *)b->FD; q.E = (R *)b->E; q.dU = (R *)b->dU; q.Cdt = (R *)b->Cdt;
@@ -460,10 +466,10 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, double 
 
 // one AdvanceStep of every box (Src/rk_step.cpp:88-720), then per-box dt
 static void enqueue_stage_r(gpl_batch *b, const double *uin, double *uout, double *vin,
-                            double *vout, const double *vbase, RKW w, int first)
+                            double *vout, const double *vbase, RKW w, int first, int stage_idx)
 {
-  if (b->fp64) enqueue_stage<double>(b, uin, uout, vin, vout, vbase, w, first);
-  else         enqueue_stage<float>(b, uin, uout, vin, vout, vbase, w, first);
+  if (b->fp64) enqueue_stage<double>(b, uin, uout, vin, vout, vbase, w, first, stage_idx);
+  else         enqueue_stage<float>(b, uin, uout, vin, vout, vbase, w, first, stage_idx);
 }
 
 static void enqueue_step(gpl_batch *b, int advance_dt)
@@ -472,13 +478,14 @@ static void enqueue_step(gpl_batch *b, int advance_dt)
   const RKW w2 = b->cfg.time_stepping == GPL_RK3 ? RKW{0.75, 0.25, 0.25} : RKW{0.5, 0.5, 0.5};
   const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
   double *f0 = b->vs[b->cur], *f1 = b->vs[(b->cur + 1) % 3], *f2 = b->vs[(b->cur + 2) % 3];
-  enqueue_stage_r(b, b->U, b->Us, f0, f1, f0, w1, 1);            // -> Us, f1
+  if (b->zcnt) launch_zone_reset(b->zcnt, b->stream);
+  enqueue_stage_r(b, b->U, b->Us, f0, f1, f0, w1, 1, 0);            // -> Us, f1
   if (b->cfg.time_stepping != GPL_RK3) {
-    enqueue_stage_r(b, b->Us, b->U, f1, f2, f0, w2, 0);          // -> U, f2
+    enqueue_stage_r(b, b->Us, b->U, f1, f2, f0, w2, 0, 1);          // -> U, f2
     b->cur = (b->cur + 2) % 3;
   } else {
-    enqueue_stage_r(b, b->Us, b->Us, f1, f2, f0, w2, 0);         // -> Us, f2
-    enqueue_stage_r(b, b->Us, b->U, f2, f1, f0, w3, 0);          // -> U, f1
+    enqueue_stage_r(b, b->Us, b->Us, f1, f2, f0, w2, 0, 1);         // -> Us, f2
+    enqueue_stage_r(b, b->Us, b->U, f2, f1, f0, w3, 0, 2);          // -> U, f1
     b->cur = (b->cur + 1) % 3;
   }
   batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
@@ -646,6 +653,27 @@ public:
   int graph_period() const override { return ::graph_period(b); }
   const void *state_key() const override { return b->vs[b->cur]; }
   int nvar() const override { return 8; }
+  int nstages() const override { return b->cfg.time_stepping == GPL_RK3 ? 3 : 2; }
+  size_t zone_cells() const override { return (size_t)b->ncell; }
+  size_t zone_active() const override
+  {
+    const BoxDesc &d = b->hbox[0];
+    return (size_t)(d.hi[0] - d.lo[0] + 1)*(d.hi[1] - d.lo[1] + 1);
+  }
+  void zone_extract(const uint32_t *src, uint32_t *dst) const override
+  {
+    const BoxDesc &d = b->hbox[0];
+    const int nx = d.hi[0] - d.lo[0] + 1, ny = d.hi[1] - d.lo[1] + 1;
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        dst[(size_t)j*nx + i] = src[d.off + (size_t)(j + d.lo[1])*d.nx + (i + d.lo[0])];
+  }
+  void zone_changed() override
+  {
+    b->zerr = zerr;
+    b->zcnt = zcnt;
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  }
 
   int diagnostics(double *out) override
   {

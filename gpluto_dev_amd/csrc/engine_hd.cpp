@@ -118,9 +118,10 @@ public:
     const int nst = cfg.time_stepping == GPL_RK3 ? 3 : 2;
     static const int kid[3] = {K_HD_X, K_HD_Y, K_HD_Z};
     double *in = cur_v, *out = cur_v == VA ? VB : VA;
+    if (zcnt) launch_zone_reset(zcnt, env.stream);
     for (int st = 1; st <= nst; st++) {
       const RKW w = st == 1 ? w1 : (st == 2 ? w2 : w3);
-      HdBufs b = {in, out, U, U0, dU, Cdt, cdt_part, env.err};
+      HdBufs b = {in, out, U, U0, dU, Cdt, cdt_part, env.err, zone_rep(st - 1)};
       for (int d = 0; d < g.dims; d++) {
         if (t) t->begin(kid[d]);
         hd_launch(d, b, g, env.ctl, w, st == 1, st == 1, cfg.limiter, cfg.solver, cfg.gamma,
@@ -140,6 +141,14 @@ public:
   const void *state_key() const override { return cur_v; }
 
   int nvar() const override { return 5; }
+  int nstages() const override { return cfg.time_stepping == GPL_RK3 ? 3 : 2; }
+  // the device cells are the active zones, i fastest
+  size_t zone_cells() const override { return n; }
+  size_t zone_active() const override { return n; }
+  void zone_extract(const uint32_t *src, uint32_t *dst) const override
+  {
+    for (size_t c = 0; c < n; c++) dst[c] = src[c];
+  }
 
   int diagnostics(double *out) override
   {

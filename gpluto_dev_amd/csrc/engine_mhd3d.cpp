@@ -281,7 +281,8 @@ public:
 
   template <typename R>
   void enqueue_stage(const double *uin, double *uout, double *vin, double *vout,
-                     const double *vbase, RKW w, int first_stage, int want_dt, LaunchTimer *t)
+                     const double *vbase, RKW w, int first_stage, int want_dt, int stage_idx,
+                     LaunchTimer *t)
   {
     // Boundary (Src/rk_step.cpp: at the start of every stage) on a padded box
     if (padded) mhd3d_boundary<R>(g, side, (R *)V, vin, env.stream);
@@ -294,6 +295,7 @@ public:
     b.emf_march = emf_march;
     b.phys = rmhd ? 1 : 0;
     b.net = net_on ? net : nullptr;
+    b.zr = zone_rep(stage_idx);
     static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
     static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
     static const int xy_seq[] = {K_SWEEPXY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
@@ -318,15 +320,16 @@ public:
     const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
     const bool rk3 = cfg.time_stepping == GPL_RK3;
     double *f0 = vs[cur], *f1 = vs[(cur + 1) % 3], *f2 = vs[(cur + 2) % 3];
+    if (zcnt) launch_zone_reset(zcnt, env.stream);
     // stage 1: U^n, faces f0 -> Us, f1
-    stage(U, Us, f0, f1, f0, w1, 1, t);
+    stage(U, Us, f0, f1, f0, w1, 1, 0, t);
     if (!rk3) {
       // stage 2: 1/2 U^n + 1/2 (Us + dt L(Us)) -> U, f2
-      stage(Us, U, f1, f2, f0, w2, 0, t);
+      stage(Us, U, f1, f2, f0, w2, 0, 1, t);
       cur = (cur + 2) % 3;
     } else {
-      stage(Us, Us, f1, f2, f0, w2, 0, t);     // -> Us, f2
-      stage(Us, U, f2, f1, f0, w3, 0, t);      // -> U, f1
+      stage(Us, Us, f1, f2, f0, w2, 0, 1, t);     // -> Us, f2
+      stage(Us, U, f2, f1, f0, w3, 0, 2, t);      // -> U, f1
       cur = (cur + 1) % 3;
     }
     if (t) t->begin(K_DT);
@@ -335,10 +338,10 @@ public:
   }
 
   void stage(const double *uin, double *uout, double *vin, double *vout, const double *vbase,
-             RKW w, int first, LaunchTimer *t)
+             RKW w, int first, int stage_idx, LaunchTimer *t)
   {
-    if (fp64) enqueue_stage<double>(uin, uout, vin, vout, vbase, w, first, first, t);
-    else      enqueue_stage<float>(uin, uout, vin, vout, vbase, w, first, first, t);
+    if (fp64) enqueue_stage<double>(uin, uout, vin, vout, vbase, w, first, first, stage_idx, t);
+    else      enqueue_stage<float>(uin, uout, vin, vout, vbase, w, first, first, stage_idx, t);
   }
 
   // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
@@ -346,6 +349,16 @@ public:
   const void *state_key() const override { return vs[cur]; }
 
   int nvar() const override { return 8; }
+  int nstages() const override { return cfg.time_stepping == GPL_RK3 ? 3 : 2; }
+  size_t zone_cells() const override { return (size_t)g.ncell; }
+  size_t zone_active() const override { return (size_t)na[0]*na[1]*na[2]; }
+  void zone_extract(const uint32_t *src, uint32_t *dst) const override
+  {
+    for (int k = 0; k < na[2]; k++)
+      for (int j = 0; j < na[1]; j++)
+        for (int i = 0; i < na[0]; i++)
+          dst[((size_t)k*na[1] + j)*na[0] + i] = src[dev_cell(i, j, k)];
+  }
 
   int diagnostics(double *out) override
   {

@@ -562,6 +562,86 @@ extern "C" uint32_t gpl_run(gpl_ctx *c, int nsteps, double first_dt, double *tim
                   [&] { return gpl_run_impl(c, nsteps, first_dt, time_out, dt_out); });
 }
 
+// ---------------------------------------------------------------------------
+// zone-by-zone error reports
+// ---------------------------------------------------------------------------
+int Engine::set_zone_reports(int on)
+{
+  const size_t cells = zone_cells();
+  if (!cells) return GPL_EUNSUPPORTED;
+  if (zerr) { (void)hipFree(zerr); zerr = nullptr; bytes -= 3*cells*sizeof(uint32_t); }
+  if (zcnt) { (void)hipFree(zcnt); zcnt = nullptr; bytes -= 4*sizeof(unsigned long long); }
+  if (on) {
+    if (dev_alloc(bytes, (void **)&zerr, 3*cells*sizeof(uint32_t)) ||
+        dev_alloc(bytes, (void **)&zcnt, 4*sizeof(unsigned long long))) {
+      zone_changed();
+      return GPL_ENOMEM;
+    }
+    if (hipMemset(zerr, 0, 3*cells*sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(zcnt, 0, 4*sizeof(unsigned long long)) != hipSuccess)
+      return GPL_EHIP;
+  }
+  zone_changed();
+  return 0;
+}
+
+int Engine::get_zone_reports(uint32_t *err, unsigned long long *counts, unsigned long long *total)
+{
+  if (!zerr) return GPL_EINVAL;
+  const int ns = nstages();
+  const size_t cells = zone_cells();
+  unsigned long long h[4];
+  if (hipMemcpy(h, zcnt, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return GPL_EHIP;
+  if (counts) for (int s = 0; s < ns; s++) counts[s] = h[s];
+  if (total) *total = h[3];
+  if (err) {
+    std::vector<uint32_t> hs(cells);
+    const size_t nact = zone_active();
+    for (int s = 0; s < ns; s++) {
+      if (hipMemcpy(hs.data(), zerr + (size_t)s*cells, cells*sizeof(uint32_t),
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return GPL_EHIP;
+      zone_extract(hs.data(), err + (size_t)s*nact);
+    }
+  }
+  return ns;
+}
+
+static int gpl_set_zone_reports_impl(gpl_ctx *c, int enable)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
+  const size_t before = c->engine->bytes;
+  const int rc = c->engine->set_zone_reports(enable);
+  if (c->graph_exec) {               // the captured kernels carry the old pointers
+    hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  c->bytes = c->bytes - before + c->engine->bytes;
+  return rc;
+}
+
+extern "C" int gpl_set_zone_reports(gpl_ctx *c, int enable)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_set_zone_reports_impl(c, enable); });
+}
+
+static int gpl_get_zone_reports_impl(gpl_ctx *c, uint32_t *err, unsigned long long *counts,
+                                     unsigned long long *total)
+{
+  if (!c) return GPL_EINVAL;
+  hipSetDevice(c->dev);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return GPL_EHIP;
+  return c->engine->get_zone_reports(err, counts, total);
+}
+
+extern "C" int gpl_get_zone_reports(gpl_ctx *c, uint32_t *err, unsigned long long *counts,
+                                    unsigned long long *total)
+{
+  return c_guard<int>(GPL_ENOMEM, [&] { return gpl_get_zone_reports_impl(c, err, counts, total); });
+}
+
 static int gpl_set_c2p_network_impl(gpl_ctx *c, const float *weights)
 {
   if (!c) return GPL_EINVAL;

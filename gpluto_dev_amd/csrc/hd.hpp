@@ -26,6 +26,7 @@ struct HdBufs {
   double *Cdt;        // [n]    C_dt accumulator (dims > 1)
   double *cdt_part;   // per-block maxima of C_dt (hd_blocks entries)
   uint32_t *err;
+  ZoneRep zr;         // this stage's zone-by-zone reports (zr.err null: off)
 };
 
 // Launch the sweep along direction dir (0..dims-1).  The last direction also

@@ -49,7 +49,7 @@ void k_hd_sweep(HdBox g, const double *__restrict__ V, double *__restrict__ Vout
                 double *__restrict__ U, double *__restrict__ U0, double *__restrict__ dU,
                 double *__restrict__ Cdt, const StepCtl *__restrict__ ctl, RKW w,
                 int first_stage, int want_cdt, double *__restrict__ cdt_part, double gamma,
-                double small_rho, double small_prs, uint32_t *__restrict__ err)
+                double small_rho, double small_prs, uint32_t *__restrict__ err, ZoneRep zr)
 {
   const int c = blockIdx.x*blockDim.x + threadIdx.x;
   double cdt_max = -1.0;
@@ -141,7 +141,9 @@ void k_hd_sweep(HdBox g, const double *__restrict__ V, double *__restrict__ Vout
           u[nv] = w.w0*U0[o] + w.w1*U[o] + w.wR*du[nv];
         }
       }
-      e |= hd_cons_to_prim(u, gamma - 1.0, small_rho, small_prs, vn);
+      const uint32_t ce = hd_cons_to_prim(u, gamma - 1.0, small_rho, small_prs, vn);
+      e |= ce;
+      zone_report(zr, c, ce);
 #pragma unroll
       for (int nv = 0; nv < 5; nv++) { U[nv*n + c] = u[nv]; Vout[nv*n + c] = vn[nv]; }
       if (want_cdt) cdt_max = cd;
@@ -163,7 +165,7 @@ static void launch_t(const HdBufs &b, const HdBox &g, const StepCtl *ctl, RKW w,
 {
   hipLaunchKernelGGL((k_hd_sweep<D, LAST, LIM, SOLVER>), dim3(hd_blocks(g)), dim3(hd_block_size(g)),
                      0, s, g, b.Vin, b.Vout, b.U, b.U0, b.dU, b.Cdt, ctl, w, first_stage, want_dt,
-                     b.cdt_part, gamma, small_rho, small_prs, b.err);
+                     b.cdt_part, gamma, small_rho, small_prs, b.err, b.zr);
 }
 
 template <int D, bool LAST>

@@ -47,6 +47,16 @@ void k_dt(const double *__restrict__ part, int nparts, double dims, StepCtl *ctl
   }
 }
 
+__global__ void k_zone_reset(unsigned long long *counts)
+{
+  if (threadIdx.x < 3) counts[threadIdx.x] = 0ull;
+}
+
+void launch_zone_reset(unsigned long long *counts, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zone_reset, dim3(1), dim3(64), 0, s, counts);
+}
+
 int mhd3d_blocks(const Box3 &g) { return (g.ncell + 255)/256; }
 
 // blocks of the z march (one C_dt partial maximum each)

@@ -1071,7 +1071,7 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               double *__restrict__ U, R *__restrict__ V,
               const StepCtl *__restrict__ ctl, RKW w, int first_stage,
               double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err,
-              const float *__restrict__ net)
+              const float *__restrict__ net, ZoneRep zr)
 {
   // no early exit: the learned con2prim guess is evaluated by whole wavefronts
   const int c0 = blockIdx.x*blockDim.x + threadIdx.x;
@@ -1125,7 +1125,10 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
       sto(U, c, u8[0]); sto(U + n, c, u8[1]); sto(U + 2*n, c, u8[2]); sto(U + 3*n, c, u8[3]);
       sto(U + 4*n, c, u8[7]);
     }
-    if (e && is_active(g, p.i, p.j, p.k)) atomicOr(err, e);
+    if (is_active(g, p.i, p.j, p.k)) {
+      if (e) atomicOr(err, e);
+      zone_report(zr, c, e);
+    }
   } else {
     const uint32_t e = mhd_cons_to_prim(u, b, gmm1, small_rho, small_prs, v);
     if (!live) return;
@@ -1134,6 +1137,7 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
       for (int nv = 0; nv < 5; nv++) sto(U + nv*n, c, u[nv]);
       if (is_active(g, p.i, p.j, p.k)) atomicOr(err, e);
     }
+    if (is_active(g, p.i, p.j, p.k)) zone_report(zr, c, e);
   }
 #pragma unroll
   for (int nv = 0; nv < 8; nv++) sto(V + nv*n, c, (R)v[nv]);
@@ -1233,7 +1237,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     hipLaunchKernelGGL((k_finish<R, PH>), dim3(nb), dim3(T), 0, s,
                        g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
                        PH == 1 ? gamma : gamma - 1.0, small_rho, small_prs, b.err,
-                       PH == 1 ? b.net : nullptr);
+                       PH == 1 ? b.net : nullptr, b.zr);
     break;
   default:
     break;

@@ -53,12 +53,12 @@ def _synth(c: np.ndarray, n, shift, kmax: int = KMAX) -> np.ndarray:
           for d in range(3)]
     # c[kx, ky, kz]
     t1 = np.einsum("xyz,cz->xyc", c, ex[2])           # [kx, ky, z]
-    t2 = np.einsum("xyc,by->xbc", t1, ex[1])          # [kx, y, z]
-    nx = n[0]
-    t2r = t2.reshape(t2.shape[0], -1)
-    out = ex[0].real @ t2r.real - ex[0].imag @ t2r.imag  # [x, y*z]
-    out = out.reshape(nx, n[1], n[2])
-    return np.ascontiguousarray(out.transpose(2, 1, 0))
+    t2 = np.einsum("xyc,by->cbx", t1, ex[1])          # [z, y, kx]
+    t2 = t2.reshape(-1, t2.shape[2])
+    # Re(t2 . ex0^T) as two real matrix products, written in [z*y, x] order
+    out = np.ascontiguousarray(t2.real) @ np.ascontiguousarray(ex[0].real.T)
+    out -= np.ascontiguousarray(t2.imag) @ np.ascontiguousarray(ex[0].imag.T)
+    return out.reshape(n[2], n[1], n[0])
 
 
 def _flat_modes(c: np.ndarray, dims: int) -> np.ndarray:
@@ -240,6 +240,46 @@ def hd_blast(n, seed: int, gamma: float = 1.4):
     r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 * (ny > 1) + (z - c[2]) ** 2 * (nz > 1)
     vc[4][r2 < 0.04] = 10.0
     return vc
+
+
+def hd_vacuum(n, vmag: float, prs: float):
+    """Two streams leaving x = 0.5 at speed vmag (1D or 2D, n = (nx, ny, 1)):
+    rho = 1 + 0.2 cos(2 pi y), p = prs, v1 = -vmag / +vmag left / right of
+    the centre, and in 2D a transverse v2 = vmag/2 sin(2 pi y) near x = 0.5.
+    The centre empties, so ConsToPrim meets negative pressures and energies
+    (its small_prs floor).  Returns vc[5, 1, ny, nx]."""
+    nx, ny, _ = n
+    y, x = np.meshgrid((np.arange(ny) + 0.5) / ny, (np.arange(nx) + 0.5) / nx, indexing="ij")
+    vc = np.zeros((5, 1, ny, nx))
+    vc[0, 0] = 1.0 + (0.2 * np.cos(2 * np.pi * y) if ny > 1 else 0.0)
+    vc[1, 0] = np.where(x < 0.5, -vmag, vmag)
+    if ny > 1:
+        vc[2, 0] = 0.5 * vmag * np.sin(2 * np.pi * y) * np.exp(-((x - 0.5) / 0.1) ** 2)
+    vc[4] = prs
+    return vc
+
+
+def mhd_lowbeta_blast(n, p_in: float, p_out: float, b0: float = 1.0, r0: float = 0.2):
+    """Blast in a strongly magnetised box (2D or 3D, n = (nx, ny, nz)): rho = 1,
+    v = 0, p = p_in inside radius r0 of the centre and p_out outside, uniform
+    B = b0 (1, 1, 0)/sqrt(2) on faces and cells.  With p_out << b0^2 the
+    pressure recovered from the total energy goes negative behind the front
+    (ConsToPrim's small_prs floor).  Returns vc[8, nz, ny, nx], vs[dims, ...]."""
+    nx, ny, nz = n
+    dims = 3 if nz > 1 else 2
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny,
+                          (np.arange(nx) + 0.5) / nx, indexing="ij")
+    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2 + ((z - 0.5) ** 2 if dims == 3 else 0.0)
+    vc = np.zeros((8, nz, ny, nx))
+    vc[0] = 1.0
+    vc[7] = np.where(r2 < r0 * r0, p_in, p_out)
+    b = b0 / np.sqrt(2.0)
+    vc[4] = b
+    vc[5] = b
+    vs = np.zeros((dims, nz, ny, nx))
+    vs[0] = b
+    vs[1] = b
+    return vc, vs
 
 
 def divb_max(vs: np.ndarray) -> float:

@@ -270,6 +270,23 @@ class StageUpdate:
                                            C.byref(mach), C.byref(divb)), "gpl_diagnostics")
         return {"min": vmin, "max": vmax, "mach_max": mach.value, "divb_max": divb.value}
 
+    def set_zone_reports(self, on: bool = True):
+        """Zone-by-zone error reports (the reference's Data::err and the
+        flagged-zone counts of Where): off by default."""
+        _lib.check(self._L.gpl_set_zone_reports(self._h, 1 if on else 0), "gpl_set_zone_reports")
+
+    def zone_reports(self):
+        """(err[stage, nz, ny, nx] uint32 of the last step, flagged zones per
+        stage of the last step, flagged zones since the reports were enabled)."""
+        nx, ny, nz = self.n
+        err = np.zeros((3, nz, ny, nx), dtype=np.uint32)
+        counts = np.zeros(3, dtype=np.uint64)
+        total = C.c_ulonglong(0)
+        ns = self._L.gpl_get_zone_reports(self._h, err.ctypes.data_as(C.c_void_p),
+                                          counts.ctypes.data_as(C.c_void_p), C.byref(total))
+        _lib.check(min(ns, 0), "gpl_get_zone_reports")
+        return err[:ns], counts[:ns].astype(np.int64), int(total.value)
+
     # -- measurement ----------------------------------------------------------
     def set_timing(self, on: bool):
         _lib.check(self._L.gpl_set_timing(self._h, 1 if on else 0), "gpl_set_timing")

@@ -184,6 +184,25 @@ uint32_t gpl_run(gpl_ctx *ctx, int nsteps, double first_dt, double *time_out, do
  * formulas bit for bit: min and max do not depend on reduction order. */
 int gpl_diagnostics(gpl_ctx *ctx, double *vmin, double *vmax, double *mach_max, double *divb_max);
 
+/* Zone-by-zone error reports: the reference's Data::err (Src/structs.hpp:678)
+ * as ConsToPrim3D writes it in every zone (Src/mappers3D.cpp:115-176), and the
+ * flagged-zone counts that Where() adds to Data::err_count
+ * (Src/debug_tools.cpp:566-684; reset every step, Src/main.cpp:423).  Off by
+ * default: the stepping calls then return only the OR of the ERR_* bits (the
+ * reference's err_tot).  enable != 0 allocates one uint32 per zone and RK
+ * stage and makes every ConsToPrim write each active zone's bits (0 when none)
+ * and count the flagged zones; 0 frees them.  A Riemann solver's NaN report
+ * (ERR_NAN | ERR_QUIT, fatal in the reference) is not recorded per zone.
+ * GPL_EUNSUPPORTED for contexts without zone reports (none at present). */
+int gpl_set_zone_reports(gpl_ctx *ctx, int enable);
+/* After a stepping call: err[s][nz][ny][nx] = the bits ConsToPrim3D left in
+ * stage s of the last step (s < 2 for RK2, s < 3 for RK3; may be NULL),
+ * counts[s] = the zones flagged in stage s of the last step (may be NULL),
+ * *total = the zones flagged since the reports were enabled (may be NULL).
+ * Returns the number of stages, or GPL_EINVAL when the reports are off. */
+int gpl_get_zone_reports(gpl_ctx *ctx, uint32_t *err, unsigned long long *counts,
+                         unsigned long long *total);
+
 /* Kernel timing with HIP events on the library's stream: enable, then query
  * the average duration of each kernel family over the recorded launches. */
 int gpl_set_timing(gpl_ctx *ctx, int enable);

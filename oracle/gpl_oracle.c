@@ -17,6 +17,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ---- reference macros (Src/macros.hpp:12-14,191-201) ------------------ */
 #define MAX(a,b)  ( (a) >= (b) ? (a) : (b) )
@@ -58,10 +61,19 @@ struct gplo_state {
   /* UCT face data, Src/MHD/CT/ct_emf.cpp:63-147 */
   double *e[3][3], *aL[3], *aR[3], *dL[3], *dR[3];
   double *Ee[3];
-  /* pencil scratch */
+  /* pencil scratch, one set per thread (pstride doubles apart; pcmax and
+     pBn pstride/NVMAX apart) */
   double *pv, *pvL, *pvR, *pflux, *pcmax, *pBn, *prhs;
+  size_t pstride;
+  int nthr;
   double *qv[2], *qL[2], *qR[2];
   int stage;
+  /* Data::err (Src/structs.hpp:678) of the active zones as ConsToPrim3D
+     leaves it in each stage of the last step ([3][nz][ny][nx]), and the
+     flagged-zone counts Where() adds to Data::err_count
+     (Src/debug_tools.cpp:587-593; reset per step, Src/main.cpp:423) */
+  uint32_t *zerr;
+  unsigned long long zcount[3], zcount_total;
 };
 
 #define IC(s,i,j,k)  (((size_t)(k)*(s)->nt[1] + (size_t)(j))*(s)->nt[0] + (size_t)(i))
@@ -595,6 +607,18 @@ static void sort_array(double *z, int n)
   }
 }
 
+/* acos and sin/cos of the resolvent cubic: the host libm unless a test
+   substitutes other implementations (gplo_set_cubic_math), to attribute a
+   difference to the math library alone */
+static double (*cubic_acos)(double) = NULL;
+static void (*cubic_sincos)(double, double *, double *) = NULL;
+
+void gplo_set_cubic_math(double (*acos_fn)(double), void (*sincos_fn)(double, double *, double *))
+{
+  cubic_acos = acos_fn;
+  cubic_sincos = sincos_fn;
+}
+
 /* CubicSolve, Src/RMHD/rmhd.hpp:390-452 (three real roots, trigonometric form) */
 static void cubic_solve(double b, double c, double d, double z[3])
 {
@@ -607,9 +631,16 @@ static void cubic_solve(double b, double c, double d, double z[3])
   double arg = -1.5*R/(Q*sQ);
   arg = MAX(-1.0, arg);
   arg = MIN( 1.0, arg);
-  double theta = acos(arg)*one_3;
-  double cs = cos(theta);
-  double sn = sqrt3*sin(theta);
+  double theta = (cubic_acos ? cubic_acos(arg) : acos(arg))*one_3;
+  double cs, sn;
+  if (cubic_sincos) {
+    double s1;
+    cubic_sincos(theta, &s1, &cs);
+    sn = sqrt3*s1;
+  } else {
+    cs = cos(theta);
+    sn = sqrt3*sin(theta);
+  }
   double p  = -b*one_3;
   z[0] = -sQ*(cs + sn) + p;
   z[1] = -sQ*(cs - sn) + p;
@@ -1591,109 +1622,130 @@ static void boundary(gplo_state *s)
  * RightHandSide + dUc accumulation (Src/update_stage.cpp:96-187 and the
  * J/K analogues).
  * ====================================================================== */
-static uint32_t sweep(gplo_state *s, int dir, double dt)
+/* one pencil (a, b) of the sweep along dir, with the calling thread's scratch */
+static uint32_t sweep_pencil(gplo_state *s, int dir, int a, int b, double dtdx, double *v,
+                             double *vL, double *vR, double *flux, double *cmax, double *Bn)
 {
   const int nvar = s->nv, mhd = is_mhd(s);
   const int t1 = (dir + 1)%3, t2 = (dir + 2)%3;
-  const int ntr = mhd ? NG : 0;           /* Src/update_stage.cpp:80-90 */
   const int n = s->nt[dir];
   uint32_t err = 0;
-  int lo[3], hi[3], a, b, i, nv;
-  for (int d = 0; d < 3; d++) { lo[d] = s->beg[d]; hi[d] = s->end[d]; }
-  for (int d = 0; d < 3; d++) if (d != dir && d < s->c.dims) { lo[d] -= ntr; hi[d] += ntr; }
-  double dtdx = dt/s->dx[dir];
-  double *v = s->pv, *vL = s->pvL, *vR = s->pvR, *flux = s->pflux, *cmax = s->pcmax;
-
-  for (b = lo[t2]; b <= hi[t2]; b++)
-  for (a = lo[t1]; a <= hi[t1]; a++) {
-    int p[3];
-    p[t1] = a; p[t2] = b;
-    /* DataToSweep, Src/data2sweep.cpp:75-115 */
-    for (i = 0; i < n; i++) {
-      p[dir] = i;
-      for (nv = 0; nv < nvar; nv++) v[i*NVMAX + nv] = VC(s, nv, p[0], p[1], p[2]);
-      if (mhd) s->pBn[i] = s->Vs[dir][IS(s, p[0], p[1], p[2])];
+  int i, nv;
+  int p[3];
+  p[t1] = a; p[t2] = b;
+  /* DataToSweep, Src/data2sweep.cpp:75-115 */
+  for (i = 0; i < n; i++) {
+    p[dir] = i;
+    for (nv = 0; nv < nvar; nv++) v[i*NVMAX + nv] = VC(s, nv, p[0], p[1], p[2]);
+    if (mhd) Bn[i] = s->Vs[dir][IS(s, p[0], p[1], p[2])];
+  }
+  /* Reconstruct, Src/States/plm_states.cpp:99-141 then 184-189 */
+  int ib = s->beg[dir] - 1, ie = s->end[dir] + 1;
+  for (nv = 0; nv < nvar; nv++)
+    for (i = ib; i <= ie; i++) {
+      double dv = plm_slope(v[(i-1)*NVMAX + nv], v[i*NVMAX + nv], v[(i+1)*NVMAX + nv], s->c.limiter);
+      vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv] + 0.5*dv;
+      vR[i*NVMAX + nv]     = v[i*NVMAX + nv] - 0.5*dv;
     }
-    /* Reconstruct, Src/States/plm_states.cpp:99-141 then 184-189 */
-    int ib = s->beg[dir] - 1, ie = s->end[dir] + 1;
-    for (nv = 0; nv < nvar; nv++)
-      for (i = ib; i <= ie; i++) {
-        double dv = plm_slope(v[(i-1)*NVMAX + nv], v[i*NVMAX + nv], v[(i+1)*NVMAX + nv], s->c.limiter);
-        vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv] + 0.5*dv;
-        vR[i*NVMAX + nv]     = v[i*NVMAX + nv] - 0.5*dv;
-      }
+  if (is_rmhd(s)) {
+    /* VelocityLimiter, Src/vel_limiter.hpp:14-31 (plm_states.cpp:158-161):
+       a cell whose reconstructed velocity is superluminal on either side
+       is flattened */
+    for (i = ib; i <= ie; i++) {
+      const double *ra = vR + i*NVMAX, *rb = vL + (i+1)*NVMAX;
+      double v2m = ra[1]*ra[1] + ra[2]*ra[2] + ra[3]*ra[3];
+      double v2p = rb[1]*rb[1] + rb[2]*rb[2] + rb[3]*rb[3];
+      if (v2m >= 1.0 || v2p >= 1.0)
+        for (nv = 0; nv < nvar; nv++) vR[i*NVMAX + nv] = vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv];
+    }
+  }
+  if (mhd) {
+    for (i = ib; i <= ie; i++) {
+      vL[(i+1)*NVMAX + BX1 + dir] = Bn[i+1];
+      vR[i*NVMAX + BX1 + dir]     = Bn[i];
+    }
+  }
+  /* Riemann solver on faces beg..end+1 (Src/MHD/hlld.cpp:100-114) */
+  for (i = s->beg[dir]; i <= s->end[dir] + 1; i++) {
+    double Sl, Sr;
+    const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
     if (is_rmhd(s)) {
-      /* VelocityLimiter, Src/vel_limiter.hpp:14-31 (plm_states.cpp:158-161):
-         a cell whose reconstructed velocity is superluminal on either side
-         is flattened */
-      for (i = ib; i <= ie; i++) {
-        const double *a = vR + i*NVMAX, *b = vL + (i+1)*NVMAX;
-        double v2m = a[1]*a[1] + a[2]*a[2] + a[3]*a[3];
-        double v2p = b[1]*b[1] + b[2]*b[2] + b[3]*b[3];
-        if (v2m >= 1.0 || v2p >= 1.0)
-          for (nv = 0; nv < nvar; nv++) vR[i*NVMAX + nv] = vL[(i+1)*NVMAX + nv] = v[i*NVMAX + nv];
-      }
+      if (s->c.solver == GPLO_HLLC)      err |= rmhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+      else if (s->c.solver == GPLO_HLLD) err |= rmhd_hlld(vl, vr, dir, &s->c, flux + i*NVMAX, &Sl, &Sr);
+      else err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+    } else if (mhd) {
+      if (s->c.solver == GPLO_HLLD)      err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+      else if (s->c.solver == GPLO_HLLC) err |= mhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+      else err |= hll(vl, vr, dir, s->c.gamma, 1, flux + i*NVMAX, &Sl, &Sr);
+    } else {
+      if (s->c.solver == GPLO_HLLC) err |= hd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
+      else err |= hll(vl, vr, dir, s->c.gamma, 0, flux + i*NVMAX, &Sl, &Sr);
     }
-    if (mhd) {
-      for (i = ib; i <= ie; i++) {
-        vL[(i+1)*NVMAX + BX1 + dir] = s->pBn[i+1];
-        vR[i*NVMAX + BX1 + dir]     = s->pBn[i];
-      }
-    }
-    /* Riemann solver on faces beg..end+1 (Src/MHD/hlld.cpp:100-114) */
-    for (i = s->beg[dir]; i <= s->end[dir] + 1; i++) {
-      double Sl, Sr;
-      const double *vl = vL + i*NVMAX, *vr = vR + i*NVMAX;
-      if (is_rmhd(s)) {
-        if (s->c.solver == GPLO_HLLC)      err |= rmhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
-        else if (s->c.solver == GPLO_HLLD) err |= rmhd_hlld(vl, vr, dir, &s->c, flux + i*NVMAX, &Sl, &Sr);
-        else err |= rmhd_hll(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
-      } else if (mhd) {
-        if (s->c.solver == GPLO_HLLD)      err |= mhd_hlld(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
-        else if (s->c.solver == GPLO_HLLC) err |= mhd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
-        else err |= hll(vl, vr, dir, s->c.gamma, 1, flux + i*NVMAX, &Sl, &Sr);
-      } else {
-        if (s->c.solver == GPLO_HLLC) err |= hd_hllc(vl, vr, dir, s->c.gamma, flux + i*NVMAX, &Sl, &Sr);
-        else err |= hll(vl, vr, dir, s->c.gamma, 0, flux + i*NVMAX, &Sl, &Sr);
-      }
-      cmax[i] = MAX(fabs(Sl), fabs(Sr));
-      if (mhd && s->c.emf == GPLO_UCT_HLLD) {
-        p[dir] = i;
-        uct_hlld_store(s, vl, vr, dir, t1, t2, Sl, Sr, IC(s, p[0], p[1], p[2]));
-      } else if (mhd) {
-        /* CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177 */
-        const int VXt = 1 + t1, VXb = 1 + t2;
-        p[dir] = i;
-        size_t id = IC(s, p[0], p[1], p[2]);
-        double aR_1 = MAX(0.0, Sr);
-        double aL_1 = MIN(0.0, Sl);
-        double scrh = 1.0/(aR_1 - aL_1);
-        s->e[t2][dir][id] =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
-        s->e[t1][dir][id] = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
-        s->aL[dir][id] =  aR_1*scrh;
-        s->aR[dir][id] = -aL_1*scrh;
-        s->dR[dir][id] = -aL_1*aR_1*scrh;
-        s->dL[dir][id] = -aL_1*aR_1*scrh;
-      }
-    }
-    /* RightHandSide on the active box only, Src/MHD/rhs.cpp:330-527 */
-    if (a < s->beg[t1] || a > s->end[t1] || b < s->beg[t2] || b > s->end[t2]) continue;
-    for (i = s->beg[dir]; i <= s->end[dir]; i++) {
+    cmax[i] = MAX(fabs(Sl), fabs(Sr));
+    if (mhd && s->c.emf == GPLO_UCT_HLLD) {
+      p[dir] = i;
+      uct_hlld_store(s, vl, vr, dir, t1, t2, Sl, Sr, IC(s, p[0], p[1], p[2]));
+    } else if (mhd) {
+      /* CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177 */
+      const int VXt = 1 + t1, VXb = 1 + t2;
       p[dir] = i;
       size_t id = IC(s, p[0], p[1], p[2]);
-      for (nv = 0; nv < nvar; nv++) {
-        double rhs = -dtdx*(flux[(i+1)*NVMAX + nv] - flux[i*NVMAX + nv]);
-        /* MHD adds src[] = 0 (Src/MHD/rhs.cpp:519); RMHD with CT has no
-           source block (Src/RMHD/rhs.cpp:14-21,400-422) */
-        if (!is_rmhd(s)) rhs += 0.0;
-        /* dUc = rhs (x); dUc += rhs (y, z): Src/update_stage.cpp:173-184,272-284,369-380 */
-        if (dir == 0) s->Rc[(size_t)nv*s->ncell + id] = rhs;
-        else          s->Rc[(size_t)nv*s->ncell + id] += rhs;
-      }
-      double cdt = 0.5*(cmax[i] + cmax[i+1])/s->dx[dir];
-      if (dir == 0) s->Cdt[id] = cdt;      /* rhs.cpp:523 */
-      else          s->Cdt[id] += cdt;     /* rhs.cpp:756,981 */
+      double aR_1 = MAX(0.0, Sr);
+      double aL_1 = MIN(0.0, Sl);
+      double scrh = 1.0/(aR_1 - aL_1);
+      s->e[t2][dir][id] =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
+      s->e[t1][dir][id] = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
+      s->aL[dir][id] =  aR_1*scrh;
+      s->aR[dir][id] = -aL_1*scrh;
+      s->dR[dir][id] = -aL_1*aR_1*scrh;
+      s->dL[dir][id] = -aL_1*aR_1*scrh;
     }
+  }
+  /* RightHandSide on the active box only, Src/MHD/rhs.cpp:330-527 */
+  if (a < s->beg[t1] || a > s->end[t1] || b < s->beg[t2] || b > s->end[t2]) return err;
+  for (i = s->beg[dir]; i <= s->end[dir]; i++) {
+    p[dir] = i;
+    size_t id = IC(s, p[0], p[1], p[2]);
+    for (nv = 0; nv < nvar; nv++) {
+      double rhs = -dtdx*(flux[(i+1)*NVMAX + nv] - flux[i*NVMAX + nv]);
+      /* MHD adds src[] = 0 (Src/MHD/rhs.cpp:519); RMHD with CT has no
+         source block (Src/RMHD/rhs.cpp:14-21,400-422) */
+      if (!is_rmhd(s)) rhs += 0.0;
+      /* dUc = rhs (x); dUc += rhs (y, z): Src/update_stage.cpp:173-184,272-284,369-380 */
+      if (dir == 0) s->Rc[(size_t)nv*s->ncell + id] = rhs;
+      else          s->Rc[(size_t)nv*s->ncell + id] += rhs;
+    }
+    double cdt = 0.5*(cmax[i] + cmax[i+1])/s->dx[dir];
+    if (dir == 0) s->Cdt[id] = cdt;      /* rhs.cpp:523 */
+    else          s->Cdt[id] += cdt;     /* rhs.cpp:756,981 */
+  }
+  return err;
+}
+
+/* the sweep along dir over the box widened by ntr transverse zones; pencils
+   are independent (each writes its own cells), so threads split them */
+static uint32_t sweep(gplo_state *s, int dir, double dt)
+{
+  const int mhd = is_mhd(s);
+  const int t1 = (dir + 1)%3, t2 = (dir + 2)%3;
+  const int ntr = mhd ? NG : 0;           /* Src/update_stage.cpp:80-90 */
+  uint32_t err = 0;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; d++) { lo[d] = s->beg[d]; hi[d] = s->end[d]; }
+  for (int d = 0; d < 3; d++) if (d != dir && d < s->c.dims) { lo[d] -= ntr; hi[d] += ntr; }
+  const double dtdx = dt/s->dx[dir];
+  const long na = hi[t1] - lo[t1] + 1, nb = hi[t2] - lo[t2] + 1;
+  long q;
+#pragma omp parallel for reduction(|:err) schedule(static)
+  for (q = 0; q < na*nb; q++) {
+#ifdef _OPENMP
+    const size_t th = (size_t)omp_get_thread_num();
+#else
+    const size_t th = 0;
+#endif
+    const size_t o = th*s->pstride, oc = th*(s->pstride/NVMAX);
+    err |= sweep_pencil(s, dir, lo[t1] + (int)(q % na), lo[t2] + (int)(q / na), dtdx,
+                        s->pv + o, s->pvL + o, s->pvR + o, s->pflux + o, s->pcmax + oc, s->pBn + oc);
   }
   return err;
 }
@@ -1706,12 +1758,13 @@ static void array_reconstruct(gplo_state *s, int dir, const double *Q, int stag,
 {
   const int t1 = (dir + 1)%3, t2 = (dir + 2)%3;
   const int n = s->nt[dir];
-  int a, b, i;
+  int b;
+#pragma omp parallel for schedule(static)
   for (b = lo[t2]; b <= hi[t2]; b++)
-  for (a = lo[t1]; a <= hi[t1]; a++) {
+  for (int a = lo[t1]; a <= hi[t1]; a++) {
     int p[3];
     p[t1] = a; p[t2] = b;
-    for (i = 1; i < n - 1; i++) {
+    for (int i = 1; i < n - 1; i++) {
       p[dir] = i;
       size_t id = IS(s, p[0], p[1], p[2]);
       int pm[3] = {p[0], p[1], p[2]}, pp[3] = {p[0], p[1], p[2]};
@@ -1757,6 +1810,7 @@ static void ct_emf_uct(gplo_state *s)
     array_reconstruct(s, bk->rdir, s->e[bk->ea][bk->eb], 0, recV, lo, hi, vL, vR);
     double *E = bk->edge == 0 ? Ex : bk->edge == 1 ? Ey : Ez;
     const int fd = bk->rdir;   /* face direction of the a/d coefficients */
+#pragma omp parallel for private(i, j) schedule(static)
     for (k = lo[2]; k <= hi[2]; k++)
     for (j = lo[1]; j <= hi[1]; j++)
     for (i = lo[0]; i <= hi[0]; i++) {
@@ -1782,6 +1836,7 @@ static void ct_update(gplo_state *s, double dt)
   int i, j, k;
   double *Ex = s->Ee[0], *Ey = s->Ee[1], *Ez = s->Ee[2];
   /* Bx1 faces: i in [ibeg, iend+1] */
+#pragma omp parallel for private(i, j) schedule(static)
   for (k = s->beg[2]; k <= s->end[2]; k++)
   for (j = s->beg[1]; j <= s->end[1]; j++)
   for (i = s->beg[0]; i <= s->end[0] + 1; i++) {
@@ -1791,6 +1846,7 @@ static void ct_update(gplo_state *s, double dt)
     s->Rs[0][IS(s,i,j,k)] = -curlE;
   }
   if (D >= 2) {
+#pragma omp parallel for private(i, j) schedule(static)
     for (k = s->beg[2]; k <= s->end[2]; k++)
     for (j = s->beg[1]; j <= s->end[1] + 1; j++)
     for (i = s->beg[0]; i <= s->end[0]; i++) {
@@ -1801,6 +1857,7 @@ static void ct_update(gplo_state *s, double dt)
     }
   }
   if (D == 3) {
+#pragma omp parallel for private(i, j) schedule(static)
     for (k = s->beg[2]; k <= s->end[2] + 1; k++)
     for (j = s->beg[1]; j <= s->end[1]; j++)
     for (i = s->beg[0]; i <= s->end[0]; i++) {
@@ -1819,7 +1876,8 @@ static uint32_t update_stage(gplo_state *s, double dt, double *inv_dt)
   for (int d = 0; d < s->c.dims; d++) err |= sweep(s, d, dt);
   if (is_mhd(s)) { ct_emf_uct(s); ct_update(s, dt); }
   if (s->stage == 1) {
-    double local = -1.0;     /* getMax, :446-458 */
+    double local = -1.0;     /* getMax, :446-458; max is exact in any order */
+#pragma omp parallel for reduction(max:local) schedule(static)
     for (int k = s->beg[2]; k <= s->end[2]; k++)
     for (int j = s->beg[1]; j <= s->end[1]; j++)
     for (int i = s->beg[0]; i <= s->end[0]; i++) local = MAX(local, s->Cdt[IC(s,i,j,k)]);
@@ -1830,11 +1888,23 @@ static uint32_t update_stage(gplo_state *s, double dt, double *inv_dt)
 
 /* ConsToPrim MHD (Src/MHD/mappers.cpp:32-128) / HD (Src/HD/mappers.cpp:33-110)
    wrapped by ConsToPrim3D (Src/mappers3D.cpp:76-194) */
+/* the zone's Data::err entry (Src/mappers3D.cpp:115,121,173,176) and its
+   count in Where (Src/debug_tools.cpp:587-593) */
+static void zone_report(gplo_state *s, int i, int j, int k, uint32_t err)
+{
+  const int nx = s->end[0] - s->beg[0] + 1, ny = s->end[1] - s->beg[1] + 1;
+  const int nz = s->end[2] - s->beg[2] + 1;
+  const size_t a = ((size_t)(k - s->beg[2])*ny + (size_t)(j - s->beg[1]))*nx + (size_t)(i - s->beg[0]);
+  s->zerr[(size_t)(s->stage - 1)*nx*ny*nz + a] = err;
+}
+
 static uint32_t cons_to_prim_3d(gplo_state *s)
 {
   const double gmm1 = s->c.gamma - 1.0;
   const int mhd = is_mhd(s), ENG = mhd ? 7 : 4;
   uint32_t err_all = 0;
+  unsigned long long flagged = 0;
+#pragma omp parallel for reduction(|:err_all) reduction(+:flagged) schedule(static)
   for (int k = s->beg[2]; k <= s->end[2]; k++)
   for (int j = s->beg[1]; j <= s->end[1]; j++)
   for (int i = s->beg[0]; i <= s->end[0]; i++) {
@@ -1851,6 +1921,8 @@ static uint32_t cons_to_prim_3d(gplo_state *s)
       if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
       if (le) for (int nv = 0; nv < s->nv; nv++) s->Uc[(size_t)nv*s->ncell + id] = u[nv];
       for (int nv = 0; nv < s->nv; nv++) s->Vc[(size_t)nv*s->ncell + id] = v[nv];
+      zone_report(s, i, j, k, err);
+      flagged += err != 0;
       err_all |= err;
       continue;
     }
@@ -1886,14 +1958,19 @@ static uint32_t cons_to_prim_3d(gplo_state *s)
     if (le & ERR_CONS2PRIM_FAIL) err |= ERR_QUIT;
     if (le) for (int nv = 0; nv < s->nv; nv++) s->Uc[(size_t)nv*s->ncell + id] = u[nv];
     for (int nv = 0; nv < s->nv; nv++) s->Vc[(size_t)nv*s->ncell + id] = v[nv];
+    zone_report(s, i, j, k, err);
+    flagged += err != 0;
     err_all |= err;
   }
+  s->zcount[s->stage - 1] += flagged;
+  s->zcount_total += flagged;
   return err_all;
 }
 
 /* CT_AverageStaggeredFields on the active box, Src/MHD/CT/ct_field_average.cpp:182-184,393-395 */
 static void ct_average(gplo_state *s)
 {
+#pragma omp parallel for schedule(static)
   for (int k = s->beg[2]; k <= s->end[2]; k++)
   for (int j = s->beg[1]; j <= s->end[1]; j++)
   for (int i = s->beg[0]; i <= s->end[0]; i++)
@@ -1916,10 +1993,12 @@ uint32_t gplo_advance_step(gplo_state *s, double dt, double *inv_dt)
     w[3][0] = 1.0/3.0; w[3][1] = 2.0/3.0; w[3][2] = 2.0/3.0;
   }
   const int D = s->c.dims, mhd = is_mhd(s);
+  s->zcount[0] = s->zcount[1] = s->zcount[2] = 0;
   for (int st = 1; st <= nstage; st++) {
     s->stage = st;
     boundary(s);
     err |= update_stage(s, dt, inv_dt);
+#pragma omp parallel for collapse(2) schedule(static)
     for (int nv = 0; nv < s->nv; nv++)
     for (int k = s->beg[2]; k <= s->end[2]; k++)
     for (int j = s->beg[1]; j <= s->end[1]; j++)
@@ -1933,6 +2012,7 @@ uint32_t gplo_advance_step(gplo_state *s, double dt, double *inv_dt)
       }
     }
     if (mhd) {   /* BOX_STAG_LOOP, Src/macros.hpp:82-84 */
+#pragma omp parallel for schedule(static)
       for (int k = s->beg[2]; k <= s->end[2] + (D == 3); k++)
       for (int j = s->beg[1]; j <= s->end[1] + (D >= 2); j++)
       for (int i = s->beg[0]; i <= s->end[0] + 1; i++) {
@@ -1951,6 +2031,17 @@ uint32_t gplo_advance_step(gplo_state *s, double dt, double *inv_dt)
     err |= cons_to_prim_3d(s);
   }
   return err;
+}
+
+int gplo_zone_errors(const gplo_state *s, uint32_t *err, unsigned long long *counts,
+                     unsigned long long *total)
+{
+  const size_t na = (size_t)s->c.nx*s->c.ny*s->c.nz;
+  const int nstage = s->c.time_stepping == 3 ? 3 : 2;
+  if (err) memcpy(err, s->zerr, (size_t)nstage*na*sizeof(uint32_t));
+  if (counts) for (int q = 0; q < nstage; q++) counts[q] = s->zcount[q];
+  if (total) *total = s->zcount_total;
+  return nstage;
 }
 
 uint32_t gplo_run(gplo_state *s, int nsteps, double first_dt, double *time_out, double *dt_out)
@@ -2005,6 +2096,7 @@ gplo_state *gplo_create(const gplo_config *cfg)
   s->Vc = calloc(nc*s->nv, 8); s->Uc = calloc(nc*s->nv, 8);
   s->U0 = calloc(nc*s->nv, 8); s->Rc = calloc(nc*s->nv, 8);
   s->Cdt = calloc(nc, 8);
+  s->zerr = calloc((size_t)3*c.nx*c.ny*c.nz, sizeof(uint32_t));
   for (int d = 0; d < 3; d++) {
     s->Vs[d] = calloc(ns, 8); s->Bs0[d] = calloc(ns, 8); s->Rs[d] = calloc(ns, 8);
     s->Ee[d] = calloc(ns, 8);
@@ -2013,9 +2105,17 @@ gplo_state *gplo_create(const gplo_config *cfg)
     for (int e = 0; e < 3; e++) s->e[d][e] = calloc(nc, 8);
   }
   int nmax = s->nt[0]; if (s->nt[1] > nmax) nmax = s->nt[1]; if (s->nt[2] > nmax) nmax = s->nt[2];
-  s->pv = calloc((size_t)(nmax + 2)*NVMAX, 8); s->pvL = calloc((size_t)(nmax + 2)*NVMAX, 8);
-  s->pvR = calloc((size_t)(nmax + 2)*NVMAX, 8); s->pflux = calloc((size_t)(nmax + 2)*NVMAX, 8);
-  s->pcmax = calloc(nmax + 2, 8); s->pBn = calloc(nmax + 2, 8); s->prhs = calloc((size_t)(nmax + 2)*NVMAX, 8);
+#ifdef _OPENMP
+  s->nthr = omp_get_max_threads();
+#else
+  s->nthr = 1;
+#endif
+  const size_t nth = (size_t)s->nthr;
+  s->pstride = (size_t)(nmax + 2)*NVMAX;
+  s->pv = calloc(nth*s->pstride, 8); s->pvL = calloc(nth*s->pstride, 8);
+  s->pvR = calloc(nth*s->pstride, 8); s->pflux = calloc(nth*s->pstride, 8);
+  s->pcmax = calloc(nth*(nmax + 2), 8); s->pBn = calloc(nth*(nmax + 2), 8);
+  s->prhs = calloc(s->pstride, 8);
   for (int q = 0; q < 2; q++) { s->qv[q] = calloc(ns, 8); s->qL[q] = calloc(ns, 8); s->qR[q] = calloc(ns, 8); }
   return s;
 }
@@ -2023,7 +2123,7 @@ gplo_state *gplo_create(const gplo_config *cfg)
 void gplo_destroy(gplo_state *s)
 {
   if (!s) return;
-  free(s->Vc); free(s->Uc); free(s->U0); free(s->Rc); free(s->Cdt);
+  free(s->Vc); free(s->Uc); free(s->U0); free(s->Rc); free(s->Cdt); free(s->zerr);
   for (int d = 0; d < 3; d++) {
     free(s->Vs[d]); free(s->Bs0[d]); free(s->Rs[d]); free(s->Ee[d]);
     free(s->aL[d]); free(s->aR[d]); free(s->dL[d]); free(s->dR[d]);

@@ -63,6 +63,13 @@ uint32_t    gplo_advance_step(gplo_state *s, double dt, double *inv_dt);
 uint32_t    gplo_run(gplo_state *s, int nsteps, double first_dt,
                      double *time_out, double *dt_out);
 int         gplo_nvar(const gplo_state *s);
+/* Zone-by-zone error reports of the last step: err[stage][nz][ny][nx] holds
+   Data::err as ConsToPrim3D left it in each stage (Src/mappers3D.cpp:115-176),
+   counts[stage] the zones Where() counted (Src/debug_tools.cpp:587-593),
+   *total the flagged zones since creation; any pointer may be NULL.  Returns
+   the number of stages. */
+int         gplo_zone_errors(const gplo_state *s, uint32_t *err, unsigned long long *counts,
+                             unsigned long long *total);
 /* pointwise RMHD pieces for tests: HLL interface solve along dir (flux[8],
    Davis speeds) and con2prim of u[8] (rewrites u like the reference) */
 uint32_t    gplo_rmhd_face(const double *vl, const double *vr, int dir, double gamma,
@@ -71,6 +78,10 @@ uint32_t    gplo_rmhd_face(const double *vl, const double *vr, int dir, double g
 uint32_t    gplo_face(int phys, int solver, const double *vl, const double *vr, int dir,
                       double gamma, double *flux, double *sl, double *sr);
 uint32_t    gplo_rmhd_c2p(double *u, double *v, double gamma, double small_rho, double small_prs);
+/* acos and sincos for the RMHD resolvent cubic (Src/RMHD/rmhd.hpp:441-444):
+   NULL (the default) calls the host libm, as the reference does */
+void        gplo_set_cubic_math(double (*acos_fn)(double),
+                                void (*sincos_fn)(double, double *, double *));
 
 #ifdef __cplusplus
 }

@@ -56,8 +56,20 @@ def lib():
         L.gplo_run.restype = C.c_uint32
         L.gplo_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]
+        L.gplo_set_cubic_math.restype = None
+        L.gplo_set_cubic_math.argtypes = [C.c_void_p, C.c_void_p]
+        L.gplo_zone_errors.restype = C.c_int
+        L.gplo_zone_errors.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_ulonglong)]
         _lib = L
     return _lib
+
+
+def set_cubic_math(acos_fn=None, sincos_fn=None):
+    """Substitute the RMHD resolvent cubic's acos and sincos (C function
+    addresses, e.g. ctypes.cast(lib.crm_acos, c_void_p)); no arguments
+    restore the host libm, the reference's."""
+    lib().gplo_set_cubic_math(acos_fn, sincos_fn)
 
 
 def _ptr(a):
@@ -121,3 +133,14 @@ class Oracle:
         t, dt = C.c_double(0.0), C.c_double(0.0)
         err = lib().gplo_run(self._h, steps, first_dt, C.byref(t), C.byref(dt))
         return int(err), t.value, dt.value
+
+    def zone_errors(self):
+        """Data::err of the active zones in each stage of the last step
+        ([stage, nz, ny, nx] uint32), the zones flagged per stage, and the
+        total flagged since creation."""
+        nx, ny, nz = self.n
+        err = np.zeros((3, nz, ny, nx), dtype=np.uint32)
+        counts = np.zeros(3, dtype=np.uint64)
+        total = C.c_ulonglong(0)
+        ns = lib().gplo_zone_errors(self._h, _ptr(err), _ptr(counts), C.byref(total))
+        return err[:ns], counts[:ns].astype(np.int64), int(total.value)

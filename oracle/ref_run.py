@@ -27,18 +27,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 
 
-def ref_binary(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL") -> str:
-    name = f"{cfg}_{ts}_{lim}" + (f"_{emf}" if "mhd" in cfg else "")
+def ref_binary(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL",
+               warn: int = 1) -> str:
+    name = f"{cfg}_{ts}_{lim}" + (f"_{emf}" if "mhd" in cfg else "") + (f"_W{warn}" if warn != 1 else "")
     return os.path.join(REF_DIR, name, "pluto")
 
 
 def build_ref(cfg: str, ts: str = "RK2", lim: str = "MC_LIM", emf: str = "UCT_HLL",
-              ref_root: str = "/root/reference") -> str:
+              ref_root: str = "/root/reference", warn: int = 1) -> str:
     """Build one reference configuration (needs the reference sources)."""
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "ref"), f"CFG={cfg}",
-                    f"TS={ts}", f"LIM={lim}", f"EMF={emf}", f"REF={ref_root}", "-j8"],
-                   check=True)
-    return ref_binary(cfg, ts, lim, emf)
+                    f"TS={ts}", f"LIM={lim}", f"EMF={emf}", f"REF={ref_root}", f"WARN={warn}",
+                    "-j8"], check=True)
+    return ref_binary(cfg, ts, lim, emf, warn)
 
 
 def write_ic(path: str, vc: np.ndarray, vs: np.ndarray | None, gamma: float, periodic: bool):
@@ -124,6 +125,46 @@ def _read_last_dump(tmp, nx, ny, nz):
             off += ncell
     assert off == raw.size, (off, raw.size)
     return np.stack(cells), (np.stack(stag) if stag else None), (upper if stag else None), t
+
+
+# ErrorShow's names (Src/debug_tools.cpp:221-238) and the bits they print
+# (Src/gpluto.hpp:247-265)
+ERR_BITS = {"ERR_NEGATIVE_RHO": 1 << 0, "ERR_NEGATIVE_ENG": 1 << 1, "ERR_NEGATIVE_PRS": 1 << 2,
+            "ERR_NEGATIVE_ENTR": 1 << 3, "ERR_CONS2PRIM_FAIL": 1 << 4, "ERR_PRESSURE_FIX": 1 << 5,
+            "ERR_ENG_SOLVE_INACCURATE": 1 << 6, "ERR_ENG_SOLVE_NO_SOLUTION": 1 << 7,
+            "ERR_SUPERLUMINAL_SPEED": 1 << 8, "ERR_TOO_MANY_ITERATIONS": 1 << 9,
+            "ERR_RIEMANN_SOLVE_FAIL": 1 << 10, "ERR_NAN": 1 << 11, "ERR_POINT_VALUE_FAIL": 1 << 12,
+            "ERR_IMEX_GUESS_FAIL": 1 << 13, "ERR_QUIT": 1 << 15,
+            "ERR_SUPERLUMINAL_RADIATION_FLUX": 1 << 16, "ERR_GAUSSIAN_SOLVE": 1 << 17,
+            "ERR_NEGATIVE_ENR": 1 << 18}
+
+
+def parse_zone_reports(log: str, nghost: int = 2) -> np.ndarray:
+    """The zone-by-zone reports of a WARNING_MESSAGES >= 3 build (Where,
+    Src/debug_tools.cpp:633-682) as rows (step, stage, i, j, k, bits, func):
+    active-zone indices (the log prints Data indices, ghosts included; an
+    absent direction prints nothing and gets 0), g_stepNumber, g_intStage,
+    the ERR_* bits, and func 0 for ConsToPrim3D, 1 for a Riemann solver."""
+    import re
+    rows = []
+    lines = log.splitlines()
+    for q, line in enumerate(lines):
+        m = re.match(r"! (\S+)\(\): zone: (.*)", line)
+        if not m:
+            continue
+        func = 0 if m.group(1) == "ConsToPrim3D" else 1
+        z = (re.match(r"i = (\d+),", m.group(2)) or re.match(r"\(i,j\) = \((\d+), (\d+)\)", m.group(2))
+             or re.match(r"\(i,j,k\) = \((\d+), (\d+), (\d+)\)", m.group(2)))
+        idx = [int(v) - nghost for v in z.groups()]
+        ijk = idx + [0] * (3 - len(idx))
+        st = re.search(r"step: (\d+); t = \S+; g_intStage = (\d+)", lines[q + 1])
+        names = re.search(r"error\(s\) found: (\S*)", lines[q + 2]).group(1)
+        bits = 0
+        for nm in names.split("/"):
+            if nm:
+                bits |= ERR_BITS[nm]
+        rows.append((int(st.group(1)), int(st.group(2)), *ijk, bits, func))
+    return np.array(rows, dtype=np.int64).reshape(-1, 7)
 
 
 def _sec_per_step(log):

@@ -82,6 +82,25 @@ CASES = [
     ("rmhd3d_12x10x10_rk2_mc_hll_walls", "rmhd3d", "RK2", "MC_LIM", (12, 10, 10), 29, 5, 1e-3, 0.4, "hll", False, 4 / 3),
 ]
 
+# ConsToPrim's floors (Src/HD/mappers.cpp:54-90, Src/MHD/mappers.cpp:53-96) and
+# the rewrite of Uc in flagged zones (Src/mappers3D.cpp:182-190): streams
+# leaving an emptying centre (negative energies and pressures) and low-beta
+# blasts (negative pressures).  The reference runs a WARNING_MESSAGES = 3 build
+# as well, whose zone-by-zone log (Where, Src/debug_tools.cpp:633-682) gives
+# every flagged zone, its step, stage and bits; its dump must equal the
+# default build's bit for bit.
+FLOOR_CASES = [
+    # name, cfg, ts, lim, n, steps, first_dt, cfl, solver, periodic, gamma, ic
+    ("hd1d_256_rk2_mc_hllc_floor", "hd1d", "RK2", "MC_LIM", (256, 1, 1), 60, 1e-4, 0.8, "hllc", False, 1.4,
+     ("hd_vacuum", 10.0, 0.01)),
+    ("hd2d_48x32_rk3_vl_hllc_floor", "hd2d", "RK3", "VANLEER_LIM", (48, 32, 1), 40, 1e-4, 0.4, "hllc", False, 1.4,
+     ("hd_vacuum", 20.0, 0.01)),
+    ("mhd3d_12x14x10_rk2_mc_floor", "mhd3d", "RK2", "MC_LIM", (12, 14, 10), 10, 1e-4, 0.4, "hlld", True, 5 / 3,
+     ("mhd_lowbeta_blast", 10.0, 1e-4)),
+    ("mhd2d_32x32_rk2_mc_floor", "mhd2d", "RK2", "MC_LIM", (32, 32, 1), 12, 1e-4, 0.4, "hlld", True, 5 / 3,
+     ("mhd_lowbeta_blast", 10.0, 1e-4)),
+]
+
 # pluto.ini [Boundary] per side (X1-beg, X1-end, ...) where not all periodic /
 # all outflow: shock tubes between walls (ReflectiveBoundary), long enough for
 # the waves to reflect
@@ -195,8 +214,40 @@ def make(case):
     print(f"{name}: t = {r.time:.9e}, wall {r.wall_s:.2f} s")
 
 
+def make_floor(case):
+    name, cfg, ts, lim, n, steps, first_dt, cfl, solver, periodic, gamma, spec = case
+    gen, a, b = spec
+    if gen == "hd_vacuum":
+        vc, vs = ic.hd_vacuum(n, a, b), None
+    else:
+        vc, vs = ic.mhd_lowbeta_blast(n, a, b)
+    ref_run.build_ref(cfg, ts, lim, warn=3)
+    runs = [ref_run.run_reference(ref_run.ref_binary(cfg, ts, lim, warn=w), vc, vs, gamma=gamma,
+                                  steps=steps, cfl=cfl, first_dt=first_dt, solver=solver,
+                                  periodic=periodic) for w in (1, 3)]
+    r, r3 = runs
+    assert np.array_equal(r.vc, r3.vc) and r.time == r3.time, "the logging build changed the result"
+    if vs is not None:
+        assert np.array_equal(r.vs, r3.vs)
+    zones = ref_run.parse_zone_reports(r3.log)
+    assert len(zones) > 0 and "ERR_QUIT" not in r3.log
+    params = dict(name=name, cfg=cfg, time_stepping=ts, limiter=lim, n=list(n), steps=steps,
+                  first_dt=first_dt, cfl=cfl, cfl_max_var=1.1, solver=solver, periodic=periodic,
+                  gamma=gamma, time=r.time, err_log=True, ic=list(spec))
+    arrays = dict(vc_in=vc, vc_out=r.vc, zones=zones)
+    if vs is not None:
+        arrays.update(vs_in=vs, vs_out=r.vs)
+        arrays.update({f"vs_upper{d}": u for d, u in enumerate(r.vs_upper)})
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=json.dumps(params), **arrays)
+    print(f"{name}: t = {r.time:.9e}, {len(zones)} flagged zone reports, bits "
+          f"{sorted(set(int(x) for x in zones[:, 5]))}")
+
+
 if __name__ == "__main__":
     only = sys.argv[1:]
+    for c in FLOOR_CASES:
+        if not only or c[0] in only:
+            make_floor(c)
     for c in CASES:
         if not only or c[0] in only:
             make(c)

@@ -26,3 +26,16 @@ def load(name):
     p = json.loads(str(d["params"]))
     arrays = {k: d[k] for k in d.files if k != "params"}
     return p, arrays
+
+
+def expected_err(arrays) -> int:
+    """The OR of the ERR_* bits the reference's run raised: from its
+    zone-by-zone log where the fixture holds one (Where, WARNING_MESSAGES 3),
+    else 0 (the fixture's run logged no error)."""
+    z = arrays.get("zones")
+    if z is None or len(z) == 0:
+        return 0
+    out = 0
+    for b in z[:, 5]:
+        out |= int(b)
+    return out
