@@ -10,13 +10,18 @@ is one cell advanced by one step.
     python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--n 256]
 
 Prints ONE JSON line.  value = FP64 zone-updates/s with the state resident in
-HBM (host wall clock around exactly --steps steps, the library stream
-synchronised on both sides).  The mixed-precision form runs the same steps
-from the same IC and is reported beside it with its L1 error against the FP64
-run.  roofline: the dominant kernel's algorithmic bytes per launch over its
-average launch duration (HIP events on the library's stream during the timed
-region).  cpu_baseline: the reference's own serial CPU build (oracle/_ref) on
-a bounded sample of the same workload.
+HBM (host wall clock around exactly --steps steps replayed from the library's
+captured HIP graph, the library stream synchronised on both sides).  The
+mixed-precision form runs the same steps from the same IC and is reported
+beside it with its L1 error against the FP64 run.  roofline: the whole step
+against HBM, zone-updates/s x the algorithmic bytes of one zone-update (each
+stage reads Vc, Vs and Uc once and writes them once: 696 B FP64), with the
+measured HBM bytes of one step (rocprofv3 FETCH_SIZE + WRITE_SIZE) beside it;
+kernel_roofline: the dominant kernel's algorithmic bytes per launch over its
+average launch duration (HIP events on the library's stream, in a second run
+of the same steps with per-launch timing on).  cpu_baseline: the oracle on all
+host cores and the reference's own serial build (oracle/_ref), each on a
+bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -78,22 +83,27 @@ def log(*a):
 
 
 def run_gpu(n, steps, warmup, precision, vc, vs, first_dt):
-    """Warm-up, then exactly ``steps`` steps timed on the host clock with HIP
-    events around every kernel on the library's stream; the final state."""
+    """Warm-up, then exactly ``steps`` steps timed on the host clock as the
+    library runs them (graph replay, dt on the device); the final state; then
+    the same number of steps again with HIP events around every kernel on the
+    library's stream (eager launches) for the per-kernel times."""
     from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
     g = StageUpdate(Definitions(), RuntimeSetup(n=(n, n, n), first_dt=first_dt), precision=precision)
     g.upload(vc, vs)
     err, t, dt = g.run(warmup, first_dt)
     if err:
         raise RuntimeError(f"warmup error bits {err:#x}")
-    g.set_timing(True)
     t0 = time.perf_counter()
     err, t1, dt1 = g.run(steps, -1.0)       # synchronous: returns after the stream drained
     wall = time.perf_counter() - t0
     if err:
         raise RuntimeError(f"error bits {err:#x}")
-    stats = g.kernel_stats()
     vcf, vsf, _ = g.download()
+    g.set_timing(True)
+    err, _, _ = g.run(steps, -1.0)
+    if err:
+        raise RuntimeError(f"error bits {err:#x}")
+    stats = g.kernel_stats()
     g.close()
     return wall, stats, vcf, vsf, t1
 
@@ -107,43 +117,71 @@ def dominant(stats, w, ncell):
     return name, avg_ms, abytes
 
 
-def traffic_from_profile(name: str, w: int):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def _traffic_table(w: int):
     path = os.path.join(ROOT, "profiles", "traffic_mhd3d_256.json")
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get("fp64" if w == 8 else "mixed", {}).get(name)
+            return json.load(f).get("fp64" if w == 8 else "mixed", {})
     except (OSError, ValueError):
+        return {}
+
+
+def traffic_from_profile(name: str, w: int):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    return _traffic_table(w).get(name)
+
+
+def step_traffic_from_profile(w: int, stages: int = 2):
+    """HBM bytes of one whole step (every kernel, once per RK2 stage) from the
+    committed rocprofv3 --pmc summary of the 256^3 workload, if any."""
+    t = _traffic_table(w)
+    return stages * sum(t.values()) if t else None
+
+
+def cpu_reference_serial(seed: int):
+    """The reference's own serial CPU build on a bounded sample (64^3, 20
+    steps of the same IC generator); None where the build is absent."""
+    from gpluto_dev_amd import ic
+    m, steps = 64, 20
+    binary = os.path.join(ROOT, "oracle", "_ref", "mhd3d_RK2_MC_LIM_UCT_HLL", "pluto")
+    if not os.path.exists(binary):
         return None
+    import ref_run
+    vc, vs = ic.mhd_turbulence((m, m, m), seed)
+    r = ref_run.run_reference(binary, vc, vs, gamma=5 / 3, steps=steps, cfl=0.4,
+                              first_dt=5e-4 * 256 / m, solver="hlld")
+    mt = re.search(r"Elapsed time \(compute\)\s+([0-9.eE+-]+)", r.log)
+    secs = float(mt.group(1)) if mt else r.wall_s
+    return {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{m}^3 box of the same seeded IC generator, {steps} RK2 steps, "
+                      f"reference serial build (g++ -O3, oracle/_ref), compute time "
+                      f"{secs:.2f} s from its own 'Elapsed time (compute)' line"}
 
 
 def cpu_baseline(seed: int):
-    """Reference serial CPU build on a bounded sample (64^3, 20 steps)."""
+    """The CPU path on all host cores: the oracle (the reference's algorithm
+    restated in C, pinned bit for bit to the reference build) with OpenMP over
+    pencils and planes, on a bounded sample (128^3, 10 steps of the same IC
+    generator); the reference's own serial build beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from gpluto_dev_amd import ic
-    m, steps = 64, 20
-    vc, vs = ic.mhd_turbulence((m, m, m), seed)
-    binary = os.path.join(ROOT, "oracle", "_ref", "mhd3d_RK2_MC_LIM_UCT_HLL", "pluto")
-    if os.path.exists(binary):
-        import ref_run
-        r = ref_run.run_reference(binary, vc, vs, gamma=5 / 3, steps=steps, cfl=0.4,
-                                  first_dt=5e-4 * 256 / m, solver="hlld")
-        mt = re.search(r"Elapsed time \(compute\)\s+([0-9.eE+-]+)", r.log)
-        secs = float(mt.group(1)) if mt else r.wall_s
-        return {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1,
-                "kind": "reference",
-                "sample": f"{m}^3 box of the same seeded IC generator, {steps} RK2 steps, "
-                          f"reference serial build (g++ -O3, oracle/_ref), compute time "
-                          f"{secs:.2f} s from its own 'Elapsed time (compute)' line"}
     import oracle
+    m, steps = 128, 10
+    vc, vs = ic.mhd_turbulence((m, m, m), seed)
     o = oracle.Oracle(phys="mhd", n=(m, m, m), gamma=5 / 3)
     o.set_state(vc, vs)
     t0 = time.perf_counter()
     o.run(steps, 5e-4 * 256 / m)
     secs = time.perf_counter() - t0
-    return {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{m}^3, {steps} RK2 steps, C oracle (serial)"}
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"value": m ** 3 * steps / secs, "unit": "zone-updates/s", "cores": cores, "kind": "port",
+           "sample": f"{m}^3 box of the same seeded IC generator, {steps} RK2 steps, C oracle "
+                     f"(OpenMP, {cores} threads), {secs:.2f} s wall"}
+    ref = cpu_reference_serial(seed)
+    if ref is not None:
+        out["reference_serial"] = ref
+    return out
 
 
 def pmc_probe(n, seed, precision):
@@ -386,6 +424,7 @@ def main():
     achieved = abytes / (avg_ms * 1e-3) / 1e9
     traffic = traffic_from_profile(name, 8)
     step_gbs = value * B_ALG["fp64"] / 1e9
+    step_traffic = step_traffic_from_profile(8) if n == 256 else None
 
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
@@ -395,11 +434,15 @@ def main():
         "config": {"workload": f"3D ideal MHD {n}^3 periodic, PLM(MC)+HLLD+UCT_HLL CT+RK2, "
                                f"gamma 5/3, CFL 0.4, seeded random-field IC (beta 10)",
                    "n": n, "zones": ncell, "precision": "fp64", "seed": a.seed},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": abytes, "avg_launch_ms": avg_ms},
-        "step_roofline": {"b_alg_per_zone_update": B_ALG["fp64"], "achieved_gbs": step_gbs,
-                          "frac": step_gbs / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "basis": "whole step: zone-updates/s x B_alg",
+                     "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": step_gbs / HBM_PEAK_GBS, "traffic": step_traffic,
+                     "b_alg_per_zone_update": B_ALG["fp64"], "zone_updates_per_step": ncell,
+                     "traffic_per_zone_update": step_traffic / ncell if step_traffic else None},
+        "kernel_roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                            "traffic": traffic, "alg_bytes_per_launch": abytes,
+                            "avg_launch_ms": avg_ms},
         "kernels_ms": {k: (v[1] / v[0] if v[0] else None) for k, v in stats.items()},
     }
     if not a.no_mixed:

@@ -83,11 +83,31 @@ public:
   bool padded = false;
   int side[6] = {GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC, GPL_PERIODIC};
   int na[3] = {0, 0, 0};      // active cells per direction
+  // Slab pipeline (FP64 fused kernels with the EMF tiles): a stage runs the
+  // three sweeps slab by slab of z-planes on the context stream while a
+  // second stream runs the EMF and the finish of the slabs whose inputs are
+  // complete, so the VALU-bound sweeps and the HBM-bound EMF and finish
+  // overlap.  The finish then writes the primitives into a second buffer
+  // (Valt), since later slabs' sweeps still read the stage's input; the
+  // buffers swap after every stage.  Each cell's arithmetic is unchanged.
+  bool pipelined = false;
+  int nslab = 0, slab_planes = 0;
+  void *Valt = nullptr;       // the other primitive buffer (pipelined)
+  void *V0 = nullptr;         // the buffer V held after upload (graph key)
+  hipStream_t s2 = nullptr;   // the EMF/finish stream (default priority)
+  static constexpr int kMaxSlab = 64;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_slab[kMaxSlab] = {};
+  std::vector<int> slab_part0;   // first C_dt partial slot of each slab's z march
+  char key_tab[3][2] = {};       // graph keys: (face buffer, primitive buffer)
 
   ~Mhd3dEngine() override
   {
-    void *ptrs[] = {V, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net, diag};
+    void *ptrs[] = {V, Valt, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net, diag};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    for (hipEvent_t e : ev_slab) if (e) (void)hipEventDestroy(e);
+    if (s2) (void)hipStreamDestroy(s2);
   }
 
   int init(const gpl_config &c, const EngineEnv &e)
@@ -103,6 +123,7 @@ public:
     for (int d = 0; d < 3; d++) { g.lo[d] = off; g.hi[d] = off + na[d] - 1; }
     n = (size_t)g.nx*g.ny*g.nz;
     g.ncell = (int)n;
+    g.kb = 0; g.ke = g.nz; g.part0 = 0;
     // dx = (xend - xbeg)/N for every cell (Src/set_grid.cpp:196)
     g.small_rho = c.small_rho;
     g.small_prs = c.small_prs;
@@ -126,8 +147,51 @@ public:
     if (const char *m = getenv("GPL_FUSED_XY")) fused_xy = atoi(m) != 0;
     fused_xy = fused_xy && fused && 20*(size_t)g.nx*rs <= 65536;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
+    // the slab pipeline: GPL_PIPE_SLABS slabs (0 or 1: off); every slab but
+    // the last starts and ends at a multiple of the y/z march and of the EMF
+    // tiles' march length
+    {
+      int want = 0;
+      if (const char *m = getenv("GPL_PIPE_SLABS")) want = atoi(m);
+      // a slab's EMF launch covers only its planes: tiles marching 8 planes
+      // keep four times as many workgroups in flight as the default 32
+      if (want >= 3 && !getenv("GPL_EMF_MARCH")) emf_march = -8;
+      const int lt = emf_march < 0 ? -emf_march : 0;
+      if (want >= 3 && fp64 && fused && !fused_xy && lt > 0 &&
+          c.ct_emf_average == GPL_UCT_HLL && want <= kMaxSlab) {
+        const int unit = march > lt ? march : lt;
+        int p = (g.nz + want - 1)/want;
+        p = (p + unit - 1)/unit*unit;
+        if (p > 0 && (g.nz + p - 1)/p >= 3) {
+          pipelined = true;
+          slab_planes = p;
+          nslab = (g.nz + p - 1)/p;
+        }
+      }
+      if (!pipelined && want >= 3 && !getenv("GPL_EMF_MARCH")) emf_march = -32;
+    }
+    if (pipelined) {
+      // C_dt partial slots: each slab's z march writes its own range
+      slab_part0.resize(nslab);
+      int total = 0;
+      for (int s = 0; s < nslab; s++) {
+        slab_part0[s] = total;
+        const int nk = (s + 1)*slab_planes < g.nz ? slab_planes : g.nz - s*slab_planes;
+        const long long nt = (long long)g.nx*g.ny*((nk + march - 1)/march);
+        total += (int)((nt + 255)/256);
+      }
+      nparts = total;
+      if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+        return GPL_EHIP;
+      for (int s = 0; s < nslab; s++)
+        if (hipEventCreateWithFlags(&ev_slab[s], hipEventDisableTiming) != hipSuccess) return GPL_EHIP;
+    }
     int rc = 0;
     rc |= dev_alloc(bytes, &V, 8*n*rs);
+    if (pipelined) rc |= dev_alloc(bytes, &Valt, 8*n*rs);
+    V0 = V;
     rc |= dev_alloc(bytes, (void **)&U, 5*n*sizeof(double));
     rc |= dev_alloc(bytes, (void **)&Us, 5*n*sizeof(double));
     for (int q = 0; q < 3; q++) rc |= dev_alloc(bytes, (void **)&vs[q], 3*n*sizeof(double));
@@ -299,6 +363,10 @@ public:
     static const int split_seq[] = {K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_RHS, K_EMF, K_FINISH};
     static const int fused_seq[] = {K_SWEEPX_F, K_SWEEPY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
     static const int xy_seq[] = {K_SWEEPXY_F, K_SWEEPZ_F, K_EMF, K_FINISH};
+    if (pipelined && !t) {
+      enqueue_stage_pipe<R>(b, w, first_stage, want_dt);
+      return;
+    }
     const int *seq = fused ? (fused_xy ? xy_seq : fused_seq) : split_seq;
     const int nk = fused ? (fused_xy ? 4 : 5) : 6;
     for (int q = 0; q < nk; q++) {
@@ -310,6 +378,58 @@ public:
                       march, cfg.gamma, cfg.small_rho, cfg.small_prs, env.stream);
       if (t) t->end(k);
     }
+  }
+
+  // One stage as a slab pipeline.  Slab s covers planes [s P, (s+1) P).  The
+  // EMF of slab s reads face records and faces of planes s P - 2 .. (s+1) P
+  // (its z stencils and the next plane's records), so it waits for the
+  // sweeps of slabs s-1, s, s+1 (slab 0 and the last slab, which wrap, for
+  // every slab); the finish of slab s reads the edge EMFs of planes s P ..
+  // (s+1) P, so it follows the EMF of slab s+1 (of slab 0 for the last).  It
+  // writes the primitives into Valt, which no sweep of the stage reads.  The
+  // stage ends when both streams have drained (the next stage's z march
+  // reads every plane).
+  template <typename R>
+  void enqueue_stage_pipe(const Mhd3dBufs<R> &b, RKW w, int first_stage, int want_dt)
+  {
+    const int solver = cfg.solver;
+    const hipStream_t sa = env.stream;
+    Mhd3dBufs<R> bf = b;
+    bf.V = (R *)Valt;                       // the finish writes the other buffer
+    auto slab_box = [&](int s) {
+      Box3 q = g;
+      q.kb = s*slab_planes;
+      q.ke = (s + 1)*slab_planes < g.nz ? (s + 1)*slab_planes : g.nz;
+      q.part0 = slab_part0[s];
+      return q;
+    };
+    auto launch = [&](int k, const Mhd3dBufs<R> &bb, int s, hipStream_t st) {
+      mhd3d_launch<R>(k, bb, slab_box(s), env.ctl, w, first_stage, want_dt, cfg.limiter, solver,
+                      march, cfg.gamma, cfg.small_rho, cfg.small_prs, st);
+    };
+    (void)hipEventRecord(ev_fork, sa);
+    (void)hipStreamWaitEvent(s2, ev_fork, 0);
+    for (int s = 0; s < nslab; s++) {
+      launch(K_SWEEPX_F, b, s, sa);
+      launch(K_SWEEPY_F, b, s, sa);
+      launch(K_SWEEPZ_F, b, s, sa);
+      (void)hipEventRecord(ev_slab[s], sa);
+    }
+    const int S = nslab;
+    for (int s = 1; s <= S - 2; s++) {
+      (void)hipStreamWaitEvent(s2, ev_slab[s + 1], 0);
+      launch(K_EMF, b, s, s2);
+      if (s >= 2) launch(K_FINISH, bf, s - 1, s2);
+    }
+    (void)hipStreamWaitEvent(s2, ev_slab[S - 1], 0);
+    launch(K_EMF, b, S - 1, s2);
+    launch(K_EMF, b, 0, s2);
+    launch(K_FINISH, bf, S - 2, s2);
+    launch(K_FINISH, bf, S - 1, s2);
+    launch(K_FINISH, bf, 0, s2);
+    (void)hipEventRecord(ev_join, s2);
+    (void)hipStreamWaitEvent(sa, ev_join, 0);
+    std::swap(V, Valt);
   }
 
   void enqueue_step(int advance_dt, LaunchTimer *t) override
@@ -345,8 +465,10 @@ public:
   }
 
   // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
-  int graph_period() const override { return 3; }
-  const void *state_key() const override { return vs[cur]; }
+  // faces: period 3; the primitive buffers of the pipeline swap every stage,
+  // so RK3's three stages need six steps to return both
+  int graph_period() const override { return pipelined && cfg.time_stepping == GPL_RK3 ? 6 : 3; }
+  const void *state_key() const override { return &key_tab[cur][V == V0 ? 0 : 1]; }
 
   int nvar() const override { return 8; }
   int nstages() const override { return cfg.time_stepping == GPL_RK3 ? 3 : 2; }

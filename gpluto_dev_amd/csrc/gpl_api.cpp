@@ -458,25 +458,36 @@ extern "C" int gpl_read_dbl(gpl_ctx *c, const char *dir, int nfile)
 // ---------------------------------------------------------------------------
 // stepping
 // ---------------------------------------------------------------------------
+// The step-control block and the error bits move between host and device on
+// the context's own stream, never through the null stream, so a context is
+// not held up by (or made to fail by) another context's work or graph capture
+// in another thread.
+static bool read_ctl(gpl_ctx *c, StepCtl *h, uint32_t *err)
+{
+  return hipMemcpyAsync(h, c->ctl, sizeof(*h), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         (!err || hipMemcpyAsync(err, c->err, sizeof(*err), hipMemcpyDeviceToHost, c->stream) == hipSuccess) &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+}
+
+static bool write_ctl(gpl_ctx *c, const StepCtl *h)
+{
+  static const uint32_t zero = 0;
+  return hipMemcpyAsync(c->ctl, h, sizeof(*h), hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+         hipMemcpyAsync(c->err, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+}
+
 static uint32_t gpl_advance_step_impl(gpl_ctx *c, double dt, double *inv_dt_hyp)
 {
   if (!c) return GPL_STEP_API_ERROR;
   hipSetDevice(c->dev);
   StepCtl h_in, h_out;
-  if (hipStreamSynchronize(c->stream) != hipSuccess ||
-      hipMemcpy(&h_in, c->ctl, sizeof(h_in), hipMemcpyDeviceToHost) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!read_ctl(c, &h_in, nullptr)) return GPL_STEP_API_ERROR;
   h_in.dt = dt;
-  const uint32_t zero = 0;
-  if (hipMemcpy(c->ctl, &h_in, sizeof(h_in), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->err, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!write_ctl(c, &h_in)) return GPL_STEP_API_ERROR;
   c->engine->enqueue_step(0, c->timing.on ? &c->timing : nullptr);
   uint32_t err = 0;
-  if (hipStreamSynchronize(c->stream) != hipSuccess ||
-      hipMemcpy(&h_out, c->ctl, sizeof(h_out), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(&err, c->err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!read_ctl(c, &h_out, &err)) return GPL_STEP_API_ERROR;
   c->timing.harvest();
   if (inv_dt_hyp) *inv_dt_hyp = h_out.inv_dt;
   return err;
@@ -518,14 +529,9 @@ static uint32_t gpl_run_impl(gpl_ctx *c, int nsteps, double first_dt, double *ti
   if (!c || nsteps < 0) return GPL_STEP_API_ERROR;
   hipSetDevice(c->dev);
   StepCtl h_in, h_out;
-  if (hipStreamSynchronize(c->stream) != hipSuccess ||
-      hipMemcpy(&h_in, c->ctl, sizeof(h_in), hipMemcpyDeviceToHost) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!read_ctl(c, &h_in, nullptr)) return GPL_STEP_API_ERROR;
   if (first_dt > 0.0) { h_in.dt = first_dt; h_in.time = 0.0; h_in.nstep = 0; }
-  const uint32_t zero = 0;
-  if (hipMemcpy(c->ctl, &h_in, sizeof(h_in), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->err, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!write_ctl(c, &h_in)) return GPL_STEP_API_ERROR;
   const bool use_graph = !c->timing.on && getenv("GPL_NO_GRAPH") == nullptr;
   const int period = c->engine->graph_period();
   int done = 0;
@@ -543,10 +549,7 @@ static uint32_t gpl_run_impl(gpl_ctx *c, int nsteps, double first_dt, double *ti
   }
   hipEventRecord(c->run_end, c->stream);
   uint32_t err = 0;
-  if (hipStreamSynchronize(c->stream) != hipSuccess ||
-      hipMemcpy(&h_out, c->ctl, sizeof(h_out), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(&err, c->err, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess)
-    return GPL_STEP_API_ERROR;
+  if (!read_ctl(c, &h_out, &err)) return GPL_STEP_API_ERROR;
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->run_beg, c->run_end);
   c->last_run_ms = ms;

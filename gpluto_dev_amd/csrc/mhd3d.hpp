@@ -18,6 +18,11 @@ struct Box3 {
   // density and pressure floors of ConsToPrim (run_config.small_rho /
   // small_prs), read by the RMHD HLLD's con2prim of the HLL state
   double small_rho, small_prs;
+  // planes kb .. ke-1 of the launch (the whole box: 0, nz) for the fused x
+  // sweep, the y and z marches, the EMF tiles and the finish kernel, and the
+  // first C_dt partial slot of the z march's blocks; a slab of planes starts
+  // at a multiple of the march lengths
+  int kb, ke, part0;
 };
 
 template <typename R>

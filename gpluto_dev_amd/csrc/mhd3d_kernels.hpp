@@ -286,7 +286,8 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
   R *sV = (R *)gpl_smem;           // [7][nx] primitives of the row
   R *sD = sV + 7*nx;               // [7][nx] left state vL(i+1) made by cell i
   const size_t n = (size_t)g.ncell;
-  const unsigned c = i + nx*blockIdx.x;
+  const unsigned row = blockIdx.x + (unsigned)g.ny*g.kb;   // j + ny*k
+  const unsigned c = i + nx*row;
   const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
   constexpr int VAR[7] = {0, 1, 2, 3, 5, 6, 7};   // B1 comes from the face
   R v0[7], dv[7];
@@ -332,7 +333,7 @@ void k_sweepx_fused(Box3 g, const R *__restrict__ V, const double *__restrict__ 
     sto(dU + nv*n, c, (R)r);
   }
   if (want_cdt) sto(Cdt, c, (R)(0.5*((double)sF[5*nx + i] + (double)sF[5*nx + ip])/g.dx));
-  if (ff.err && face_reported<0>(g, i, (int)blockIdx.x % g.ny, (int)blockIdx.x / g.ny))
+  if (ff.err && face_reported<0>(g, i, (int)(row % g.ny), (int)(row / g.ny)))
     atomicOr(err, ff.err);
 }
 
@@ -528,10 +529,11 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
   constexpr int VAR[7] = {0, 1, 2, 3, D == 1 ? 4 : 4, D == 1 ? 6 : 5, 7};
   // block order: x fastest, then the other transverse index, then the
   // segment, so that concurrent blocks stream whole planes
+  // planes kb .. ke-1: for y the transverse index a, for z whole segments
   const int t = blockIdx.x*blockDim.x + threadIdx.x;
   const int nd = D == 1 ? g.ny : g.nz;          // cells along the sweep
-  const int na = D == 1 ? g.nz : g.ny;          // the other transverse extent
-  const int nseg = (nd + L - 1)/L;
+  const int na = D == 1 ? g.ke - g.kb : g.ny;   // the other transverse extent
+  const int nseg = D == 1 ? (nd + L - 1)/L : (g.ke - g.kb + L - 1)/L;
   double cdt_max = -1.0;
   uint32_t e = 0;
   if (t < g.nx*na*nseg) {
@@ -551,7 +553,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
     R *fd0 = FD + (size_t)D*uct_nrec<SOLVER>()*n;
     const int i = t % g.nx;
     const int r = t / g.nx;
-    const int a = r % na, seg = r / na;
+    const int a = (D == 1 ? g.kb : 0) + r % na, seg = (D == 1 ? 0 : g.kb/L) + r / na;
     const int sd = D == 1 ? g.nx : g.nx*g.ny;     // stride along D
     const int base = D == 1 ? i + g.nx*g.ny*a : i + g.nx*a;
     const int s0 = seg*L, s1 = s0 + L < nd ? s0 + L : nd;
@@ -657,7 +659,7 @@ void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ V
   if (e) atomicOr(err, e);
   if (D == 2 && want_cdt) {
     const double m = block_max(cdt_max);
-    if (threadIdx.x == 0) cdt_part[blockIdx.x] = m;
+    if (threadIdx.x == 0) cdt_part[g.part0 + blockIdx.x] = m;
   }
 }
 
@@ -806,14 +808,14 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int ntx = (nx + TX - 1)/TX, nty = (ny + TY - 1)/TY;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int txb = t % ntx, rr = t / ntx, tyb = rr % nty, seg = rr / nty;
+  const int txb = t % ntx, rr = t / ntx, tyb = rr % nty, seg = g.kb/L + rr / nty;
   const int x0 = txb*TX, y0 = tyb*TY;
   const int tw = nx - x0 < TX ? nx - x0 : TX, th = ny - y0 < TY ? ny - y0 : TY;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const bool act = tx < tw && ty < th;
   const size_t n = (size_t)g.ncell;
   const unsigned pl = nx*ny;
-  const int k0 = seg*L, k1 = k0 + L < nz ? k0 + L : nz;
+  const int k0 = seg*L, k1 = k0 + L < g.ke ? k0 + L : g.ke;
   // uniform array bases (scalar registers) + 32-bit cell offsets
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
   const R *ex0 = FD, *ex1 = FD + n, *fx2 = FD + 2*n, *fx3 = FD + 3*n, *fx4 = FD + 4*n;
@@ -1074,9 +1076,10 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               const float *__restrict__ net, ZoneRep zr)
 {
   // no early exit: the learned con2prim guess is evaluated by whole wavefronts
-  const int c0 = blockIdx.x*blockDim.x + threadIdx.x;
-  const bool live = c0 < g.ncell;
-  const int c = live ? c0 : g.ncell - 1;
+  const int c_beg = g.kb*g.nx*g.ny, c_end = g.ke*g.nx*g.ny;   // planes kb .. ke-1
+  const int c0 = c_beg + blockIdx.x*blockDim.x + threadIdx.x;
+  const bool live = c0 < c_end;
+  const int c = live ? c0 : c_end - 1;
   const size_t n = (size_t)g.ncell;
   const Cell p = cell_of(c, g);
   const double dt = ctl->dt;
@@ -1154,6 +1157,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
 {
   const int T = 256;
   const int nb = (g.ncell + T - 1)/T;
+  const int nk = g.ke - g.kb;          // planes of the fused kernels' launch
   switch (which) {
   case K_SWEEP_X:
     hipLaunchKernelGGL((k_sweep<R, 0, LIM, SOLVER>), dim3(nb), dim3(T), 0, s,
@@ -1174,7 +1178,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   case K_SWEEPX_F: {
     const size_t shm = 14*(size_t)g.nx*sizeof(R);
-    hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER, PH>), dim3(g.ny*g.nz), dim3(g.nx), shm, s,
+    hipLaunchKernelGGL((k_sweepx_fused<R, LIM, SOLVER, PH>), dim3(g.ny*nk), dim3(g.nx), shm, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
     break;
   }
@@ -1186,14 +1190,14 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   }
   case K_SWEEPY_F: {
-    const int nt = g.nx*g.nz*((g.ny + L - 1)/L);
+    const int nt = g.nx*nk*((g.ny + L - 1)/L);
     hipLaunchKernelGGL((k_sweep_march<R, 1, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.Uin, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
     break;
   }
   case K_SWEEPZ_F: {
-    const int nt = g.nx*g.ny*((g.nz + L - 1)/L);
+    const int nt = g.nx*g.ny*((nk + L - 1)/L);
     hipLaunchKernelGGL((k_sweep_march<R, 2, LIM, SOLVER, L, PH>), dim3((nt + T - 1)/T), dim3(T), 0, s,
                        g, b.V, b.Vin, b.FD, b.dU, b.Cdt, b.Uin, b.U, b.U0, ctl, w, first_stage, want_dt,
                        b.cdt_part, (R)gamma, b.err);
@@ -1211,7 +1215,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
       hipLaunchKernelGGL((k_emf_march<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
     } else if (b.emf_march < 0) {        // LDS-staged tiles marching -emf_march planes
       const int lt = -b.emf_march;
-      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((g.nz + lt - 1)/lt);
+      const int nt = ((g.nx + 63)/64)*((g.ny + 3)/4)*((nk + lt - 1)/lt);
       if (lt == 4)
         hipLaunchKernelGGL((k_emf_tile<R, LIM, 4>), dim3(nt), dim3(256), 0, s, g, b.Vin, b.FD, b.E);
       else if (lt == 8)
@@ -1234,7 +1238,7 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
   }
   case K_FINISH:
     // the RMHD mapper takes gamma, the MHD one gamma - 1
-    hipLaunchKernelGGL((k_finish<R, PH>), dim3(nb), dim3(T), 0, s,
+    hipLaunchKernelGGL((k_finish<R, PH>), dim3((nk*g.nx*g.ny + T - 1)/T), dim3(T), 0, s,
                        g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
                        PH == 1 ? gamma : gamma - 1.0, small_rho, small_prs, b.err,
                        PH == 1 ? b.net : nullptr, b.zr);

@@ -11,11 +11,3 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
-
-
-def gpu_available() -> bool:
-    try:
-        import torch  # noqa: F401  (device count only; no HIP context is created)
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False

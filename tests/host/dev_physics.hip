@@ -11,7 +11,7 @@ using namespace gpl;
 
 namespace {
 
-__global__ void k_crmath(const double *x, int n, int mode, double *o1, double *o2)
+__global__ void __launch_bounds__(256) k_crmath(const double *x, int n, int mode, double *o1, double *o2)
 {
   const int q = blockIdx.x*blockDim.x + threadIdx.x;
   if (q >= n) return;
@@ -42,7 +42,7 @@ __device__ void face_n(int phys, int solver, const double *vl, const double *vr,
   }
 }
 
-__global__ void k_face(int phys, int solver, const double *vl, const double *vr, int n, int dir,
+__global__ void __launch_bounds__(256) k_face(int phys, int solver, const double *vl, const double *vr, int n, int dir,
                        double gamma, double *flux, double *sl, double *sr, uint32_t *err)
 {
   const int q = blockIdx.x*blockDim.x + threadIdx.x;
@@ -58,7 +58,25 @@ __global__ void k_face(int phys, int solver, const double *vl, const double *vr,
   err[q] = ff.err;
 }
 
-__global__ void k_c2p(double *u, double *v, int n, double gamma, uint32_t *err)
+// the RMHD HLLC solver alone in its kernel (the solver of the stage kernels
+// is instantiated alone per kernel too)
+template <int N>
+__global__ void __launch_bounds__(256) k_face_rmhd_hllc(const double *vl, const double *vr, int n, double gamma,
+                                 double *flux, double *sl, double *sr, uint32_t *err)
+{
+  const int q = blockIdx.x*blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double a[8], b[8];
+  for (int v = 0; v < 8; v++) { a[v] = vl[8*(size_t)q + v]; b[v] = vr[8*(size_t)q + v]; }
+  FaceFlux<double> ff;
+  rmhd_hllc<N>(a, b, gamma, ff);
+  for (int v = 0; v < 8; v++) flux[8*(size_t)q + v] = ff.f[v];
+  sl[q] = ff.Sl;
+  sr[q] = ff.Sr;
+  err[q] = ff.err;
+}
+
+__global__ void __launch_bounds__(256) k_c2p(double *u, double *v, int n, double gamma, uint32_t *err)
 {
   const int q = blockIdx.x*blockDim.x + threadIdx.x;
   if (q >= n) return;
@@ -119,6 +137,27 @@ extern "C" int dp_face(int phys, int solver, const double *vl, const double *vr,
   if (d.rc != hipSuccess) return (int)d.rc;
   hipLaunchKernelGGL(k_face, dim3(blocks(n)), dim3(256), 0, 0, phys, solver, a, b, n, dir, gamma,
                      f, l, r, e);
+  d.rc = hipDeviceSynchronize();
+  d.out(flux, f, 8*(size_t)n);
+  d.out(sl, l, n);
+  d.out(sr, r, n);
+  d.out(err, e, n);
+  return (int)d.rc;
+}
+
+extern "C" int dp_face_rmhd_hllc(const double *vl, const double *vr, int n, int dir, double gamma,
+                                 double *flux, double *sl, double *sr, uint32_t *err)
+{
+  if (n <= 0) return 0;
+  Dev d;
+  const double *a = d.in(vl, 8*(size_t)n), *b = d.in(vr, 8*(size_t)n);
+  double *f = d.in<double>(nullptr, 8*(size_t)n), *l = d.in<double>(nullptr, n);
+  double *r = d.in<double>(nullptr, n);
+  uint32_t *e = d.in<uint32_t>(nullptr, n);
+  if (d.rc != hipSuccess) return (int)d.rc;
+  if (dir == 0) hipLaunchKernelGGL(k_face_rmhd_hllc<0>, dim3(blocks(n)), dim3(256), 0, 0, a, b, n, gamma, f, l, r, e);
+  else if (dir == 1) hipLaunchKernelGGL(k_face_rmhd_hllc<1>, dim3(blocks(n)), dim3(256), 0, 0, a, b, n, gamma, f, l, r, e);
+  else hipLaunchKernelGGL(k_face_rmhd_hllc<2>, dim3(blocks(n)), dim3(256), 0, 0, a, b, n, gamma, f, l, r, e);
   d.rc = hipDeviceSynchronize();
   d.out(flux, f, 8*(size_t)n);
   d.out(sl, l, n);
