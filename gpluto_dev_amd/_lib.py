@@ -19,7 +19,8 @@ GPL_RK2, GPL_RK3 = 2, 3
 GPL_PERIODIC, GPL_OUTFLOW, GPL_REFLECTIVE = 1, 2, 3
 GPL_FP64, GPL_MIXED = 0, 1
 GPL_LAYOUT_ACTIVE, GPL_LAYOUT_ARY4D = 0, 1
-GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED, GPL_EIO = -1, -2, -3, -4, -5
+GPL_EINVAL, GPL_EHIP, GPL_ENOMEM, GPL_EUNSUPPORTED, GPL_EIO, GPL_ESCRATCH = -1, -2, -3, -4, -5, -6
+GPL_C2P_NOALLOC_MAX = 65536
 GPL_STEP_API_ERROR = 0x80000000     # stepping calls: the call itself failed
 
 # every symbol include/gpluto_amd.h declares
@@ -33,7 +34,8 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_c2p_net_size", "gpl_c2p_net_make_data", "gpl_c2p_net_train", "gpl_c2p_net_grad",
            "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics",
            "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl",
-           "gpl_write_dbl_async", "gpl_wait_dbl", "gpl_set_zone_reports", "gpl_get_zone_reports")
+           "gpl_write_dbl_async", "gpl_wait_dbl", "gpl_set_zone_reports", "gpl_get_zone_reports",
+           "gpl_c2p_solve_noalloc")
 
 
 class GplConfig(C.Structure):
@@ -132,6 +134,9 @@ def load() -> C.CDLL:
     L.gpl_c2p_solve.argtypes = [ip, fp, dp, ip, C.c_double, dp, C.POINTER(C.c_int),
                                 C.POINTER(C.c_uint32), dp, C.POINTER(C.c_int)]
     L.gpl_c2p_solve.restype = ip
+    L.gpl_c2p_solve_noalloc.argtypes = [ip, fp, dp, ip, C.c_double, dp, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_int)]
+    L.gpl_c2p_solve_noalloc.restype = ip
     L.gpl_set_c2p_network.argtypes = [vp, fp]
     L.gpl_set_c2p_network.restype = ip
     L.gpl_diagnostics.argtypes = [vp, dp, dp, dp, dp]

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace gpl {
@@ -396,5 +397,83 @@ extern "C" int gpl_c2p_solve(int device, const float *weights, const double *u, 
     }
   }
   if (ms_out) *ms_out = ms;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// gpl_c2p_solve without allocation: the states, results and weights are
+// staged in one file-scope device array instead of hipMalloc'd buffers.  The
+// host reaches the array only through the address hipGetSymbolAddress
+// returns for the current device (each device holds its own copy).
+// ---------------------------------------------------------------------------
+namespace gpl {
+namespace {
+
+constexpr size_t kNoAllocMax = GPL_C2P_NOALLOC_MAX;
+constexpr size_t kScrU = 0;                                        // [max][8] double
+constexpr size_t kScrV = kScrU + kNoAllocMax*8*sizeof(double);     // [max][8] double
+constexpr size_t kScrP = kScrV + kNoAllocMax*8*sizeof(double);     // [max] int
+constexpr size_t kScrE = kScrP + kNoAllocMax*sizeof(int);          // [max] uint32
+constexpr size_t kScrR = kScrE + kNoAllocMax*sizeof(uint32_t);     // [max] int
+constexpr size_t kScrW = kScrR + kNoAllocMax*sizeof(int);          // [NET_SIZE] float
+constexpr size_t kScrBytes = kScrW + (NET_SIZE*sizeof(float) + 255)/256*256;
+static_assert(kScrV % 256 == 0 && kScrP % 256 == 0 && kScrE % 256 == 0 && kScrR % 256 == 0 &&
+              kScrW % 256 == 0, "scratch regions are 256-byte aligned");
+
+__device__ __align__(256) unsigned char scratch[kScrBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+unsigned char *scratch_dev[kMaxDevices];   // the array's address on each device
+
+// the scratch address on the current device (looked up once per device)
+int scratch_on(int device, unsigned char **out)
+{
+  if (device < 0 || device >= kMaxDevices) return GPL_EINVAL;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GPL_EHIP;
+    scratch_dev[device] = (unsigned char *)p;
+  }
+  *out = scratch_dev[device];
+  return 0;
+}
+
+}  // namespace
+}  // namespace gpl
+
+extern "C" int gpl_c2p_solve_noalloc(int device, const float *weights, const double *u, int n,
+                                     double gamma, double *v_out, int *passes_out, uint32_t *err_out,
+                                     int *restart_out)
+{
+  using namespace gpl;
+  if (!u || !v_out || !passes_out || !err_out || n < 1) return GPL_EINVAL;
+  if ((size_t)n > kNoAllocMax) return GPL_ESCRATCH;
+  if (int rc = set_device(device)) return rc;
+  unsigned char *base = nullptr;
+  if (int rc = scratch_on(device, &base)) return rc;
+  double *du = (double *)(base + kScrU), *dv = (double *)(base + kScrV);
+  int *dp = (int *)(base + kScrP), *dr = (int *)(base + kScrR);
+  uint32_t *de = (uint32_t *)(base + kScrE);
+  float *dw = (float *)(base + kScrW);
+  if ((weights && hipMemcpy(dw, weights, NET_SIZE*sizeof(float), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(du, u, (size_t)n*8*sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return GPL_EHIP;
+  // every output slot below n is written by the kernel: nothing to clear
+  hipLaunchKernelGGL(k_net_solve, dim3((n + 255)/256), dim3(256), 0, 0, weights ? dw : nullptr, du, n,
+                     gamma, dv, dp, de, (restart_out && weights) ? dr : nullptr);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpy(v_out, dv, (size_t)n*8*sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(passes_out, dp, (size_t)n*sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(err_out, de, (size_t)n*sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    return GPL_EHIP;
+  if (restart_out) {
+    if (!weights) {
+      for (int i = 0; i < n; i++) restart_out[i] = 0;
+    } else if (hipMemcpy(restart_out, dr, (size_t)n*sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      return GPL_EHIP;
+    }
+  }
   return 0;
 }

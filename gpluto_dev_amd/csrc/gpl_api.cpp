@@ -731,6 +731,7 @@ extern "C" const char *gpl_strerror(int code)
   case GPL_ENOMEM: return "out of device or host memory";
   case GPL_EUNSUPPORTED: return "configuration not supported";
   case GPL_EIO: return "dump or restart file error";
+  case GPL_ESCRATCH: return "request larger than the fixed scratch of a no-allocation entry point";
   default: return "unknown error";
   }
 }

@@ -60,6 +60,7 @@ extern "C" {
 #define GPL_ENOMEM       (-3)   /* device memory, or host memory or threads for the host side */
 #define GPL_EUNSUPPORTED (-4)
 #define GPL_EIO          (-5)   /* a dump or restart file could not be opened, read or written */
+#define GPL_ESCRATCH     (-6)   /* more than a no-allocation entry point's fixed scratch holds */
 
 /* The selections of definitions.hpp and pluto.ini that the stage update
  * reads (Src/structs.hpp:397-470 RunConfig/Runtime, Src/grid_input.hpp). */
@@ -292,6 +293,19 @@ int gpl_set_c2p_network(gpl_ctx *ctx, const float *weights);
 int gpl_c2p_solve(int device, const float *weights, const double *u, int n, double gamma,
                   double *v_out, int *passes_out, uint32_t *err_out, double *ms_out,
                   int *restart_out);
+
+/* gpl_c2p_solve that allocates nothing.  No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The states, results
+ * and weights are staged in a fixed device array of the library (one per
+ * device, about 9.2 MB, held until the process ends) instead of buffers
+ * allocated per call, so n is at most GPL_C2P_NOALLOC_MAX (larger n:
+ * GPL_ESCRATCH, nothing run).  The same kernel computes the same bits as
+ * gpl_c2p_solve; there is no kernel time output.  Calls that overlap on one
+ * device share that array and race; calls on different devices do not. */
+#define GPL_C2P_NOALLOC_MAX 65536
+int gpl_c2p_solve_noalloc(int device, const float *weights, const double *u, int n, double gamma,
+                          double *v_out, int *passes_out, uint32_t *err_out, int *restart_out);
 
 #ifdef __cplusplus
 }
