@@ -889,6 +889,14 @@ GPL_HD void rmhd_hll(const double *vl, const double *vr, double gamma, FaceFlux<
 // Src/gpluto.hpp:877-878, Src/RMHD/rmhd.hpp:143-145).  Every failed star
 // construction (no real or subluminal contact speed, a superluminal or
 // non-positive star state, a non-finite value) falls back to the HLL flux.
+//
+// Written without data-dependent branches: both star constructions (Bx -> 0
+// and general) and every test are evaluated for every face, and the flux is
+// selected at the end.  Each selected value is the same expression as in the
+// branching form, so the bits are the reference's; on the GPU no value has to
+// survive a divergent branch.  (The branching form was miscompiled for the
+// GPU: the HLL components needed by faces falling back after a superluminal
+// star state shared registers with the star velocities.)
 template <int N>
 GPL_HD void rmhd_hllc(const double *vl, const double *vr, double gamma, FaceFlux<double> &out)
 {
@@ -908,132 +916,106 @@ GPL_HD void rmhd_hllc(const double *vl, const double *vr, double gamma, FaceFlux
   out.Sl = Sl; out.Sr = Sr;
   rmhd_flux<double, N>(vl, ul, hL, fl, pl);
   rmhd_flux<double, N>(vr, ur, hR, fr, pr);
-  double *flux = out.f;
-  double Uhll[8], Fhll[8];
-  double scrh = 1.0/(Sr - Sl);
+  double Uhll[8], Fhll[8], Fh[8];
+  const double rdS = 1.0/(Sr - Sl);
   for (int nv = 0; nv < 8; nv++) {
     Uhll[nv] = Sr*ur[nv] - Sl*ul[nv] + fl[nv] - fr[nv];
-    Uhll[nv] *= scrh;
+    Uhll[nv] *= rdS;
     Fhll[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
-    Fhll[nv] *= scrh;
+    Fhll[nv] *= rdS;
+    Fh[nv] = Fhll[nv];             // the HLL flux, the fallback
   }
-  if (Sl >= 0.0) {
-    for (int nv = 0; nv < 8; nv++) flux[nv] = fl[nv];
-  } else if (Sr <= 0.0) {
-    for (int nv = 0; nv < 8; nv++) flux[nv] = fr[nv];
-  } else {
-    bool to_hll = false;
-    double usl[8], usr[8], vxs = 0.0;
-    const double vxl = vl[VXn], vxr = vr[VXn];
-    const double Bx = Uhll[BXn], Bys = Uhll[BXt], Bzs = Uhll[BXb];
-    Uhll[ENG] += Uhll[0];           // the star states are derived with the total energy
-    Fhll[ENG] += Fhll[0];
-    // coefficients of the quadratic for the contact speed v* (Bx -> 0: Eq. [47])
-    double a, b, c, BtFBt, FBt2;
-    const bool degenerate = rabs(Bx) < BX_MIN;
-    if (degenerate) {
-      BtFBt = FBt2 = 0.0;
-      a = Fhll[ENG];
-      b = - (Fhll[MXn] + Uhll[ENG]);
-      c = Uhll[MXn];
-    } else {
-      BtFBt = Uhll[BXt]*Fhll[BXt] + Uhll[BXb]*Fhll[BXb];
-      const double Bt2 = Uhll[BXt]*Uhll[BXt] + Uhll[BXb]*Uhll[BXb];
-      FBt2 = Fhll[BXt]*Fhll[BXt] + Fhll[BXb]*Fhll[BXb];
-      a = Fhll[ENG] - BtFBt;
-      b = Bt2 + FBt2 - (Fhll[MXn] + Uhll[ENG]);
-      c = Uhll[MXn] - BtFBt;
-    }
-    const double b2 = b*b;
-    if (!risfinite(a) || !risfinite(b) || !risfinite(c) || !risfinite(b2) || !(b2 > 0.0)) {
-      to_hll = true;
-    } else {
-      const double disc = 1.0 - 4.0*a*c/b2;
-      if (!risfinite(disc) || disc < 0.0) {
-        to_hll = true;
-      } else {
-        scrh = 1.0 + sqrt(disc);
-        const double den = b*scrh;
-        if (!risfinite(den) || den == 0.0) {
-          to_hll = true;
-        } else {
-          vxs = -2.0*c/den;
-          if (!risfinite(vxs) || !(rabs(vxs) < 1.0) || !(Sl < vxs && vxs < Sr)) to_hll = true;
-        }
-      }
-    }
-    if (!to_hll) {
-      if (degenerate) {              // (vy, vz) do not enter
-        const double alpha_l = (Sl - vxl)/(Sl - vxs);
-        const double alpha_r = (Sr - vxr)/(Sr - vxs);
-        usl[0] = ul[0]*alpha_l;
-        usr[0] = ur[0]*alpha_r;
-        const double ps = Fhll[MXn] - Fhll[ENG]*vxs;
-        usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs)/(Sl - vxs);
-        usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs)/(Sr - vxs);
-        usl[MXn] = (usl[ENG] + ps)*vxs;
-        usr[MXn] = (usr[ENG] + ps)*vxs;
-        usl[MXt] = ul[MXt]*alpha_l;
-        usr[MXt] = ur[MXt]*alpha_r;
-        usl[MXb] = ul[MXb]*alpha_l;
-        usr[MXb] = ur[MXb]*alpha_r;
-        usl[MXn] += usl[0]*vxs;      // reduced energy
-        usr[MXn] += usr[0]*vxs;
-        usl[BXn] = Bx;
-        usr[BXn] = Bx;
-        usl[BXt] = ul[BXt]*alpha_l;
-        usr[BXt] = ur[BXt]*alpha_r;
-        usl[BXb] = ul[BXb]*alpha_l;
-        usr[BXb] = ur[BXb]*alpha_r;
-      } else {
-        const double vys = (Bys*vxs - Fhll[BXt])/Bx;
-        const double vzs = (Bzs*vxs - Fhll[BXb])/Bx;
-        double gammas_2 = vxs*vxs + vys*vys + vzs*vzs;
-        gammas_2 = 1.0 - gammas_2;
-        if (!risfinite(gammas_2) || gammas_2 <= 0.0) to_hll = true;     // superluminal
-        if (!to_hll) {
-          const double vBs = vxs*Bx + vys*Bys + vzs*Bzs;
-          const double alpha_l = (Sl - vxl)/(Sl - vxs);
-          const double alpha_r = (Sr - vxr)/(Sr - vxs);
-          usl[0] = ul[0]*alpha_l;
-          usr[0] = ur[0]*alpha_r;
-          const double ps = (BtFBt - Fhll[ENG])*vxs + Bx*Bx + Fhll[MXn] - FBt2;
-          usl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs - vBs*Bx)/(Sl - vxs);
-          usr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs - vBs*Bx)/(Sr - vxs);
-          usl[MXn] = (usl[ENG] + ps)*vxs - vBs*Bx;
-          usr[MXn] = (usr[ENG] + ps)*vxs - vBs*Bx;
-          usl[MXt] = (Sl*ul[MXt] - fl[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sl - vxs);
-          usr[MXt] = (Sr*ur[MXt] - fr[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sr - vxs);
-          usl[MXb] = (Sl*ul[MXb] - fl[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sl - vxs);
-          usr[MXb] = (Sr*ur[MXb] - fr[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sr - vxs);
-          usl[MXn] += usl[0]*vxs;    // reduced energy
-          usr[MXn] += usr[0]*vxs;
-          usl[BXn] = usr[BXn] = Bx;
-          usl[BXt] = usr[BXt] = Bys;
-          usl[BXb] = usr[BXb] = Bzs;
-        }
-      }
-      if (!to_hll) {
-        if (!(usl[0] > 0.0) || !(usr[0] > 0.0)) to_hll = true;
-        for (int nv = 0; nv < 8; nv++)
-          if (!risfinite(usl[nv]) || !risfinite(usr[nv])) to_hll = true;
-      }
-    }
-    if (!to_hll) {
-      const bool left = vxs > 0.0;
-      for (int nv = 0; nv < 8; nv++)
-        flux[nv] = left ? fl[nv] + Sl*(usl[nv] - ul[nv]) : fr[nv] + Sr*(usr[nv] - ur[nv]);
-      for (int nv = 0; nv < 8; nv++) if (!risfinite(flux[nv])) to_hll = true;
-      if (!risfinite(left ? pl : pr)) to_hll = true;
-    }
-    if (to_hll) {
-      scrh = 1.0/(Sr - Sl);
-      for (int nv = 0; nv < 8; nv++) {
-        flux[nv]  = Sl*Sr*(ur[nv] - ul[nv]) + Sr*fl[nv] - Sl*fr[nv];
-        flux[nv] *= scrh;
-      }
-    }
+  const double vxl = vl[VXn], vxr = vr[VXn];
+  const double Bx = Uhll[BXn], Bys = Uhll[BXt], Bzs = Uhll[BXb];
+  Uhll[ENG] += Uhll[0];             // the star states are derived with the total energy
+  Fhll[ENG] += Fhll[0];
+  // coefficients of the quadratic for the contact speed v* (Bx -> 0: Eq. [47])
+  const bool degenerate = rabs(Bx) < BX_MIN;
+  const double BtFBt_g = Uhll[BXt]*Fhll[BXt] + Uhll[BXb]*Fhll[BXb];
+  const double Bt2_g   = Uhll[BXt]*Uhll[BXt] + Uhll[BXb]*Uhll[BXb];
+  const double FBt2_g  = Fhll[BXt]*Fhll[BXt] + Fhll[BXb]*Fhll[BXb];
+  const double BtFBt = degenerate ? 0.0 : BtFBt_g;
+  const double FBt2  = degenerate ? 0.0 : FBt2_g;
+  const double a = degenerate ? Fhll[ENG] : Fhll[ENG] - BtFBt_g;
+  const double b = degenerate ? -(Fhll[MXn] + Uhll[ENG]) : Bt2_g + FBt2_g - (Fhll[MXn] + Uhll[ENG]);
+  const double c = degenerate ? Uhll[MXn] : Uhll[MXn] - BtFBt_g;
+  const double b2 = b*b;
+  // the tests combine with | (not ||): no short-circuit branch
+  bool to_hll = !risfinite(a) | !risfinite(b) | !risfinite(c) | !risfinite(b2) | !(b2 > 0.0);
+  const double disc = 1.0 - 4.0*a*c/b2;
+  to_hll |= !risfinite(disc) | (disc < 0.0);
+  const double den = b*(1.0 + sqrt(disc));
+  to_hll |= !risfinite(den) | (den == 0.0);
+  const double vxs = -2.0*c/den;
+  to_hll |= !risfinite(vxs) | !(rabs(vxs) < 1.0) | !((Sl < vxs) & (vxs < Sr));
+  const double alpha_l = (Sl - vxl)/(Sl - vxs);
+  const double alpha_r = (Sr - vxr)/(Sr - vxs);
+  double usl[8], usr[8];
+  // Bx -> 0: (vy, vz) do not enter
+  double dl[8], dr[8];
+  {
+    const double ps = Fhll[MXn] - Fhll[ENG]*vxs;
+    dl[0] = ul[0]*alpha_l;
+    dr[0] = ur[0]*alpha_r;
+    dl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs)/(Sl - vxs);
+    dr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs)/(Sr - vxs);
+    dl[MXn] = (dl[ENG] + ps)*vxs;
+    dr[MXn] = (dr[ENG] + ps)*vxs;
+    dl[MXt] = ul[MXt]*alpha_l;
+    dr[MXt] = ur[MXt]*alpha_r;
+    dl[MXb] = ul[MXb]*alpha_l;
+    dr[MXb] = ur[MXb]*alpha_r;
+    dl[MXn] += dl[0]*vxs;            // reduced energy
+    dr[MXn] += dr[0]*vxs;
+    dl[BXn] = Bx;
+    dr[BXn] = Bx;
+    dl[BXt] = ul[BXt]*alpha_l;
+    dr[BXt] = ur[BXt]*alpha_r;
+    dl[BXb] = ul[BXb]*alpha_l;
+    dr[BXb] = ur[BXb]*alpha_r;
   }
+  // general Bx
+  double gl[8], gr[8];
+  const double vys = (Bys*vxs - Fhll[BXt])/Bx;
+  const double vzs = (Bzs*vxs - Fhll[BXb])/Bx;
+  double gammas_2 = vxs*vxs + vys*vys + vzs*vzs;
+  gammas_2 = 1.0 - gammas_2;
+  const bool superluminal = !risfinite(gammas_2) | (gammas_2 <= 0.0);
+  {
+    const double vBs = vxs*Bx + vys*Bys + vzs*Bzs;
+    const double ps = (BtFBt_g - Fhll[ENG])*vxs + Bx*Bx + Fhll[MXn] - FBt2_g;
+    gl[0] = ul[0]*alpha_l;
+    gr[0] = ur[0]*alpha_r;
+    gl[ENG] = (Sl*ul[ENG] - fl[ENG] + ps*vxs - vBs*Bx)/(Sl - vxs);
+    gr[ENG] = (Sr*ur[ENG] - fr[ENG] + ps*vxs - vBs*Bx)/(Sr - vxs);
+    gl[MXn] = (gl[ENG] + ps)*vxs - vBs*Bx;
+    gr[MXn] = (gr[ENG] + ps)*vxs - vBs*Bx;
+    gl[MXt] = (Sl*ul[MXt] - fl[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sl - vxs);
+    gr[MXt] = (Sr*ur[MXt] - fr[MXt] - Bx*(Bys*gammas_2 + vBs*vys))/(Sr - vxs);
+    gl[MXb] = (Sl*ul[MXb] - fl[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sl - vxs);
+    gr[MXb] = (Sr*ur[MXb] - fr[MXb] - Bx*(Bzs*gammas_2 + vBs*vzs))/(Sr - vxs);
+    gl[MXn] += gl[0]*vxs;            // reduced energy
+    gr[MXn] += gr[0]*vxs;
+    gl[BXn] = gr[BXn] = Bx;
+    gl[BXt] = gr[BXt] = Bys;
+    gl[BXb] = gr[BXb] = Bzs;
+  }
+  to_hll |= !degenerate & superluminal;
+  for (int nv = 0; nv < 8; nv++) {
+    usl[nv] = degenerate ? dl[nv] : gl[nv];
+    usr[nv] = degenerate ? dr[nv] : gr[nv];
+  }
+  to_hll |= !(usl[0] > 0.0) | !(usr[0] > 0.0);
+  for (int nv = 0; nv < 8; nv++) to_hll |= !risfinite(usl[nv]) | !risfinite(usr[nv]);
+  const bool left = vxs > 0.0;
+  double fs[8];
+  for (int nv = 0; nv < 8; nv++)
+    fs[nv] = left ? fl[nv] + Sl*(usl[nv] - ul[nv]) : fr[nv] + Sr*(usr[nv] - ur[nv]);
+  for (int nv = 0; nv < 8; nv++) to_hll |= !risfinite(fs[nv]);
+  to_hll |= !risfinite(left ? pl : pr);
+  double *flux = out.f;
+  for (int nv = 0; nv < 8; nv++)
+    flux[nv] = Sl >= 0.0 ? fl[nv] : (Sr <= 0.0 ? fr[nv] : (to_hll ? Fh[nv] : fs[nv]));
   uint32_t err = 0;
   for (int nv = 0; nv < 8; nv++) if (!risfinite(flux[nv])) err |= ERR_NAN | ERR_QUIT;
   out.err = err;
