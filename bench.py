@@ -206,6 +206,55 @@ def pmc_probe(n, seed, precision):
                       "copy_write_bytes": copy_bytes}), flush=True)
 
 
+def _rmhd_states(m: int, seed: int, steps: int = 10):
+    """(conserved, primitive) pairs [m^3, 8] of an RMHD run of the 192^3
+    workload's kind at m^3: its final Uc (with the cell field) and Vc."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+    sigma = float(10.0 ** np.random.Generator(np.random.PCG64(seed + 1000)).uniform(-1.0, 1.0))
+    vc, vs = ic.rmhd_turbulence((m, m, m), seed, lorentz_max=10.0, sigma=sigma)
+    g = StageUpdate(Definitions(physics="RMHD"),
+                    RuntimeSetup(n=(m, m, m), solver="hll", gamma=4.0 / 3.0, cfl=0.4))
+    g.upload(vc, vs)
+    g.run(steps, 1e-4)
+    v, _, u = g.download()
+    g.close()
+    prim = np.ascontiguousarray(v.reshape(8, -1).T)
+    cons = np.ascontiguousarray(u.reshape(8, -1).T)
+    cons[:, 4:7] = prim[:, 4:7]
+    return cons, prim, sigma
+
+
+def c2p_probe(mode: str, seed: int):
+    """The con2prim solve for counters: 'train' trains the network on a 64^3
+    prior run (seed + 1), takes the final states of a 128^3 run (seed) and
+    saves both to the temporary directory; 'solve' loads them and runs the
+    batched solve from the A27 and from the learned start, twice each."""
+    import tempfile
+    from gpluto_dev_amd import c2p_net
+    path = os.path.join(tempfile.gettempdir(), "gpl_c2p_probe.npz")
+    if mode == "train":
+        cons, prim, _ = _rmhd_states(64, seed + 1)
+        x, y = c2p_net.pairs(cons, prim)
+        w, loss = c2p_net.fit(x, y, steps=3000, batch=4096, lr=3e-3, seed=seed)
+        u, _, _ = _rmhd_states(128, seed)
+        np.savez(path, w=w, u=u, loss=loss)
+    else:
+        with np.load(path) as z:
+            w, u, loss = z["w"], z["u"], float(z["loss"])
+    for _ in range(2):
+        _, pa, _, ma = c2p_net.solve(u)
+        _, pn, _, mn, rs = c2p_net.solve(u, w, restarts=True)
+
+    def wave_max(p):
+        return float(p[:p.size // 64 * 64].reshape(-1, 64).max(1).mean())
+
+    print(json.dumps({"c2p_probe": mode, "states": int(u.shape[0]), "loss": loss,
+                      "passes": [float(pa.mean()), float(pn.mean())],
+                      "wave_max_passes": [wave_max(pa), wave_max(pn)], "restart_rate": float(rs.mean()),
+                      "kernel_ms": [ma, mn]}), flush=True)
+
+
 def run_batch2d(seed: int, steps: int = 100):
     """The batched 2D workload: 64 periodic boxes, N log-uniform in [32, 512),
     beta ~ U[1, 100], PLM(MC) + HLLD + UCT_HLL + RK2, CFL 0.4, per-box dt,
@@ -250,18 +299,27 @@ def run_rmhd(seed: int, n: int = 192, steps: int = 10):
     """The 3D RMHD workload: 192^3 periodic, gamma 4/3, PLM(MC) + HLL + UCT_HLL
     + RK2, CFL 0.4, 10 steps, Lorentz factors up to 10 and magnetisation
     sigma log-uniform in [0.1, 10] by seed.  The stage update runs with the
-    reference's Eq. (A27) con2prim start and with the learned start (network
-    trained here on the GPU); Newton passes and restart rate of both starts
-    are measured on the final conserved states of the plain run."""
+    reference's Eq. (A27) con2prim start and with the learned start: an
+    8-64-64-1 BF16 network trained here on the GPU on the (conserved,
+    primitive) pairs of a prior run (96^3, the next seed, 10 steps).  Newton
+    passes and restart rate of both starts are measured on the final
+    conserved states of the plain run."""
     from gpluto_dev_amd import c2p_net, ic
     from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+
     sigma = float(10.0 ** np.random.Generator(np.random.PCG64(seed + 1000)).uniform(-1.0, 1.0))
     vc, vs = ic.rmhd_turbulence((n, n, n), seed, lorentz_max=10.0, sigma=sigma)
+    # the prior run whose states train the network
+    m = 96
     t0 = time.perf_counter()
-    w, loss = c2p_net.train(nsamples=1 << 18, steps=3000, batch=4096, lr=3e-3, seed=seed)
+    cons, prim, psigma = _rmhd_states(m, seed + 1, steps)
+    x, y = c2p_net.pairs(cons, prim)
+    w, loss = c2p_net.fit(x, y, steps=3000, batch=4096, lr=3e-3, seed=seed)
     train_s = time.perf_counter() - t0
     res = {"n": n, "zones": n ** 3, "steps": steps, "sigma": sigma, "lorentz_max": 10.0,
-           "train": {"seconds": train_s, "loss": loss, "samples": 1 << 18, "steps": 3000}}
+           "train": {"seconds": train_s, "loss": loss, "pairs": int(len(y)), "steps": 3000,
+                     "source": f"prior run: {m}^3, seed {seed + 1}, sigma {psigma:.3g}, "
+                               f"{steps} steps, final (Uc, Vc)"}}
     finals = {}
     for use in (False, True):
         g = StageUpdate(Definitions(physics="RMHD"),
@@ -284,10 +342,14 @@ def run_rmhd(seed: int, n: int = 192, steps: int = 10):
     u = np.ascontiguousarray(finals[False][2].reshape(8, -1).T)
     _, pa, ea, ms_a = c2p_net.solve(u)
     _, pn, en, ms_n, rs = c2p_net.solve(u, w, restarts=True)
+    def wave_max(p):     # a wavefront runs Newton until its slowest lane is done
+        return float(p[:p.size // 64 * 64].reshape(-1, 64).max(1).mean())
+
     res["con2prim"] = {"states": int(u.shape[0]),
-                       "a27": {"mean_passes": float(pa.mean()), "failed": int(np.count_nonzero(ea)),
-                               "kernel_ms": ms_a},
-                       "learned": {"mean_passes": float(pn.mean()), "failed": int(np.count_nonzero(en)),
+                       "a27": {"mean_passes": float(pa.mean()), "wave_max_passes": wave_max(pa),
+                               "failed": int(np.count_nonzero(ea)), "kernel_ms": ms_a},
+                       "learned": {"mean_passes": float(pn.mean()), "wave_max_passes": wave_max(pn),
+                                   "failed": int(np.count_nonzero(en)),
                                    "restart_rate": float(rs.mean()), "kernel_ms": ms_n}}
     return res
 
@@ -399,6 +461,8 @@ def main():
                     help="also run the 512^3 FP64 vs mixed comparison (about 1.5 min of IC generation)")
     ap.add_argument("--dump", action="store_true",
                     help="also time one dbl dump in the 256^3 run, synchronous and overlapped")
+    ap.add_argument("--c2p-probe", choices=["train", "solve"], default=None,
+                    help="profiling helper: the RMHD con2prim solve, A27 and learned start")
     ap.add_argument("--pmc-probe", choices=["fp64", "mixed"], default=None,
                     help="run one step of the workload plus a 1 GiB device copy (k_copy_probe, the "
                          "HBM byte-counter calibration) for rocprofv3 --pmc, then exit")
@@ -410,6 +474,9 @@ def main():
     from gpluto_dev_amd import ic
     n = a.n
     ncell = n ** 3
+    if a.c2p_probe:
+        c2p_probe(a.c2p_probe, a.seed)
+        return
     if a.pmc_probe:
         pmc_probe(n, a.seed, a.pmc_probe)
         return

@@ -35,7 +35,7 @@ EXPORTS = ("gpl_default_config", "gpl_create", "gpl_destroy", "gpl_upload", "gpl
            "gpl_c2p_net_infer", "gpl_c2p_solve", "gpl_set_c2p_network", "gpl_diagnostics",
            "gpl_set_step", "gpl_get_step", "gpl_write_dbl", "gpl_read_dbl",
            "gpl_write_dbl_async", "gpl_wait_dbl", "gpl_set_zone_reports", "gpl_get_zone_reports",
-           "gpl_c2p_solve_noalloc")
+           "gpl_c2p_solve_noalloc", "gpl_c2p_net_pairs", "gpl_c2p_net_fit")
 
 
 class GplConfig(C.Structure):
@@ -127,6 +127,10 @@ def load() -> C.CDLL:
     L.gpl_c2p_net_make_data.restype = ip
     L.gpl_c2p_net_train.argtypes = [ip, C.c_uint64, ip, ip, ip, C.c_float, C.c_double, fp, fp]
     L.gpl_c2p_net_train.restype = ip
+    L.gpl_c2p_net_pairs.argtypes = [ip, dp, dp, ip, C.c_double, fp, fp]
+    L.gpl_c2p_net_pairs.restype = ip
+    L.gpl_c2p_net_fit.argtypes = [ip, C.c_uint64, fp, fp, ip, ip, ip, C.c_float, fp, fp]
+    L.gpl_c2p_net_fit.restype = ip
     L.gpl_c2p_net_grad.argtypes = [ip, fp, fp, fp, ip, fp, fp]
     L.gpl_c2p_net_grad.restype = ip
     L.gpl_c2p_net_infer.argtypes = [ip, fp, fp, ip, fp]

@@ -30,7 +30,11 @@ ARCH = os.environ.get("GPL_OFFLOAD_ARCH", "gfx950")
 # FP64 arithmetic in the same files keeps denormals).
 MIXED = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
          "-fgpu-flush-denormals-to-zero"]
-SOURCES = [("mhd3d_fp64.hip", []),
+# the learned con2prim guess's matrix-core code (c2p_net.hpp): MFMA results
+# in ordinary VGPRs, which the ReLU and BF16 conversion read directly, instead
+# of accumulation registers copied out one v_accvgpr_read per float
+MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+SOURCES = [("mhd3d_fp64.hip", MFMA_VGPR),
            ("mhd3d_mixed.hip", MIXED),
            ("mhd3d_common.hip", []),
            ("diagnostics.hip", []),
@@ -42,7 +46,7 @@ SOURCES = [("mhd3d_fp64.hip", []),
            ("batch2d_common.hip", []),
            ("batch2d_boundary.hip", []),
            ("engine_batch2d.cpp", []),
-           ("c2p_net.hip", []),
+           ("c2p_net.hip", MFMA_VGPR),
            ("engine_mhd3d.cpp", []),
            ("engine_hd.cpp", []),
            ("dbl_io.cpp", []),

@@ -1,7 +1,7 @@
 """Learned initial guess for the RMHD con2prim Newton solve, over the HIP
-library: data generation, GPU training, matrix-core inference and the
-batched solve (A27 guess, the reference's, or the learned guess with the
-A27 restart)."""
+library: training pairs (sampled states, or a run's own states), GPU
+training, matrix-core inference and the batched solve (A27 guess, the
+reference's, or the learned guess with the A27 restart)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -19,18 +19,45 @@ def _d(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+NF = 8          # input features
+
+
 def size() -> int:
     return _lib.load().gpl_c2p_net_size()
 
 
 def make_data(n: int, seed: int, gamma: float = 4.0 / 3.0, device: int = 0):
-    """(u[n, 8], x[n, 4], y[n]): conserved states, raw features, log(W/W_A27)."""
+    """(u[n, 8], x[n, 8], y[n]): conserved states, raw features, log(W/W_A27)."""
     u = np.empty((n, 8))
-    x = np.empty((n, 4), dtype=np.float32)
+    x = np.empty((n, NF), dtype=np.float32)
     y = np.empty(n, dtype=np.float32)
     _lib.check(_lib.load().gpl_c2p_net_make_data(device, seed, n, gamma, _d(u), _f(x), _f(y)),
                "gpl_c2p_net_make_data")
     return u, x, y
+
+
+def pairs(u, v, gamma: float = 4.0 / 3.0, device: int = 0):
+    """(x[n, 8], y[n]) of given (conserved u[n, 8], primitive v[n, 8]) pairs,
+    e.g. the states of an earlier run."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n = u.shape[0]
+    x = np.empty((n, NF), dtype=np.float32)
+    y = np.empty(n, dtype=np.float32)
+    _lib.check(_lib.load().gpl_c2p_net_pairs(device, _d(u), _d(v), n, gamma, _f(x), _f(y)),
+               "gpl_c2p_net_pairs")
+    return x, y
+
+
+def fit(x, y, steps: int = 3000, batch: int = 4096, lr: float = 3e-3, seed: int = 1, device: int = 0):
+    """Train on given (x, y); (weights, last mini-batch loss)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    w = np.empty(size(), dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _lib.check(_lib.load().gpl_c2p_net_fit(device, seed, _f(x), _f(y), len(y), steps, batch, lr,
+                                           _f(w), _f(loss)), "gpl_c2p_net_fit")
+    return w, float(loss[0])
 
 
 def train(nsamples: int = 1 << 18, steps: int = 3000, batch: int = 4096, lr: float = 3e-3,

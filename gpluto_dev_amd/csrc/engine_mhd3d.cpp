@@ -71,7 +71,8 @@ public:
   int emf_march = -32;
   double *cdt_part = nullptr;
   int nparts = 0;
-  float *net = nullptr;       // RMHD learned con2prim guess, when enabled
+  NetPack *net = nullptr;     // RMHD learned con2prim guess (matrix-core form), when enabled
+  float *net_w = nullptr;     // its weight vector
   bool net_on = false;
   double *diag = nullptr;     // diagnostics partials (DIAG_BLOCKS rows) and result row
   // A box with an outflow or reflective side runs padded: the device box is the reference's
@@ -102,7 +103,7 @@ public:
 
   ~Mhd3dEngine() override
   {
-    void *ptrs[] = {V, Valt, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net, diag};
+    void *ptrs[] = {V, Valt, U, Us, vs[0], vs[1], vs[2], F, CM, FD, E, dU, Cdt, cdt_part, net, net_w, diag};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
@@ -502,8 +503,11 @@ public:
   {
     if (!rmhd) return GPL_EUNSUPPORTED;
     if (!w) { net_on = false; return 0; }
-    if (!net && dev_alloc(bytes, (void **)&net, NET_SIZE*sizeof(float))) return GPL_ENOMEM;
-    if (hipMemcpy(net, w, NET_SIZE*sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return GPL_EHIP;
+    if (!net && dev_alloc(bytes, (void **)&net, sizeof(NetPack))) return GPL_ENOMEM;
+    if (!net_w && dev_alloc(bytes, (void **)&net_w, NET_SIZE*sizeof(float))) return GPL_ENOMEM;
+    if (net_pack_upload(w, net_w, net, env.stream) ||
+        hipStreamSynchronize(env.stream) != hipSuccess)
+      return GPL_EHIP;
     net_on = true;
     return 0;
   }

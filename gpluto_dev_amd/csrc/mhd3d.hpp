@@ -7,6 +7,8 @@
 
 namespace gpl {
 
+struct NetPack;
+
 struct Box3 {
   int nx, ny, nz;
   int ncell;
@@ -49,7 +51,7 @@ struct Mhd3dBufs {
   uint32_t *err;
   int emf_march;   // EMF kernel: 0 one edge per thread, n > 0 register z-march of n, -n LDS tiles marching n
   int phys;        // 0: ideal MHD, 1: relativistic MHD (FP64, fused kernels)
-  const float *net; // RMHD: learned con2prim guess (nullptr: the reference's A27 guess)
+  const NetPack *net; // RMHD: learned con2prim guess (nullptr: the reference's A27 guess)
   ZoneRep zr;       // this stage's zone-by-zone reports (zr.err null: off)
 };
 

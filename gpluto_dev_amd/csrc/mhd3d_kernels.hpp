@@ -1066,14 +1066,14 @@ __device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const
 // it needs for the average are recomputed from the same inputs (bitwise the
 // value their owner writes).  Vs is double-buffered (Vin -> Vout).
 // ---------------------------------------------------------------------------
-template <typename R, int PH>
+template <typename R, int PH, bool NET = false>
 __global__ __launch_bounds__(256)
 void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               double *__restrict__ Vout, const double *__restrict__ Bs0,
               double *__restrict__ U, R *__restrict__ V,
               const StepCtl *__restrict__ ctl, RKW w, int first_stage,
               double gmm1, double small_rho, double small_prs, uint32_t *__restrict__ err,
-              const float *__restrict__ net, ZoneRep zr)
+              const NetPack *__restrict__ net, ZoneRep zr)
 {
   // no early exit: the learned con2prim guess is evaluated by whole wavefronts
   const int c_beg = g.kb*g.nx*g.ny, c_end = g.ke*g.nx*g.ny;   // planes kb .. ke-1
@@ -1115,14 +1115,13 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
     // anything (mappers3D.cpp:182-190).  gmm1 carries gamma here.
     double u8[8] = {u[0], u[1], u[2], u[3], b[0], b[1], b[2], u[4]};
     double scale = 1.0;
-    if (net) {                     // uniform: a kernel argument
-      float x[4];
-      net_features(u8, x);
-      scale = exp((double)(net[NET_YSC]*net_eval64(net, x)));
+    if constexpr (NET) {           // a separate kernel: the drop-in keeps its registers
+      float x[NET_F];
+      net_features(u8, rmhd_guess_a27(u8), x);
+      scale = (double)__expf(net_eval64(net, x));
     }
     uint32_t local = 0;
-    const uint32_t e = rmhd_cons_to_prim(u8, v, gmm1, small_rho, small_prs, local, net != nullptr,
-                                         scale);
+    const uint32_t e = rmhd_cons_to_prim(u8, v, gmm1, small_rho, small_prs, local, NET, scale);
     if (!live) return;
     if (local) {
       sto(U, c, u8[0]); sto(U + n, c, u8[1]); sto(U + 2*n, c, u8[2]); sto(U + 3*n, c, u8[3]);
@@ -1238,10 +1237,14 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
   }
   case K_FINISH:
     // the RMHD mapper takes gamma, the MHD one gamma - 1
-    hipLaunchKernelGGL((k_finish<R, PH>), dim3((nk*g.nx*g.ny + T - 1)/T), dim3(T), 0, s,
-                       g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
-                       PH == 1 ? gamma : gamma - 1.0, small_rho, small_prs, b.err,
-                       PH == 1 ? b.net : nullptr, b.zr);
+    if (PH == 1 && b.net)
+      hipLaunchKernelGGL((k_finish<R, PH, PH == 1>), dim3((nk*g.nx*g.ny + T - 1)/T), dim3(T), 0, s,
+                         g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage, gamma,
+                         small_rho, small_prs, b.err, b.net, b.zr);
+    else
+      hipLaunchKernelGGL((k_finish<R, PH, false>), dim3((nk*g.nx*g.ny + T - 1)/T), dim3(T), 0, s,
+                         g, b.E, b.Vin, b.Vout, b.Bs0, b.U, b.V, ctl, w, first_stage,
+                         PH == 1 ? gamma : gamma - 1.0, small_rho, small_prs, b.err, nullptr, b.zr);
     break;
   default:
     break;

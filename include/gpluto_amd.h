@@ -262,24 +262,34 @@ size_t gpl_batch_device_bytes(const gpl_batch *b);
 /* ---------------------------------------------------------------------------
  * Learned con2prim initial guess for RMHD.  No counterpart in the reference,
  * whose RMHD_EnergySolve (Src/RMHD/rmhd_energy_solve.cpp:86-96) always starts
- * Newton from the Eq. (A27) root.  A 4-32-32-1 network (weights: a float
- * vector of gpl_c2p_net_size() entries) corrects that guess; Newton, its
- * tolerance and the A27 restart on failure stay the judge.
+ * Newton from the Eq. (A27) root.  An 8-64-64-1 ReLU network of eight
+ * scale-free invariants of the conserved state (weights: a float vector of
+ * gpl_c2p_net_size() entries) corrects that guess; Newton, its tolerance and
+ * the A27 restart on failure stay the judge.
  * ------------------------------------------------------------------------- */
 int gpl_c2p_net_size(void);
 /* n (conserved, target) pairs from sampled primitive states: u[n][8]
- * (D, m1..3, B1..3, E - D), raw features x[n][4], y[n] = log(W/W_A27). */
+ * (D, m1..3, B1..3, E - D), raw features x[n][8], y[n] = log(W/W_A27). */
 int gpl_c2p_net_make_data(int device, uint64_t seed, int n, double gamma, double *u_out,
                           float *x_out, float *y_out);
-/* Train on the GPU (FP32 forward/backward kernels, Adam); writes the weights
- * and the last mini-batch loss (normalised target). */
+/* Features x[n][8] and targets y[n] of given (conserved u[n][8], primitive
+ * v[n][8]) pairs, e.g. the states of an earlier run (downloaded Uc, Vc). */
+int gpl_c2p_net_pairs(int device, const double *u, const double *v, int n, double gamma,
+                      float *x_out, float *y_out);
+/* Train on (x, y) on the GPU (FP32 forward/backward kernels, Adam, the
+ * samples shuffled by seed); writes the weights and the last mini-batch loss
+ * (normalised target). */
+int gpl_c2p_net_fit(int device, uint64_t seed, const float *x, const float *y, int n, int steps,
+                    int batch, float lr, float *weights_out, float *loss_out);
+/* gpl_c2p_net_make_data with nsamples sampled states, then gpl_c2p_net_fit. */
 int gpl_c2p_net_train(int device, uint64_t seed, int nsamples, int steps, int batch, float lr,
                       double gamma, float *weights_out, float *loss_out);
 /* Gradient of the mean-squared error over (x, y) -- the training kernel on
  * one batch -- and its loss. */
 int gpl_c2p_net_grad(int device, const float *weights, const float *x, const float *y, int n,
                      float *grad_out, float *loss_out);
-/* Matrix-core inference (FP16 MFMA, FP32 accumulate): y_out[n] = net(x). */
+/* Matrix-core inference (BF16 MFMA, FP32 accumulate): y_out[n] = ysc * net(x),
+ * the learned correction of log W_A27. */
 int gpl_c2p_net_infer(int device, const float *weights, const float *x, int n, float *y_out);
 /* Use the learned guess inside an RMHD context's con2prim (weights copied;
  * NULL restores the reference's A27 start, the bitwise drop-in).  With it on,

@@ -1,8 +1,9 @@
 """The learned con2prim guess (RMHD): GPU training gradients against the CPU
-reference (oracle/net_ref.py, pinned by finite differences), matrix-core
-inference against the FP32 forward pass, and what the trained network gains
-over the reference's Eq. (A27) guess -- same converged primitives (the
-Newton tolerance), fewer Newton passes."""
+reference (oracle/net_ref.py, pinned by finite differences), BF16 matrix-core
+inference against the float64 forward pass, and what the trained network
+gains over the reference's Eq. (A27) guess -- same converged primitives (the
+Newton tolerance), fewer Newton passes -- trained on sampled states or on the
+states of an earlier run."""
 import numpy as np
 import pytest
 
@@ -48,10 +49,13 @@ def test_matrix_core_inference_matches_fp32_forward():
     _, x, y = c2p_net.make_data(5000, seed=13)
     w = net_ref.random_weights(x, y, seed=2)
     out = c2p_net.infer(w, x)
-    ref = net_ref.forward(w, x)
-    # FP16 weights and activations with FP32 accumulation
-    assert np.max(np.abs(out - ref)) < 2e-2 * max(1.0, np.max(np.abs(ref)))
-    assert np.corrcoef(out, ref)[0, 1] > 0.9999
+    ref = net_ref.forward(w, x) * w[net_ref.YSC]
+    # BF16 weights, inputs and activations (8 significant bits) with FP32
+    # accumulation: errors of a few 2^-8 of the output's scale
+    scale = np.max(np.abs(ref))
+    print(f"max |out - ref| / max |ref| = {np.max(np.abs(out - ref)) / scale:.2e}", flush=True)
+    assert np.max(np.abs(out - ref)) < 3e-2 * scale
+    assert np.corrcoef(out, ref)[0, 1] > 0.999
 
 
 def test_trained_guess_saves_newton_passes_at_the_same_answer():
@@ -92,3 +96,40 @@ def test_rmhd_step_with_learned_guess_agrees_with_drop_in():
     for q in range(8):
         l1 = np.sum(np.abs(v1[q] - v0[q])) / np.sum(np.abs(v0[q]))
         assert l1 < 1e-9, (q, l1)
+
+
+def test_guess_trained_on_an_earlier_run():
+    """Pairs from one RMHD run (its final conserved and primitive states), the
+    network trained on them, and the guess used on another run's states:
+    fewer Newton passes, the same answers."""
+    from gpluto_dev_amd import ic
+    from gpluto_dev_amd.stage import Definitions, RuntimeSetup, StageUpdate
+
+    def run_states(seed):
+        n = (32, 32, 32)
+        vc, vs = ic.rmhd_turbulence(n, seed, lorentz_max=4.0, sigma=1.0)
+        g = StageUpdate(Definitions(physics="RMHD"), RuntimeSetup(n=n, solver="hll", gamma=4 / 3))
+        g.upload(vc, vs)
+        err, _, _ = g.run(4, 1e-3)
+        v, _, u = g.download()
+        g.close()
+        assert err == 0
+        prim = np.stack([v[q].ravel() for q in range(8)], 1)
+        cons = np.stack([u[q].ravel() for q in (0, 1, 2, 3)] + [v[q].ravel() for q in (4, 5, 6)]
+                        + [u[7].ravel()], 1)
+        return cons, prim
+
+    u1, v1 = run_states(41)
+    x1, y1 = c2p_net.pairs(u1, v1)
+    assert np.all(np.isfinite(x1)) and np.all(np.isfinite(y1))
+    w, loss = c2p_net.fit(x1, y1, steps=2000, batch=4096, lr=3e-3, seed=3)
+    u2, _ = run_states(42)                       # another run: held out
+    va, pa, ea, _ = c2p_net.solve(u2)
+    vn, pn, en, _, rs = c2p_net.solve(u2, w, restarts=True)
+    print(f"loss {loss:.4f}; Newton passes A27 {pa.mean():.3f} learned {pn.mean():.3f}; "
+          f"restarts {rs.mean():.2e}", flush=True)
+    assert np.all(ea == 0) and np.all(en == 0)
+    assert pn.mean() < pa.mean()
+    for q in (0, 1, 2, 3, 7):
+        scale = np.maximum(np.abs(va[:, q]), 1e-3 * np.max(np.abs(va[:, q])))
+        assert np.max(np.abs(vn[:, q] - va[:, q]) / scale) < 1e-8, q

@@ -7,7 +7,8 @@ import numpy as np
 def test_reference_gradient_matches_finite_differences():
     import net_ref
     rng = np.random.default_rng(3)
-    x = rng.normal(size=(64, 4)) * np.array([2.0, 0.5, 3.0, 0.3]) + np.array([1.0, 0.2, -1.0, 0.5])
+    x = (rng.normal(size=(64, 8)) * np.array([2.0, 0.5, 3.0, 0.3, 1.0, 0.7, 2.5, 0.4])
+         + np.array([1.0, 0.2, -1.0, 0.5, 0.0, -0.3, 2.0, 1.0]))
     y = rng.normal(size=64) * 0.05
     w = net_ref.random_weights(x, y, seed=5).astype(np.float64)
     _, g = net_ref.loss_and_grad(w, x, y)
