@@ -14,7 +14,7 @@
 // Biases ride on a constant-1 input slot (layer 1) or a constant-1 K block
 // (layer 2 and the output), so the accumulators start at zero.
 //
-// The plain solve stays the judge: the Newton iteration and its tolerance
+// The plain solve stays the arbiter: the Newton iteration and its tolerance
 // are unchanged, and a state whose Newton run from the learned guess fails
 // is solved again from the A27 guess.
 #pragma once
@@ -195,7 +195,7 @@ __device__ __forceinline__ float net_eval64(const NetPack *__restrict__ P, const
   return P->ysc*(lane < 32 ? o0 : o1);
 }
 
-// Newton from the learned guess, the A27 guess as the fallback and judge.
+// Newton from the learned guess, the A27 guess as the fallback.
 // Returns the solve's error bits (0 on success); *passes counts every pass.
 __device__ __forceinline__ uint32_t rmhd_energy_solve_guided(const double *u, double *v, double gamma,
                                                              double wa, float corr, int *passes,

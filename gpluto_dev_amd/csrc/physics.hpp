@@ -1164,7 +1164,7 @@ GPL_HD uint32_t rmhd_cons_to_prim(double *u, double *v, double gamma, double sma
   if (u[0] < 0.0) { u[0] = small_rho; errm |= ERR_NEGATIVE_RHO; }
   if (u[7] < 0.0) { fix = true; errm |= ERR_NEGATIVE_ENG; }
   if (!fix) {
-    // guided: Newton from guess_scale * W_A27 first, the A27 start as the judge
+    // guided: Newton from guess_scale * W_A27 first, the A27 start as the fallback
     uint32_t e = guided ? rmhd_energy_solve(u, v, gamma, nullptr, guess_scale*rmhd_guess_a27(u)) : 1u;
     if (e) e = rmhd_energy_solve(u, v, gamma);
     if (e) fix = true;

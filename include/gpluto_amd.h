@@ -265,7 +265,7 @@ size_t gpl_batch_device_bytes(const gpl_batch *b);
  * Newton from the Eq. (A27) root.  An 8-64-64-1 ReLU network of eight
  * scale-free invariants of the conserved state (weights: a float vector of
  * gpl_c2p_net_size() entries) corrects that guess; Newton, its tolerance and
- * the A27 restart on failure stay the judge.
+ * the A27 restart on failure decide the result.
  * ------------------------------------------------------------------------- */
 int gpl_c2p_net_size(void);
 /* n (conserved, target) pairs from sampled primitive states: u[n][8]

@@ -1,5 +1,5 @@
 """The CPU reference of the con2prim-guess network (oracle/net_ref.py) is
-pinned by central finite differences of its own loss, so that it can judge
+pinned by central finite differences of its own loss, so that it can check
 the GPU training kernel's gradients."""
 import numpy as np
 
