@@ -19,12 +19,12 @@
 //    table; sin r and cos r by their Taylor series to r^11 and r^10, the two
 //    leading correction terms in double-double; the addition formulas in
 //    double-double.
-//  - acos(x): one Newton step in double-double from the library's acos (or
-//    asin) value, whose error (a few ulp) becomes ~2^-104 after the step.
+//  - acos(x): one Newton step in double-double from the library's acos of
+//    |x|, whose error (a few ulp) becomes ~2^-104 after the step.
 //    |x| <= 1/2: theta = theta0 + (cos theta0 - x)/sin theta0.  |x| > 1/2:
 //    acos x = 2 phi (x > 0) or pi - 2 phi (x < 0) with sin^2 phi = h =
-//    (1 - |x|)/2 (exact), phi = phi0 - (sin^2 phi0 - h)/sin(2 phi0), which
-//    keeps full relative accuracy as x -> +-1.
+//    (1 - |x|)/2 (exact), phi = phi0 - (sin^2 phi0 - h)/sin(2 phi0),
+//    phi0 = acos(|x|)/2, which keeps full relative accuracy as x -> +-1.
 // Explicit fma() calls are the error-free products; they are exact on both
 // sides regardless of -ffp-contract.
 #pragma once
@@ -130,32 +130,39 @@ GPL_CR_HD void sincos_cr(double x, double *sn, double *cs)
   *cs = c.hi + c.lo;
 }
 
-// The Newton step of acos_cr for -1 < x < 1 from its seed: acos(x) for
-// |x| <= 1/2, asin(sqrt((1 - |x|)/2)) otherwise.  A seed a few ulp off still
-// gives the correctly rounded result (the step squares its relative error).
+// The Newton step of acos_cr for -1 < x < 1 from its seed (acos_seed).  A
+// seed a few ulp off still gives the correctly rounded result (the step
+// squares its relative error).  Both forms of the step are evaluated from one
+// double-double sincos of the seed and the right one is selected, so lanes of
+// a wavefront that take different forms do not run the expensive part twice.
+//  |x| <= 1/2: theta = theta0 + (cos theta0 - x)/sin theta0, theta0 in [pi/3, 2pi/3];
+//  |x| >  1/2: acos x = 2 phi (x > 0) or pi - 2 phi (x < 0) with sin^2 phi = h =
+//              (1 - |x|)/2 (exact), phi = phi0 - (sin^2 phi0 - h)/sin(2 phi0),
+//              phi0 in (0, pi/6], which keeps full relative accuracy as x -> +-1.
 GPL_CR_HD double acos_refine(double x, double seed)
 {
   dd s, c;
-  if (fabs(x) <= 0.5) {
-    dd_sincos(seed, s, c);                         // seed in [pi/3, 2 pi/3]
-    const double d = (c.hi - x) + c.lo;            // cos t0 - x
-    return seed + d/s.hi;
-  }
-  const double h = (x > 0.0 ? 1.0 - x : 1.0 + x)*0.5;   // sin^2 phi, exact
-  dd_sincos(seed, s, c);                           // seed phi0 in (0, pi/6]
+  dd_sincos(seed, s, c);
+  const double d1 = (c.hi - x) + c.lo;             // cos t0 - x
+  const double r_mid = seed + d1/s.hi;
+  const double h = (1.0 - fabs(x))*0.5;            // sin^2 phi, exact
   const dd s2 = dd_mul(s, s);
-  const double d = (s2.hi - h) + s2.lo;            // sin^2 phi0 - h
-  const double delta = -d/(2.0*s.hi*c.hi);
-  if (x > 0.0) return 2.0*(seed + delta);          // exact doubling of RN(phi)
+  const double d2 = (s2.hi - h) + s2.lo;           // sin^2 phi0 - h
+  const double delta = -d2/(2.0*s.hi*c.hi);
+  const double r_pos = 2.0*(seed + delta);         // exact doubling of RN(phi)
   dd t = two_sum(kPiHi, -2.0*seed);                // pi - 2 phi
   t.lo += kPiLo - 2.0*delta;
-  return t.hi + t.lo;
+  const double r_neg = t.hi + t.lo;
+  return fabs(x) <= 0.5 ? r_mid : (x > 0.0 ? r_pos : r_neg);
 }
 
+// The seed of the Newton step from one library acos of |x| (a few ulp):
+// theta0 = acos x for |x| <= 1/2, phi0 = acos(|x|)/2 otherwise.
 GPL_CR_HD double acos_seed(double x)
 {
-  if (fabs(x) <= 0.5) return acos(x);
-  return asin(sqrt((x > 0.0 ? 1.0 - x : 1.0 + x)*0.5));
+  const double t = acos(fabs(x));
+  if (fabs(x) <= 0.5) return x >= 0.0 ? t : kPiHi - t;
+  return 0.5*t;
 }
 
 // acos(x) correctly rounded for -1 <= x <= 1; NaN takes the library's.
