@@ -146,7 +146,7 @@ public:
     }
     fused_xy = !fp64;
     if (const char *m = getenv("GPL_FUSED_XY")) fused_xy = atoi(m) != 0;
-    fused_xy = fused_xy && fused && 20*(size_t)g.nx*rs <= 65536;
+    fused_xy = fused_xy && fused && !rmhd && 20*(size_t)g.nx*rs <= 65536;
     nparts = fused ? mhd3d_march_blocks(g, march) : mhd3d_blocks(g);
     // the slab pipeline: GPL_PIPE_SLABS slabs (0 or 1: off); every slab but
     // the last starts and ends at a multiple of the y/z march and of the EMF

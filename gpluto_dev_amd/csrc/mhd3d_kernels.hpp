@@ -1182,10 +1182,14 @@ static void launch_t(int which, const Mhd3dBufs<R> &b, const Box3 &g, const Step
     break;
   }
   case K_SWEEPXY_F: {
-    const size_t shm = 20*(size_t)g.nx*sizeof(R);
-    const int nb = g.nz*((g.ny + L - 1)/L);
-    hipLaunchKernelGGL((k_sweep_xy<R, LIM, SOLVER, L, PH>), dim3(nb), dim3(g.nx), shm, s,
-                       g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
+    // ideal MHD only: the RMHD x+y kernel would hold more registers than a
+    // wavefront has (the engine never selects it for RMHD)
+    if constexpr (PH == 0) {
+      const size_t shm = 20*(size_t)g.nx*sizeof(R);
+      const int nb = g.nz*((g.ny + L - 1)/L);
+      hipLaunchKernelGGL((k_sweep_xy<R, LIM, SOLVER, L, PH>), dim3(nb), dim3(g.nx), shm, s,
+                         g, b.V, b.Vin, b.FD, b.dU, b.Cdt, ctl, (R)gamma, want_dt, b.err);
+    }
     break;
   }
   case K_SWEEPY_F: {
