@@ -203,8 +203,8 @@ public:
       rc |= dev_alloc(bytes, &F, 15*n*rs);
       rc |= dev_alloc(bytes, &CM, 3*n*rs);
     }
-    // UCT face records: 5 per direction, 6 with UCT_HLLD (dL and dR apart)
-    rc |= dev_alloc(bytes, &FD, (c.ct_emf_average == GPL_UCT_HLLD ? 18 : 15)*n*rs);
+    // UCT face records: 4 per direction (et, eb, aL_1, aR_1), 6 with UCT_HLLD
+    rc |= dev_alloc(bytes, &FD, (c.ct_emf_average == GPL_UCT_HLLD ? 18 : 12)*n*rs);
     rc |= dev_alloc(bytes, &E, 3*n*rs);
     rc |= dev_alloc(bytes, (void **)&cdt_part, (size_t)nparts*sizeof(double));
     rc |= dev_alloc(bytes, (void **)&diag, (size_t)(DIAG_BLOCKS + 1)*DIAG_MAXQ*sizeof(double));

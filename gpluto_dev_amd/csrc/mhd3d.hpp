@@ -43,7 +43,7 @@ struct Mhd3dBufs {
   const double *Bs0; // [3][n] faces at the start of the step (second stage on)
   R *F;            // [3][5][n]
   R *CM;           // [3][n]
-  R *FD;           // [3][5][n]
+  R *FD;           // [3][4][n] UCT face records ([3][6][n] with UCT_HLLD)
   R *E;            // [3][n]
   R *dU;           // [5][n] flux-divergence accumulator within a stage (fused path)
   R *Cdt;          // [n]    C_dt accumulator within a stage (fused path)

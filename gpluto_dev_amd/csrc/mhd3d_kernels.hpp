@@ -10,7 +10,7 @@
 //                  so it is never stored
 //   Vs[3][ncell]   face-normal field of the lower face of every cell (FP64)
 //   F [3][5][ncell], CM[3][ncell]   interface flux / max speed  (type R)
-//   FD[3][5][ncell] UCT face data et, eb, aL, aR, d             (type R)
+//   FD[3][4][ncell] UCT face data et, eb, min(0,Sl), max(0,Sr)  (type R)
 //                   (UCT_HLLD: FD[3][6][ncell], et, eb, aL, aR, dL, dR)
 //   E [3][ncell]   edge EMFs Ex(i,j-1/2,k-1/2), Ey(i-1/2,j,k-1/2),
 //                  Ez(i-1/2,j-1/2,k)                            (type R)
@@ -133,10 +133,38 @@ __device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceF
   else mhd_hll<R, N>(vl, vr, gamma, ff);
 }
 
-// Face records of the UCT EMF average per direction: et, eb, aL, aR, d for
-// UCT_HLL (dL = dR), et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with
-// CT_Store_UCTHlld).
-template <int SOLVER> __host__ __device__ constexpr int uct_nrec() { return SOLVER == 4 ? 6 : 5; }
+// Face records of the UCT EMF average per direction: et, eb and the two
+// clipped signal speeds aL_1 = min(0, Sl), aR_1 = max(0, Sr) for UCT_HLL,
+// from which the EMF kernels recompute the weights aL, aR and d (= dL = dR)
+// with CT_Store_UCTHll's own expressions (uct_coef: the same operations on
+// the same values, so the same bits, and one record per face less through
+// HBM); et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with
+// CT_Store_UCTHlld), whose weights come from the HLLD states.
+template <int SOLVER> __host__ __device__ constexpr int uct_nrec() { return SOLVER == 4 ? 6 : 4; }
+
+// aL, aR, d of CT_Store_UCTHll (Src/MHD/CT/ct_store.hpp:156-177) from the
+// stored aL_1, aR_1: the expressions of uct_hll_store.
+template <typename R>
+__device__ __forceinline__ void uct_coef(R aL_1, R aR_1, R &aL, R &aR, R &d)
+{
+  const R scrh = R(1)/(aR_1 - aL_1);
+  aL =  aR_1*scrh;
+  aR = -aL_1*scrh;
+  d  = -aL_1*aR_1*scrh;
+}
+
+// The NQ weights of one face for the EMF average from its records fdir[2..]:
+// computed from the clipped speeds (UCT_HLL, NQ = 3) or loaded (UCT_HLLD).
+template <typename R, int NQ>
+__device__ __forceinline__ void uct_weights(const R *fdir, unsigned c, size_t n, R *w)
+{
+  if constexpr (NQ == 4) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[q] = ldo(fdir + (2 + q)*n, c);
+  } else {
+    uct_coef<R>(ldo(fdir + 2*n, c), ldo(fdir + 3*n, c), w[0], w[1], w[2]);
+  }
+}
 
 template <typename R, int N, int SOLVER>
 __device__ __forceinline__ void uct_store(const R *vl, const R *vr, R gamma, const FaceFlux<R> &ff,
@@ -150,7 +178,8 @@ __device__ __forceinline__ void uct_store(const R *vl, const R *vr, R gamma, con
   } else {
     R et, eb, aL, aR, dd;
     uct_hll_store<R, N>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-    sto(fd, c, et); sto(fd + n, c, eb); sto(fd + 2*n, c, aL); sto(fd + 3*n, c, aR); sto(fd + 4*n, c, dd);
+    sto(fd, c, et); sto(fd + n, c, eb);
+    sto(fd + 2*n, c, rmin(R(0), ff.Sl)); sto(fd + 3*n, c, rmax(R(0), ff.Sr));
   }
 }
 
@@ -718,7 +747,7 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
   const int pl = nx*ny;
   const int k0 = seg*L, k1 = k0 + L < nz ? k0 + L : nz;
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
-  const R *FDx = FD, *FDy = FD + 5*n, *FDz = FD + 10*n;
+  const R *FDx = FD, *FDy = FD + 4*n, *FDz = FD + 8*n;   // UCT_HLL: 4 records per face
   int xo[4], yo[4];               // in-plane offsets of the x- and y-stencils (m2, m1, 0, p1)
 #pragma unroll
   for (int o = 0; o < 4; o++) {
@@ -737,8 +766,8 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
   }
   {
     const int c = col + wrapm(k0 - 1, nz)*pl;
-#pragma unroll
-    for (int q = 0; q < 3; q++) { cxm[q] = FDx[(2 + q)*n + c]; cym[q] = FDy[(2 + q)*n + c]; }
+    uct_weights<R, 3>(FDx, c, n, cxm);
+    uct_weights<R, 3>(FDy, c, n, cym);
   }
   for (int k = k0; k < k1; k++) {
     const int base = k*pl;
@@ -749,14 +778,11 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
     }
     const int c = col + base;
     R cx[3], cy[3], cz[3], cxj[3], cyi[3], czi[3], czj[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      cx[q] = FDx[(2 + q)*n + c];  cy[q] = FDy[(2 + q)*n + c];  cz[q] = FDz[(2 + q)*n + c];
-      cxj[q] = FDx[(2 + q)*n + coljm + base];
-      cyi[q] = FDy[(2 + q)*n + colim + base];
-      czi[q] = FDz[(2 + q)*n + colim + base];
-      czj[q] = FDz[(2 + q)*n + coljm + base];
-    }
+    uct_weights<R, 3>(FDx, c, n, cx);  uct_weights<R, 3>(FDy, c, n, cy);  uct_weights<R, 3>(FDz, c, n, cz);
+    uct_weights<R, 3>(FDx, coljm + base, n, cxj);
+    uct_weights<R, 3>(FDy, colim + base, n, cyi);
+    uct_weights<R, 3>(FDz, colim + base, n, czi);
+    uct_weights<R, 3>(FDz, coljm + base, n, czj);
     R q4[4], e4[4];
     // X1a: By along x, e[K][J] (FDy slot 1), x-face records at (j-1, j)
 #pragma unroll
@@ -818,9 +844,11 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   const int k0 = seg*L, k1 = k0 + L < g.ke ? k0 + L : g.ke;
   // uniform array bases (scalar registers) + 32-bit cell offsets
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
-  const R *ex0 = FD, *ex1 = FD + n, *fx2 = FD + 2*n, *fx3 = FD + 3*n, *fx4 = FD + 4*n;
-  const R *ey0 = FD + 5*n, *ey1 = FD + 6*n, *fy2 = FD + 7*n, *fy3 = FD + 8*n, *fy4 = FD + 9*n;
-  const R *ez0 = FD + 10*n, *ez1 = FD + 11*n, *fz2 = FD + 12*n, *fz3 = FD + 13*n, *fz4 = FD + 14*n;
+  // UCT_HLL records per face: et, eb, aL_1, aR_1 (the weights are computed when
+  // a face is staged)
+  const R *ex0 = FD, *ex1 = FD + n, *fx2 = FD + 2*n, *fx3 = FD + 3*n;
+  const R *ey0 = FD + 4*n, *ey1 = FD + 5*n, *fy2 = FD + 6*n, *fy3 = FD + 7*n;
+  const R *ez0 = FD + 8*n, *ez1 = FD + 9*n, *fz2 = FD + 10*n, *fz3 = FD + 11*n;
   R *Ex = E, *Ey = E + n, *Ez = E + 2*n;
   const unsigned col = (x0 + (act ? tx : 0)) + nx*(y0 + (act ? ty : 0));
   // halo task of this lane: 1 = y-halo row (waves 0-2: y0-2, y0-1, y0+th),
@@ -844,34 +872,34 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   }
   R zbx[4], zex[4], zby[4], zey[4];   // z-stencils of X3a (Bx, e[J][I]) and X3b (By, e[I][J])
   R zn[4] = {0, 0, 0, 0};             // their next plane
-  R cxm[3], cym[3];                   // x- and y-face records of plane k-1
-  R o[14];                            // own cell of the staged plane
-  R ha[4], hq[6];                     // halo stencil values and face records
+  R cxm[3], cym[3];                   // x- and y-face weights of plane k-1
+  R o[11];                            // own cell of the staged plane
+  R ha[4], hq[4];                     // halo stencil values and face records
   auto load_plane = [&](int k) {
     const unsigned b = (unsigned)k*pl;
     if (act) {
       const unsigned c = col + b;
       o[0] = ldo(ey1, c); o[1] = (R)ldo(Bz, c); o[2] = ldo(ez0, c);
       o[3] = ldo(ex0, c); o[4] = ldo(ez1, c);
-      o[5] = ldo(fx2, c); o[6] = ldo(fx3, c); o[7] = ldo(fx4, c);
-      o[8] = ldo(fy2, c); o[9] = ldo(fy3, c); o[10] = ldo(fy4, c);
-      o[11] = ldo(fz2, c); o[12] = ldo(fz3, c); o[13] = ldo(fz4, c);
+      o[5] = ldo(fx2, c); o[6] = ldo(fx3, c);
+      o[7] = ldo(fy2, c); o[8] = ldo(fy3, c);
+      o[9] = ldo(fz2, c); o[10] = ldo(fz3, c);
     }
     const unsigned c = ho + b;
     if (task == 1) {
       ha[0] = (R)ldo(Bx, c); ha[1] = ldo(ex0, c); ha[2] = (R)ldo(Bz, c); ha[3] = ldo(ez1, c);
       if (hf) {
-        hq[0] = ldo(fx2, c); hq[1] = ldo(fx3, c); hq[2] = ldo(fx4, c);
-        hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+        hq[0] = ldo(fx2, c); hq[1] = ldo(fx3, c);
+        hq[2] = ldo(fz2, c); hq[3] = ldo(fz3, c);
       }
     } else if (task == 2) {
       ha[0] = (R)ldo(By, c); ha[1] = ldo(ey1, c); ha[2] = (R)ldo(Bz, c); ha[3] = ldo(ez0, c);
       if (hf) {
-        hq[0] = ldo(fy2, c); hq[1] = ldo(fy3, c); hq[2] = ldo(fy4, c);
-        hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+        hq[0] = ldo(fy2, c); hq[1] = ldo(fy3, c);
+        hq[2] = ldo(fz2, c); hq[3] = ldo(fz3, c);
       }
     } else if (task == 3) {
-      hq[3] = ldo(fz2, c); hq[4] = ldo(fz3, c); hq[5] = ldo(fz4, c);
+      hq[2] = ldo(fz2, c); hq[3] = ldo(fz3, c);
     }
   };
   if (act) {
@@ -882,8 +910,8 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       zby[q] = (R)ldo(By, c); zey[q] = ldo(ey0, c);
     }
     const unsigned c = col + wrapm(k0 - 1, nz)*pl;
-    cxm[0] = ldo(fx2, c); cxm[1] = ldo(fx3, c); cxm[2] = ldo(fx4, c);
-    cym[0] = ldo(fy2, c); cym[1] = ldo(fy3, c); cym[2] = ldo(fy4, c);
+    uct_coef<R>(ldo(fx2, c), ldo(fx3, c), cxm[0], cxm[1], cxm[2]);
+    uct_coef<R>(ldo(fy2, c), ldo(fy3, c), cym[0], cym[1], cym[2]);
   }
   load_plane(k0);
   for (int k = k0; k < k1; k++) {
@@ -893,30 +921,42 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       xs[2][ty][tx + 2] = o[1];    xs[3][ty][tx + 2] = o[2];
       ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = o[3];
       ys[2][ty + 2][tx] = o[1];    ys[3][ty + 2][tx] = o[4];
+      R w[3];
+      uct_coef<R>(o[5], o[6], w[0], w[1], w[2]);
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        cxs[q][ty + 1][tx] = o[5 + q];
-        cys[q][ty][tx + 1] = o[8 + q];
-        czs[q][ty + 1][tx + 1] = o[11 + q];
-      }
+      for (int q = 0; q < 3; q++) cxs[q][ty + 1][tx] = w[q];
+      uct_coef<R>(o[7], o[8], w[0], w[1], w[2]);
+#pragma unroll
+      for (int q = 0; q < 3; q++) cys[q][ty][tx + 1] = w[q];
+      uct_coef<R>(o[9], o[10], w[0], w[1], w[2]);
+#pragma unroll
+      for (int q = 0; q < 3; q++) czs[q][ty + 1][tx + 1] = w[q];
     }
     if (task == 1) {
 #pragma unroll
       for (int a = 0; a < 4; a++) ys[a][hr][hc] = ha[a];
       if (hf) {
+        R wa[3], wz[3];
+        uct_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
+        uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
-        for (int q = 0; q < 3; q++) { cxs[q][0][hc] = hq[q]; czs[q][0][hc + 1] = hq[3 + q]; }
+        for (int q = 0; q < 3; q++) { cxs[q][0][hc] = wa[q]; czs[q][0][hc + 1] = wz[q]; }
       }
     } else if (task == 2) {
 #pragma unroll
       for (int a = 0; a < 4; a++) xs[a][hr][hc] = ha[a];
       if (hf) {
+        R wa[3], wz[3];
+        uct_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
+        uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
-        for (int q = 0; q < 3; q++) { cys[q][hr][0] = hq[q]; czs[q][hr + 1][0] = hq[3 + q]; }
+        for (int q = 0; q < 3; q++) { cys[q][hr][0] = wa[q]; czs[q][hr + 1][0] = wz[q]; }
       }
     } else if (task == 3) {
+      R wz[3];
+      uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
-      for (int q = 0; q < 3; q++) czs[q][0][0] = hq[3 + q];
+      for (int q = 0; q < 3; q++) czs[q][0][0] = wz[q];
     }
     __syncthreads();
     // loads of plane k+1 (and the z-stencil's plane k+2) fly during plane k
@@ -973,7 +1013,7 @@ template <typename R, int LIMV, int NQ = 3>
 __global__ __launch_bounds__(256)
 void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
 {
-  constexpr int NR = NQ + 2;       // records per face: et, eb and NQ coefficients
+  constexpr int NR = NQ == 4 ? 6 : 4;   // records per face: et, eb, then the weights' data
   const int i = blockIdx.x*64 + (threadIdx.x & 63), j = blockIdx.y*4 + (threadIdx.x >> 6);
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
@@ -984,11 +1024,7 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
   const double *Bx = Vs, *By = Vs + n, *Bz = Vs + 2*n;
   const R *ex0 = FD, *ex1 = FD + n, *ey0 = FD + NR*n, *ey1 = FD + (NR + 1)*n;
   const R *ez0 = FD + 2*NR*n, *ez1 = FD + (2*NR + 1)*n;
-  const R *fx[NQ], *fy[NQ], *fz[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    fx[q] = FD + (2 + q)*n; fy[q] = FD + (NR + 2 + q)*n; fz[q] = FD + (2*NR + 2 + q)*n;
-  }
+  const R *fdx = FD, *fdy = FD + NR*n, *fdz = FD + 2*NR*n;
   unsigned xo[4], yo[4], zo[4];
 #pragma unroll
   for (int o = 0; o < 4; o++) {
@@ -997,16 +1033,13 @@ void k_emf_edge(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
     zo[o] = i + nx*j + wrapm(k + o - 2, nz)*pl;
   }
   R cx[NQ], cy[NQ], cz[NQ], cxj[NQ], cyi[NQ], czi[NQ], czj[NQ], cxk[NQ], cyk[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    cx[q] = ldo(fx[q], c);  cy[q] = ldo(fy[q], c);  cz[q] = ldo(fz[q], c);
-    cxj[q] = ldo(fx[q], yo[1]);
-    cyi[q] = ldo(fy[q], xo[1]);
-    czi[q] = ldo(fz[q], xo[1]);
-    czj[q] = ldo(fz[q], yo[1]);
-    cxk[q] = ldo(fx[q], zo[1]);
-    cyk[q] = ldo(fy[q], zo[1]);
-  }
+  uct_weights<R, NQ>(fdx, c, n, cx);  uct_weights<R, NQ>(fdy, c, n, cy);  uct_weights<R, NQ>(fdz, c, n, cz);
+  uct_weights<R, NQ>(fdx, yo[1], n, cxj);
+  uct_weights<R, NQ>(fdy, xo[1], n, cyi);
+  uct_weights<R, NQ>(fdz, xo[1], n, czi);
+  uct_weights<R, NQ>(fdz, yo[1], n, czj);
+  uct_weights<R, NQ>(fdx, zo[1], n, cxk);
+  uct_weights<R, NQ>(fdy, zo[1], n, cyk);
   R q4[4], e4[4];
 #pragma unroll
   for (int o = 0; o < 4; o++) { q4[o] = (R)ldo(By, xo[o]); e4[o] = ldo(ey1, xo[o]); }
