@@ -43,7 +43,7 @@ struct Batch2dBufs {
   const double *Vin; // [2][ncell] lower x- and y-faces at stage start
   double *Vout;      // [2][ncell] faces after the stage (a third buffer)
   const double *Bs0; // [2][ncell] faces at the start of the step (second stage on)
-  R *FD;             // [2][5][ncell] UCT face data et, eb, aL, aR, d; UCT_HLLD:
+  R *FD;             // [2][4][ncell] UCT face data et, eb, min(0,Sl), max(0,Sr); UCT_HLLD:
                      // [2][6][ncell] et, eb, aL, aR, dL, dR
   R *E;              // [ncell] Ez at the lower-left corner of every cell
   R *dU;             // [6][ncell] x-sweep flux divergence

@@ -55,10 +55,12 @@ __device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
   Rr = q2 - R(0.5)*plm_slope<R, LIM>(q1, q2, q3);
 }
 
-// Face records per direction: et, eb, aL, aR, d for UCT_HLL (dL = dR), and
-// et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with CT_Store_UCTHlld,
+// Face records per direction: et, eb and the clipped speeds min(0, Sl),
+// max(0, Sr) for UCT_HLL, from which the EMF kernel recomputes aL, aR and
+// d = dL = dR (uct_hll_coef, the reference's expressions); et, eb, aL, aR,
+// dL, dR for UCT_HLLD (SOLVER 4: HLLD with CT_Store_UCTHlld,
 // Src/MHD/CT/ct_store.hpp:254-354).
-template <int SOLVER> __host__ __device__ constexpr int nrec() { return SOLVER == 4 ? 6 : 5; }
+template <int SOLVER> __host__ __device__ constexpr int nrec() { return SOLVER == 4 ? 6 : 4; }
 
 template <typename R, int N, int SOLVER>
 __device__ __forceinline__ void store_rec(const R *vl, const R *vr, R gamma, const FaceFlux<R> &ff,
@@ -72,7 +74,8 @@ __device__ __forceinline__ void store_rec(const R *vl, const R *vr, R gamma, con
   } else {
     R et, eb, aL, aR, dd;
     uct_hll_store<R, N>(vl, vr, ff.Sl, ff.Sr, et, eb, aL, aR, dd);
-    fd[0*n + c] = et; fd[1*n + c] = eb; fd[2*n + c] = aL; fd[3*n + c] = aR; fd[4*n + c] = dd;
+    fd[0*n + c] = et; fd[1*n + c] = eb;
+    fd[2*n + c] = rmin(R(0), ff.Sl); fd[3*n + c] = rmax(R(0), ff.Sr);
   }
 }
 
@@ -297,7 +300,7 @@ void k2_emf(Batch2dBufs<R> b)
 {
   const size_t n = (size_t)b.ncell;
   const double *Bx = b.Vin, *By = b.Vin + n;
-  const R *FDx = b.FD, *FDy = b.FD + (NQ + 2)*n;
+  const R *FDx = b.FD, *FDy = b.FD + (NQ == 4 ? 6 : 4)*n;
   for (int gi = blockIdx.x; gi < b.ntiles; gi += gridDim.x) {
     const Item2 t2 = b.tiles[gi];
     const BoxDesc bx = b.box[t2.box];
@@ -311,11 +314,18 @@ void k2_emf(Batch2dBufs<R> b)
       yo[o] = bx.off + i + NX*b2::wrapm(j + o - 2, NY);
     }
     R cx[NQ], cy[NQ], cxj[NQ], cyi[NQ];
+    if constexpr (NQ == 4) {
 #pragma unroll
-    for (int s = 0; s < NQ; s++) {
-      cx[s] = FDx[(2 + s)*n + c];  cy[s] = FDy[(2 + s)*n + c];
-      cxj[s] = FDx[(2 + s)*n + yo[1]];
-      cyi[s] = FDy[(2 + s)*n + xo[1]];
+      for (int s = 0; s < NQ; s++) {
+        cx[s] = FDx[(2 + s)*n + c];  cy[s] = FDy[(2 + s)*n + c];
+        cxj[s] = FDx[(2 + s)*n + yo[1]];
+        cyi[s] = FDy[(2 + s)*n + xo[1]];
+      }
+    } else {
+      uct_hll_coef<R>(FDx[2*n + c], FDx[3*n + c], cx[0], cx[1], cx[2]);
+      uct_hll_coef<R>(FDy[2*n + c], FDy[3*n + c], cy[0], cy[1], cy[2]);
+      uct_hll_coef<R>(FDx[2*n + yo[1]], FDx[3*n + yo[1]], cxj[0], cxj[1], cxj[2]);
+      uct_hll_coef<R>(FDy[2*n + xo[1]], FDy[3*n + xo[1]], cyi[0], cyi[1], cyi[2]);
     }
     R q4[4], e4[4];
 #pragma unroll
@@ -409,7 +419,7 @@ static void b2_launch_t(int phase, const Batch2dBufs<R> &b, RKW w, int first_sta
                        want_dt, (R)gamma);
     break;
   case B2_EMF:
-    hipLaunchKernelGGL((k2_emf<R, LIM, b2::nrec<SOLVER>() - 2>), dim3(blocks), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k2_emf<R, LIM, SOLVER == 4 ? 4 : 3>), dim3(blocks), dim3(256), 0, s, b);
     break;
   case B2_FINISH:
     hipLaunchKernelGGL((k2_finish<R>), dim3(blocks), dim3(256), 0, s, b, w, first_stage, gamma - 1.0,

@@ -184,7 +184,7 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
   rc |= dev_alloc(b->bytes, (void **)&b->Us, 6*nc*sizeof(double));
   for (int q = 0; q < 3; q++) rc |= dev_alloc(b->bytes, (void **)&b->vs[q], 2*nc*sizeof(double));
-  rc |= dev_alloc(b->bytes, &b->FD, 2*(b->ksolver == 4 ? 6 : 5)*nc*rs);
+  rc |= dev_alloc(b->bytes, &b->FD, 2*(b->ksolver == 4 ? 6 : 4)*nc*rs);
   rc |= dev_alloc(b->bytes, &b->E, nc*rs);
   rc |= dev_alloc(b->bytes, &b->dU, 6*nc*rs);
   rc |= dev_alloc(b->bytes, &b->Cdt, nc*rs);

@@ -136,22 +136,11 @@ __device__ __forceinline__ void riemann(const R *vl, const R *vr, R gamma, FaceF
 // Face records of the UCT EMF average per direction: et, eb and the two
 // clipped signal speeds aL_1 = min(0, Sl), aR_1 = max(0, Sr) for UCT_HLL,
 // from which the EMF kernels recompute the weights aL, aR and d (= dL = dR)
-// with CT_Store_UCTHll's own expressions (uct_coef: the same operations on
-// the same values, so the same bits, and one record per face less through
+// with CT_Store_UCTHll's own expressions (uct_hll_coef: the same operations
+// on the same values, so the same bits, and one record per face less through
 // HBM); et, eb, aL, aR, dL, dR for UCT_HLLD (SOLVER 4: HLLD with
 // CT_Store_UCTHlld), whose weights come from the HLLD states.
 template <int SOLVER> __host__ __device__ constexpr int uct_nrec() { return SOLVER == 4 ? 6 : 4; }
-
-// aL, aR, d of CT_Store_UCTHll (Src/MHD/CT/ct_store.hpp:156-177) from the
-// stored aL_1, aR_1: the expressions of uct_hll_store.
-template <typename R>
-__device__ __forceinline__ void uct_coef(R aL_1, R aR_1, R &aL, R &aR, R &d)
-{
-  const R scrh = R(1)/(aR_1 - aL_1);
-  aL =  aR_1*scrh;
-  aR = -aL_1*scrh;
-  d  = -aL_1*aR_1*scrh;
-}
 
 // The NQ weights of one face for the EMF average from its records fdir[2..]:
 // computed from the clipped speeds (UCT_HLL, NQ = 3) or loaded (UCT_HLLD).
@@ -162,7 +151,7 @@ __device__ __forceinline__ void uct_weights(const R *fdir, unsigned c, size_t n,
 #pragma unroll
     for (int q = 0; q < 4; q++) w[q] = ldo(fdir + (2 + q)*n, c);
   } else {
-    uct_coef<R>(ldo(fdir + 2*n, c), ldo(fdir + 3*n, c), w[0], w[1], w[2]);
+    uct_hll_coef<R>(ldo(fdir + 2*n, c), ldo(fdir + 3*n, c), w[0], w[1], w[2]);
   }
 }
 
@@ -910,8 +899,8 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       zby[q] = (R)ldo(By, c); zey[q] = ldo(ey0, c);
     }
     const unsigned c = col + wrapm(k0 - 1, nz)*pl;
-    uct_coef<R>(ldo(fx2, c), ldo(fx3, c), cxm[0], cxm[1], cxm[2]);
-    uct_coef<R>(ldo(fy2, c), ldo(fy3, c), cym[0], cym[1], cym[2]);
+    uct_hll_coef<R>(ldo(fx2, c), ldo(fx3, c), cxm[0], cxm[1], cxm[2]);
+    uct_hll_coef<R>(ldo(fy2, c), ldo(fy3, c), cym[0], cym[1], cym[2]);
   }
   load_plane(k0);
   for (int k = k0; k < k1; k++) {
@@ -922,13 +911,13 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       ys[0][ty + 2][tx] = zbx[2];  ys[1][ty + 2][tx] = o[3];
       ys[2][ty + 2][tx] = o[1];    ys[3][ty + 2][tx] = o[4];
       R w[3];
-      uct_coef<R>(o[5], o[6], w[0], w[1], w[2]);
+      uct_hll_coef<R>(o[5], o[6], w[0], w[1], w[2]);
 #pragma unroll
       for (int q = 0; q < 3; q++) cxs[q][ty + 1][tx] = w[q];
-      uct_coef<R>(o[7], o[8], w[0], w[1], w[2]);
+      uct_hll_coef<R>(o[7], o[8], w[0], w[1], w[2]);
 #pragma unroll
       for (int q = 0; q < 3; q++) cys[q][ty][tx + 1] = w[q];
-      uct_coef<R>(o[9], o[10], w[0], w[1], w[2]);
+      uct_hll_coef<R>(o[9], o[10], w[0], w[1], w[2]);
 #pragma unroll
       for (int q = 0; q < 3; q++) czs[q][ty + 1][tx + 1] = w[q];
     }
@@ -937,8 +926,8 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       for (int a = 0; a < 4; a++) ys[a][hr][hc] = ha[a];
       if (hf) {
         R wa[3], wz[3];
-        uct_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
-        uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
+        uct_hll_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
+        uct_hll_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
         for (int q = 0; q < 3; q++) { cxs[q][0][hc] = wa[q]; czs[q][0][hc + 1] = wz[q]; }
       }
@@ -947,14 +936,14 @@ void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD,
       for (int a = 0; a < 4; a++) xs[a][hr][hc] = ha[a];
       if (hf) {
         R wa[3], wz[3];
-        uct_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
-        uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
+        uct_hll_coef<R>(hq[0], hq[1], wa[0], wa[1], wa[2]);
+        uct_hll_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
         for (int q = 0; q < 3; q++) { cys[q][hr][0] = wa[q]; czs[q][hr + 1][0] = wz[q]; }
       }
     } else if (task == 3) {
       R wz[3];
-      uct_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
+      uct_hll_coef<R>(hq[2], hq[3], wz[0], wz[1], wz[2]);
 #pragma unroll
       for (int q = 0; q < 3; q++) czs[q][0][0] = wz[q];
     }

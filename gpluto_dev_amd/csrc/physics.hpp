@@ -436,6 +436,20 @@ GPL_HD void uct_hlld_store(const R *vl, const R *vr, R gamma, R Sl, R Sr, R &et,
   eb = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
 }
 
+// The weights aL, aR and the diffusion coefficient d of CT_Store_UCTHll
+// (Src/MHD/CT/ct_store.hpp:156-177) from the clipped signal speeds
+// aL_1 = min(0, Sl), aR_1 = max(0, Sr).  The sweeps store aL_1, aR_1 and the
+// EMF kernels call this, so the weights come out of the same operations on
+// the same values as in the reference's store.
+template <typename R>
+GPL_HD void uct_hll_coef(R aL_1, R aR_1, R &aL, R &aR, R &d)
+{
+  const R scrh = R(1)/(aR_1 - aL_1);
+  aL =  aR_1*scrh;
+  aR = -aL_1*scrh;
+  d  = -aL_1*aR_1*scrh;
+}
+
 // CT_Store_UCTHll<dir>, Src/MHD/CT/ct_store.hpp:156-177.  Per face:
 //   et = e[DIRb][DIRn] (upwinded transverse velocity v_t),
 //   eb = e[DIRt][DIRn] (minus upwinded v_b),
@@ -449,9 +463,7 @@ GPL_HD void uct_hll_store(const R *vl, const R *vr, R Sl, R Sr, R &et, R &eb, R 
   R scrh = R(1)/(aR_1 - aL_1);
   et =  (aR_1*vl[VXt] - aL_1*vr[VXt])*scrh;
   eb = -(aR_1*vl[VXb] - aL_1*vr[VXb])*scrh;
-  aL =  aR_1*scrh;
-  aR = -aL_1*scrh;
-  d  = -aL_1*aR_1*scrh;
+  uct_hll_coef<R>(aL_1, aR_1, aL, aR, d);
 }
 
 // ConsToPrim MHD (Src/MHD/mappers.cpp:32-128) for the hydro part; the cell
