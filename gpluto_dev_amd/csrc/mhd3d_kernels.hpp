@@ -811,7 +811,7 @@ void k_emf_march(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD
 // are issued before plane k is computed, so their latency hides behind it.
 // The z-stencils and the face records of plane k-1 stay in registers.
 template <typename R, int LIMV, int L>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 1)))
 void k_emf_tile(Box3 g, const double *__restrict__ Vs, const R *__restrict__ FD, R *__restrict__ E)
 {
   constexpr int TX = 64, TY = 4;
