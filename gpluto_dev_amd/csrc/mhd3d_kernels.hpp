@@ -535,7 +535,7 @@ void k_sweep_xy(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
 // (getMax, update_stage.cpp:446-458).
 // ---------------------------------------------------------------------------
 template <typename R, int D, int LIM, int SOLVER, int L, int PH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && D == 1 ? 4 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && D == 1 ? 4 : (PH == 1 && SOLVER == 1 ? 2 : 1))))
 void k_sweep_march(Box3 g, const R *__restrict__ V, const double *__restrict__ Vs,
                    R *__restrict__ FD, R *__restrict__ dU, R *__restrict__ Cdt,
                    const double *Uin, double *U, const double *U0,
