@@ -50,29 +50,30 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # the whole box; the figure is the mean over the two stages of an RK2 step,
 # because the kernels' average launch time is too.  Stage 1 writes its state
 # to a second buffer and stage 2 reads both, so no stage copies U^n or the
-# faces of U^n aside.
-#   k_sweepx_fused : V 7w + face 8 -> FD 5w + dU 5w (+ C_dt w, stage 1)
-#                    = 18w + 8 | 17w + 8                           -> 17.5w + 8
-#   k_sweepy_fused : V 7w + face 8 + dU 5w (+ C_dt w) -> FD 5w + dU 5w (+ C_dt w)
-#                    = 24w + 8 | 22w + 8                           -> 23w + 8
+# faces of U^n aside.  FD holds 4 UCT_HLL records per face (et, eb and the
+# two clipped signal speeds).
+#   k_sweepx_fused : V 7w + face 8 -> FD 4w + dU 5w (+ C_dt w, stage 1)
+#                    = 17w + 8 | 16w + 8                           -> 16.5w + 8
+#   k_sweepy_fused : V 7w + face 8 + dU 5w (+ C_dt w) -> FD 4w + dU 5w (+ C_dt w)
+#                    = 23w + 8 | 21w + 8                           -> 22w + 8
 #   k_sweepz_fused : V 7w + face 8 + dU 5w + U 40 (+ C_dt w | + U^n 40)
-#                    -> FD 5w + U 40
-#                    = 18w + 88 | 17w + 128                        -> 17.5w + 108
+#                    -> FD 4w + U 40
+#                    = 17w + 88 | 16w + 128                        -> 16.5w + 108
 #   k_sweepxy_fused (mixed form: the x and y sweeps in one row march):
-#                    V 8w + faces 16 -> FD 10w + dU 5w (+ C_dt w, stage 1)
-#                    = 24w + 16 | 23w + 16                         -> 23.5w + 16
-#   k_emf          : FD 15w + faces 24 -> E 3w                     =  18w + 24
+#                    V 8w + faces 16 -> FD 8w + dU 5w (+ C_dt w, stage 1)
+#                    = 22w + 16 | 21w + 16                         -> 21.5w + 16
+#   k_emf          : FD 12w + faces 24 -> E 3w                     =  15w + 24
 #   k_finish       : E 3w + faces 24 + U 40 (+ faces of U^n 24, stage 2)
 #                    -> faces 24 + V 8w                            =  11w + 100
-#   k_sweep_* (split, rows > 512): V 8w + face 8 -> F 5w + CM w + FD 5w = 19w + 8
+#   k_sweep_* (split, rows > 512): V 8w + face 8 -> F 5w + CM w + FD 4w = 18w + 8
 #   k_rhs_update   : F 15w + CM 3w + U 40 (+ U^n 40, stage 2) -> U 40 = 18w + 100
-ALG_BYTES = {"k_sweepx_fused": (17.5, 8), "k_sweepy_fused": (23, 8), "k_sweepxy_fused": (23.5, 16),
-             "k_sweepz_fused": (17.5, 108), "k_emf": (18, 24), "k_finish": (11, 100),
+ALG_BYTES = {"k_sweepx_fused": (16.5, 8), "k_sweepy_fused": (22, 8), "k_sweepxy_fused": (21.5, 16),
+             "k_sweepz_fused": (16.5, 108), "k_emf": (15, 24), "k_finish": (11, 100),
              "k_rhs_update": (18, 100)}
 
 
 def alg_bytes_per_cell(kernel: str, w: int) -> float:
-    a, b = ALG_BYTES.get(kernel, (19, 8) if kernel.startswith("k_sweep") else (0, 0))
+    a, b = ALG_BYTES.get(kernel, (18, 8) if kernel.startswith("k_sweep") else (0, 0))
     return a * w + b
 
 
