@@ -194,6 +194,20 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
   return x*q + (x < r ? x : r) + b / NXCD;
 }
 
+// Runs of RUN consecutive logical blocks per XCD: block b (on XCD b % 8) of
+// group g = b / (8 RUN) takes logical block g 8 RUN + (b % 8) RUN + (b / 8) % RUN.
+// A one-row block then reads the next row of its plane (RUN - 1 times in
+// RUN) and the same row of the next plane (when a plane holds a multiple of
+// 8 RUN blocks) through its own L2.  The last partial group keeps its numbers.
+template <int RUN>
+__device__ __forceinline__ int xcd_runs(int b, int nb)
+{
+  constexpr int G = 8*RUN;
+  if (b >= nb - nb % G) return b;
+  const int g = b / G, r = b - g*G;
+  return g*G + (r % 8)*RUN + r / 8;
+}
+
 // ---------------------------------------------------------------------------
 // Sweep along D: one thread per interface (lower face of cell c).
 // DataToSweep + Reconstruct (plm_states.cpp:99-189) + Riemann solver
@@ -1099,7 +1113,7 @@ void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
 {
   // no early exit: the learned con2prim guess is evaluated by whole wavefronts
   const int c_beg = g.kb*g.nx*g.ny, c_end = g.ke*g.nx*g.ny;   // planes kb .. ke-1
-  const int c0 = c_beg + blockIdx.x*blockDim.x + threadIdx.x;
+  const int c0 = c_beg + xcd_runs<8>(blockIdx.x, gridDim.x)*blockDim.x + threadIdx.x;
   const bool live = c0 < c_end;
   const int c = live ? c0 : c_end - 1;
   const size_t n = (size_t)g.ncell;
