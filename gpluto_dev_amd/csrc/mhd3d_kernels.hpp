@@ -1103,7 +1103,7 @@ __device__ __forceinline__ FaceR ct_rs(int i, int j, int k, const Box3 &g, const
 // value their owner writes).  Vs is double-buffered (Vin -> Vout).
 // ---------------------------------------------------------------------------
 template <typename R, int PH, bool NET = false>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PH == 1 && !NET ? 6 : 1)))
 void k_finish(Box3 g, const R *__restrict__ E, const double *__restrict__ Vin,
               double *__restrict__ Vout, const double *__restrict__ Bs0,
               double *__restrict__ U, R *__restrict__ V,
