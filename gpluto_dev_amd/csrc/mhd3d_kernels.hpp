@@ -708,7 +708,7 @@ __device__ __forceinline__ void rec4(R q0, R q1, R q2, R q3, R &L, R &Rr)
 // one UCT contribution (ct_emf_average.cpp:636-643):
 // (aL vL bL + aR vR bR) +/- (d bR - d bL), from preloaded values: face-field stencil q[0..3] and
 // transverse-velocity stencil e[0..3] along the reconstruction direction, and
-// the (aL, aR, d) or (aL, aR, dL, dR) records of the two faces the edge
+// the (aL, aR, d) or (aL, aR, dL, dR) weights of the two faces the edge
 // averages over (fa is the displaced face, fb the face of the edge's own cell).
 template <typename R, int LIMV, int NQ = 3>
 __device__ __forceinline__ R uct_val(const R *q, const R *e, const R *fa, const R *fb, int sgn)
