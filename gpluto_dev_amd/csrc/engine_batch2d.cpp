@@ -77,6 +77,14 @@ struct gpl_batch {
   Item4 *yitems[3] = {nullptr, nullptr, nullptr};   // per march length
   int *part_beg[3] = {nullptr, nullptr, nullptr};
   int ny_items[3] = {0, 0, 0};
+  // Two box groups, boxes [0, split) and [split, nbox), on two streams: the
+  // groups share no cell, so their phases run side by side and fill each
+  // other's latency-bound stretches and tails (split 0: one group).  Item
+  // ranges of the groups: [beg[0], beg[1]) and [beg[1], beg[2]).
+  int split = 0;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int xbeg[3] = {}, tbeg[3] = {}, ybeg[3][3] = {};
   // launch shape
   int march_idx = 1, bpc_idx = 2;
   bool tune_pending = true;
@@ -97,6 +105,9 @@ static void batch_free(gpl_batch *b)
                   b->part_beg[2]};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_join) hipEventDestroy(b->ev_join);
+  if (b->s2) hipStreamDestroy(b->s2);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->owns_env) {
@@ -135,6 +146,7 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   // boxes and work lists
   std::vector<Item4> xi;
   std::vector<Item2> ti;
+  std::vector<int> xstart, tstart;   // first item of every box
   int off = 0;
   for (int q = 0; q < 4; q++) b->side[q] = side_type(*cfg, q);
   b->padded = !all_periodic(*cfg);
@@ -150,6 +162,8 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
     b->hbox.push_back(d);
     if (d.nx > b->maxp) b->maxp = d.nx;
     if (d.ny > b->maxp) b->maxp = d.ny;
+    xstart.push_back((int)xi.size());
+    tstart.push_back((int)ti.size());
     for (int j = 0; j < d.ny; j++)
       for (int i0 = 0; i0 < d.nx; i0 += 63) xi.push_back({q, j, i0, d.nx - i0 < 63 ? d.nx - i0 : 63});
     for (int s = 0; s < d.nx*d.ny; s += 256) ti.push_back({q, s});
@@ -163,6 +177,7 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
   b->nx_items = (int)xi.size();
   b->ntiles = (int)ti.size();
   int max_y = 0;
+  std::vector<int> yb[3];
   for (int m = 0; m < 3; m++) {
     const int L = kMarch[m];
     std::vector<Item4> yi;
@@ -175,10 +190,35 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
         for (int i0 = 0; i0 < sx; i0 += 64) yi.push_back({q, i0, s, 0});
     }
     pb.push_back((int)yi.size());
+    yb[m] = pb;
     rc |= upload_vec(&b->yitems[m], yi, b->bytes);
     rc |= upload_vec(&b->part_beg[m], pb, b->bytes);
     b->ny_items[m] = (int)yi.size();
     if ((int)yi.size() > max_y) max_y = (int)yi.size();
+  }
+  // the box groups: the first box at which half of the cells are reached
+  // (a batch of its own with two boxes or more; GPL_BATCH_STREAMS=1 keeps one)
+  {
+    const char *e = getenv("GPL_BATCH_STREAMS");
+    if (!env && nbox >= 2 && !(e && atoi(e) == 1)) {
+      long long acc = 0;
+      int q = 0;
+      while (q < nbox - 1 && 2*(acc + (long long)b->hbox[q].nx*b->hbox[q].ny) <= total) {
+        acc += (long long)b->hbox[q].nx*b->hbox[q].ny;
+        q++;
+      }
+      b->split = q > 0 ? q : 1;
+    }
+    const int s = b->split;
+    b->xbeg[0] = 0; b->xbeg[1] = s ? xstart[s] : b->nx_items; b->xbeg[2] = b->nx_items;
+    b->tbeg[0] = 0; b->tbeg[1] = s ? tstart[s] : b->ntiles; b->tbeg[2] = b->ntiles;
+    for (int m = 0; m < 3; m++) {
+      b->ybeg[m][0] = 0; b->ybeg[m][1] = yb[m][s ? s : nbox]; b->ybeg[m][2] = b->ny_items[m];
+    }
+    if (s && (hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess))
+      return GPL_EHIP;
   }
   rc |= dev_alloc(b->bytes, &b->V, 8*nc*rs);
   rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
@@ -442,26 +482,36 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, double 
   q.V = (R *)b->V; q.Uin = uin; q.U = uout; q.U0 = b->U;
   q.Vin = vin; q.Vout = vout; q.Bs0 = vbase;
   q.FD = (R *)b->FD; q.E = (R *)b->E; q.dU = (R *)b->dU; q.Cdt = (R *)b->Cdt;
-  q.cdt_part = b->cdt_part; q.err = b->err; q.ctl = b->ctl; q.box = b->box;
-  q.xitems = b->xitems; q.yitems = b->yitems[b->march_idx]; q.tiles = b->tiles;
-  q.nx_items = b->nx_items; q.ny_items = b->ny_items[b->march_idx]; q.ntiles = b->ntiles;
+  q.err = b->err; q.ctl = b->ctl; q.box = b->box;
   q.ncell = b->ncell;
-  const int march = kMarch[b->march_idx];
-  const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
-  const int gt = grid_for(b, q.ntiles);
+  const int m = b->march_idx, march = kMarch[m];
   const gpl_config &c = b->cfg;
-  // Boundary (Src/rk_step.cpp: at the start of every stage) on padded boxes
-  if (b->padded)
-    batch2d_boundary<R>(b->box, b->nbox, b->maxp, b->side, (R *)b->V, vin, (size_t)b->ncell,
-                        b->stream);
-  batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gx,
-                    c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gy,
-                    c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_EMF, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
-                    c.gamma, c.small_rho, c.small_prs, b->stream);
-  batch2d_launch<R>(B2_FINISH, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
-                    c.gamma, c.small_rho, c.small_prs, b->stream);
+  // the box groups, each on its own stream; the kernels index the items of
+  // their group and the C_dt partial slots that go with them
+  for (int gi = 0; gi < (b->split ? 2 : 1); gi++) {
+    const hipStream_t st = gi ? b->s2 : b->stream;
+    const int box0 = gi ? b->split : 0, nbg = b->split ? (gi ? b->nbox - b->split : b->split) : b->nbox;
+    q.xitems = b->xitems + b->xbeg[gi]; q.nx_items = b->xbeg[gi + 1] - b->xbeg[gi];
+    q.yitems = b->yitems[m] + b->ybeg[m][gi]; q.ny_items = b->ybeg[m][gi + 1] - b->ybeg[m][gi];
+    q.cdt_part = b->cdt_part + b->ybeg[m][gi];
+    q.tiles = b->tiles + b->tbeg[gi]; q.ntiles = b->tbeg[gi + 1] - b->tbeg[gi];
+    if (!b->split) {
+      q.nx_items = b->nx_items; q.ny_items = b->ny_items[m]; q.ntiles = b->ntiles;
+    }
+    const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
+    const int gt = grid_for(b, q.ntiles);
+    // Boundary (Src/rk_step.cpp: at the start of every stage) on padded boxes
+    if (b->padded)
+      batch2d_boundary<R>(b->box + box0, nbg, b->maxp, b->side, (R *)b->V, vin, (size_t)b->ncell, st);
+    batch2d_launch<R>(B2_SWEEP_X, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gx,
+                      c.gamma, c.small_rho, c.small_prs, st);
+    batch2d_launch<R>(B2_SWEEP_Y, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gy,
+                      c.gamma, c.small_rho, c.small_prs, st);
+    batch2d_launch<R>(B2_EMF, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
+                      c.gamma, c.small_rho, c.small_prs, st);
+    batch2d_launch<R>(B2_FINISH, q, w, first_stage, first_stage, c.limiter, b->ksolver, march, gt,
+                      c.gamma, c.small_rho, c.small_prs, st);
+  }
 }
 
 // one AdvanceStep of every box (Src/rk_step.cpp:88-720), then per-box dt
@@ -479,6 +529,10 @@ static void enqueue_step(gpl_batch *b, int advance_dt)
   const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
   double *f0 = b->vs[b->cur], *f1 = b->vs[(b->cur + 1) % 3], *f2 = b->vs[(b->cur + 2) % 3];
   if (b->zcnt) launch_zone_reset(b->zcnt, b->stream);
+  if (b->split) {                      // the second group's stream joins here
+    (void)hipEventRecord(b->ev_fork, b->stream);
+    (void)hipStreamWaitEvent(b->s2, b->ev_fork, 0);
+  }
   enqueue_stage_r(b, b->U, b->Us, f0, f1, f0, w1, 1, 0);            // -> Us, f1
   if (b->cfg.time_stepping != GPL_RK3) {
     enqueue_stage_r(b, b->Us, b->U, f1, f2, f0, w2, 0, 1);          // -> U, f2
@@ -488,8 +542,18 @@ static void enqueue_step(gpl_batch *b, int advance_dt)
     enqueue_stage_r(b, b->Us, b->U, f2, f1, f0, w3, 0, 2);          // -> U, f1
     b->cur = (b->cur + 1) % 3;
   }
-  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
+  if (!b->split) {
+    batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
+                      b->cfg.cfl_max_var, advance_dt, b->stream);
+    return;
+  }
+  // per group: each box's dt reads only its own partial slots
+  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->split, b->ctl, b->cfg.cfl,
                     b->cfg.cfl_max_var, advance_dt, b->stream);
+  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx] + b->split, b->nbox - b->split,
+                    b->ctl + b->split, b->cfg.cfl, b->cfg.cfl_max_var, advance_dt, b->s2);
+  (void)hipEventRecord(b->ev_join, b->s2);
+  (void)hipStreamWaitEvent(b->stream, b->ev_join, 0);
 }
 
 // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
@@ -514,7 +578,10 @@ static int build_graph(gpl_batch *b)
   return 0;
 }
 
-// Time every launch shape on one step each (eager launches); keep the fastest.
+// Time every launch shape on one step each and keep the fastest.  Each
+// trial step is captured as a graph of its own and launched once, so it is
+// timed as the replayed steps run, without the host's launch gaps; it is a
+// step of the run like any other.
 static int autotune(gpl_batch *b, int *done)
 {
   double best = 1e30;
@@ -522,10 +589,20 @@ static int autotune(gpl_batch *b, int *done)
   for (int m = 0; m < 3; m++)
     for (int p = 0; p < 4; p++) {
       b->march_idx = m; b->bpc_idx = p;
-      hipEventRecord(b->ev0, b->stream);
+      hipGraph_t graph;
+      hipGraphExec_t exec = nullptr;
+      if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return GPL_EHIP;
       enqueue_step(b, 1);
+      if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) return GPL_EHIP;
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return GPL_EHIP;
+      hipEventRecord(b->ev0, b->stream);
+      const hipError_t l = hipGraphLaunch(exec, b->stream);
       hipEventRecord(b->ev1, b->stream);
-      if (hipEventSynchronize(b->ev1) != hipSuccess) return GPL_EHIP;
+      const hipError_t sy = hipEventSynchronize(b->ev1);
+      hipGraphExecDestroy(exec);
+      if (l != hipSuccess || sy != hipSuccess) return GPL_EHIP;
       float ms = 0.f;
       hipEventElapsedTime(&ms, b->ev0, b->ev1);
       if (ms < best) { best = ms; bm = m; bp = p; }

@@ -7,7 +7,9 @@ within 1e-5 relative L1 of the FP64 batch.  Both EMF averages: UCT_HLL, and
 UCT_HLLD with HLLD (against two more reference fixtures).  Periodic boxes, and
 boxes with outflow and reflective sides, alone and mixed with periodic
 directions (the padded boxes, against the oracle and two more reference
-fixtures, the evolved upper boundary faces included)."""
+fixtures, the evolved upper boundary faces included).  A batch runs its boxes
+as two groups on two streams; one stream (GPL_BATCH_STREAMS=1) gives the same
+bits."""
 import numpy as np
 import pytest
 
@@ -162,6 +164,33 @@ def test_fp32_batch_within_1e5_of_fp64(emf, sides):
             l1 = np.sum(np.abs(v32[v] - v64[v])) / np.sum(np.abs(v64[v]))
             assert l1 <= 1e-5, f"box {b} var {v}: rel L1 {l1:.2e}"
         assert (ic.divb_max(s32) if sides is None else _divb_interior(s32)) < 1e-10
+
+
+@pytest.mark.parametrize("precision,emf,sides", [("fp64", "UCT_HLL", None), ("mixed", "UCT_HLLD", None),
+                                                 ("mixed", "UCT_HLL", ("outflow", "reflective") * 2 + ("outflow",) * 2)])
+def test_two_box_groups_match_one_stream(precision, emf, sides, monkeypatch):
+    """The two box groups on two streams (the default) against the whole
+    batch on one stream: the same bits, times and dts, through the autotuned
+    graph replay and the eager steps after it."""
+    sizes = [(64, 64), (150, 150), (37, 37), (256, 256), (5, 5), (100, 37)]
+    ics = _ics(sizes, 13)
+    out = {}
+    for streams in ("1", "2"):
+        monkeypatch.setenv("GPL_BATCH_STREAMS", streams)
+        print(f"case {precision} {emf} {sides} streams={streams}", flush=True)
+        g = _batch(sizes, precision=precision, emf=emf, sides=sides)
+        for b, (vc, vs) in enumerate(ics):
+            g.upload(b, vc, vs)
+        err, t, dt = g.run(61, 5e-4)
+        assert err == 0
+        out[streams] = (t, dt, [g.download(b) for b in range(len(sizes))])
+        g.close()
+    (t1, dt1, s1), (t2, dt2, s2) = out["1"], out["2"]
+    assert np.array_equal(t1, t2) and np.array_equal(dt1, dt2)
+    for b, ((a1, f1, u1), (a2, f2, u2)) in enumerate(zip(s1, s2)):
+        assert np.array_equal(a1, a2), f"box {b} {sizes[b]}: cells differ"
+        assert np.array_equal(f1, f2), f"box {b} {sizes[b]}: faces differ"
+        assert np.array_equal(u1, u2), f"box {b} {sizes[b]}: Uc differs"
 
 
 def test_uct_hlld_needs_hlld():
