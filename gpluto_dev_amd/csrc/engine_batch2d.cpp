@@ -77,14 +77,17 @@ struct gpl_batch {
   Item4 *yitems[3] = {nullptr, nullptr, nullptr};   // per march length
   int *part_beg[3] = {nullptr, nullptr, nullptr};
   int ny_items[3] = {0, 0, 0};
-  // Two box groups, boxes [0, split) and [split, nbox), on two streams: the
-  // groups share no cell, so their phases run side by side and fill each
-  // other's latency-bound stretches and tails (split 0: one group).  Item
-  // ranges of the groups: [beg[0], beg[1]) and [beg[1], beg[2]).
-  int split = 0;
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int xbeg[3] = {}, tbeg[3] = {}, ybeg[3][3] = {};
+  // Box groups on streams of their own: group g holds boxes [gbox[g],
+  // gbox[g+1]) with about the same number of cells.  The groups share no
+  // cell, so their phases run side by side and fill each other's
+  // latency-bound stretches and tails.  Group g's items are [beg[g],
+  // beg[g+1]) of each work list; group 0 runs on the batch's stream.
+  static constexpr int kMaxGroups = 4;
+  int ng = 1;
+  int gbox[kMaxGroups + 1] = {};
+  hipStream_t gs[kMaxGroups] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups] = {};
+  int xbeg[kMaxGroups + 1] = {}, tbeg[kMaxGroups + 1] = {}, ybeg[3][kMaxGroups + 1] = {};
   // launch shape
   int march_idx = 1, bpc_idx = 2;
   bool tune_pending = true;
@@ -106,8 +109,10 @@ static void batch_free(gpl_batch *b)
   for (void *p : ptrs) if (p) (void)hipFree(p);
   if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
-  if (b->ev_join) hipEventDestroy(b->ev_join);
-  if (b->s2) hipStreamDestroy(b->s2);
+  for (int g = 1; g < gpl_batch::kMaxGroups; g++) {
+    if (b->ev_join[g]) hipEventDestroy(b->ev_join[g]);
+    if (b->gs[g]) hipStreamDestroy(b->gs[g]);
+  }
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   if (b->owns_env) {
@@ -196,29 +201,44 @@ static int batch_init(gpl_batch *b, const gpl_config *cfg, int nbox, const int *
     b->ny_items[m] = (int)yi.size();
     if ((int)yi.size() > max_y) max_y = (int)yi.size();
   }
-  // the box groups: the first box at which half of the cells are reached
-  // (a batch of its own with two boxes or more; GPL_BATCH_STREAMS=1 keeps one)
+  // the box groups (a batch of its own: GPL_BATCH_STREAMS groups, 2 by
+  // default, at most one per box): group g starts at the first box whose
+  // middle lies past g/ng of the cells
   {
-    const char *e = getenv("GPL_BATCH_STREAMS");
-    if (!env && nbox >= 2 && !(e && atoi(e) == 1)) {
-      long long acc = 0;
-      int q = 0;
-      while (q < nbox - 1 && 2*(acc + (long long)b->hbox[q].nx*b->hbox[q].ny) <= total) {
+    int ng = 2;
+    if (const char *e = getenv("GPL_BATCH_STREAMS")) ng = atoi(e);
+    if (ng < 1) ng = 1;
+    if (ng > gpl_batch::kMaxGroups) ng = gpl_batch::kMaxGroups;
+    if (ng > nbox) ng = nbox;
+    if (env) ng = 1;
+    b->ng = ng;
+    b->gbox[0] = 0;
+    long long acc = 0;
+    int q = 0;
+    for (int g = 1; g < ng; g++) {
+      // at least one box per group, and room for one per later group
+      while (q < nbox - (ng - g) &&
+             (q < b->gbox[g - 1] + 1 ||
+              (long long)ng*(2*acc + (long long)b->hbox[q].nx*b->hbox[q].ny) <= 2LL*g*total)) {
         acc += (long long)b->hbox[q].nx*b->hbox[q].ny;
         q++;
       }
-      b->split = q > 0 ? q : 1;
+      b->gbox[g] = q;
     }
-    const int s = b->split;
-    b->xbeg[0] = 0; b->xbeg[1] = s ? xstart[s] : b->nx_items; b->xbeg[2] = b->nx_items;
-    b->tbeg[0] = 0; b->tbeg[1] = s ? tstart[s] : b->ntiles; b->tbeg[2] = b->ntiles;
-    for (int m = 0; m < 3; m++) {
-      b->ybeg[m][0] = 0; b->ybeg[m][1] = yb[m][s ? s : nbox]; b->ybeg[m][2] = b->ny_items[m];
+    b->gbox[ng] = nbox;
+    xstart.push_back(b->nx_items);
+    tstart.push_back(b->ntiles);
+    for (int g = 0; g <= ng; g++) {
+      b->xbeg[g] = xstart[b->gbox[g]];
+      b->tbeg[g] = tstart[b->gbox[g]];
+      for (int m = 0; m < 3; m++) b->ybeg[m][g] = yb[m][b->gbox[g]];
     }
-    if (s && (hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking) != hipSuccess ||
-              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-              hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess))
-      return GPL_EHIP;
+    b->gs[0] = b->stream;
+    if (ng > 1 && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) return GPL_EHIP;
+    for (int g = 1; g < ng; g++)
+      if (hipStreamCreateWithFlags(&b->gs[g], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&b->ev_join[g], hipEventDisableTiming) != hipSuccess)
+        return GPL_EHIP;
   }
   rc |= dev_alloc(b->bytes, &b->V, 8*nc*rs);
   rc |= dev_alloc(b->bytes, (void **)&b->U, 6*nc*sizeof(double));
@@ -488,16 +508,13 @@ static void enqueue_stage(gpl_batch *b, const double *uin, double *uout, double 
   const gpl_config &c = b->cfg;
   // the box groups, each on its own stream; the kernels index the items of
   // their group and the C_dt partial slots that go with them
-  for (int gi = 0; gi < (b->split ? 2 : 1); gi++) {
-    const hipStream_t st = gi ? b->s2 : b->stream;
-    const int box0 = gi ? b->split : 0, nbg = b->split ? (gi ? b->nbox - b->split : b->split) : b->nbox;
+  for (int gi = 0; gi < b->ng; gi++) {
+    const hipStream_t st = b->gs[gi];
+    const int box0 = b->gbox[gi], nbg = b->gbox[gi + 1] - b->gbox[gi];
     q.xitems = b->xitems + b->xbeg[gi]; q.nx_items = b->xbeg[gi + 1] - b->xbeg[gi];
     q.yitems = b->yitems[m] + b->ybeg[m][gi]; q.ny_items = b->ybeg[m][gi + 1] - b->ybeg[m][gi];
     q.cdt_part = b->cdt_part + b->ybeg[m][gi];
     q.tiles = b->tiles + b->tbeg[gi]; q.ntiles = b->tbeg[gi + 1] - b->tbeg[gi];
-    if (!b->split) {
-      q.nx_items = b->nx_items; q.ny_items = b->ny_items[m]; q.ntiles = b->ntiles;
-    }
     const int gx = grid_for(b, (q.nx_items + 3)/4), gy = grid_for(b, (q.ny_items + 3)/4);
     const int gt = grid_for(b, q.ntiles);
     // Boundary (Src/rk_step.cpp: at the start of every stage) on padded boxes
@@ -529,9 +546,9 @@ static void enqueue_step(gpl_batch *b, int advance_dt)
   const RKW w3 = {1.0/3.0, 2.0/3.0, 2.0/3.0};
   double *f0 = b->vs[b->cur], *f1 = b->vs[(b->cur + 1) % 3], *f2 = b->vs[(b->cur + 2) % 3];
   if (b->zcnt) launch_zone_reset(b->zcnt, b->stream);
-  if (b->split) {                      // the second group's stream joins here
+  if (b->ng > 1) {                     // the other groups' streams join here
     (void)hipEventRecord(b->ev_fork, b->stream);
-    (void)hipStreamWaitEvent(b->s2, b->ev_fork, 0);
+    for (int g = 1; g < b->ng; g++) (void)hipStreamWaitEvent(b->gs[g], b->ev_fork, 0);
   }
   enqueue_stage_r(b, b->U, b->Us, f0, f1, f0, w1, 1, 0);            // -> Us, f1
   if (b->cfg.time_stepping != GPL_RK3) {
@@ -542,18 +559,14 @@ static void enqueue_step(gpl_batch *b, int advance_dt)
     enqueue_stage_r(b, b->Us, b->U, f2, f1, f0, w3, 0, 2);          // -> U, f1
     b->cur = (b->cur + 1) % 3;
   }
-  if (!b->split) {
-    batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->nbox, b->ctl, b->cfg.cfl,
-                      b->cfg.cfl_max_var, advance_dt, b->stream);
-    return;
-  }
   // per group: each box's dt reads only its own partial slots
-  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx], b->split, b->ctl, b->cfg.cfl,
-                    b->cfg.cfl_max_var, advance_dt, b->stream);
-  batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx] + b->split, b->nbox - b->split,
-                    b->ctl + b->split, b->cfg.cfl, b->cfg.cfl_max_var, advance_dt, b->s2);
-  (void)hipEventRecord(b->ev_join, b->s2);
-  (void)hipStreamWaitEvent(b->stream, b->ev_join, 0);
+  for (int g = 0; g < b->ng; g++)
+    batch2d_launch_dt(b->cdt_part, b->part_beg[b->march_idx] + b->gbox[g], b->gbox[g + 1] - b->gbox[g],
+                      b->ctl + b->gbox[g], b->cfg.cfl, b->cfg.cfl_max_var, advance_dt, b->gs[g]);
+  for (int g = 1; g < b->ng; g++) {
+    (void)hipEventRecord(b->ev_join[g], b->gs[g]);
+    (void)hipStreamWaitEvent(b->stream, b->ev_join[g], 0);
+  }
 }
 
 // the face buffers rotate with period 3 (RK2: cur += 2, RK3: cur += 1)
