@@ -220,8 +220,10 @@ const char *gpl_strerror(int code);
 
 /* ---------------------------------------------------------------------------
  * Batched 2D MHD: many independent boxes of uneven sizes in one device
- * state, advanced together (one launch per phase over all boxes,
- * one HIP graph per step), each box with its own dt.  No counterpart in the
+ * state, advanced together (one launch per phase over each of two groups
+ * of boxes, the groups on two streams of the batch; one HIP graph per
+ * step), each box with its own dt.  GPL_BATCH_STREAMS=n in the environment
+ * at gpl_batch_create sets the number of groups (1..4).  No counterpart in the
  * reference, which runs one domain per process; each box computes exactly
  * what gpl_create/gpl_run would compute for it alone (FP64: bit for bit).
  * ------------------------------------------------------------------------- */
