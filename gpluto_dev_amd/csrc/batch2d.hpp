@@ -1,6 +1,7 @@
 // Declarations of the batched 2D MHD stage kernels: many independent boxes
-// of uneven sizes advanced together, one launch per phase, each launch a
-// persistent grid striding over a per-batch work list.  Boxes are periodic,
+// of uneven sizes advanced together, one launch per phase and group of boxes,
+// each launch a persistent grid striding over the group's slice of a work
+// list.  Boxes are periodic,
 // or, with an outflow or reflective side, stored as the reference's own array
 // (two ghost layers per side and one spare layer for the outermost face) and
 // computed on as a periodic box of that size, with the ghosts refilled at

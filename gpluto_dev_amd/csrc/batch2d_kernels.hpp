@@ -5,10 +5,11 @@
 // refills the ghosts before every stage); C_dt and error bits come from the
 // reference's cells and faces only.
 //
-// Every phase is one launch over all boxes: a persistent grid whose
-// workgroups stride over a per-batch work list (sweep items are one
-// wavefront each, cell tiles one workgroup each), so boxes from 32^2 to
-// 512^2 share the chip in a single launch and the step is one HIP graph.
+// Every phase is one launch over a group of boxes: a persistent grid whose
+// workgroups stride over the group's slice of a work list (sweep items are
+// one wavefront each, cell tiles one workgroup each), so boxes from 32^2 to
+// 512^2 share the chip in a single launch; the host runs the groups on
+// streams of their own and the step is one HIP graph.
 // The per-cell arithmetic is the 3D path's, restricted to DIMENSIONS 2 with
 // three field components (reference 2D MHD, Src/MHD/mhd.hpp:21-41): B1, B2
 // live on faces (CT), B3 is cell-centred and evolved by its flux.

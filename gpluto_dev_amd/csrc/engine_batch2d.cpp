@@ -1,7 +1,8 @@
 // Batched 2D ideal MHD: many independent boxes of uneven sizes (periodic, or
 // stored with the reference's ghost layers when a side is outflow or
-// reflective) in one device state, advanced together (one launch per phase over all boxes,
-// one HIP graph per step), each box with its own dt.  Launch shapes (the
+// reflective) in one device state, advanced together (one launch per phase
+// over each group of boxes, the groups on streams of their own, one HIP graph
+// per step), each box with its own dt.  Launch shapes (the
 // y-march length and the persistent grid size) are chosen by an autotuner
 // that times the candidates on the run's own first steps: every candidate
 // computes the same bits, so tuning never changes a result.
