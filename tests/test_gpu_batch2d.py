@@ -169,13 +169,13 @@ def test_fp32_batch_within_1e5_of_fp64(emf, sides):
 @pytest.mark.parametrize("precision,emf,sides", [("fp64", "UCT_HLL", None), ("mixed", "UCT_HLLD", None),
                                                  ("mixed", "UCT_HLL", ("outflow", "reflective") * 2 + ("outflow",) * 2)])
 def test_two_box_groups_match_one_stream(precision, emf, sides, monkeypatch):
-    """The two box groups on two streams (the default) against the whole
-    batch on one stream: the same bits, times and dts, through the autotuned
-    graph replay and the eager steps after it."""
+    """The two box groups on two streams (the default), and four groups,
+    against the whole batch on one stream: the same bits, times and dts,
+    through the autotuned graph replay and the eager steps after it."""
     sizes = [(64, 64), (150, 150), (37, 37), (256, 256), (5, 5), (100, 37)]
     ics = _ics(sizes, 13)
     out = {}
-    for streams in ("1", "2"):
+    for streams in ("1", "2", "4"):
         monkeypatch.setenv("GPL_BATCH_STREAMS", streams)
         print(f"case {precision} {emf} {sides} streams={streams}", flush=True)
         g = _batch(sizes, precision=precision, emf=emf, sides=sides)
@@ -185,12 +185,14 @@ def test_two_box_groups_match_one_stream(precision, emf, sides, monkeypatch):
         assert err == 0
         out[streams] = (t, dt, [g.download(b) for b in range(len(sizes))])
         g.close()
-    (t1, dt1, s1), (t2, dt2, s2) = out["1"], out["2"]
-    assert np.array_equal(t1, t2) and np.array_equal(dt1, dt2)
-    for b, ((a1, f1, u1), (a2, f2, u2)) in enumerate(zip(s1, s2)):
-        assert np.array_equal(a1, a2), f"box {b} {sizes[b]}: cells differ"
-        assert np.array_equal(f1, f2), f"box {b} {sizes[b]}: faces differ"
-        assert np.array_equal(u1, u2), f"box {b} {sizes[b]}: Uc differs"
+    t1, dt1, s1 = out["1"]
+    for streams in ("2", "4"):
+        t2, dt2, s2 = out[streams]
+        assert np.array_equal(t1, t2) and np.array_equal(dt1, dt2), f"{streams} groups: time/dt differ"
+        for b, ((a1, f1, u1), (a2, f2, u2)) in enumerate(zip(s1, s2)):
+            assert np.array_equal(a1, a2), f"{streams} groups, box {b} {sizes[b]}: cells differ"
+            assert np.array_equal(f1, f2), f"{streams} groups, box {b} {sizes[b]}: faces differ"
+            assert np.array_equal(u1, u2), f"{streams} groups, box {b} {sizes[b]}: Uc differs"
 
 
 def test_uct_hlld_needs_hlld():
